@@ -1,0 +1,131 @@
+/*
+ * littleharsh_amd.h -- C-ABI of the MI355X (gfx950) solver for the Littleharsh channel-DNS
+ * time-step hot path.  Plain pointers and sizes only; every entry point returns an int
+ * status (LH_OK = 0) and, on failure, leaves a message for lh_last_error().
+ *
+ * The boundary sits at the RK3 substep: one lh_substep call replaces the body of the
+ * reference's `do kRK = 1,3` loop, littleharsh.f90:166-188, and lh_step also covers the
+ * constant-pressure-gradient flow-rate bookkeeping after it, littleharsh.f90:193-197.
+ *
+ * Conventions (single resolution, every column in the reference's middle band):
+ *   ny    reference input ny (input.in:19), dsty canopy planes per wall (input.in:62)
+ *   u,w   planes j = -dsty .. ny+dsty       (jlim(:,ugrid,2), start.f90:1158-1159)
+ *   v     planes j = -dsty .. ny+dsty-1     (jlim(:,vgrid,2), start.f90:1165-1166)
+ *   p     planes j = -dsty+1 .. ny+dsty-1   (jlim(:,pgrid,2), start.f90:1172-1173)
+ *   column c = k*(nx/2+1) + i, i = 0..nx/2, k = 0..nz-1 in FFT order (columns_i/k at one
+ *   rank, start.f90:1105-1119).
+ * Host state arrays are in the reference's own layout: complex(8) f(j, column), i.e.
+ * interleaved (re, im) doubles with j fastest (type cfield, declaration.f90:133-135).
+ *
+ * One handle per process and device; calls on one handle must not overlap.  Work is
+ * issued on one HIP stream owned by the handle and every call is synchronous at return.
+ */
+#ifndef LITTLEHARSH_AMD_H
+#define LITTLEHARSH_AMD_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LH_OK 0
+#define LH_ERR_ARG 1      /* invalid argument or unsupported size                      */
+#define LH_ERR_HIP 2      /* HIP runtime error                                         */
+#define LH_ERR_NUMERIC 3  /* non-finite value detected                                 */
+
+/* Precision of the nonlinear (FFT + product) stage.  The tridiagonal solves, the pressure
+ * Poisson problem, the mean-flow mode and the statistics always run in FP64. */
+#define LH_PREC_FP64 0
+#define LH_PREC_MIXED 1
+
+typedef struct lh_config {
+  /* sizes: Fourier modes N(1,1), N(2,1) and collocation points Ngal(1,1), Ngal(2,1)
+   * (start.f90:97-118); the solver requires px = 3*nx/2, pz = 3*nz/2 */
+  int nx, nz, px, pz;
+  int ny, dsty;
+  /* physics (start.f90:69-76, 140-151) */
+  double re, alp, bet;
+  int flag_ctpress;   /* 1: constant mean pressure gradient mpgx; 0: constant flow rate */
+  double mpgx;        /* initial mean pressure gradient */
+  double qxt;         /* target flow rate QxT for flag_ctpress == 0 (start.f90:1546-1548) */
+  double ddthetavi;   /* 1/dtheta (y_grid_canopy.f90:43) */
+  /* metric arrays over the full reference index ranges (y_grid_canopy.f90:46-52):
+   * yu, dthdyu: ny+2*dsty+1 values (j = -dsty..ny+dsty);
+   * yv, dthdyv: ny+2*dsty values   (j = -dsty..ny+dsty-1);
+   * dyu2i, dyv2i: Fortran dyu2i(3, j) -- three weights per plane, weight index fastest */
+  const double *yu, *yv, *dthdyu, *dthdyv, *dyu2i, *dyv2i;
+  const double *gridweighting;  /* gridweighting(midband,1:2) (start.f90:500-501) */
+  const double *rk;             /* aRK(1:3), bRK(1:3), gRK(1:3), cRK(1:3), dRK(1:3) */
+  /* wavenumbers from def_k (start.f90:688-701): k1x = Im k1F_x (nx/2+1), k2x = k2F_x,
+   * k1z = Im k1F_z (nz, FFT order), k2z = k2F_z */
+  const double *k1x, *k2x, *k1z, *k2z;
+  /* immersed-boundary points list_ib(3, nlist, grid): 1-based i in 1..px, k in 1..pz,
+   * plane j (geom_sym_canopy.f90:98-234 after the getbounds plane filter) */
+  int nlist_u, nlist_v;
+  const int *list_u, *list_v;
+  int tip_u[2], tip_v[2];  /* tip planes (tridLU_3D_IB.f90:237, 317) */
+  /* canopy tiling for the conditional statistics (start.f90:456-469) */
+  int ntilex, ntilez;
+  int band_v[2];           /* N(3,1), N(3,2): v-grid band limits (y_grid_canopy.f90:152-163) */
+  int precision;           /* LH_PREC_FP64 or LH_PREC_MIXED */
+} lh_config;
+
+typedef struct lh_handle lh_handle;
+
+/* Allocate device state and upload the set-up; replaces the solver part of start()
+ * (start.f90:392-652: FFT initialisation, RK tables, LU_buildP).  All pointers are copied
+ * and may be freed on return. */
+int lh_create(const lh_config* cfg, lh_handle** out);
+int lh_destroy(lh_handle* h);
+const char* lh_last_error(const lh_handle* h);
+
+/* Upload / download u1, u2, u3, p (reference layout, see above).  Nu arrays may be NULL;
+ * they carry the previous substep's nonlinear term (only read when cRK(kRK) != 0). */
+int lh_set_state(lh_handle* h, const double* u1, const double* u2, const double* u3,
+                 const double* p);
+int lh_get_state(lh_handle* h, double* u1, double* u2, double* u3, double* p);
+int lh_set_nonlinear(lh_handle* h, const double* nu1, const double* nu2, const double* nu3);
+int lh_get_nonlinear(lh_handle* h, double* nu1, double* nu2, double* nu3);
+
+/* One RK3 substep kRK in 1..3 with time step dt: RHS0_u1/u2/u3, nonlinear, solveU/V/U,
+ * meanflow_ctU (flag_ctpress == 0), solveP, v_corr (littleharsh.f90:166-188).
+ * dt <= 0 uses the device CFL time step of the last lh_cfl_dt / lh_step_cfl. */
+int lh_substep(lh_handle* h, int krk, double dt);
+
+/* One full RK3 step (three substeps) followed by meanflow_ctP when flag_ctpress == 1
+ * (littleharsh.f90:164-197).  Launches are replayed from a HIP graph per dt mode. */
+int lh_step(lh_handle* h, double dt);
+
+/* CFL-controlled step: dt = cfl / max(|u|/dx + |v|/dy + |w|/dz) over the physical points
+ * of the state at the start of the step, evaluated on the device (no host round trip);
+ * *dt_used (may be NULL) receives the step taken. */
+int lh_step_cfl(lh_handle* h, double cfl, double* dt_used);
+int lh_cfl_dt(lh_handle* h, double cfl, double* dt);
+
+/* Scalars: mean pressure gradient mpgx, flow rate Qx (flowrateIm, littleharsh.f90:412-431),
+ * the last dt and the last CFL maximum. */
+int lh_get_scalars(lh_handle* h, double* mpgx, double* qx, double* dt, double* cflmax);
+
+/* max |div u| over the p grid (error, littleharsh.f90:323-352). */
+int lh_max_div(lh_handle* h, double* err);
+
+/* Statistics (stats.f90:1-108): per-plane sums over the physical points of
+ * u, u^2, v, v^2, w, w^2, p, p^2, u*v_itp, u*w, v*w_itp, wx, wx^2.  Output layout
+ * (13 rows, each over the full plane range of its grid; rows 1-2, 5-6, 9-10 on the u
+ * grid, 3-4, 11-13 on the v grid, 7-8 on the p grid) -- see lh_stats_read. */
+int lh_stats_accumulate(lh_handle* h);
+/* out: 13 * (ny+2*dsty+1) doubles, row r at out[r*(ny+2*dsty+1) + (j+dsty)]; *nstat gets
+ * the number of accumulated samples.  reset != 0 clears the accumulators. */
+int lh_stats_read(lh_handle* h, double* out, int* nstat, int reset);
+/* Conditional statistics folded onto one canopy tile (stats.f90:110-253): out holds
+ * 13 arrays of (dnx, dnz, nplanes_band) for the bottom band then the top band. */
+int lh_cond_stats_read(lh_handle* h, double* out, long long out_len, int reset);
+
+/* Spectra (spectra.f90:1-56): sp(j, column) += |f|^2 for u, v, w, p and Re(u conj v_itp).
+ * out: 5 arrays in reference layout sp(j, column) over the u/v/u/u/p grids. */
+int lh_spectra_accumulate(lh_handle* h);
+int lh_spectra_read(lh_handle* h, double* out, int reset);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* LITTLEHARSH_AMD_H */

@@ -1,0 +1,643 @@
+// lh_api.hip -- C-ABI of the MI355X solver (declarations in include/littleharsh_amd.h).
+//
+// The handle owns one HIP stream, the device-resident state in [plane][column] layout and
+// all work buffers.  lh_substep issues the kernels of one RK3 substep:
+//   NL stage   : k_zinv (u,w,v) -> k_xpass_nl (products, CFL max) -> k_zfwd (6 products)
+//                -> k_assemble (RHS0 + nonlinear + rhs of solveU/V, /axz)
+//   IB stage   : k_zinv (rhs and Lu_y on canopy planes) -> k_xpass_ib -> k_zfwd
+//   column     : k_prep (LU of the shared velocity matrices) -> k_vel (Thomas u,w,v)
+//                -> k_ctU (constant flow rate only) -> k_pressure (solveP + v_corr)
+// lh_step replays a captured HIP graph of three substeps (+ meanflow_ctP).
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/littleharsh_amd.h"
+#include "lh_kernels.hpp"
+
+using namespace lh;
+
+struct lh_handle {
+  std::string err;
+  int dev = 0;
+  hipStream_t st = nullptr;
+  Dims dm{};
+  int prec = LH_PREC_FP64;
+  int flag_ctpress = 1;
+  double rk[15];
+  double gw[2];
+  int tip_u[2], tip_v[2];
+  int ntilex = 1, ntilez = 1, band_v[2] = {0, 0};
+  // metrics (device, slot indexed)
+  double *yu = nullptr, *yv = nullptr, *dthdyu = nullptr, *dthdyv = nullptr, *dyu2i = nullptr,
+         *dyv2i = nullptr;
+  double *k1x = nullptr, *k2x = nullptr, *k1z = nullptr, *k2z = nullptr;
+  double* scal = nullptr;
+  // state
+  double2 *u1 = nullptr, *u2 = nullptr, *u3 = nullptr, *p = nullptr;
+  double2 *nu1 = nullptr, *nu2 = nullptr, *nu3 = nullptr;
+  double2 *r1 = nullptr, *r2 = nullptr, *r3 = nullptr, *psi = nullptr, *tipl = nullptr;
+  double2* ph = nullptr;                       // product spectra [6][nyu-2][ncol]
+  void *zu = nullptr, *zw = nullptr, *zv = nullptr, *zp = nullptr;
+  double *dg1 = nullptr, *dg2 = nullptr, *dg3 = nullptr;
+  double *ayu = nullptr, *ayv = nullptr, *ctwork = nullptr;
+  void *twx = nullptr, *twz = nullptr;         // twiddles in the NL precision
+  void *twx64 = nullptr, *twz64 = nullptr;     // FP64 twiddles (IB, statistics)
+  // canopy
+  unsigned char* masks = nullptr;
+  int nmask = 0;
+  int *ibu_planes = nullptr, *ibv_planes = nullptr;
+  int nibu = 0, nibv = 0;
+  IBEntry* ibent = nullptr;
+  int nibent = 0;
+  // host copies
+  std::vector<int> h_ibu, h_ibv;
+  std::vector<int> mask_of_u, mask_of_v;
+  // graphs
+  hipGraphExec_t gexec[2] = {nullptr, nullptr};   // [0] fixed dt, [1] device CFL dt
+  double graph_dt = -1.0;
+};
+
+static int fail(lh_handle* h, int code, const std::string& msg) {
+  if (h) h->err = msg;
+  return code;
+}
+
+#define HIPCHK(h, x)                                                                         \
+  do {                                                                                       \
+    hipError_t e_ = (x);                                                                     \
+    if (e_ != hipSuccess)                                                                    \
+      return fail(h, LH_ERR_HIP, std::string(#x) + ": " + hipGetErrorString(e_));            \
+  } while (0)
+
+template <typename T>
+static hipError_t dalloc(T** p, size_t n) {
+  return hipMalloc((void**)p, n * sizeof(T) + 64);
+}
+
+static bool supported_len(int n) {
+  switch (n) {
+    case 32: case 48: case 64: case 72: case 96: case 144: case 384: case 576: return true;
+    default: return false;
+  }
+}
+
+// ---------------------------------------------------------------------------------------
+// templated launchers, dispatched on the transform lengths
+// ---------------------------------------------------------------------------------------
+template <int N> constexpr int zlines() { return N >= 384 ? 4 : (Plan<N>::T >= 8 ? 16 : 32); }
+
+template <int PZ, typename TR>
+static void launch_zinv_t(lh_handle* h, const ZJobs& jobs, int njobs, int maxn, const void* tw) {
+  constexpr int L = zlines<PZ>();
+  dim3 grid((h->dm.mx + L - 1) / L, maxn, njobs);
+  size_t sh = (size_t)L * (PZ + 1) * sizeof(cx<TR>);
+  hipLaunchKernelGGL((k_zinv<PZ, L, TR>), grid, dim3(L * Plan<PZ>::T), sh, h->st, jobs, h->dm,
+                     (const cx<TR>*)tw);
+}
+template <int PZ, typename TR>
+static void launch_zfwd_t(lh_handle* h, const ZJobs& jobs, int njobs, int maxn, const void* tw) {
+  constexpr int L = zlines<PZ>();
+  dim3 grid((h->dm.mx + L - 1) / L, maxn, njobs);
+  size_t sh = (size_t)L * (PZ + 1) * sizeof(cx<TR>);
+  hipLaunchKernelGGL((k_zfwd<PZ, L, TR>), grid, dim3(L * Plan<PZ>::T), sh, h->st, jobs, h->dm,
+                     (const cx<TR>*)tw);
+}
+template <int PX, typename TR>
+static void launch_xnl_t(lh_handle* h, const XNL& a, const void* tw) {
+  dim3 grid(h->dm.pz / 2, h->dm.nyu - 2);
+  size_t sh = (size_t)6 * (PX + 1) * sizeof(cx<TR>);
+  hipLaunchKernelGGL((k_xpass_nl<PX, TR>), grid, dim3(6 * Plan<PX>::T), sh, h->st, a, h->dm,
+                     (const cx<TR>*)tw);
+}
+template <int PX, typename TR>
+static void launch_xib_t(lh_handle* h, const XIB& a, int nent, const void* tw) {
+  dim3 grid(h->dm.pz / 2, nent);
+  size_t sh = (size_t)2 * (PX + 1) * sizeof(cx<TR>);
+  hipLaunchKernelGGL((k_xpass_ib<PX, TR>), grid, dim3(2 * Plan<PX>::T), sh, h->st, a, h->dm,
+                     (const cx<TR>*)tw);
+}
+
+#define LH_DISPATCH_LEN(n, F, TR, ...)          \
+  switch (n) {                                   \
+    case 32: F<32, TR>(__VA_ARGS__); break;      \
+    case 48: F<48, TR>(__VA_ARGS__); break;      \
+    case 64: F<64, TR>(__VA_ARGS__); break;      \
+    case 72: F<72, TR>(__VA_ARGS__); break;      \
+    case 96: F<96, TR>(__VA_ARGS__); break;      \
+    case 144: F<144, TR>(__VA_ARGS__); break;    \
+    case 384: F<384, TR>(__VA_ARGS__); break;    \
+    case 576: F<576, TR>(__VA_ARGS__); break;    \
+    default: break;                              \
+  }
+
+static void launch_zinv(lh_handle* h, const ZJobs& jobs, int njobs, int maxn, bool fp64) {
+  if (fp64 || h->prec == LH_PREC_FP64) {
+    LH_DISPATCH_LEN(h->dm.pz, launch_zinv_t, double, h, jobs, njobs, maxn, fp64 ? h->twz64 : h->twz)
+  } else {
+    LH_DISPATCH_LEN(h->dm.pz, launch_zinv_t, float, h, jobs, njobs, maxn, h->twz)
+  }
+}
+static void launch_zfwd(lh_handle* h, const ZJobs& jobs, int njobs, int maxn, bool fp64) {
+  if (fp64 || h->prec == LH_PREC_FP64) {
+    LH_DISPATCH_LEN(h->dm.pz, launch_zfwd_t, double, h, jobs, njobs, maxn, fp64 ? h->twz64 : h->twz)
+  } else {
+    LH_DISPATCH_LEN(h->dm.pz, launch_zfwd_t, float, h, jobs, njobs, maxn, h->twz)
+  }
+}
+
+// ---------------------------------------------------------------------------------------
+// substep
+// ---------------------------------------------------------------------------------------
+static void issue_substep(lh_handle* h, int krk /*0-based*/, bool cfl_mode) {
+  const Dims& dm = h->dm;
+  const double aR = h->rk[krk], bR = h->rk[3 + krk], gR = h->rk[6 + krk], cR = h->rk[9 + krk],
+               dR = h->rk[12 + krk];
+  const double bre = bR / dm.re;
+  const int nlev = dm.nyu - 2;
+  const bool f64 = h->prec == LH_PREC_FP64;
+  // ---- NL stage: inverse z transforms of u, w (all u planes) and v (all v planes)
+  {
+    ZJobs jobs{};
+    jobs.j[0] = ZJob{h->u1, h->zu, nullptr, 0, dm.nyu, 0};
+    jobs.j[1] = ZJob{h->u3, h->zw, nullptr, 0, dm.nyu, 0};
+    jobs.j[2] = ZJob{h->u2, h->zv, nullptr, 0, dm.nyv, 0};
+    launch_zinv(h, jobs, 3, dm.nyu, false);
+  }
+  if (krk == 0)
+    hipLaunchKernelGGL(k_set_scalar, dim3(1), dim3(1), 0, h->st, h->scal, (int)S_CFLMAX, 0.0);
+  {
+    XNL a{h->zu, h->zw, h->zv, h->zp, h->dthdyu, h->scal, krk == 0 ? 1 : 0};
+    if (f64) {
+      LH_DISPATCH_LEN(dm.px, launch_xnl_t, double, h, a, h->twx)
+    } else {
+      LH_DISPATCH_LEN(dm.px, launch_xnl_t, float, h, a, h->twx)
+    }
+  }
+  if (krk == 0 && cfl_mode) hipLaunchKernelGGL(k_cfl_dt, dim3(1), dim3(1), 0, h->st, h->scal);
+  {
+    ZJobs jobs{};
+    // six products, each [nlev] slots of zp -> ph[q]
+    // (four jobs per launch: two launches)
+    for (int q0 = 0; q0 < 6; q0 += 3) {
+      for (int q = 0; q < 3; ++q) {
+        const int qq = q0 + q;
+        const size_t zslot = (size_t)qq * nlev;
+        jobs.j[q] = ZJob{(const double2*)h->zp, h->ph + (size_t)qq * nlev * dm.ncol, nullptr, 0,
+                         nlev, (int)zslot};
+      }
+      launch_zfwd(h, jobs, 3, nlev, false);
+    }
+  }
+  {
+    ASM a{};
+    a.u1 = h->u1; a.u2 = h->u2; a.u3 = h->u3; a.p = h->p;
+    a.nu1 = h->nu1; a.nu2 = h->nu2; a.nu3 = h->nu3;
+    a.ph = h->ph; a.r1 = h->r1; a.r2 = h->r2; a.r3 = h->r3; a.tipl = h->tipl;
+    a.dyu2i = h->dyu2i; a.dyv2i = h->dyv2i; a.dthdyu = h->dthdyu; a.dthdyv = h->dthdyv;
+    a.k1x = h->k1x; a.k2x = h->k2x; a.k1z = h->k1z; a.k2z = h->k2z; a.scal = h->scal;
+    a.tip_u0 = h->tip_u[0] + dm.d; a.tip_u1 = h->tip_u[1] + dm.d;
+    a.tip_v0 = h->tip_v[0] + dm.d; a.tip_v1 = h->tip_v[1] + dm.d;
+    a.C1 = aR / dm.re; a.C2 = -cR; a.dR = dR; a.g = gR; a.bre = bre;
+    dim3 grid((dm.ncol + 255) / 256, dm.nyu);
+    hipLaunchKernelGGL(k_assemble, grid, dim3(256), 0, h->st, a, dm);
+  }
+  // ---- IB stage (FP64 transforms): canopy planes of rhs u, w, v and Lu_y on tip planes
+  if (h->nibent > 0) {
+    ZJobs jobs{};
+    int slot = 0;
+    jobs.j[0] = ZJob{h->r1, h->zp, h->ibu_planes, 0, h->nibu, slot}; slot += h->nibu;
+    jobs.j[1] = ZJob{h->r3, h->zp, h->ibu_planes, 0, h->nibu, slot}; slot += h->nibu;
+    jobs.j[2] = ZJob{h->r2, h->zp, h->ibv_planes, 0, h->nibv, slot}; slot += h->nibv;
+    jobs.j[3] = ZJob{h->tipl, h->zp, nullptr, 0, 6, slot};
+    int maxn = h->nibu > h->nibv ? h->nibu : h->nibv;
+    if (maxn < 6) maxn = 6;
+    launch_zinv(h, jobs, 4, maxn, true);
+    XIB a{h->zp, h->ibent, h->masks, h->scal, bre};
+    LH_DISPATCH_LEN(dm.px, launch_xib_t, double, h, a, h->nibent, h->twx64)
+    ZJobs fj{};
+    slot = 0;
+    fj.j[0] = ZJob{(const double2*)h->zp, h->r1, h->ibu_planes, 0, h->nibu, slot}; slot += h->nibu;
+    fj.j[1] = ZJob{(const double2*)h->zp, h->r3, h->ibu_planes, 0, h->nibu, slot}; slot += h->nibu;
+    fj.j[2] = ZJob{(const double2*)h->zp, h->r2, h->ibv_planes, 0, h->nibv, slot};
+    launch_zfwd(h, fj, 3, h->nibu > h->nibv ? h->nibu : h->nibv, true);
+  }
+  // ---- column stage
+  {
+    PREP a{h->dyu2i, h->dyv2i, h->scal, h->ayu, h->ayv, h->gw[0], h->gw[1], bre, dm.nyu, dm.nyv};
+    hipLaunchKernelGGL(k_prep, dim3(1), dim3(64), 0, h->st, a);
+  }
+  {
+    VEL a{h->r1, h->r2, h->r3, h->u1, h->u2, h->u3, h->ayu, h->ayv};
+    hipLaunchKernelGGL(k_vel, dim3((dm.ncol + 255) / 256, 3), dim3(256), 0, h->st, a, dm);
+  }
+  if (h->flag_ctpress == 0) {
+    CTU a{h->u1, h->scal, h->ctwork, h->dyu2i, h->yv, h->gw[0], h->gw[1], bre, dR, dm.ny, dm.d,
+          dm.ncol};
+    hipLaunchKernelGGL(k_ctU, dim3(1), dim3(64), 0, h->st, a);
+  }
+  {
+    PRES a{h->u1, h->u2, h->u3, h->p, h->psi, h->dg1, h->dg2, h->dg3, h->dthdyu, h->dthdyv,
+           h->k1x, h->k1z, h->scal, aR + bR, h->gw[0], h->gw[1]};
+    hipLaunchKernelGGL(k_pressure, dim3((dm.ncol + 255) / 256), dim3(256), 0, h->st, a, dm);
+  }
+}
+
+static void issue_step(lh_handle* h, bool cfl_mode) {
+  for (int k = 0; k < 3; ++k) issue_substep(h, k, cfl_mode);
+  if (h->flag_ctpress == 1)
+    hipLaunchKernelGGL(k_flowrate, dim3(1), dim3(64), 0, h->st, h->u1, h->yv, h->scal, h->dm.ny,
+                       h->dm.d, h->dm.ncol);
+}
+
+// ---------------------------------------------------------------------------------------
+// C-ABI
+// ---------------------------------------------------------------------------------------
+extern "C" {
+
+const char* lh_last_error(const lh_handle* h) { return h ? h->err.c_str() : "null handle"; }
+
+int lh_create(const lh_config* c, lh_handle** out) {
+  if (!c || !out) return LH_ERR_ARG;
+  *out = nullptr;
+  lh_handle* h = new lh_handle();
+  auto bad = [&](const std::string& m) { h->err = m; *out = h; return LH_ERR_ARG; };
+  if (c->nx <= 0 || c->nz <= 0 || c->nx % 2 || c->nz % 2) return bad("nx, nz must be positive and even");
+  if (c->px != 3 * c->nx / 2 || c->pz != 3 * c->nz / 2 || c->nx % 4 || c->nz % 4)
+    return bad("px = 3*nx/2 and pz = 3*nz/2 are required (nx, nz multiples of 4)");
+  if (!supported_len(c->px) || !supported_len(c->pz)) return bad("unsupported collocation size");
+  if (c->dsty < 1 || c->ny < 4) return bad("ny >= 4 and dsty >= 1 required");
+  if (c->precision != LH_PREC_FP64 && c->precision != LH_PREC_MIXED) return bad("bad precision");
+  if (!c->yu || !c->yv || !c->dthdyu || !c->dthdyv || !c->dyu2i || !c->dyv2i || !c->gridweighting ||
+      !c->rk || !c->k1x || !c->k2x || !c->k1z || !c->k2z)
+    return bad("null set-up array");
+  if ((c->nlist_u > 0 && !c->list_u) || (c->nlist_v > 0 && !c->list_v)) return bad("null list");
+  Dims& dm = h->dm;
+  dm.nx = c->nx; dm.nz = c->nz; dm.px = c->px; dm.pz = c->pz;
+  dm.mx = c->nx / 2 + 1; dm.ncol = dm.mx * c->nz;
+  dm.ny = c->ny; dm.d = c->dsty;
+  dm.nyu = c->ny + 2 * c->dsty + 1; dm.nyv = c->ny + 2 * c->dsty; dm.nyp = c->ny + 2 * c->dsty - 1;
+  dm.dxi = c->px * c->alp / (2.0 * M_PI);
+  dm.dzi = c->pz * c->bet / (2.0 * M_PI);
+  dm.re = c->re; dm.ddthetavi = c->ddthetavi;
+  h->prec = c->precision;
+  h->flag_ctpress = c->flag_ctpress;
+  memcpy(h->rk, c->rk, sizeof(h->rk));
+  h->gw[0] = c->gridweighting[0]; h->gw[1] = c->gridweighting[1];
+  h->tip_u[0] = c->tip_u[0]; h->tip_u[1] = c->tip_u[1];
+  h->tip_v[0] = c->tip_v[0]; h->tip_v[1] = c->tip_v[1];
+  h->ntilex = c->ntilex > 0 ? c->ntilex : 1; h->ntilez = c->ntilez > 0 ? c->ntilez : 1;
+  h->band_v[0] = c->band_v[0]; h->band_v[1] = c->band_v[1];
+  for (int t = 0; t < 2; ++t) {
+    if (c->tip_u[t] < -dm.d + 1 || c->tip_u[t] > dm.ny + dm.d - 1) return bad("tip_u out of range");
+    if (c->tip_v[t] < -dm.d + 1 || c->tip_v[t] > dm.ny + dm.d - 2) return bad("tip_v out of range");
+  }
+  *out = h;
+  HIPCHK(h, hipGetDevice(&h->dev));
+  HIPCHK(h, hipStreamCreate(&h->st));
+  const size_t nu = (size_t)dm.nyu * dm.ncol, nv = (size_t)dm.nyv * dm.ncol,
+               np = (size_t)dm.nyp * dm.ncol;
+  const int nlev = dm.nyu - 2;
+  HIPCHK(h, dalloc(&h->yu, dm.nyu)); HIPCHK(h, dalloc(&h->dthdyu, dm.nyu));
+  HIPCHK(h, dalloc(&h->yv, dm.nyv)); HIPCHK(h, dalloc(&h->dthdyv, dm.nyv));
+  HIPCHK(h, dalloc(&h->dyu2i, 3 * dm.nyu)); HIPCHK(h, dalloc(&h->dyv2i, 3 * dm.nyv));
+  HIPCHK(h, dalloc(&h->k1x, dm.mx)); HIPCHK(h, dalloc(&h->k2x, dm.mx));
+  HIPCHK(h, dalloc(&h->k1z, dm.nz)); HIPCHK(h, dalloc(&h->k2z, dm.nz));
+  HIPCHK(h, dalloc(&h->scal, S_NSCAL));
+  HIPCHK(h, hipMemcpy(h->yu, c->yu, dm.nyu * 8, hipMemcpyHostToDevice));
+  HIPCHK(h, hipMemcpy(h->dthdyu, c->dthdyu, dm.nyu * 8, hipMemcpyHostToDevice));
+  HIPCHK(h, hipMemcpy(h->yv, c->yv, dm.nyv * 8, hipMemcpyHostToDevice));
+  HIPCHK(h, hipMemcpy(h->dthdyv, c->dthdyv, dm.nyv * 8, hipMemcpyHostToDevice));
+  HIPCHK(h, hipMemcpy(h->dyu2i, c->dyu2i, 3 * dm.nyu * 8, hipMemcpyHostToDevice));
+  HIPCHK(h, hipMemcpy(h->dyv2i, c->dyv2i, 3 * dm.nyv * 8, hipMemcpyHostToDevice));
+  HIPCHK(h, hipMemcpy(h->k1x, c->k1x, dm.mx * 8, hipMemcpyHostToDevice));
+  HIPCHK(h, hipMemcpy(h->k2x, c->k2x, dm.mx * 8, hipMemcpyHostToDevice));
+  HIPCHK(h, hipMemcpy(h->k1z, c->k1z, dm.nz * 8, hipMemcpyHostToDevice));
+  HIPCHK(h, hipMemcpy(h->k2z, c->k2z, dm.nz * 8, hipMemcpyHostToDevice));
+  {
+    double s[S_NSCAL] = {0};
+    s[S_MPGX] = c->mpgx;
+    s[S_QXT] = c->qxt;
+    HIPCHK(h, hipMemcpy(h->scal, s, sizeof(s), hipMemcpyHostToDevice));
+  }
+  // state and work
+  HIPCHK(h, dalloc(&h->u1, nu)); HIPCHK(h, dalloc(&h->u3, nu)); HIPCHK(h, dalloc(&h->u2, nv));
+  HIPCHK(h, dalloc(&h->p, np)); HIPCHK(h, dalloc(&h->psi, np));
+  HIPCHK(h, dalloc(&h->nu1, nu)); HIPCHK(h, dalloc(&h->nu3, nu)); HIPCHK(h, dalloc(&h->nu2, nv));
+  HIPCHK(h, dalloc(&h->r1, nu)); HIPCHK(h, dalloc(&h->r3, nu)); HIPCHK(h, dalloc(&h->r2, nv));
+  HIPCHK(h, dalloc(&h->tipl, 6 * (size_t)dm.ncol));
+  HIPCHK(h, dalloc(&h->ph, 6 * (size_t)nlev * dm.ncol));
+  HIPCHK(h, dalloc(&h->dg1, np)); HIPCHK(h, dalloc(&h->dg2, np)); HIPCHK(h, dalloc(&h->dg3, np));
+  HIPCHK(h, dalloc(&h->ayu, 3 * dm.nyu)); HIPCHK(h, dalloc(&h->ayv, 3 * dm.nyv));
+  HIPCHK(h, dalloc(&h->ctwork, 4 * (dm.ny + 2)));
+  const size_t zel = h->prec == LH_PREC_FP64 ? sizeof(double2) : sizeof(float2);
+  const size_t zslot = (size_t)dm.pz * dm.mx;
+  HIPCHK(h, hipMalloc(&h->zu, zel * zslot * dm.nyu));
+  HIPCHK(h, hipMalloc(&h->zw, zel * zslot * dm.nyu));
+  HIPCHK(h, hipMalloc(&h->zv, zel * zslot * dm.nyv));
+  // zp also serves as the FP64 scratch of the IB stage
+  {
+    size_t nzp = zel * zslot * 6 * (size_t)nlev;
+    size_t nib = sizeof(double2) * zslot * (3 * (size_t)dm.nyu + 6);
+    HIPCHK(h, hipMalloc(&h->zp, nzp > nib ? nzp : nib));
+  }
+  HIPCHK(h, hipMemset(h->u1, 0, nu * 16)); HIPCHK(h, hipMemset(h->u3, 0, nu * 16));
+  HIPCHK(h, hipMemset(h->u2, 0, nv * 16)); HIPCHK(h, hipMemset(h->p, 0, np * 16));
+  HIPCHK(h, hipMemset(h->nu1, 0, nu * 16)); HIPCHK(h, hipMemset(h->nu3, 0, nu * 16));
+  HIPCHK(h, hipMemset(h->nu2, 0, nv * 16));
+  // twiddles exp(-2 pi i m/N)
+  auto mk_tw = [&](int n, void** d64, void** dnl) -> int {
+    std::vector<double2> t(n);
+    std::vector<float2> tf(n);
+    for (int m = 0; m < n; ++m) {
+      const double a = 2.0 * M_PI * (double)m / (double)n;
+      t[m].x = cos(a); t[m].y = -sin(a);
+      tf[m].x = (float)t[m].x; tf[m].y = (float)t[m].y;
+    }
+    if (hipMalloc(d64, n * sizeof(double2)) != hipSuccess) return 1;
+    if (hipMemcpy(*d64, t.data(), n * sizeof(double2), hipMemcpyHostToDevice) != hipSuccess) return 1;
+    if (h->prec == LH_PREC_FP64) {
+      *dnl = *d64;
+    } else {
+      if (hipMalloc(dnl, n * sizeof(float2)) != hipSuccess) return 1;
+      if (hipMemcpy(*dnl, tf.data(), n * sizeof(float2), hipMemcpyHostToDevice) != hipSuccess) return 1;
+    }
+    return 0;
+  };
+  if (mk_tw(dm.px, &h->twx64, &h->twx) || mk_tw(dm.pz, &h->twz64, &h->twz))
+    return fail(h, LH_ERR_HIP, "twiddle upload failed");
+  // canopy masks: one per distinct footprint; planes -> mask index
+  {
+    const size_t pp = (size_t)dm.px * dm.pz;
+    std::vector<std::vector<unsigned char>> pu(dm.nyu), pv(dm.nyv);
+    for (int n = 0; n < c->nlist_u; ++n) {
+      const int i = c->list_u[3 * n], k = c->list_u[3 * n + 1], j = c->list_u[3 * n + 2];
+      if (i < 1 || i > dm.px || k < 1 || k > dm.pz || j < -dm.d + 1 || j > dm.ny + dm.d - 1)
+        return fail(h, LH_ERR_ARG, "list_u point out of range");
+      auto& m = pu[j + dm.d];
+      if (m.empty()) m.assign(pp, 0);
+      m[(size_t)(k - 1) * dm.px + (i - 1)] = 1;
+    }
+    for (int n = 0; n < c->nlist_v; ++n) {
+      const int i = c->list_v[3 * n], k = c->list_v[3 * n + 1], j = c->list_v[3 * n + 2];
+      if (i < 1 || i > dm.px || k < 1 || k > dm.pz || j < -dm.d + 1 || j > dm.ny + dm.d - 2)
+        return fail(h, LH_ERR_ARG, "list_v point out of range");
+      auto& m = pv[j + dm.d];
+      if (m.empty()) m.assign(pp, 0);
+      m[(size_t)(k - 1) * dm.px + (i - 1)] = 1;
+    }
+    std::vector<std::vector<unsigned char>> uniq;
+    auto idx_of = [&](const std::vector<unsigned char>& m) {
+      for (size_t q = 0; q < uniq.size(); ++q)
+        if (uniq[q] == m) return (int)q;
+      uniq.push_back(m);
+      return (int)uniq.size() - 1;
+    };
+    h->mask_of_u.assign(dm.nyu, -1);
+    h->mask_of_v.assign(dm.nyv, -1);
+    for (int s = 0; s < dm.nyu; ++s)
+      if (!pu[s].empty()) { h->mask_of_u[s] = idx_of(pu[s]); h->h_ibu.push_back(s); }
+    for (int s = 0; s < dm.nyv; ++s)
+      if (!pv[s].empty()) { h->mask_of_v[s] = idx_of(pv[s]); h->h_ibv.push_back(s); }
+    h->nmask = (int)uniq.size();
+    h->nibu = (int)h->h_ibu.size();
+    h->nibv = (int)h->h_ibv.size();
+    std::vector<IBEntry> ent;
+    int slot = 0;
+    const int su0 = 0, sw0 = h->nibu, sv0 = 2 * h->nibu, st0 = 2 * h->nibu + h->nibv;
+    auto tip_slot = [&](int grid, int s) -> int {   // Lu_y slot (tipl rows 0..5) or -1
+      if (grid == 0) { if (s == h->tip_u[0] + dm.d) return 0; if (s == h->tip_u[1] + dm.d) return 1; }
+      if (grid == 2) { if (s == h->tip_u[0] + dm.d) return 2; if (s == h->tip_u[1] + dm.d) return 3; }
+      if (grid == 1) { if (s == h->tip_v[0] + dm.d) return 4; if (s == h->tip_v[1] + dm.d) return 5; }
+      return -1;
+    };
+    for (int q = 0; q < h->nibu; ++q) {
+      const int s = h->h_ibu[q];
+      int ts = tip_slot(0, s);
+      ent.push_back(IBEntry{su0 + q, ts >= 0 ? st0 + ts : -1, h->mask_of_u[s], 0});
+    }
+    for (int q = 0; q < h->nibu; ++q) {
+      const int s = h->h_ibu[q];
+      int ts = tip_slot(2, s);
+      ent.push_back(IBEntry{sw0 + q, ts >= 0 ? st0 + ts : -1, h->mask_of_u[s], 0});
+    }
+    for (int q = 0; q < h->nibv; ++q) {
+      const int s = h->h_ibv[q];
+      int ts = tip_slot(1, s);
+      ent.push_back(IBEntry{sv0 + q, ts >= 0 ? st0 + ts : -1, h->mask_of_v[s], 0});
+    }
+    (void)slot;
+    h->nibent = (int)ent.size();
+    if (h->nmask > 0) {
+      HIPCHK(h, hipMalloc((void**)&h->masks, pp * h->nmask));
+      for (int q = 0; q < h->nmask; ++q)
+        HIPCHK(h, hipMemcpy(h->masks + pp * q, uniq[q].data(), pp, hipMemcpyHostToDevice));
+    }
+    if (h->nibent > 0) {
+      HIPCHK(h, hipMalloc((void**)&h->ibent, sizeof(IBEntry) * h->nibent));
+      HIPCHK(h, hipMemcpy(h->ibent, ent.data(), sizeof(IBEntry) * h->nibent, hipMemcpyHostToDevice));
+    }
+    if (h->nibu) {
+      HIPCHK(h, hipMalloc((void**)&h->ibu_planes, sizeof(int) * h->nibu));
+      HIPCHK(h, hipMemcpy(h->ibu_planes, h->h_ibu.data(), sizeof(int) * h->nibu, hipMemcpyHostToDevice));
+    }
+    if (h->nibv) {
+      HIPCHK(h, hipMalloc((void**)&h->ibv_planes, sizeof(int) * h->nibv));
+      HIPCHK(h, hipMemcpy(h->ibv_planes, h->h_ibv.data(), sizeof(int) * h->nibv, hipMemcpyHostToDevice));
+    }
+  }
+  // pressure LU factors, once (start.f90:628-630)
+  {
+    BUILDP a{h->dg1, h->dg2, h->dg3, h->dthdyu, h->dthdyv, h->yu, h->k2x, h->k2z};
+    hipLaunchKernelGGL(k_buildP, dim3((dm.ncol + 255) / 256), dim3(256), 0, h->st, a, dm);
+    HIPCHK(h, hipGetLastError());
+  }
+  HIPCHK(h, hipStreamSynchronize(h->st));
+  return LH_OK;
+}
+
+int lh_destroy(lh_handle* h) {
+  if (!h) return LH_OK;
+  if (h->st) hipStreamSynchronize(h->st);
+  for (auto& g : h->gexec)
+    if (g) hipGraphExecDestroy(g);
+  void* ptrs[] = {h->yu, h->yv, h->dthdyu, h->dthdyv, h->dyu2i, h->dyv2i, h->k1x, h->k2x, h->k1z,
+                  h->k2z, h->scal, h->u1, h->u2, h->u3, h->p, h->nu1, h->nu2, h->nu3, h->r1,
+                  h->r2, h->r3, h->psi, h->tipl, h->ph, h->zu, h->zw, h->zv, h->zp, h->dg1,
+                  h->dg2, h->dg3, h->ayu, h->ayv, h->ctwork, h->masks, h->ibu_planes,
+                  h->ibv_planes, h->ibent};
+  for (void* q : ptrs)
+    if (q) hipFree(q);
+  if (h->twx && h->twx != h->twx64) hipFree(h->twx);
+  if (h->twz && h->twz != h->twz64) hipFree(h->twz);
+  if (h->twx64) hipFree(h->twx64);
+  if (h->twz64) hipFree(h->twz64);
+  if (h->st) hipStreamDestroy(h->st);
+  delete h;
+  return LH_OK;
+}
+
+// reference layout f(j, column) <-> device [plane][column]
+static int upload_field(lh_handle* h, double2* dst, const double* src, int nj) {
+  const int ncol = h->dm.ncol;
+  std::vector<double2> t((size_t)nj * ncol);
+  const double2* s = reinterpret_cast<const double2*>(src);
+  for (int c = 0; c < ncol; ++c)
+    for (int j = 0; j < nj; ++j) t[(size_t)j * ncol + c] = s[(size_t)c * nj + j];
+  HIPCHK(h, hipMemcpy(dst, t.data(), t.size() * sizeof(double2), hipMemcpyHostToDevice));
+  return LH_OK;
+}
+static int download_field(lh_handle* h, double* dst, const double2* src, int nj) {
+  const int ncol = h->dm.ncol;
+  std::vector<double2> t((size_t)nj * ncol);
+  HIPCHK(h, hipMemcpy(t.data(), src, t.size() * sizeof(double2), hipMemcpyDeviceToHost));
+  double2* d = reinterpret_cast<double2*>(dst);
+  for (int c = 0; c < ncol; ++c)
+    for (int j = 0; j < nj; ++j) d[(size_t)c * nj + j] = t[(size_t)j * ncol + c];
+  return LH_OK;
+}
+
+int lh_set_state(lh_handle* h, const double* u1, const double* u2, const double* u3, const double* p) {
+  if (!h) return LH_ERR_ARG;
+  HIPCHK(h, hipStreamSynchronize(h->st));
+  int r;
+  if (u1 && (r = upload_field(h, h->u1, u1, h->dm.nyu))) return r;
+  if (u2 && (r = upload_field(h, h->u2, u2, h->dm.nyv))) return r;
+  if (u3 && (r = upload_field(h, h->u3, u3, h->dm.nyu))) return r;
+  if (p && (r = upload_field(h, h->p, p, h->dm.nyp))) return r;
+  return LH_OK;
+}
+
+int lh_get_state(lh_handle* h, double* u1, double* u2, double* u3, double* p) {
+  if (!h) return LH_ERR_ARG;
+  HIPCHK(h, hipStreamSynchronize(h->st));
+  int r;
+  if (u1 && (r = download_field(h, u1, h->u1, h->dm.nyu))) return r;
+  if (u2 && (r = download_field(h, u2, h->u2, h->dm.nyv))) return r;
+  if (u3 && (r = download_field(h, u3, h->u3, h->dm.nyu))) return r;
+  if (p && (r = download_field(h, p, h->p, h->dm.nyp))) return r;
+  return LH_OK;
+}
+
+// Nu in the reference's allocation range jlim(1)+1 : jlim(2)-1 (littleharsh.f90:122-124)
+static int xfer_nl(lh_handle* h, double2* dev, double* host, int nj, bool up) {
+  const int ncol = h->dm.ncol;
+  std::vector<double2> t((size_t)nj * ncol);
+  if (!up) HIPCHK(h, hipMemcpy(t.data(), dev, t.size() * 16, hipMemcpyDeviceToHost));
+  double2* hh = reinterpret_cast<double2*>(host);
+  for (int c = 0; c < ncol; ++c)
+    for (int r = 0; r < nj - 2; ++r) {
+      if (up) t[(size_t)(r + 1) * ncol + c] = hh[(size_t)c * (nj - 2) + r];
+      else hh[(size_t)c * (nj - 2) + r] = t[(size_t)(r + 1) * ncol + c];
+    }
+  if (up) HIPCHK(h, hipMemcpy(dev, t.data(), t.size() * 16, hipMemcpyHostToDevice));
+  return LH_OK;
+}
+
+int lh_set_nonlinear(lh_handle* h, const double* n1, const double* n2, const double* n3) {
+  if (!h) return LH_ERR_ARG;
+  HIPCHK(h, hipStreamSynchronize(h->st));
+  int r;
+  if (n1 && (r = xfer_nl(h, h->nu1, (double*)n1, h->dm.nyu, true))) return r;
+  if (n2 && (r = xfer_nl(h, h->nu2, (double*)n2, h->dm.nyv, true))) return r;
+  if (n3 && (r = xfer_nl(h, h->nu3, (double*)n3, h->dm.nyu, true))) return r;
+  return LH_OK;
+}
+
+int lh_get_nonlinear(lh_handle* h, double* n1, double* n2, double* n3) {
+  if (!h) return LH_ERR_ARG;
+  HIPCHK(h, hipStreamSynchronize(h->st));
+  int r;
+  if (n1 && (r = xfer_nl(h, h->nu1, n1, h->dm.nyu, false))) return r;
+  if (n2 && (r = xfer_nl(h, h->nu2, n2, h->dm.nyv, false))) return r;
+  if (n3 && (r = xfer_nl(h, h->nu3, n3, h->dm.nyu, false))) return r;
+  return LH_OK;
+}
+
+int lh_substep(lh_handle* h, int krk, double dt) {
+  if (!h) return LH_ERR_ARG;
+  if (krk < 1 || krk > 3) return fail(h, LH_ERR_ARG, "kRK must be 1, 2 or 3");
+  if (dt > 0.0) hipLaunchKernelGGL(k_set_scalar, dim3(1), dim3(1), 0, h->st, h->scal, (int)S_DT, dt);
+  issue_substep(h, krk - 1, false);
+  HIPCHK(h, hipGetLastError());
+  HIPCHK(h, hipStreamSynchronize(h->st));
+  return LH_OK;
+}
+
+static int run_graph(lh_handle* h, int mode) {
+  if (!h->gexec[mode]) {
+    hipGraph_t g;
+    HIPCHK(h, hipStreamBeginCapture(h->st, hipStreamCaptureModeThreadLocal));
+    issue_step(h, mode == 1);
+    HIPCHK(h, hipStreamEndCapture(h->st, &g));
+    HIPCHK(h, hipGraphInstantiate(&h->gexec[mode], g, nullptr, nullptr, 0));
+    HIPCHK(h, hipGraphDestroy(g));
+  }
+  HIPCHK(h, hipGraphLaunch(h->gexec[mode], h->st));
+  return LH_OK;
+}
+
+int lh_step(lh_handle* h, double dt) {
+  if (!h) return LH_ERR_ARG;
+  if (!(dt > 0.0)) return fail(h, LH_ERR_ARG, "dt must be positive");
+  hipLaunchKernelGGL(k_set_scalar, dim3(1), dim3(1), 0, h->st, h->scal, (int)S_DT, dt);
+  int r = run_graph(h, 0);
+  if (r) return r;
+  HIPCHK(h, hipGetLastError());
+  HIPCHK(h, hipStreamSynchronize(h->st));
+  return LH_OK;
+}
+
+int lh_step_cfl(lh_handle* h, double cfl, double* dt_used) {
+  if (!h) return LH_ERR_ARG;
+  if (!(cfl > 0.0)) return fail(h, LH_ERR_ARG, "cfl must be positive");
+  hipLaunchKernelGGL(k_set_scalar, dim3(1), dim3(1), 0, h->st, h->scal, (int)S_CFL, cfl);
+  int r = run_graph(h, 1);
+  if (r) return r;
+  HIPCHK(h, hipGetLastError());
+  HIPCHK(h, hipStreamSynchronize(h->st));
+  if (dt_used) HIPCHK(h, hipMemcpy(dt_used, h->scal + S_DT, 8, hipMemcpyDeviceToHost));
+  return LH_OK;
+}
+
+int lh_get_scalars(lh_handle* h, double* mpgx, double* qx, double* dt, double* cflmax) {
+  if (!h) return LH_ERR_ARG;
+  double s[S_NSCAL];
+  HIPCHK(h, hipStreamSynchronize(h->st));
+  HIPCHK(h, hipMemcpy(s, h->scal, sizeof(s), hipMemcpyDeviceToHost));
+  if (mpgx) *mpgx = s[S_MPGX];
+  if (qx) *qx = s[S_QX];
+  if (dt) *dt = s[S_DT];
+  if (cflmax) *cflmax = s[S_CFLMAX];
+  return LH_OK;
+}
+
+int lh_max_div(lh_handle* h, double* err) {
+  if (!h || !err) return LH_ERR_ARG;
+  hipLaunchKernelGGL(k_set_scalar, dim3(1), dim3(1), 0, h->st, h->scal, (int)S_MAXDIV, 0.0);
+  hipLaunchKernelGGL(k_maxdiv, dim3((h->dm.ncol + 255) / 256), dim3(256), 0, h->st, h->u1, h->u2,
+                     h->u3, h->dthdyu, h->k1x, h->k1z, h->scal, h->dm);
+  HIPCHK(h, hipGetLastError());
+  HIPCHK(h, hipStreamSynchronize(h->st));
+  HIPCHK(h, hipMemcpy(err, h->scal + S_MAXDIV, 8, hipMemcpyDeviceToHost));
+  return LH_OK;
+}
+
+int lh_cfl_dt(lh_handle* h, double cfl, double* dt) {
+  if (!h || !dt) return LH_ERR_ARG;
+  (void)cfl;
+  return fail(h, LH_ERR_ARG, "lh_cfl_dt: not available yet");
+}
+
+int lh_stats_accumulate(lh_handle* h) { return fail(h, LH_ERR_ARG, "statistics not available yet"); }
+int lh_stats_read(lh_handle* h, double*, int*, int) { return fail(h, LH_ERR_ARG, "statistics not available yet"); }
+int lh_cond_stats_read(lh_handle* h, double*, long long, int) { return fail(h, LH_ERR_ARG, "statistics not available yet"); }
+int lh_spectra_accumulate(lh_handle* h) { return fail(h, LH_ERR_ARG, "spectra not available yet"); }
+int lh_spectra_read(lh_handle* h, double*, int) { return fail(h, LH_ERR_ARG, "spectra not available yet"); }
+
+}  // extern "C"

@@ -1,0 +1,234 @@
+// lh_fft.hpp -- batched LDS-staged Stockham FFT for gfx950, FP64 or FP32.
+//
+// A "line" of N complex values lives in LDS; T threads of the workgroup work on one line.
+// Each stage of radix R reads the line with stride N/R (conflict-free, consecutive
+// butterflies on consecutive threads), twiddles, does an in-register DFT_R and writes the
+// autosorted (Stockham) positions back to the same LDS line.  Every stage brackets its
+// read and write phases with __syncthreads(), so a workgroup of L lines x T threads
+// transforms L lines at once.  Radices are composed at compile time from 2, 3 and 4.
+//
+// Conventions: forward = sum x_n e^{-2 pi i nk/N} (no scaling), inverse = e^{+...}; the
+// 1/N of the reference's forward rft/cft (rft_buff.f:63-73, cft_buff.f:44-45) is applied
+// by the callers.
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace lh {
+
+template <typename T> struct cvec;
+template <> struct cvec<double> { using type = double2; };
+template <> struct cvec<float> { using type = float2; };
+template <typename T> using cx = typename cvec<T>::type;
+
+template <typename C> __device__ __forceinline__ C mk(decltype(C::x) a, decltype(C::x) b) {
+  C r; r.x = a; r.y = b; return r;
+}
+template <typename C> __device__ __forceinline__ C cadd(C a, C b) { return mk<C>(a.x + b.x, a.y + b.y); }
+template <typename C> __device__ __forceinline__ C csub(C a, C b) { return mk<C>(a.x - b.x, a.y - b.y); }
+template <typename C> __device__ __forceinline__ C cmul(C a, C b) {
+  return mk<C>(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+// a * conj(b)
+template <typename C> __device__ __forceinline__ C cmulc(C a, C b) {
+  return mk<C>(a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y);
+}
+// multiply by -i (forward) or +i (inverse)
+template <bool INV, typename C> __device__ __forceinline__ C rot(C a) {
+  return INV ? mk<C>(-a.y, a.x) : mk<C>(a.y, -a.x);
+}
+
+// W_R^m = exp(-2 pi i m / R) for the composite radices, cos/sin(2 pi m/R) to double precision
+__device__ constexpr double kW6[6][2] = {{1.0, -0.0}, {5.00000000000000111022e-01, -8.66025403784438596588e-01}, {-4.99999999999999777955e-01, -8.66025403784438707611e-01}, {-1.0, -1.22464679914735320717e-16}, {-5.00000000000000444089e-01, 8.66025403784438374544e-01}, {5.00000000000000111022e-01, 8.66025403784438596588e-01}};
+__device__ constexpr double kW8[8][2] = {{1.0, -0.0}, {7.07106781186547572737e-01, -7.07106781186547461715e-01}, {6.12323399573676603587e-17, -1.0}, {-7.07106781186547461715e-01, -7.07106781186547572737e-01}, {-1.0, -1.22464679914735320717e-16}, {-7.07106781186547683760e-01, 7.07106781186547461715e-01}, {-1.83697019872102968750e-16, 1.0}, {7.07106781186547350693e-01, 7.07106781186547683760e-01}};
+__device__ constexpr double kW9[9][2] = {{1.0, -0.0}, {7.66044443118978013452e-01, -6.42787609686539251896e-01}, {1.73648177666930414453e-01, -9.84807753012208020316e-01}, {-4.99999999999999777955e-01, -8.66025403784438707611e-01}, {-9.39692620785908316883e-01, -3.42020143325668879442e-01}, {-9.39692620785908427905e-01, 3.42020143325668657397e-01}, {-5.00000000000000444089e-01, 8.66025403784438374544e-01}, {1.73648177666929970364e-01, 9.84807753012208131338e-01}, {7.66044443118977791407e-01, 6.42787609686539584963e-01}};
+__device__ constexpr double kW12[12][2] = {{1.0, -0.0}, {8.66025403784438707611e-01, -4.99999999999999944489e-01}, {5.00000000000000111022e-01, -8.66025403784438596588e-01}, {6.12323399573676603587e-17, -1.0}, {-4.99999999999999777955e-01, -8.66025403784438707611e-01}, {-8.66025403784438707611e-01, -4.99999999999999944489e-01}, {-1.0, -1.22464679914735320717e-16}, {-8.66025403784438818633e-01, 4.99999999999999722444e-01}, {-5.00000000000000444089e-01, 8.66025403784438374544e-01}, {-1.83697019872102968750e-16, 1.0}, {5.00000000000000111022e-01, 8.66025403784438596588e-01}, {8.66025403784438374544e-01, 5.00000000000000444089e-01}};
+__device__ constexpr double kW16[16][2] = {{1.0, -0.0}, {9.23879532511286738483e-01, -3.82683432365089781779e-01}, {7.07106781186547572737e-01, -7.07106781186547461715e-01}, {3.82683432365089837290e-01, -9.23879532511286738483e-01}, {6.12323399573676603587e-17, -1.0}, {-3.82683432365089726268e-01, -9.23879532511286738483e-01}, {-7.07106781186547461715e-01, -7.07106781186547572737e-01}, {-9.23879532511286738483e-01, -3.82683432365089892802e-01}, {-1.0, -1.22464679914735320717e-16}, {-9.23879532511286849505e-01, 3.82683432365089670757e-01}, {-7.07106781186547683760e-01, 7.07106781186547461715e-01}, {-3.82683432365090336891e-01, 9.23879532511286516439e-01}, {-1.83697019872102968750e-16, 1.0}, {3.82683432365090003824e-01, 9.23879532511286627461e-01}, {7.07106781186547350693e-01, 7.07106781186547683760e-01}, {9.23879532511286516439e-01, 3.82683432365090392402e-01}};
+
+template <int R> __device__ __forceinline__ double wre(int m);
+template <int R> __device__ __forceinline__ double wim(int m);
+#define LH_WTAB(R)                                                                   \
+  template <> __device__ __forceinline__ double wre<R>(int m) { return kW##R[m][0]; } \
+  template <> __device__ __forceinline__ double wim<R>(int m) { return kW##R[m][1]; }
+LH_WTAB(6)
+LH_WTAB(8)
+LH_WTAB(9)
+LH_WTAB(12)
+LH_WTAB(16)
+#undef LH_WTAB
+
+template <int R> struct split;
+template <> struct split<6> { static constexpr int A = 2, B = 3; };
+template <> struct split<8> { static constexpr int A = 4, B = 2; };
+template <> struct split<9> { static constexpr int A = 3, B = 3; };
+template <> struct split<12> { static constexpr int A = 4, B = 3; };
+template <> struct split<16> { static constexpr int A = 4, B = 4; };
+
+// In-register DFT of size R
+template <int R, bool INV, typename C> struct DFT {
+  // composite: x[n2 + B*n1] -> X[k1 + A*k2]
+  static __device__ __forceinline__ void run(C* v) {
+    constexpr int A = split<R>::A, B = split<R>::B;
+    using T = decltype(C::x);
+    C y[R];
+#pragma unroll
+    for (int n2 = 0; n2 < B; ++n2) {
+      C t[A];
+#pragma unroll
+      for (int n1 = 0; n1 < A; ++n1) t[n1] = v[n2 + B * n1];
+      DFT<A, INV, C>::run(t);
+#pragma unroll
+      for (int k1 = 0; k1 < A; ++k1) {
+        const int m = (n2 * k1) % R;
+        if (m == 0) {
+          y[n2 * A + k1] = t[k1];
+        } else {
+          const C w = mk<C>((T)wre<R>(m), INV ? -(T)wim<R>(m) : (T)wim<R>(m));
+          y[n2 * A + k1] = cmul(t[k1], w);
+        }
+      }
+    }
+#pragma unroll
+    for (int k1 = 0; k1 < A; ++k1) {
+      C t[B];
+#pragma unroll
+      for (int n2 = 0; n2 < B; ++n2) t[n2] = y[n2 * A + k1];
+      DFT<B, INV, C>::run(t);
+#pragma unroll
+      for (int k2 = 0; k2 < B; ++k2) v[k1 + A * k2] = t[k2];
+    }
+  }
+};
+template <bool INV, typename C> struct DFT<1, INV, C> {
+  static __device__ __forceinline__ void run(C*) {}
+};
+template <bool INV, typename C> struct DFT<2, INV, C> {
+  static __device__ __forceinline__ void run(C* v) {
+    const C a = v[0], b = v[1];
+    v[0] = cadd(a, b);
+    v[1] = csub(a, b);
+  }
+};
+template <bool INV, typename C> struct DFT<3, INV, C> {
+  static __device__ __forceinline__ void run(C* v) {
+    using T = decltype(C::x);
+    const T s = (T)0.86602540378443864676;
+    const C t1 = cadd(v[1], v[2]);
+    const C t2 = mk<C>(v[0].x - (T)0.5 * t1.x, v[0].y - (T)0.5 * t1.y);
+    const C dd = csub(v[1], v[2]);
+    const C r = rot<INV>(mk<C>(s * dd.x, s * dd.y));  // -i s (v1-v2) forward
+    v[0] = cadd(v[0], t1);
+    v[1] = cadd(t2, r);
+    v[2] = csub(t2, r);
+  }
+};
+template <bool INV, typename C> struct DFT<4, INV, C> {
+  static __device__ __forceinline__ void run(C* v) {
+    const C s0 = cadd(v[0], v[2]), s1 = csub(v[0], v[2]);
+    const C s2 = cadd(v[1], v[3]), s3 = rot<INV>(csub(v[1], v[3]));
+    v[0] = cadd(s0, s2);
+    v[2] = csub(s0, s2);
+    v[1] = cadd(s1, s3);
+    v[3] = csub(s1, s3);
+  }
+};
+
+// One Stockham stage of radix R on a line of N values; NS = product of earlier radices.
+// tw[m] = exp(-2 pi i m/N) (same precision as the line), m < N.
+template <int N, int T, int R, int NS, bool INV, typename C>
+__device__ __forceinline__ void stage(C* line, int t, const C* __restrict__ tw) {
+  constexpr int NB = N / R;
+  constexpr int PER = (NB + T - 1) / T;
+  C v[PER][R];
+#pragma unroll
+  for (int q = 0; q < PER; ++q) {
+    const int b = t + q * T;
+    if (PER * T == NB || b < NB) {
+#pragma unroll
+      for (int r = 0; r < R; ++r) v[q][r] = line[b + r * NB];
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < PER; ++q) {
+    const int b = t + q * T;
+    if (PER * T == NB || b < NB) {
+      const int j = b % NS;
+      if (NS > 1) {
+#pragma unroll
+        for (int r = 1; r < R; ++r) {
+          const C w = tw[j * r * (N / (NS * R))];
+          v[q][r] = INV ? cmulc(v[q][r], w) : cmul(v[q][r], w);
+        }
+      }
+      DFT<R, INV, C>::run(v[q]);
+      const int idx = (b / NS) * NS * R + j;
+#pragma unroll
+      for (int r = 0; r < R; ++r) line[idx + r * NS] = v[q][r];
+    }
+  }
+  __syncthreads();
+}
+
+// Plans: threads per line T and the radix sequence for each supported length.
+template <int N> struct Plan;
+template <> struct Plan<576> {
+  static constexpr int T = 48;
+  template <bool INV, typename C> static __device__ void run(C* l, int t, const C* tw) {
+    stage<576, T, 12, 1, INV>(l, t, tw);
+    stage<576, T, 12, 12, INV>(l, t, tw);
+    stage<576, T, 4, 144, INV>(l, t, tw);
+  }
+};
+template <> struct Plan<384> {
+  static constexpr int T = 48;
+  template <bool INV, typename C> static __device__ void run(C* l, int t, const C* tw) {
+    stage<384, T, 8, 1, INV>(l, t, tw);
+    stage<384, T, 12, 8, INV>(l, t, tw);
+    stage<384, T, 4, 96, INV>(l, t, tw);
+  }
+};
+template <> struct Plan<144> {
+  static constexpr int T = 12;
+  template <bool INV, typename C> static __device__ void run(C* l, int t, const C* tw) {
+    stage<144, T, 12, 1, INV>(l, t, tw);
+    stage<144, T, 12, 12, INV>(l, t, tw);
+  }
+};
+template <> struct Plan<96> {
+  static constexpr int T = 8;
+  template <bool INV, typename C> static __device__ void run(C* l, int t, const C* tw) {
+    stage<96, T, 4, 1, INV>(l, t, tw);
+    stage<96, T, 4, 4, INV>(l, t, tw);
+    stage<96, T, 6, 16, INV>(l, t, tw);
+  }
+};
+template <> struct Plan<72> {
+  static constexpr int T = 6;
+  template <bool INV, typename C> static __device__ void run(C* l, int t, const C* tw) {
+    stage<72, T, 12, 1, INV>(l, t, tw);
+    stage<72, T, 6, 12, INV>(l, t, tw);
+  }
+};
+template <> struct Plan<64> {
+  static constexpr int T = 4;
+  template <bool INV, typename C> static __device__ void run(C* l, int t, const C* tw) {
+    stage<64, T, 16, 1, INV>(l, t, tw);
+    stage<64, T, 4, 16, INV>(l, t, tw);
+  }
+};
+template <> struct Plan<48> {
+  static constexpr int T = 4;
+  template <bool INV, typename C> static __device__ void run(C* l, int t, const C* tw) {
+    stage<48, T, 12, 1, INV>(l, t, tw);
+    stage<48, T, 4, 12, INV>(l, t, tw);
+  }
+};
+template <> struct Plan<32> {
+  static constexpr int T = 4;
+  template <bool INV, typename C> static __device__ void run(C* l, int t, const C* tw) {
+    stage<32, T, 8, 1, INV>(l, t, tw);
+    stage<32, T, 4, 8, INV>(l, t, tw);
+  }
+};
+
+}  // namespace lh

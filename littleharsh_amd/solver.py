@@ -1,0 +1,108 @@
+"""Host-side driver of one channel on the GPU through the C-ABI (include/littleharsh_amd.h).
+
+Mirrors the reference host loop (littleharsh.f90:160-201): ``step`` is one pass of the
+``do kRK = 1,3`` loop plus meanflow_ctP; ``substep`` is one body of that loop.  State
+arrays use the reference layout complex(8) f(j, column): numpy complex128 of shape
+(ncol, nplanes).  Any non-zero status raises with the library's message, the analogue
+of the reference's ``write(*,*) ...; stop``.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import capi
+from .channel import ChannelSetup
+
+_dp = ctypes.POINTER(ctypes.c_double)
+
+
+def _p(a):
+    return a.ctypes.data_as(_dp) if a is not None else None
+
+
+class LHError(RuntimeError):
+    pass
+
+
+class Channel:
+    def __init__(self, setup: ChannelSetup, precision: int = capi.LH_PREC_FP64):
+        self.s = setup
+        self._lib = capi.load_library()
+        cfg, keep = capi.make_config(setup, precision)
+        h = ctypes.c_void_p()
+        r = self._lib.lh_create(ctypes.byref(cfg), ctypes.byref(h))
+        self._h = h
+        if r != capi.LH_OK:
+            msg = self._lib.lh_last_error(h).decode() if h.value else "lh_create failed"
+            if h.value:
+                self._lib.lh_destroy(h)
+            self._h = None
+            raise LHError(f"lh_create: {msg} (status {r})")
+        del keep
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.lh_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def _chk(self, r, what):
+        if r != capi.LH_OK:
+            raise LHError(f"{what}: {self._lib.lh_last_error(self._h).decode()} (status {r})")
+
+    # ---- state
+    def set_state(self, u, v, w, p):
+        arrs = [np.ascontiguousarray(a, dtype=np.complex128) for a in (u, v, w, p)]
+        s = self.s
+        for a, n in zip(arrs, (s.nyu, s.nyv, s.nyu, s.nyp)):
+            if a.shape != (s.ncol, n):
+                raise ValueError(f"state array shape {a.shape} != {(s.ncol, n)}")
+        self._chk(self._lib.lh_set_state(self._h, *[_p(a) for a in arrs]), "lh_set_state")
+
+    def get_state(self):
+        s = self.s
+        u = np.zeros((s.ncol, s.nyu), complex)
+        v = np.zeros((s.ncol, s.nyv), complex)
+        w = np.zeros((s.ncol, s.nyu), complex)
+        p = np.zeros((s.ncol, s.nyp), complex)
+        self._chk(self._lib.lh_get_state(self._h, _p(u), _p(v), _p(w), _p(p)), "lh_get_state")
+        return u, v, w, p
+
+    def get_nonlinear(self):
+        s = self.s
+        n1 = np.zeros((s.ncol, s.nyu - 2), complex)
+        n2 = np.zeros((s.ncol, s.nyv - 2), complex)
+        n3 = np.zeros((s.ncol, s.nyu - 2), complex)
+        self._chk(self._lib.lh_get_nonlinear(self._h, _p(n1), _p(n2), _p(n3)), "lh_get_nonlinear")
+        return n1, n2, n3
+
+    def set_nonlinear(self, n1, n2, n3):
+        arrs = [np.ascontiguousarray(a, dtype=np.complex128) for a in (n1, n2, n3)]
+        self._chk(self._lib.lh_set_nonlinear(self._h, *[_p(a) for a in arrs]), "lh_set_nonlinear")
+
+    # ---- time stepping
+    def substep(self, krk: int, dt: float):
+        self._chk(self._lib.lh_substep(self._h, int(krk), float(dt)), "lh_substep")
+
+    def step(self, dt: float):
+        self._chk(self._lib.lh_step(self._h, float(dt)), "lh_step")
+
+    def step_cfl(self, cfl: float) -> float:
+        dt = ctypes.c_double()
+        self._chk(self._lib.lh_step_cfl(self._h, float(cfl), ctypes.byref(dt)), "lh_step_cfl")
+        return dt.value
+
+    def scalars(self):
+        m, q, d, c = (ctypes.c_double() for _ in range(4))
+        self._chk(self._lib.lh_get_scalars(self._h, ctypes.byref(m), ctypes.byref(q),
+                                           ctypes.byref(d), ctypes.byref(c)), "lh_get_scalars")
+        return dict(mpgx=m.value, qx=q.value, dt=d.value, cflmax=c.value)
+
+    def max_div(self) -> float:
+        e = ctypes.c_double()
+        self._chk(self._lib.lh_max_div(self._h, ctypes.byref(e)), "lh_max_div")
+        return e.value

@@ -1,0 +1,147 @@
+"""ctypes front end of the CPU oracle (oracle/lh_oracle.c).
+
+Test infrastructure only: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py as the checker of the HIP library, never by the product.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+from littleharsh_amd.capi import make_config, LHConfig
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+SO = os.path.join(HERE, "liblh_oracle.so")
+_dp = ctypes.POINTER(ctypes.c_double)
+_lib = None
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", HERE, "-B"], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(SO):
+            build()
+        L = ctypes.CDLL(SO)
+        vp = ctypes.c_void_p
+        L.lho_create.argtypes = [ctypes.POINTER(LHConfig)]
+        L.lho_create.restype = vp
+        L.lho_destroy.argtypes = [vp]
+        L.lho_set_state.argtypes = [vp, _dp, _dp, _dp, _dp]
+        L.lho_get_state.argtypes = [vp, _dp, _dp, _dp, _dp]
+        L.lho_set_nonlinear.argtypes = [vp, _dp, _dp, _dp]
+        L.lho_get_nonlinear.argtypes = [vp, _dp, _dp, _dp]
+        L.lho_get_scalars.argtypes = [vp, _dp, _dp, _dp]
+        L.lho_substep.argtypes = [vp, ctypes.c_int, ctypes.c_double]
+        L.lho_step.argtypes = [vp, ctypes.c_double]
+        L.lho_max_div.argtypes = [vp]
+        L.lho_max_div.restype = ctypes.c_double
+        L.lho_flowrate.argtypes = [vp]
+        L.lho_flowrate.restype = ctypes.c_double
+        L.lho_four_to_phys.argtypes = [vp, _dp, _dp]
+        L.lho_phys_to_four.argtypes = [vp, _dp, _dp]
+        L.lho_cfft.argtypes = [ctypes.c_int, _dp, _dp, ctypes.c_int]
+        for name in ("lho_stats_accumulate", "lho_spectra_accumulate"):
+            if hasattr(L, name):
+                getattr(L, name).argtypes = [vp]
+        if hasattr(L, "lho_stats_read"):
+            L.lho_stats_read.argtypes = [vp, _dp]
+            L.lho_stats_read.restype = ctypes.c_int
+        if hasattr(L, "lho_cond_stats_read"):
+            L.lho_cond_stats_read.argtypes = [vp, _dp, ctypes.c_longlong]
+            L.lho_cond_stats_read.restype = ctypes.c_longlong
+        if hasattr(L, "lho_spectra_read"):
+            L.lho_spectra_read.argtypes = [vp, _dp]
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return a.ctypes.data_as(_dp) if a is not None else None
+
+
+class Oracle:
+    """One channel advanced on the CPU by the restatement in lh_oracle.c."""
+
+    def __init__(self, setup):
+        self.s = setup
+        cfg, keep = make_config(setup)
+        self._h = lib().lho_create(ctypes.byref(cfg))
+        if not self._h:
+            raise RuntimeError("lho_create failed (unsupported FFT length?)")
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().lho_destroy(self._h)
+            self._h = None
+
+    def set_state(self, u, v, w, p):
+        arrs = [np.ascontiguousarray(a, dtype=np.complex128) for a in (u, v, w, p)]
+        lib().lho_set_state(self._h, *[_p(a) for a in arrs])
+
+    def get_state(self):
+        s = self.s
+        u = np.zeros((s.ncol, s.nyu), complex)
+        v = np.zeros((s.ncol, s.nyv), complex)
+        w = np.zeros((s.ncol, s.nyu), complex)
+        p = np.zeros((s.ncol, s.nyp), complex)
+        lib().lho_get_state(self._h, _p(u), _p(v), _p(w), _p(p))
+        return u, v, w, p
+
+    def get_nonlinear(self):
+        s = self.s
+        n1 = np.zeros((s.ncol, s.nyu - 2), complex)
+        n2 = np.zeros((s.ncol, s.nyv - 2), complex)
+        n3 = np.zeros((s.ncol, s.nyu - 2), complex)
+        lib().lho_get_nonlinear(self._h, _p(n1), _p(n2), _p(n3))
+        return n1, n2, n3
+
+    def set_nonlinear(self, n1, n2, n3):
+        arrs = [np.ascontiguousarray(a, dtype=np.complex128) for a in (n1, n2, n3)]
+        lib().lho_set_nonlinear(self._h, *[_p(a) for a in arrs])
+
+    def scalars(self):
+        m, q, c = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+        lib().lho_get_scalars(self._h, ctypes.byref(m), ctypes.byref(q), ctypes.byref(c))
+        return dict(mpgx=m.value, qx=q.value, cflmax=c.value)
+
+    def substep(self, krk, dt):
+        r = lib().lho_substep(self._h, krk, dt)
+        if r:
+            raise RuntimeError(f"lho_substep failed ({r})")
+
+    def step(self, dt):
+        r = lib().lho_step(self._h, dt)
+        if r:
+            raise RuntimeError(f"lho_step failed ({r})")
+
+    def max_div(self):
+        return lib().lho_max_div(self._h)
+
+    def flowrate(self):
+        return lib().lho_flowrate(self._h)
+
+    def four_to_phys(self, sp):
+        sp = np.ascontiguousarray(sp, dtype=np.complex128)
+        ph = np.zeros((self.s.pz, self.s.px))
+        lib().lho_four_to_phys(self._h, _p(sp), _p(ph))
+        return ph
+
+    def phys_to_four(self, ph):
+        ph = np.ascontiguousarray(ph, dtype=np.float64)
+        sp = np.zeros(self.s.ncol, complex)
+        lib().lho_phys_to_four(self._h, _p(ph), _p(sp))
+        return sp
+
+
+def cfft(x: np.ndarray, sign: int) -> np.ndarray:
+    x = np.ascontiguousarray(x, dtype=np.complex128)
+    out = np.zeros_like(x)
+    lib().lho_cfft(len(x), _p(x), _p(out), sign)
+    return out
