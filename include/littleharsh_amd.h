@@ -125,6 +125,27 @@ int lh_cond_stats_read(lh_handle* h, double* out, long long out_len, int reset);
 int lh_spectra_accumulate(lh_handle* h);
 int lh_spectra_read(lh_handle* h, double* out, int reset);
 
+/* Per-kernel timing (no counterpart in the reference; measurement support).  While
+ * enabled, lh_step/lh_step_cfl issue their kernels directly (no graph) with a HIP event
+ * pair around every launch on the handle's stream; durations accumulate per kernel
+ * class.  lh_timing_read returns the accumulated milliseconds, the launch count and the
+ * algorithmic bytes one launch of that class moves (each input read once, each output
+ * written once). */
+#define LH_K_ZINV 0   /* inverse z transforms of u, w, v                  */
+#define LH_K_XNL 1    /* x transforms + products                          */
+#define LH_K_ZFWD 2   /* forward z transforms of the products             */
+#define LH_K_ASM 3    /* RHS0 + nonlinear assembly + rhs of the solves    */
+#define LH_K_IBZ 4    /* canopy-plane z transforms                         */
+#define LH_K_XIB 5    /* canopy-plane x transforms + forcing               */
+#define LH_K_VEL 6    /* velocity Thomas sweeps                            */
+#define LH_K_PRES 7   /* pressure solve + v_corr                          */
+#define LH_K_MISC 8   /* small kernels: LU factors, mean flow, scalars     */
+#define LH_K_COUNT 9
+int lh_timing_enable(lh_handle* h, int enable);
+int lh_timing_read(lh_handle* h, int kernel, double* total_ms, long long* launches,
+                   double* bytes_per_launch);
+int lh_timing_reset(lh_handle* h);
+
 #ifdef __cplusplus
 }
 #endif

@@ -116,6 +116,9 @@ def load_library():
     lib.lh_cond_stats_read.argtypes = [vp, _dp, ctypes.c_longlong, ctypes.c_int]
     lib.lh_spectra_accumulate.argtypes = [vp]
     lib.lh_spectra_read.argtypes = [vp, _dp, ctypes.c_int]
+    lib.lh_timing_enable.argtypes = [vp, ctypes.c_int]
+    lib.lh_timing_reset.argtypes = [vp]
+    lib.lh_timing_read.argtypes = [vp, ctypes.c_int, _dp, ctypes.POINTER(ctypes.c_longlong), _dp]
     _lib = lib
     return lib
 
@@ -125,4 +128,7 @@ EXPORTED = [
     "lh_set_nonlinear", "lh_get_nonlinear", "lh_substep", "lh_step", "lh_step_cfl",
     "lh_cfl_dt", "lh_get_scalars", "lh_max_div", "lh_stats_accumulate", "lh_stats_read",
     "lh_cond_stats_read", "lh_spectra_accumulate", "lh_spectra_read",
+    "lh_timing_enable", "lh_timing_reset", "lh_timing_read",
 ]
+
+KERNEL_NAMES = ["zinv", "xpass_nl", "zfwd", "assemble", "ib_z", "ib_x", "vel", "pressure", "misc"]

@@ -61,7 +61,54 @@ struct lh_handle {
   // graphs
   hipGraphExec_t gexec[2] = {nullptr, nullptr};   // [0] fixed dt, [1] device CFL dt
   double graph_dt = -1.0;
+  // per-kernel timing
+  bool timing = false;
+  std::vector<hipEvent_t> evpool;
+  size_t evnext = 0;
+  struct Pend { int k; hipEvent_t a, b; };
+  std::vector<Pend> pending;
+  double tms[LH_K_COUNT] = {0};
+  long long tcnt[LH_K_COUNT] = {0};
+  double tbytes[LH_K_COUNT] = {0};
 };
+
+// event pair around one launch (timing mode only; never inside graph capture)
+static int tbeg(lh_handle* h, int k, double bytes) {
+  if (!h->timing) return -1;
+  if (h->evnext + 2 > h->evpool.size()) {
+    for (int q = 0; q < 64; ++q) {
+      hipEvent_t e;
+      if (hipEventCreate(&e) != hipSuccess) return -1;
+      h->evpool.push_back(e);
+    }
+  }
+  hipEvent_t a = h->evpool[h->evnext++], b = h->evpool[h->evnext++];
+  (void)hipEventRecord(a, h->st);
+  h->pending.push_back({k, a, b});
+  h->tbytes[k] += bytes;
+  return (int)h->pending.size() - 1;
+}
+static void tend(lh_handle* h, int idx) {
+  if (idx < 0) return;
+  (void)hipEventRecord(h->pending[idx].b, h->st);
+}
+static void tflush(lh_handle* h) {
+  for (auto& q : h->pending) {
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, q.a, q.b) == hipSuccess) {
+      h->tms[q.k] += ms;
+      h->tcnt[q.k] += 1;
+    }
+  }
+  h->pending.clear();
+  h->evnext = 0;
+}
+#define TIMED(h, K, BYTES, ...)        \
+  do {                                 \
+    int ti_ = tbeg(h, K, BYTES);       \
+    __VA_ARGS__;                       \
+    tend(h, ti_);                      \
+  } while (0)
 
 static int fail(lh_handle* h, int code, const std::string& msg) {
   if (h) h->err = msg;
@@ -161,37 +208,43 @@ static void issue_substep(lh_handle* h, int krk /*0-based*/, bool cfl_mode) {
   const double bre = bR / dm.re;
   const int nlev = dm.nyu - 2;
   const bool f64 = h->prec == LH_PREC_FP64;
+  const double zel = f64 ? 16.0 : 8.0;                       // bytes per Z element
+  const double zsl = zel * dm.pz * dm.mx;                    // bytes per Z slot
+  const double spl = 16.0 * dm.ncol;                         // bytes per spectral plane
+  const double nuvw = 2.0 * dm.nyu + dm.nyv;                 // u, w, v planes
   // ---- NL stage: inverse z transforms of u, w (all u planes) and v (all v planes)
   {
     ZJobs jobs{};
     jobs.j[0] = ZJob{h->u1, h->zu, nullptr, 0, dm.nyu, 0};
     jobs.j[1] = ZJob{h->u3, h->zw, nullptr, 0, dm.nyu, 0};
     jobs.j[2] = ZJob{h->u2, h->zv, nullptr, 0, dm.nyv, 0};
-    launch_zinv(h, jobs, 3, dm.nyu, false);
+    TIMED(h, LH_K_ZINV, nuvw * (spl + zsl), launch_zinv(h, jobs, 3, dm.nyu, false));
   }
   if (krk == 0)
-    hipLaunchKernelGGL(k_set_scalar, dim3(1), dim3(1), 0, h->st, h->scal, (int)S_CFLMAX, 0.0);
+    TIMED(h, LH_K_MISC, 0.0,
+          hipLaunchKernelGGL(k_set_scalar, dim3(1), dim3(1), 0, h->st, h->scal, (int)S_CFLMAX, 0.0));
   {
     XNL a{h->zu, h->zw, h->zv, h->zp, h->dthdyu, h->scal, krk == 0 ? 1 : 0};
-    if (f64) {
-      LH_DISPATCH_LEN(dm.px, launch_xnl_t, double, h, a, h->twx)
-    } else {
-      LH_DISPATCH_LEN(dm.px, launch_xnl_t, float, h, a, h->twx)
-    }
+    TIMED(h, LH_K_XNL, nuvw * zsl + 6.0 * nlev * zsl, {
+      if (f64) {
+        LH_DISPATCH_LEN(dm.px, launch_xnl_t, double, h, a, h->twx)
+      } else {
+        LH_DISPATCH_LEN(dm.px, launch_xnl_t, float, h, a, h->twx)
+      }
+    });
   }
-  if (krk == 0 && cfl_mode) hipLaunchKernelGGL(k_cfl_dt, dim3(1), dim3(1), 0, h->st, h->scal);
+  if (krk == 0 && cfl_mode)
+    TIMED(h, LH_K_MISC, 0.0, hipLaunchKernelGGL(k_cfl_dt, dim3(1), dim3(1), 0, h->st, h->scal));
   {
     ZJobs jobs{};
-    // six products, each [nlev] slots of zp -> ph[q]
-    // (four jobs per launch: two launches)
+    // six products, three per launch: zp slots [q*nlev, (q+1)*nlev) -> ph[q]
     for (int q0 = 0; q0 < 6; q0 += 3) {
       for (int q = 0; q < 3; ++q) {
         const int qq = q0 + q;
-        const size_t zslot = (size_t)qq * nlev;
         jobs.j[q] = ZJob{(const double2*)h->zp, h->ph + (size_t)qq * nlev * dm.ncol, nullptr, 0,
-                         nlev, (int)zslot};
+                         nlev, qq * nlev};
       }
-      launch_zfwd(h, jobs, 3, nlev, false);
+      TIMED(h, LH_K_ZFWD, 3.0 * nlev * (zsl + spl), launch_zfwd(h, jobs, 3, nlev, false));
     }
   }
   {
@@ -205,10 +258,14 @@ static void issue_substep(lh_handle* h, int krk /*0-based*/, bool cfl_mode) {
     a.tip_v0 = h->tip_v[0] + dm.d; a.tip_v1 = h->tip_v[1] + dm.d;
     a.C1 = aR / dm.re; a.C2 = -cR; a.dR = dR; a.g = gR; a.bre = bre;
     dim3 grid((dm.ncol + 255) / 256, dm.nyu);
-    hipLaunchKernelGGL(k_assemble, grid, dim3(256), 0, h->st, a, dm);
+    // in: u, v, w, p, Nu_old, 6 product spectra; out: Nu_new, rhs, tip Lu_y
+    const double by = spl * (nuvw + dm.nyp + nuvw + 6.0 * nlev + nuvw + nuvw + 6.0);
+    TIMED(h, LH_K_ASM, by, hipLaunchKernelGGL(k_assemble, grid, dim3(256), 0, h->st, a, dm));
   }
   // ---- IB stage (FP64 transforms): canopy planes of rhs u, w, v and Lu_y on tip planes
   if (h->nibent > 0) {
+    const double z64 = 16.0 * dm.pz * dm.mx;
+    const double nib = 2.0 * h->nibu + h->nibv;
     ZJobs jobs{};
     int slot = 0;
     jobs.j[0] = ZJob{h->r1, h->zp, h->ibu_planes, 0, h->nibu, slot}; slot += h->nibu;
@@ -217,42 +274,49 @@ static void issue_substep(lh_handle* h, int krk /*0-based*/, bool cfl_mode) {
     jobs.j[3] = ZJob{h->tipl, h->zp, nullptr, 0, 6, slot};
     int maxn = h->nibu > h->nibv ? h->nibu : h->nibv;
     if (maxn < 6) maxn = 6;
-    launch_zinv(h, jobs, 4, maxn, true);
+    TIMED(h, LH_K_IBZ, (nib + 6.0) * (spl + z64), launch_zinv(h, jobs, 4, maxn, true));
     XIB a{h->zp, h->ibent, h->masks, h->scal, bre};
-    LH_DISPATCH_LEN(dm.px, launch_xib_t, double, h, a, h->nibent, h->twx64)
+    TIMED(h, LH_K_XIB, (2.0 * nib + 6.0) * z64,
+          { LH_DISPATCH_LEN(dm.px, launch_xib_t, double, h, a, h->nibent, h->twx64) });
     ZJobs fj{};
     slot = 0;
     fj.j[0] = ZJob{(const double2*)h->zp, h->r1, h->ibu_planes, 0, h->nibu, slot}; slot += h->nibu;
     fj.j[1] = ZJob{(const double2*)h->zp, h->r3, h->ibu_planes, 0, h->nibu, slot}; slot += h->nibu;
     fj.j[2] = ZJob{(const double2*)h->zp, h->r2, h->ibv_planes, 0, h->nibv, slot};
-    launch_zfwd(h, fj, 3, h->nibu > h->nibv ? h->nibu : h->nibv, true);
+    TIMED(h, LH_K_IBZ, nib * (spl + z64),
+          launch_zfwd(h, fj, 3, h->nibu > h->nibv ? h->nibu : h->nibv, true));
   }
   // ---- column stage
   {
     PREP a{h->dyu2i, h->dyv2i, h->scal, h->ayu, h->ayv, h->gw[0], h->gw[1], bre, dm.nyu, dm.nyv};
-    hipLaunchKernelGGL(k_prep, dim3(1), dim3(64), 0, h->st, a);
+    TIMED(h, LH_K_MISC, 0.0, hipLaunchKernelGGL(k_prep, dim3(1), dim3(64), 0, h->st, a));
   }
   {
     VEL a{h->r1, h->r2, h->r3, h->u1, h->u2, h->u3, h->ayu, h->ayv};
-    hipLaunchKernelGGL(k_vel, dim3((dm.ncol + 255) / 256, 3), dim3(256), 0, h->st, a, dm);
+    TIMED(h, LH_K_VEL, 2.0 * nuvw * spl,
+          hipLaunchKernelGGL(k_vel, dim3((dm.ncol + 255) / 256, 3), dim3(256), 0, h->st, a, dm));
   }
   if (h->flag_ctpress == 0) {
     CTU a{h->u1, h->scal, h->ctwork, h->dyu2i, h->yv, h->gw[0], h->gw[1], bre, dR, dm.ny, dm.d,
           dm.ncol};
-    hipLaunchKernelGGL(k_ctU, dim3(1), dim3(64), 0, h->st, a);
+    TIMED(h, LH_K_MISC, 0.0, hipLaunchKernelGGL(k_ctU, dim3(1), dim3(64), 0, h->st, a));
   }
   {
     PRES a{h->u1, h->u2, h->u3, h->p, h->psi, h->dg1, h->dg2, h->dg3, h->dthdyu, h->dthdyv,
            h->k1x, h->k1z, h->scal, aR + bR, h->gw[0], h->gw[1]};
-    hipLaunchKernelGGL(k_pressure, dim3((dm.ncol + 255) / 256), dim3(256), 0, h->st, a, dm);
+    // in: u, v, w, p, three LU factor arrays (8 B each); out: u, v, w, p
+    const double by = spl * (2.0 * (nuvw + dm.nyp)) + 3.0 * 8.0 * dm.ncol * (double)dm.nyp;
+    TIMED(h, LH_K_PRES, by,
+          hipLaunchKernelGGL(k_pressure, dim3((dm.ncol + 255) / 256), dim3(256), 0, h->st, a, dm));
   }
 }
 
 static void issue_step(lh_handle* h, bool cfl_mode) {
   for (int k = 0; k < 3; ++k) issue_substep(h, k, cfl_mode);
   if (h->flag_ctpress == 1)
-    hipLaunchKernelGGL(k_flowrate, dim3(1), dim3(64), 0, h->st, h->u1, h->yv, h->scal, h->dm.ny,
-                       h->dm.d, h->dm.ncol);
+    TIMED(h, LH_K_MISC, 0.0,
+          hipLaunchKernelGGL(k_flowrate, dim3(1), dim3(64), 0, h->st, h->u1, h->yv, h->scal,
+                             h->dm.ny, h->dm.d, h->dm.ncol));
 }
 
 // ---------------------------------------------------------------------------------------
@@ -465,7 +529,8 @@ int lh_destroy(lh_handle* h) {
   if (!h) return LH_OK;
   if (h->st) hipStreamSynchronize(h->st);
   for (auto& g : h->gexec)
-    if (g) hipGraphExecDestroy(g);
+    if (g) (void)hipGraphExecDestroy(g);
+  for (auto e : h->evpool) (void)hipEventDestroy(e);
   void* ptrs[] = {h->yu, h->yv, h->dthdyu, h->dthdyv, h->dyu2i, h->dyv2i, h->k1x, h->k2x, h->k1z,
                   h->k2z, h->scal, h->u1, h->u2, h->u3, h->p, h->nu1, h->nu2, h->nu3, h->r1,
                   h->r2, h->r3, h->psi, h->tipl, h->ph, h->zu, h->zw, h->zv, h->zp, h->dg1,
@@ -566,10 +631,15 @@ int lh_substep(lh_handle* h, int krk, double dt) {
   issue_substep(h, krk - 1, false);
   HIPCHK(h, hipGetLastError());
   HIPCHK(h, hipStreamSynchronize(h->st));
+  tflush(h);
   return LH_OK;
 }
 
 static int run_graph(lh_handle* h, int mode) {
+  if (h->timing) {   // per-launch events cannot live in a graph: issue directly
+    issue_step(h, mode == 1);
+    return LH_OK;
+  }
   if (!h->gexec[mode]) {
     hipGraph_t g;
     HIPCHK(h, hipStreamBeginCapture(h->st, hipStreamCaptureModeThreadLocal));
@@ -590,6 +660,7 @@ int lh_step(lh_handle* h, double dt) {
   if (r) return r;
   HIPCHK(h, hipGetLastError());
   HIPCHK(h, hipStreamSynchronize(h->st));
+  tflush(h);
   return LH_OK;
 }
 
@@ -601,6 +672,7 @@ int lh_step_cfl(lh_handle* h, double cfl, double* dt_used) {
   if (r) return r;
   HIPCHK(h, hipGetLastError());
   HIPCHK(h, hipStreamSynchronize(h->st));
+  tflush(h);
   if (dt_used) HIPCHK(h, hipMemcpy(dt_used, h->scal + S_DT, 8, hipMemcpyDeviceToHost));
   return LH_OK;
 }
@@ -632,6 +704,31 @@ int lh_cfl_dt(lh_handle* h, double cfl, double* dt) {
   if (!h || !dt) return LH_ERR_ARG;
   (void)cfl;
   return fail(h, LH_ERR_ARG, "lh_cfl_dt: not available yet");
+}
+
+int lh_timing_enable(lh_handle* h, int enable) {
+  if (!h) return LH_ERR_ARG;
+  HIPCHK(h, hipStreamSynchronize(h->st));
+  tflush(h);
+  h->timing = enable != 0;
+  return LH_OK;
+}
+
+int lh_timing_reset(lh_handle* h) {
+  if (!h) return LH_ERR_ARG;
+  HIPCHK(h, hipStreamSynchronize(h->st));
+  tflush(h);
+  for (int k = 0; k < LH_K_COUNT; ++k) { h->tms[k] = 0; h->tcnt[k] = 0; h->tbytes[k] = 0; }
+  return LH_OK;
+}
+
+int lh_timing_read(lh_handle* h, int kernel, double* total_ms, long long* launches,
+                   double* bytes_per_launch) {
+  if (!h || kernel < 0 || kernel >= LH_K_COUNT) return LH_ERR_ARG;
+  if (total_ms) *total_ms = h->tms[kernel];
+  if (launches) *launches = h->tcnt[kernel];
+  if (bytes_per_launch) *bytes_per_launch = h->tcnt[kernel] ? h->tbytes[kernel] / h->tcnt[kernel] : 0.0;
+  return LH_OK;
 }
 
 int lh_stats_accumulate(lh_handle* h) { return fail(h, LH_ERR_ARG, "statistics not available yet"); }

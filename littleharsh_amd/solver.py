@@ -106,3 +106,19 @@ class Channel:
         e = ctypes.c_double()
         self._chk(self._lib.lh_max_div(self._h, ctypes.byref(e)), "lh_max_div")
         return e.value
+
+    # ---- per-kernel timing (HIP events on the handle's stream)
+    def timing(self, enable: bool):
+        self._chk(self._lib.lh_timing_enable(self._h, 1 if enable else 0), "lh_timing_enable")
+
+    def timing_reset(self):
+        self._chk(self._lib.lh_timing_reset(self._h), "lh_timing_reset")
+
+    def timing_read(self):
+        out = {}
+        for k, name in enumerate(capi.KERNEL_NAMES):
+            ms, n, by = ctypes.c_double(), ctypes.c_longlong(), ctypes.c_double()
+            self._chk(self._lib.lh_timing_read(self._h, k, ctypes.byref(ms), ctypes.byref(n),
+                                               ctypes.byref(by)), "lh_timing_read")
+            out[name] = dict(ms=ms.value, launches=n.value, bytes_per_launch=by.value)
+        return out
