@@ -2,11 +2,11 @@
 //
 // The handle owns one HIP stream, the device-resident state in [plane][column] layout and
 // all work buffers.  lh_substep issues the kernels of one RK3 substep:
-//   NL stage   : k_zinv (u,w,v) -> k_xpass_nl (products, CFL max) -> k_zfwd (6 products)
-//                -> k_assemble (RHS0 + nonlinear + rhs of solveU/V, /axz)
+//   NL stage   : k_zinv (u,w,v) -> k_xnl (x transforms, products, CFL max, recombination)
+//                -> k_zasm (forward z transforms + RHS0 + nonlinear + rhs of solveU/V)
 //   IB stage   : k_zinv (rhs and Lu_y on canopy planes) -> k_xpass_ib -> k_zfwd
-//   column     : k_prep (LU of the shared velocity matrices) -> k_vel (Thomas u,w,v)
-//                -> k_ctU (constant flow rate only) -> k_pressure (solveP + v_corr)
+//   column     : k_prep (LU of the shared velocity matrices) -> k_column (Thomas u,w,v,
+//                constant-flow-rate correction, solveP, v_corr)
 // lh_step replays a captured HIP graph of three substeps (+ meanflow_ctP).
 #include <hip/hip_runtime.h>
 #include <math.h>
@@ -19,6 +19,8 @@
 
 #include "../../include/littleharsh_amd.h"
 #include "lh_kernels.hpp"
+#include "lh_nl.hpp"
+#include "lh_col.hpp"
 
 using namespace lh;
 
@@ -155,12 +157,39 @@ static void launch_zfwd_t(lh_handle* h, const ZJobs& jobs, int njobs, int maxn, 
   hipLaunchKernelGGL((k_zfwd<PZ, L, TR>), grid, dim3(L * Plan<PZ>::T), sh, h->st, jobs, h->dm,
                      (const cx<TR>*)tw);
 }
+constexpr int kLevelsPerBlock = 16;
 template <int PX, typename TR>
-static void launch_xnl_t(lh_handle* h, const XNL& a, const void* tw) {
+static void launch_xnl_t(lh_handle* h, const XNL2& a, const void* tw) {
+  const int nlev = h->dm.nyu - 2;
+  dim3 grid(h->dm.pz / 2, (nlev + a.jc - 1) / a.jc);
+  size_t sh = ((size_t)6 * (PX + 1) + Plan<PX>::template TwL<cx<TR>>::SIZE) * sizeof(cx<TR>);
+  hipLaunchKernelGGL((k_xnl<PX, TR>), grid, dim3(6 * Plan<PX>::T), sh, h->st, a, h->dm,
+                     (const cx<TR>*)tw);
+}
+template <int PX, typename TR>
+static void launch_xraw_t(lh_handle* h, const XRAW& a, const void* tw) {
   dim3 grid(h->dm.pz / 2, h->dm.nyu - 2);
   size_t sh = (size_t)6 * (PX + 1) * sizeof(cx<TR>);
-  hipLaunchKernelGGL((k_xpass_nl<PX, TR>), grid, dim3(6 * Plan<PX>::T), sh, h->st, a, h->dm,
+  hipLaunchKernelGGL((k_xraw<PX, TR>), grid, dim3(6 * Plan<PX>::T), sh, h->st, a, h->dm,
                      (const cx<TR>*)tw);
+}
+constexpr int kZasmCols = 4;
+template <int PZ, typename TR>
+static void launch_zasm_t(lh_handle* h, const ZASM& a, const void* tw) {
+  constexpr int L = kZasmCols;
+  const int nlev = h->dm.nyu - 2;
+  dim3 grid(a.nib * nlev * 3);
+  size_t sh = (size_t)2 * L * (PZ + 1) * sizeof(cx<TR>);
+  hipLaunchKernelGGL((k_zasm<PZ, L, TR>), grid, dim3(2 * L * Plan<PZ>::T), sh, h->st, a, h->dm,
+                     (const cx<TR>*)tw);
+}
+template <int PZ, typename TR>
+static void attr_zasm_t(int* dummy) {
+  constexpr int L = kZasmCols;
+  (void)dummy;
+  (void)hipFuncSetAttribute((const void*)k_zasm<PZ, L, TR>,
+                            hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)(2 * L * (PZ + 1) * sizeof(cx<TR>)));
 }
 template <int PX, typename TR>
 static void launch_xib_t(lh_handle* h, const XIB& a, int nent, const void* tw) {
@@ -224,43 +253,38 @@ static void issue_substep(lh_handle* h, int krk /*0-based*/, bool cfl_mode) {
     TIMED(h, LH_K_MISC, 0.0,
           hipLaunchKernelGGL(k_set_scalar, dim3(1), dim3(1), 0, h->st, h->scal, (int)S_CFLMAX, 0.0));
   {
-    XNL a{h->zu, h->zw, h->zv, h->zp, h->dthdyu, h->scal, krk == 0 ? 1 : 0};
+    XRAW a{h->zu, h->zw, h->zv, h->zp, h->dthdyu, h->scal, krk == 0 ? 1 : 0};
     TIMED(h, LH_K_XNL, nuvw * zsl + 6.0 * nlev * zsl, {
       if (f64) {
-        LH_DISPATCH_LEN(dm.px, launch_xnl_t, double, h, a, h->twx)
+        LH_DISPATCH_LEN(dm.px, launch_xraw_t, double, h, a, h->twx)
       } else {
-        LH_DISPATCH_LEN(dm.px, launch_xnl_t, float, h, a, h->twx)
+        LH_DISPATCH_LEN(dm.px, launch_xraw_t, float, h, a, h->twx)
       }
     });
   }
   if (krk == 0 && cfl_mode)
     TIMED(h, LH_K_MISC, 0.0, hipLaunchKernelGGL(k_cfl_dt, dim3(1), dim3(1), 0, h->st, h->scal));
   {
-    ZJobs jobs{};
-    // six products, three per launch: zp slots [q*nlev, (q+1)*nlev) -> ph[q]
-    for (int q0 = 0; q0 < 6; q0 += 3) {
-      for (int q = 0; q < 3; ++q) {
-        const int qq = q0 + q;
-        jobs.j[q] = ZJob{(const double2*)h->zp, h->ph + (size_t)qq * nlev * dm.ncol, nullptr, 0,
-                         nlev, qq * nlev};
-      }
-      TIMED(h, LH_K_ZFWD, 3.0 * nlev * (zsl + spl), launch_zfwd(h, jobs, 3, nlev, false));
-    }
-  }
-  {
-    ASM a{};
+    ZASM a{};
+    a.zo = h->zp;
     a.u1 = h->u1; a.u2 = h->u2; a.u3 = h->u3; a.p = h->p;
     a.nu1 = h->nu1; a.nu2 = h->nu2; a.nu3 = h->nu3;
-    a.ph = h->ph; a.r1 = h->r1; a.r2 = h->r2; a.r3 = h->r3; a.tipl = h->tipl;
+    a.r1 = h->r1; a.r2 = h->r2; a.r3 = h->r3; a.tipl = h->tipl;
     a.dyu2i = h->dyu2i; a.dyv2i = h->dyv2i; a.dthdyu = h->dthdyu; a.dthdyv = h->dthdyv;
     a.k1x = h->k1x; a.k2x = h->k2x; a.k1z = h->k1z; a.k2z = h->k2z; a.scal = h->scal;
     a.tip_u0 = h->tip_u[0] + dm.d; a.tip_u1 = h->tip_u[1] + dm.d;
     a.tip_v0 = h->tip_v[0] + dm.d; a.tip_v1 = h->tip_v[1] + dm.d;
+    a.nib = (dm.mx + kZasmCols - 1) / kZasmCols;
     a.C1 = aR / dm.re; a.C2 = -cR; a.dR = dR; a.g = gR; a.bre = bre;
-    dim3 grid((dm.ncol + 255) / 256, dm.nyu);
-    // in: u, v, w, p, Nu_old, 6 product spectra; out: Nu_new, rhs, tip Lu_y
-    const double by = spl * (nuvw + dm.nyp + nuvw + 6.0 * nlev + nuvw + nuvw + 6.0);
-    TIMED(h, LH_K_ASM, by, hipLaunchKernelGGL(k_assemble, grid, dim3(256), 0, h->st, a, dm));
+    // in: six product Z rows per level, u/v/w, p, Nu_old; out: Nu_new, rhs
+    const double by = 6.0 * nlev * zsl + spl * (nuvw + dm.nyp + 3.0 * nuvw);
+    TIMED(h, LH_K_ASM, by, {
+      if (f64) {
+        LH_DISPATCH_LEN(dm.pz, launch_zasm_t, double, h, a, h->twz)
+      } else {
+        LH_DISPATCH_LEN(dm.pz, launch_zasm_t, float, h, a, h->twz)
+      }
+    });
   }
   // ---- IB stage (FP64 transforms): canopy planes of rhs u, w, v and Lu_y on tip planes
   if (h->nibent > 0) {
@@ -292,22 +316,21 @@ static void issue_substep(lh_handle* h, int krk /*0-based*/, bool cfl_mode) {
     TIMED(h, LH_K_MISC, 0.0, hipLaunchKernelGGL(k_prep, dim3(1), dim3(64), 0, h->st, a));
   }
   {
-    VEL a{h->r1, h->r2, h->r3, h->u1, h->u2, h->u3, h->ayu, h->ayv};
+    COL a{};
+    a.r1 = h->r1; a.r2 = h->r2; a.r3 = h->r3;
+    a.u1 = h->u1; a.u2 = h->u2; a.u3 = h->u3; a.p = h->p; a.psi = h->psi;
+    a.ayu = h->ayu; a.ayv = h->ayv; a.dg1 = h->dg1; a.dg2 = h->dg2; a.dg3 = h->dg3;
+    a.dthdyu = h->dthdyu; a.dthdyv = h->dthdyv; a.k1x = h->k1x; a.k1z = h->k1z;
+    a.scal = h->scal; a.ab = aR + bR; a.gw0 = h->gw[0]; a.gw1 = h->gw[1];
+    a.ctu = h->flag_ctpress == 0 ? 1 : 0;
+    a.ct = CTU{h->u1, h->scal, h->ctwork, h->dyu2i, h->yv, h->gw[0], h->gw[1], bre, dR, dm.ny,
+               dm.d, dm.ncol};
+    // in: rhs u, w, v, p, LU factors (8 B each); out: u, w, v, p
+    const double by = spl * (2.0 * nuvw + 2.0 * dm.nyp) + 3.0 * 8.0 * dm.ncol * (double)dm.nyp;
     TIMED(h, LH_K_VEL, 2.0 * nuvw * spl,
-          hipLaunchKernelGGL(k_vel, dim3((dm.ncol + 255) / 256, 3), dim3(256), 0, h->st, a, dm));
-  }
-  if (h->flag_ctpress == 0) {
-    CTU a{h->u1, h->scal, h->ctwork, h->dyu2i, h->yv, h->gw[0], h->gw[1], bre, dR, dm.ny, dm.d,
-          dm.ncol};
-    TIMED(h, LH_K_MISC, 0.0, hipLaunchKernelGGL(k_ctU, dim3(1), dim3(64), 0, h->st, a));
-  }
-  {
-    PRES a{h->u1, h->u2, h->u3, h->p, h->psi, h->dg1, h->dg2, h->dg3, h->dthdyu, h->dthdyv,
-           h->k1x, h->k1z, h->scal, aR + bR, h->gw[0], h->gw[1]};
-    // in: u, v, w, p, three LU factor arrays (8 B each); out: u, v, w, p
-    const double by = spl * (2.0 * (nuvw + dm.nyp)) + 3.0 * 8.0 * dm.ncol * (double)dm.nyp;
-    TIMED(h, LH_K_PRES, by,
-          hipLaunchKernelGGL(k_pressure, dim3((dm.ncol + 255) / 256), dim3(256), 0, h->st, a, dm));
+          hipLaunchKernelGGL(k_vel<8>, dim3((dm.ncol + 255) / 256, 3), dim3(256), 0, h->st, a, dm));
+    TIMED(h, LH_K_PRES, by - 2.0 * nuvw * spl + 2.0 * nuvw * spl,
+          hipLaunchKernelGGL(k_pres<8>, dim3((dm.ncol + 255) / 256), dim3(256), 0, h->st, a, dm));
   }
 }
 
@@ -364,6 +387,11 @@ int lh_create(const lh_config* c, lh_handle** out) {
   *out = h;
   HIPCHK(h, hipGetDevice(&h->dev));
   HIPCHK(h, hipStreamCreate(&h->st));
+  {
+    int dummy = 0;
+    LH_DISPATCH_LEN(c->pz, attr_zasm_t, double, &dummy)
+    LH_DISPATCH_LEN(c->pz, attr_zasm_t, float, &dummy)
+  }
   const size_t nu = (size_t)dm.nyu * dm.ncol, nv = (size_t)dm.nyv * dm.ncol,
                np = (size_t)dm.nyp * dm.ncol;
   const int nlev = dm.nyu - 2;

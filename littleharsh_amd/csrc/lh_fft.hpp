@@ -37,6 +37,12 @@ template <bool INV, typename C> __device__ __forceinline__ C rot(C a) {
   return INV ? mk<C>(-a.y, a.x) : mk<C>(a.y, -a.x);
 }
 
+// Workgroup barrier for LDS hand-offs only: waits for this wave's LDS operations, not for
+// its outstanding global loads, so prefetched loads stay in flight across FFT stages.
+__device__ __forceinline__ void lds_bar() {
+  asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+}
+
 // W_R^m = exp(-2 pi i m / R) for the composite radices, cos/sin(2 pi m/R) to double precision
 __device__ constexpr double kW6[6][2] = {{1.0, -0.0}, {5.00000000000000111022e-01, -8.66025403784438596588e-01}, {-4.99999999999999777955e-01, -8.66025403784438707611e-01}, {-1.0, -1.22464679914735320717e-16}, {-5.00000000000000444089e-01, 8.66025403784438374544e-01}, {5.00000000000000111022e-01, 8.66025403784438596588e-01}};
 __device__ constexpr double kW8[8][2] = {{1.0, -0.0}, {7.07106781186547572737e-01, -7.07106781186547461715e-01}, {6.12323399573676603587e-17, -1.0}, {-7.07106781186547461715e-01, -7.07106781186547572737e-01}, {-1.0, -1.22464679914735320717e-16}, {-7.07106781186547683760e-01, 7.07106781186547461715e-01}, {-1.83697019872102968750e-16, 1.0}, {7.07106781186547350693e-01, 7.07106781186547683760e-01}};
@@ -81,6 +87,12 @@ template <int R, bool INV, typename C> struct DFT {
         const int m = (n2 * k1) % R;
         if (m == 0) {
           y[n2 * A + k1] = t[k1];
+        } else if (4 * m == R) {            // W = -i (forward) / +i (inverse)
+          y[n2 * A + k1] = rot<INV>(t[k1]);
+        } else if (2 * m == R) {            // W = -1
+          y[n2 * A + k1] = mk<C>(-t[k1].x, -t[k1].y);
+        } else if (4 * m == 3 * R) {        // W = +i (forward) / -i (inverse)
+          y[n2 * A + k1] = rot<!INV>(t[k1]);
         } else {
           const C w = mk<C>((T)wre<R>(m), INV ? -(T)wim<R>(m) : (T)wim<R>(m));
           y[n2 * A + k1] = cmul(t[k1], w);
@@ -132,10 +144,52 @@ template <bool INV, typename C> struct DFT<4, INV, C> {
   }
 };
 
+// Per-thread twiddles of one stage: the thread's butterflies b = t + q*T are fixed for a
+// given line position t, so their twiddles W_N^{(b%NS)*r*N/(NS*R)} are loaded once into
+// registers and reused for every line the thread transforms.
+template <int N, int T, int R, int NS, typename C>
+struct StageTw {
+  static constexpr int NB = N / R;
+  static constexpr int PER = (NB + T - 1) / T;
+  C w[PER][R > 1 ? R - 1 : 1];
+  __device__ __forceinline__ void load(int t, const C* __restrict__ tw) {
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+      const int b = t + q * T;
+      const int j = (b < NB ? b : 0) % NS;
+#pragma unroll
+      for (int r = 1; r < R; ++r) w[q][r - 1] = tw[j * r * (N / (NS * R))];
+    }
+  }
+  __device__ __forceinline__ C get(int q, int r) const { return w[q][r - 1]; }
+};
+
+// The same twiddles kept in an LDS table (one row of PER*(R-1) values per line position t)
+// when registers are the occupancy limit.
+template <int N, int T, int R, int NS, typename C>
+struct StageTwL {
+  static constexpr int NB = N / R;
+  static constexpr int PER = (NB + T - 1) / T;
+  static constexpr int CNT = PER * (R - 1);
+  static constexpr int SIZE = T * CNT;
+  const C* p;
+  __device__ __forceinline__ void init(C* tab, int t, const C* __restrict__ tw, bool fill) {
+    p = tab + t * CNT;
+    if (fill) {
+      for (int q = 0; q < PER; ++q) {
+        const int b = t + q * T;
+        const int j = (b < NB ? b : 0) % NS;
+        for (int r = 1; r < R; ++r) tab[t * CNT + q * (R - 1) + r - 1] = tw[j * r * (N / (NS * R))];
+      }
+    }
+  }
+  __device__ __forceinline__ C get(int q, int r) const { return p[q * (R - 1) + r - 1]; }
+};
+
 // One Stockham stage of radix R on a line of N values; NS = product of earlier radices.
-// tw[m] = exp(-2 pi i m/N) (same precision as the line), m < N.
-template <int N, int T, int R, int NS, bool INV, typename C>
-__device__ __forceinline__ void stage(C* line, int t, const C* __restrict__ tw) {
+// w: the thread's twiddles for this stage (unused when NS == 1).
+template <int N, int T, int R, int NS, bool INV, typename C, typename W>
+__device__ __forceinline__ void stage(C* line, int t, const W& w) {
   constexpr int NB = N / R;
   constexpr int PER = (NB + T - 1) / T;
   C v[PER][R];
@@ -147,18 +201,16 @@ __device__ __forceinline__ void stage(C* line, int t, const C* __restrict__ tw) 
       for (int r = 0; r < R; ++r) v[q][r] = line[b + r * NB];
     }
   }
-  __syncthreads();
+  lds_bar();
 #pragma unroll
   for (int q = 0; q < PER; ++q) {
     const int b = t + q * T;
     if (PER * T == NB || b < NB) {
       const int j = b % NS;
-      if (NS > 1) {
+      if constexpr (NS > 1) {
 #pragma unroll
-        for (int r = 1; r < R; ++r) {
-          const C w = tw[j * r * (N / (NS * R))];
-          v[q][r] = INV ? cmulc(v[q][r], w) : cmul(v[q][r], w);
-        }
+        for (int r = 1; r < R; ++r)
+          v[q][r] = INV ? cmulc(v[q][r], w.get(q, r)) : cmul(v[q][r], w.get(q, r));
       }
       DFT<R, INV, C>::run(v[q]);
       const int idx = (b / NS) * NS * R + j;
@@ -166,69 +218,66 @@ __device__ __forceinline__ void stage(C* line, int t, const C* __restrict__ tw) 
       for (int r = 0; r < R; ++r) line[idx + r * NS] = v[q][r];
     }
   }
-  __syncthreads();
+  lds_bar();
 }
 
-// Plans: threads per line T and the radix sequence for each supported length.
+struct NoTw {};
+
+// Plans: threads per line T, the radix sequence, and the per-thread twiddle set Tw<C>.
+#define LH_PLAN2(NN, TT, R1, R2)                                                        \
+  template <> struct Plan<NN> {                                                         \
+    static constexpr int T = TT;                                                        \
+    template <typename C> struct Tw {                                                   \
+      StageTw<NN, TT, R2, R1, C> s2;                                                    \
+      __device__ __forceinline__ void load(int t, const C* tw) { s2.load(t, tw); }      \
+    };                                                                                  \
+    template <typename C> struct TwL {                                                  \
+      StageTwL<NN, TT, R2, R1, C> s2;                                                   \
+      static constexpr int SIZE = StageTwL<NN, TT, R2, R1, C>::SIZE;                    \
+      __device__ __forceinline__ void init(C* tab, int t, const C* tw, bool fill) {     \
+        s2.init(tab, t, tw, fill);                                                      \
+      }                                                                                 \
+    };                                                                                  \
+    template <bool INV, typename C, typename W> static __device__ void run(C* l, int t, const W& w) { \
+      stage<NN, TT, R1, 1, INV>(l, t, NoTw{});                                          \
+      stage<NN, TT, R2, R1, INV>(l, t, w.s2);                                           \
+    }                                                                                   \
+  };
+#define LH_PLAN3(NN, TT, R1, R2, R3)                                                    \
+  template <> struct Plan<NN> {                                                         \
+    static constexpr int T = TT;                                                        \
+    template <typename C> struct Tw {                                                   \
+      StageTw<NN, TT, R2, R1, C> s2;                                                    \
+      StageTw<NN, TT, R3, R1 * R2, C> s3;                                               \
+      __device__ __forceinline__ void load(int t, const C* tw) { s2.load(t, tw); s3.load(t, tw); } \
+    };                                                                                  \
+    template <typename C> struct TwL {                                                  \
+      StageTwL<NN, TT, R2, R1, C> s2;                                                   \
+      StageTwL<NN, TT, R3, R1 * R2, C> s3;                                              \
+      static constexpr int SIZE = StageTwL<NN, TT, R2, R1, C>::SIZE +                   \
+                                  StageTwL<NN, TT, R3, R1 * R2, C>::SIZE;               \
+      __device__ __forceinline__ void init(C* tab, int t, const C* tw, bool fill) {     \
+        s2.init(tab, t, tw, fill);                                                      \
+        s3.init(tab + StageTwL<NN, TT, R2, R1, C>::SIZE, t, tw, fill);                  \
+      }                                                                                 \
+    };                                                                                  \
+    template <bool INV, typename C, typename W> static __device__ void run(C* l, int t, const W& w) { \
+      stage<NN, TT, R1, 1, INV>(l, t, NoTw{});                                          \
+      stage<NN, TT, R2, R1, INV>(l, t, w.s2);                                           \
+      stage<NN, TT, R3, R1 * R2, INV>(l, t, w.s3);                                      \
+    }                                                                                   \
+  };
+
 template <int N> struct Plan;
-template <> struct Plan<576> {
-  static constexpr int T = 48;
-  template <bool INV, typename C> static __device__ void run(C* l, int t, const C* tw) {
-    stage<576, T, 12, 1, INV>(l, t, tw);
-    stage<576, T, 12, 12, INV>(l, t, tw);
-    stage<576, T, 4, 144, INV>(l, t, tw);
-  }
-};
-template <> struct Plan<384> {
-  static constexpr int T = 48;
-  template <bool INV, typename C> static __device__ void run(C* l, int t, const C* tw) {
-    stage<384, T, 8, 1, INV>(l, t, tw);
-    stage<384, T, 12, 8, INV>(l, t, tw);
-    stage<384, T, 4, 96, INV>(l, t, tw);
-  }
-};
-template <> struct Plan<144> {
-  static constexpr int T = 12;
-  template <bool INV, typename C> static __device__ void run(C* l, int t, const C* tw) {
-    stage<144, T, 12, 1, INV>(l, t, tw);
-    stage<144, T, 12, 12, INV>(l, t, tw);
-  }
-};
-template <> struct Plan<96> {
-  static constexpr int T = 8;
-  template <bool INV, typename C> static __device__ void run(C* l, int t, const C* tw) {
-    stage<96, T, 4, 1, INV>(l, t, tw);
-    stage<96, T, 4, 4, INV>(l, t, tw);
-    stage<96, T, 6, 16, INV>(l, t, tw);
-  }
-};
-template <> struct Plan<72> {
-  static constexpr int T = 6;
-  template <bool INV, typename C> static __device__ void run(C* l, int t, const C* tw) {
-    stage<72, T, 12, 1, INV>(l, t, tw);
-    stage<72, T, 6, 12, INV>(l, t, tw);
-  }
-};
-template <> struct Plan<64> {
-  static constexpr int T = 4;
-  template <bool INV, typename C> static __device__ void run(C* l, int t, const C* tw) {
-    stage<64, T, 16, 1, INV>(l, t, tw);
-    stage<64, T, 4, 16, INV>(l, t, tw);
-  }
-};
-template <> struct Plan<48> {
-  static constexpr int T = 4;
-  template <bool INV, typename C> static __device__ void run(C* l, int t, const C* tw) {
-    stage<48, T, 12, 1, INV>(l, t, tw);
-    stage<48, T, 4, 12, INV>(l, t, tw);
-  }
-};
-template <> struct Plan<32> {
-  static constexpr int T = 4;
-  template <bool INV, typename C> static __device__ void run(C* l, int t, const C* tw) {
-    stage<32, T, 8, 1, INV>(l, t, tw);
-    stage<32, T, 4, 8, INV>(l, t, tw);
-  }
-};
+LH_PLAN3(576, 48, 12, 12, 4)
+LH_PLAN3(384, 48, 8, 12, 4)
+LH_PLAN2(144, 12, 12, 12)
+LH_PLAN3(96, 8, 4, 4, 6)
+LH_PLAN2(72, 6, 12, 6)
+LH_PLAN2(64, 4, 16, 4)
+LH_PLAN2(48, 4, 12, 4)
+LH_PLAN2(32, 4, 8, 4)
+#undef LH_PLAN2
+#undef LH_PLAN3
 
 }  // namespace lh

@@ -1,0 +1,544 @@
+// lh_nl.hpp -- nonlinear-term kernels of the RK3 substep (gfx950).
+//
+// The nonlinear term of the reference lives in the absent FOU3D.f90 (call at
+// littleharsh.f90:171); it is restated as the 3/2-dealiased divergence form on the
+// staggered grid:
+//   Nu1(j) = ikx uu(j) + ikz uw(j) + m_u(j) (uv(j) - uv(j-1))      u centres
+//   Nu3(j) = ikx uw(j) + ikz ww(j) + m_u(j) (vw(j) - vw(j-1))      u centres
+//   Nu2(j) = ikx uv(j) + ikz vw(j) + m_v(j) (vv(j+1) - vv(j))      v faces
+// with m = ddthetavi*dthdy (divergence, littleharsh.f90:238-241), v averaged to centres and
+// u, w to faces (theta midpoints), products of the zero-padded physical fields.
+//
+// Because the z transform is linear and ikx is constant along a z line, each Nu is
+// F_z(A) + ikz F_z(B) with A, B combined before the z transform:
+//   u:  A1 = ikx uu + m_u (uv - uv_prev),   B1 = uw
+//   w:  A3 = ikx uw + m_u (vw - vw_prev),   B3 = ww
+//   v:  A2 = ikx uv_prev + m_v (vv - vv_prev), B2 = vw_prev   (face j-1 at level j)
+// The x pass (k_xnl) produces these six rows per level; the forward z pass (k_zasm)
+// transforms them and assembles RHS0 + nonlinear + the rhs of solveU/solveV in one go.
+#pragma once
+#include <type_traits>
+#include "lh_kernels.hpp"
+
+namespace lh {
+
+struct XNL2 {
+  const void* zu;       // Z slots per u plane
+  const void* zw;
+  const void* zv;       // Z slots per v plane
+  void* zo;             // [6][nlev][pz][mx]
+  const double* dthdyu;
+  const double* dthdyv;
+  const double* k1x;
+  double* scal;
+  int want_cfl;
+  int jc;               // levels per block
+};
+
+// Block: one pair of z rows (z0, z0+1) and a chunk of consecutive levels.  Physical
+// u, w at the current centre and v at the face below stay in registers (each thread owns
+// fixed x positions); the Z-space products uv, vw, vv of the previous level stay in
+// registers of the thread that owns that kx index.
+template <int PX, typename TR>
+__global__ void __launch_bounds__(6 * Plan<PX>::T) __attribute__((amdgpu_waves_per_eu(3)))
+    k_xnl(XNL2 a, Dims dm, const cx<TR>* __restrict__ tw) {
+  using C = cx<TR>;
+  constexpr int T = Plan<PX>::T;
+  constexpr int NT = 6 * T;
+  constexpr int LS = PX + 1;
+  constexpr int XPT = (PX + NT - 1) / NT;       // x positions per thread
+  extern __shared__ char smem_raw[];
+  C* lds = reinterpret_cast<C*>(smem_raw);
+  __shared__ double red[NT / 64 + 1];
+  const int tid = threadIdx.x;
+  const int z0 = 2 * blockIdx.x;
+  const int nlev = dm.nyu - 2;
+  const int l0 = blockIdx.y * a.jc;
+  const int l1 = min(l0 + a.jc, nlev);
+  const int mx = dm.mx;
+  const size_t slot = (size_t)dm.pz * mx;
+  const C* zu = reinterpret_cast<const C*>(a.zu);
+  const C* zw = reinterpret_cast<const C*>(a.zw);
+  const C* zv = reinterpret_cast<const C*>(a.zv);
+  C* zo = reinterpret_cast<C*>(a.zo);
+  const int ln = tid / T, t = tid % T;
+  // twiddles in LDS after the six lines (filled by the threads of line 0)
+  typename Plan<PX>::template TwL<C> w;
+  w.init(lds + 6 * LS, t, tw, ln == 0);
+  const TR half = (TR)0.5;
+
+  // rows z0, z0+1 of up to six source slots: loads go to registers first (issue), then are
+  // packed into LDS lines (commit); the three per-level loads are issued one level ahead.
+  constexpr int MAXI = (6 * (PX / 3 + 1) + NT - 1) / NT;
+  constexpr int MAX3 = (3 * (PX / 3 + 1) + NT - 1) / NT;
+  auto issue = [&](const C* const* srcs, int nl, C* va, C* vb, auto cnt) {
+    constexpr int M = decltype(cnt)::value;
+#pragma unroll
+    for (int q = 0; q < M; ++q) {
+      const int e = tid + q * NT;
+      if (e < nl * mx) {
+        const int f = e / mx, n = e % mx;
+        const C* A = srcs[f] + (size_t)z0 * mx;
+        va[q] = A[n];
+        vb[q] = A[mx + n];
+      }
+    }
+  };
+  auto commit = [&](int nl, const C* va, const C* vb, auto cnt) {
+    constexpr int M = decltype(cnt)::value;
+#pragma unroll
+    for (int q = 0; q < M; ++q) {
+      const int e = tid + q * NT;
+      if (e < nl * mx) {
+        const int f = e / mx, n = e % mx;
+        C* line = lds + f * LS;
+        if (n == 0) {
+          line[0] = mk<C>(va[q].x, vb[q].x);
+        } else {
+          line[n] = mk<C>(va[q].x - vb[q].y, va[q].y + vb[q].x);
+          line[PX - n] = mk<C>(va[q].x + vb[q].y, -va[q].y + vb[q].x);
+        }
+      }
+    }
+    const int nzr = PX - 2 * mx + 1;   // positions mx .. PX-mx are zero
+    for (int e = tid; e < nl * nzr; e += NT) {
+      const int f = e / nzr, n = mx + e % nzr;
+      lds[f * LS + n] = mk<C>(0, 0);
+    }
+  };
+  using CntAll = std::integral_constant<int, MAXI>;
+  using Cnt3 = std::integral_constant<int, MAX3>;
+  // all six lines are transformed; when only nl < 6 carry data the others hold
+  // finite left-overs that are overwritten before they are read again
+  auto fft_lines = [&](int nl, auto inv_tag) {
+    constexpr bool INV = decltype(inv_tag)::value;
+    (void)nl;
+    Plan<PX>::template run<INV>(lds + ln * LS, t, w);
+  };
+  using Inv = std::true_type;
+  using Fwd = std::false_type;
+
+  // Z-space products of the previous level for this thread's kx index (rows z0, z0+1)
+  C puv0 = mk<C>(0, 0), puv1 = mk<C>(0, 0), pvw0 = mk<C>(0, 0), pvw1 = mk<C>(0, 0);
+  C pvv0 = mk<C>(0, 0), pvv1 = mk<C>(0, 0);
+  // physical u(j), w(j), v(j-1) at this thread's x positions (Re: row z0, Im: row z0+1)
+  C Uc[XPT], Wc[XPT], Vb[XPT];
+
+  auto unpack = [&](const C* line, int k, C& ra, C& rb) { unpack_rows<PX>(line, k, ra, rb); };
+
+  // ---------------- prologue ----------------
+  {
+    const int su0 = l0 + 1;
+    {
+      C va[MAXI], vb[MAXI];
+      const int sm = l0 > 0 ? su0 - 1 : su0;
+      const C* srcs[6] = {zu + sm * slot, zu + su0 * slot, zw + sm * slot, zw + su0 * slot,
+                          zv + (su0 - (l0 > 0 ? 2 : 1)) * slot, zv + (su0 - 1) * slot};
+      issue(srcs, 6, va, vb, CntAll{});
+      commit(6, va, vb, CntAll{});
+    }
+    lds_bar();
+    fft_lines(6, Inv{});
+#pragma unroll
+    for (int q = 0; q < XPT; ++q) {
+      const int x = tid + q * NT;
+      if (x < PX) {
+        const C Um = lds[0 * LS + x], U0 = lds[1 * LS + x], Wm = lds[2 * LS + x], W0 = lds[3 * LS + x];
+        const C Vm = lds[4 * LS + x], V0 = lds[5 * LS + x];
+        Uc[q] = U0; Wc[q] = W0; Vb[q] = V0;
+        if (l0 > 0) {
+          const C vc = mk<C>(half * (Vm.x + V0.x), half * (Vm.y + V0.y));
+          const C uf = mk<C>(half * (Um.x + U0.x), half * (Um.y + U0.y));
+          const C wf = mk<C>(half * (Wm.x + W0.x), half * (Wm.y + W0.y));
+          lds[0 * LS + x] = mk<C>(uf.x * V0.x, uf.y * V0.y);    // uv(j0-1)
+          lds[1 * LS + x] = mk<C>(V0.x * wf.x, V0.y * wf.y);    // vw(j0-1)
+          lds[2 * LS + x] = mk<C>(vc.x * vc.x, vc.y * vc.y);    // vv(j0-1)
+        }
+      }
+    }
+    if (l0 > 0) {
+      lds_bar();
+      fft_lines(3, Fwd{});
+      if (tid < mx) {
+        unpack(lds + 0 * LS, tid, puv0, puv1);
+        unpack(lds + 1 * LS, tid, pvw0, pvw1);
+        unpack(lds + 2 * LS, tid, pvv0, pvv1);
+      }
+    }
+    lds_bar();
+  }
+
+  double cmax = 0.0;
+  // ---------------- levels ----------------
+  for (int l = l0; l < l1; ++l) {
+    const int su = l + 1;
+    {
+      C pa[MAX3], pb[MAX3];
+      const C* srcs[3] = {zu + (su + 1) * slot, zw + (su + 1) * slot, zv + su * slot};
+      issue(srcs, 3, pa, pb, Cnt3{});
+      commit(3, pa, pb, Cnt3{});
+    }
+    lds_bar();
+    fft_lines(3, Inv{});
+    const double dyi = a.dthdyu[su];
+#pragma unroll
+    for (int q = 0; q < XPT; ++q) {
+      const int x = tid + q * NT;
+      if (x < PX) {
+        const C Ua = lds[0 * LS + x], Wa = lds[1 * LS + x], Vt = lds[2 * LS + x];
+        const C U = Uc[q], W = Wc[q], Vd = Vb[q];
+        const C vc = mk<C>(half * (Vd.x + Vt.x), half * (Vd.y + Vt.y));
+        const C uf = mk<C>(half * (U.x + Ua.x), half * (U.y + Ua.y));
+        const C wf = mk<C>(half * (W.x + Wa.x), half * (W.y + Wa.y));
+        lds[0 * LS + x] = mk<C>(U.x * U.x, U.y * U.y);          // uu(j)
+        lds[1 * LS + x] = mk<C>(U.x * W.x, U.y * W.y);          // uw(j)
+        lds[2 * LS + x] = mk<C>(W.x * W.x, W.y * W.y);          // ww(j)
+        lds[3 * LS + x] = mk<C>(vc.x * vc.x, vc.y * vc.y);      // vv(j)
+        lds[4 * LS + x] = mk<C>(uf.x * Vt.x, uf.y * Vt.y);      // uv(j)
+        lds[5 * LS + x] = mk<C>(Vt.x * wf.x, Vt.y * wf.y);      // vw(j)
+        if (a.want_cfl) {
+          const double s0 = fabs((double)U.x) * dm.dxi + fabs((double)vc.x) * dyi + fabs((double)W.x) * dm.dzi;
+          const double s1 = fabs((double)U.y) * dm.dxi + fabs((double)vc.y) * dyi + fabs((double)W.y) * dm.dzi;
+          cmax = fmax(cmax, fmax(s0, s1));
+        }
+        Uc[q] = Ua; Wc[q] = Wa; Vb[q] = Vt;
+      }
+    }
+    lds_bar();
+    fft_lines(6, Fwd{});
+    if (tid < mx) {
+      const int k = tid;
+      C uu0, uu1, uw0, uw1, ww0, ww1, vv0, vv1, uv0, uv1, vw0, vw1;
+      unpack(lds + 0 * LS, k, uu0, uu1);
+      unpack(lds + 1 * LS, k, uw0, uw1);
+      unpack(lds + 2 * LS, k, ww0, ww1);
+      unpack(lds + 3 * LS, k, vv0, vv1);
+      unpack(lds + 4 * LS, k, uv0, uv1);
+      unpack(lds + 5 * LS, k, vw0, vw1);
+      const TR kx = (TR)a.k1x[k];
+      const TR mu = (TR)(dm.ddthetavi * a.dthdyu[su]);
+      const TR mv = (TR)(dm.ddthetavi * a.dthdyv[su - 1]);
+      auto ik = [&](C v) { return mk<C>(-kx * v.y, kx * v.x); };
+      auto axpy = [&](C p, TR m, C d) { return mk<C>(p.x + m * d.x, p.y + m * d.y); };
+      const size_t base = (size_t)l * slot + (size_t)z0 * mx + k;
+      const size_t ps = (size_t)nlev * slot;   // stride between output kinds
+      // u centre j: A1, B1
+      zo[0 * ps + base] = axpy(ik(uu0), mu, csub(uv0, puv0));
+      zo[0 * ps + base + mx] = axpy(ik(uu1), mu, csub(uv1, puv1));
+      zo[1 * ps + base] = uw0;
+      zo[1 * ps + base + mx] = uw1;
+      // w centre j: A3, B3
+      zo[2 * ps + base] = axpy(ik(uw0), mu, csub(vw0, pvw0));
+      zo[2 * ps + base + mx] = axpy(ik(uw1), mu, csub(vw1, pvw1));
+      zo[3 * ps + base] = ww0;
+      zo[3 * ps + base + mx] = ww1;
+      // v face j-1: A2, B2
+      zo[4 * ps + base] = axpy(ik(puv0), mv, csub(vv0, pvv0));
+      zo[4 * ps + base + mx] = axpy(ik(puv1), mv, csub(vv1, pvv1));
+      zo[5 * ps + base] = pvw0;
+      zo[5 * ps + base + mx] = pvw1;
+      puv0 = uv0; puv1 = uv1; pvw0 = vw0; pvw1 = vw1; pvv0 = vv0; pvv1 = vv1;
+    }
+    lds_bar();
+  }
+  if (a.want_cfl) {
+    for (int off = 32; off > 0; off >>= 1) cmax = fmax(cmax, __shfl_down(cmax, off, 64));
+    if ((tid & 63) == 0) red[tid / 64] = cmax;
+    __syncthreads();
+    if (tid == 0) {
+      double m = 0.0;
+      for (int q = 0; q < (NT + 63) / 64; ++q) m = fmax(m, red[q]);
+      atomic_max_pos(a.scal + S_CFLMAX, m);
+    }
+  }
+}
+
+// One-level x pass (one row pair, one level per block): the six raw products
+// Nonlinear x pass: one row pair and one level j (centre j and face j) per block.
+//   lines 0..5 <- U(j), U(j+1), W(j), W(j+1), V(j-1), V(j)
+//   products   -> uu_c(j), uw_c(j), ww_c(j), vv_c(j), uv_f(j), vw_f(j)
+struct XRAW {
+  const void* zu;
+  const void* zw;
+  const void* zv;
+  void* zp;                  // [6][level][pz][mx]
+  const double* dthdyu;      // per u plane
+  double* scal;
+  int want_cfl;
+};
+
+template <int PX, typename TR>
+__global__ void __launch_bounds__(6 * Plan<PX>::T)
+    k_xraw(XRAW a, Dims dm, const cx<TR>* __restrict__ tw) {
+  using C = cx<TR>;
+  constexpr int T = Plan<PX>::T;
+  constexpr int LS = PX + 1;
+  extern __shared__ char smem_raw[];
+  C* lds = reinterpret_cast<C*>(smem_raw);
+  __shared__ double red[16];
+  const int z0 = 2 * blockIdx.x;
+  const int level = blockIdx.y;            // centre / face j = level - d + 1
+  const int su = level + 1;                // u slot of centre j
+  const int mx = dm.mx, pz = dm.pz;
+  const size_t slot = (size_t)pz * mx;
+  const C* zu = reinterpret_cast<const C*>(a.zu);
+  const C* zw = reinterpret_cast<const C*>(a.zw);
+  const C* zv = reinterpret_cast<const C*>(a.zv);
+  const C* src[6] = {zu + su * slot, zu + (su + 1) * slot, zw + su * slot, zw + (su + 1) * slot,
+                     zv + (su - 1) * slot, zv + su * slot};
+  {
+    // loads of the packed half spectra first, then the LDS writes
+    constexpr int NT = 6 * T;
+    constexpr int M = (6 * (PX / 3 + 1) + NT - 1) / NT;
+    C va[M], vb[M];
+#pragma unroll
+    for (int q = 0; q < M; ++q) {
+      const int e = threadIdx.x + q * NT;
+      if (e < 6 * mx) {
+        const C* A = src[e / mx] + (size_t)z0 * mx;
+        va[q] = A[e % mx];
+        vb[q] = A[mx + e % mx];
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < M; ++q) {
+      const int e = threadIdx.x + q * NT;
+      if (e < 6 * mx) {
+        const int f = e / mx, n = e % mx;
+        C* line = lds + f * LS;
+        if (n == 0) {
+          line[0] = mk<C>(va[q].x, vb[q].x);
+        } else {
+          line[n] = mk<C>(va[q].x - vb[q].y, va[q].y + vb[q].x);
+          line[PX - n] = mk<C>(va[q].x + vb[q].y, -va[q].y + vb[q].x);
+        }
+      }
+    }
+    const int nzr = PX - 2 * mx + 1;
+    for (int e = threadIdx.x; e < 6 * nzr; e += NT) lds[(e / nzr) * LS + mx + e % nzr] = mk<C>(0, 0);
+  }
+  lds_bar();
+  const int ln = threadIdx.x / T, t = threadIdx.x % T;
+  typename Plan<PX>::template Tw<C> w;
+  w.load(t, tw);
+  Plan<PX>::template run<true>(lds + ln * LS, t, w);
+  // products (each thread owns its x positions in all six lines)
+  const TR half = (TR)0.5;
+  const double dyi = a.dthdyu[su];
+  double cmax = 0.0;
+  for (int x = threadIdx.x; x < PX; x += blockDim.x) {
+    const C U = lds[0 * LS + x], Ua = lds[1 * LS + x], W = lds[2 * LS + x], Wa = lds[3 * LS + x];
+    const C Vb = lds[4 * LS + x], Vt = lds[5 * LS + x];
+    C pr[6];
+    {
+      const TR vc0 = half * (Vb.x + Vt.x), vc1 = half * (Vb.y + Vt.y);
+      const TR uf0 = half * (U.x + Ua.x), uf1 = half * (U.y + Ua.y);
+      const TR wf0 = half * (W.x + Wa.x), wf1 = half * (W.y + Wa.y);
+      pr[0] = mk<C>(U.x * U.x, U.y * U.y);
+      pr[1] = mk<C>(U.x * W.x, U.y * W.y);
+      pr[2] = mk<C>(W.x * W.x, W.y * W.y);
+      pr[3] = mk<C>(vc0 * vc0, vc1 * vc1);
+      pr[4] = mk<C>(uf0 * Vt.x, uf1 * Vt.y);
+      pr[5] = mk<C>(Vt.x * wf0, Vt.y * wf1);
+      if (a.want_cfl) {
+        const double s0 = fabs((double)U.x) * dm.dxi + fabs((double)vc0) * dyi + fabs((double)W.x) * dm.dzi;
+        const double s1 = fabs((double)U.y) * dm.dxi + fabs((double)vc1) * dyi + fabs((double)W.y) * dm.dzi;
+        cmax = fmax(cmax, fmax(s0, s1));
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < 6; ++q) lds[q * LS + x] = pr[q];
+  }
+  lds_bar();
+  Plan<PX>::template run<false>(lds + ln * LS, t, w);
+  C* zp = reinterpret_cast<C*>(a.zp);
+  const size_t nlev = dm.nyu - 2;
+  for (int e = threadIdx.x; e < 6 * mx; e += blockDim.x) {
+    const int q = e / mx, k = e % mx;
+    C va, vb;
+    unpack_rows<PX>(lds + q * LS, k, va, vb);
+    C* dst = zp + ((size_t)q * nlev + level) * slot + (size_t)z0 * mx;
+    dst[k] = va;
+    dst[mx + k] = vb;
+  }
+  if (a.want_cfl) {
+    // block max (wave64 shuffles, then LDS across waves)
+    for (int off = 32; off > 0; off >>= 1) cmax = fmax(cmax, __shfl_down(cmax, off, 64));
+    const int wv = threadIdx.x / 64, lane = threadIdx.x % 64;
+    if (lane == 0) red[wv] = cmax;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double m = 0.0;
+      for (int w = 0; w < (int)((blockDim.x + 63) / 64); ++w) m = fmax(m, red[w]);
+      atomic_max_pos(a.scal + S_CFLMAX, m);
+    }
+  }
+}
+
+// Forward z transform of the combined rows + RHS0 + nonlinear + rhs of solveU/solveV:
+// one block per (L consecutive kx columns, level, field).  Block ids are remapped so that
+// the two blocks sharing a 128-byte line of every [plane][column] row run on one XCD.
+struct ZASM {
+  const void* zo;
+  const double2 *u1, *u2, *u3, *p;
+  double2 *nu1, *nu2, *nu3;
+  double2 *r1, *r2, *r3, *tipl;
+  const double *dyu2i, *dyv2i, *dthdyu, *dthdyv, *k1x, *k2x, *k1z, *k2z, *scal;
+  int tip_u0, tip_u1, tip_v0, tip_v1;
+  int nib;        // column blocks
+  double C1, C2, dR, g, bre;
+};
+
+template <int PZ, int L, typename TR>
+__global__ void __launch_bounds__(2 * L * Plan<PZ>::T)
+    k_zasm(ZASM a, Dims dm, const cx<TR>* __restrict__ tw) {
+  using C = cx<TR>;
+  constexpr int T = Plan<PZ>::T;
+  constexpr int NT = 2 * L * T;
+  constexpr int LS = PZ + 1;
+  extern __shared__ char smem_raw[];
+  C* lds = reinterpret_cast<C*>(smem_raw);
+  const int tid = threadIdx.x;
+  const int nlev = dm.nyu - 2;
+  // XCD-aware remap of the linear block id (8 XCDs, round-robin dispatch)
+  const int nblk = gridDim.x;
+  int bid = blockIdx.x;
+  const int per = nblk / 8;
+  if (bid < per * 8) bid = (bid % 8) * per + bid / 8;
+  const int ib = bid % a.nib;
+  const int rest = bid / a.nib;
+  const int l = rest % nlev;
+  const int f = rest / nlev;                  // 0: u1, 1: u3, 2: v
+  const int i0 = ib * L;
+  const int mx = dm.mx, nz = dm.nz, h = nz / 2, ncol = dm.ncol;
+  const size_t slot = (size_t)PZ * mx;
+  // raw products [6][nlev][pz][mx]: 0 uu, 1 uw, 2 ww, 3 vv, 4 uv, 5 vw (centre/face of level)
+  // u centre j (level l):  A = ikx uu(l) + m_u (uv(l) - uv(l-1)),   B = uw(l)
+  // w centre j (level l):  A = ikx uw(l) + m_u (vw(l) - vw(l-1)),   B = ww(l)
+  // v face j   (level l):  A = ikx uv(l) + m_v (vv(l+1) - vv(l)),   B = vw(l)
+  const C* zo = reinterpret_cast<const C*>(a.zo);
+  const size_t ps = (size_t)nlev * slot;
+  const C* zl = zo + (size_t)l * slot;
+  const int l_lo = f < 2 ? l - 1 : l;         // second operand of the y difference
+  const int l_hi = f < 2 ? l : l + 1;
+  const bool has_lo = l_lo >= 0 && l_hi < nlev;
+  const int q0 = f == 0 ? 0 : (f == 1 ? 1 : 4);       // ikx term
+  const int qd = f == 0 ? 4 : (f == 1 ? 5 : 3);       // y-difference term
+  const int qb = f == 0 ? 1 : (f == 1 ? 2 : 5);       // B
+  const TR mlev = (TR)(dm.ddthetavi * (f < 2 ? a.dthdyu[l + 1] : a.dthdyv[l + 1]));
+  {
+    constexpr int ITZ = (L * PZ + NT - 1) / NT;
+    C r0[ITZ], rh[ITZ], rl[ITZ], rb[ITZ];
+#pragma unroll
+    for (int q = 0; q < ITZ; ++q) {
+      const int e = tid + q * NT;
+      const int l2 = e % L, z = e / L, i = i0 + l2;
+      if (e < L * PZ && i < mx) {
+        const size_t o = (size_t)z * mx + i;
+        r0[q] = zl[q0 * ps + o];
+        rb[q] = zl[qb * ps + o];
+        rh[q] = zo[qd * ps + (size_t)l_hi * slot + o];
+        rl[q] = has_lo ? zo[qd * ps + (size_t)l_lo * slot + o] : mk<C>(0, 0);
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < ITZ; ++q) {
+      const int e = tid + q * NT;
+      const int l2 = e % L, z = e / L, i = i0 + l2;
+      if (e < L * PZ) {
+        C A = mk<C>(0, 0), B = mk<C>(0, 0);
+        if (i < mx) {
+          const TR kx = (TR)a.k1x[i];
+          A = mk<C>(-kx * r0[q].y + mlev * (rh[q].x - rl[q].x), kx * r0[q].x + mlev * (rh[q].y - rl[q].y));
+          B = rb[q];
+        }
+        lds[(2 * l2) * LS + z] = A;
+        lds[(2 * l2 + 1) * LS + z] = B;
+      }
+    }
+  }
+  __syncthreads();
+  {
+    const int ln = tid / T, t = tid % T;
+    typename Plan<PZ>::template Tw<C> w;
+    w.load(t, tw);
+    Plan<PZ>::template run<false>(lds + ln * LS, t, w);
+  }
+  // assembly, one (column, plane) output per item; loads of all items are issued first
+  const double dt = a.scal[S_DT];
+  const double beta = a.bre * dt;
+  const double sz = 1.0 / (double)PZ;
+  const double2 zero = d2(0.0, 0.0);
+  const int s = l + 1;                        // output row: u slot of centre j / v slot of face j
+  double2* __restrict__ rr = f == 0 ? a.r1 : (f == 1 ? a.r3 : a.r2);
+  const double2* __restrict__ uu = f == 0 ? a.u1 : (f == 1 ? a.u3 : a.u2);
+  double2* __restrict__ nu = f == 0 ? a.nu1 : (f == 1 ? a.nu3 : a.nu2);
+  const double2* __restrict__ pp = a.p;
+  constexpr int NZ = 2 * PZ / 3;
+  constexpr int ITEMS = (L * NZ + NT - 1) / NT;
+  const double mpgx = a.scal[S_MPGX];
+  const double* wy = (f < 2 ? a.dyu2i : a.dyv2i) + 3 * s;
+  const double w0 = wy[0], w1 = wy[1], w2 = wy[2];
+  const double C3v = f == 2 ? -a.dR * dm.ddthetavi * a.dthdyv[s] : 0.0;
+  const bool tip = f < 2 ? (s == a.tip_u0 || s == a.tip_u1) : (s == a.tip_v0 || s == a.tip_v1);
+  const int tipr = f < 2 ? 2 * f + (s == a.tip_u0 ? 0 : 1) : 4 + (s == a.tip_v0 ? 0 : 1);
+  const bool active = !(f == 2 && l == nlev - 1);   // face ny+d-1 is the wall
+  double2 vm[ITEMS], v0[ITEMS], vp[ITEMS], pa[ITEMS], pb[ITEMS], no[ITEMS];
+#pragma unroll
+  for (int q = 0; q < ITEMS; ++q) {
+    const int e = tid + q * NT;
+    const int l2 = e % L, k = e / L, i = i0 + l2;
+    if (e < L * NZ && i < mx && active) {
+      const int c = k * mx + i;
+      const size_t o = (size_t)s * ncol + c;
+      vm[q] = uu[o - ncol];
+      v0[q] = uu[o];
+      vp[q] = uu[o + ncol];
+      pa[q] = pp[(size_t)(s - 1) * ncol + c];
+      pb[q] = f == 2 ? pp[o] : zero;
+      no[q] = nu[o];
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < ITEMS; ++q) {
+    const int e = tid + q * NT;
+    const int l2 = e % L, k = e / L, i = i0 + l2;
+    if (e >= L * NZ || i >= mx) continue;
+    const int c = k * mx + i;
+    // BC rows handled by the first/last level blocks
+    if (f < 2) {
+      if (l == 0) rr[c] = zero;
+      if (l == nlev - 1) rr[(size_t)(nlev + 1) * ncol + c] = zero;
+    } else {
+      if (l == 0) { rr[c] = zero; rr[(size_t)nlev * ncol + c] = zero; }
+    }
+    if (!active) continue;
+    const int kk = k < h ? k : k + (PZ - nz);
+    const C A = lds[(2 * l2) * LS + kk], B = lds[(2 * l2 + 1) * LS + kk];
+    const double kx = a.k1x[i], kz = a.k1z[k], k2 = a.k2x[i] + a.k2z[k];
+    const double axz = 1.0 - beta * (a.k2x[i] + a.k2z[k]);
+    // Nu = (F(A) + ikz F(B)) / pz
+    const double2 nn = d2(sz * ((double)A.x - kz * (double)B.y), sz * ((double)A.y + kz * (double)B.x));
+    const size_t o = (size_t)s * ncol + c;
+    const double2 Lu = dadd(dadd(drm(w0, vm[q]), drm(k2 + w1, v0[q])), drm(w2, vp[q]));
+    double2 du;
+    if (f < 2) {
+      du = dadd(dadd(drm(a.C1, Lu), drm(a.C2, no[q])), dim(-a.dR * (f == 0 ? kx : kz), pa[q]));
+      if (f == 0 && c == 0) du.x += -a.dR * mpgx;
+    } else {
+      du = dadd(dadd(drm(a.C1, Lu), drm(a.C2, no[q])), drm(C3v, dsub(pb[q], pa[q])));
+    }
+    nu[o] = nn;
+    du = dsub(du, drm(a.g, nn));
+    double2 r = dadd(v0[q], drm(dt, du));
+    r.x /= axz;
+    r.y /= axz;
+    rr[o] = r;
+    if (tip) {
+      const double2 Luy = dadd(dadd(drm(w0, vm[q]), drm(w1, v0[q])), drm(w2, vp[q]));
+      a.tipl[(size_t)tipr * ncol + c] = Luy;
+    }
+  }
+}
+
+}  // namespace lh
