@@ -88,10 +88,12 @@ __device__ void ctu_column(const CTU& a) {
   u11[0] = u11[0] * m[1];
   for (int j = 1; j <= n; ++j) u11[j] = (u11[j] - m[3 * j] * u11[j - 1]) * m[3 * j + 1];
   for (int j = n - 1; j >= 0; --j) u11[j] = u11[j] - m[3 * j + 2] * u11[j + 1];
+  // flowrateRe(Qx1, u11) declares its dummy u(1:nn+2) while u11 is (0:nn+2)
+  // (littleharsh.f90:443 vs 472-474): inside it u(j) is u11(j-1)
   double qx1 = 0.0, qx0 = 0.0;
   for (int j = 1; j <= n - 1; ++j) {
     const double dy = a.yv[j + a.d] - a.yv[j - 1 + a.d];
-    qx1 += dy * u11[j];
+    qx1 += dy * u11[j - 1];
     qx0 += dy * a.u1[(size_t)(j + a.d) * a.ncol].x;
   }
   const double g = (a.scal[S_QXT] - qx0) / qx1;

@@ -278,45 +278,6 @@ struct CTU {
   int ny, d, ncol;
 };
 
-__global__ void k_ctU(CTU a) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  const int n = a.ny;
-  const double dt = a.scal[S_DT];
-  double* u11 = a.work;
-  double* m = a.work + (n + 1);
-  const double bdt = -a.dR * dt;
-  u11[0] = 0.0;
-  for (int j = 1; j <= n - 1; ++j) u11[j] = bdt;
-  u11[n] = 0.0;
-  const double beta = a.bre * dt;
-  m[0] = 0.0; m[1] = 1.0; m[2] = a.gw0;
-  for (int j = 1; j <= n - 1; ++j) {
-    const double* w = a.dyu2i + 3 * (j + a.d);
-    m[3 * j] = -beta * w[0];
-    m[3 * j + 1] = 1.0 - beta * w[1];
-    m[3 * j + 2] = -beta * w[2];
-  }
-  m[3 * n] = a.gw1; m[3 * n + 1] = 1.0; m[3 * n + 2] = 0.0;
-  m[1] = 1.0 / m[1];
-  for (int j = 1; j <= n; ++j) {
-    m[3 * (j - 1) + 2] = m[3 * (j - 1) + 2] * m[3 * (j - 1) + 1];
-    m[3 * j + 1] = 1.0 / (m[3 * j + 1] - m[3 * j] * m[3 * (j - 1) + 2]);
-  }
-  u11[0] = u11[0] * m[1];
-  for (int j = 1; j <= n; ++j) u11[j] = (u11[j] - m[3 * j] * u11[j - 1]) * m[3 * j + 1];
-  for (int j = n - 1; j >= 0; --j) u11[j] = u11[j] - m[3 * j + 2] * u11[j + 1];
-  double qx1 = 0.0, qx0 = 0.0;
-  for (int j = 1; j <= n - 1; ++j) {
-    const double dy = a.yv[j + a.d] - a.yv[j - 1 + a.d];
-    qx1 += dy * u11[j];
-    qx0 += dy * a.u1[(size_t)(j + a.d) * a.ncol].x;
-  }
-  const double g = (a.scal[S_QXT] - qx0) / qx1;
-  a.scal[S_MPGX] = a.scal[S_MPGX] + g;
-  a.scal[S_DGX] = g;
-  for (int j = 0; j <= n; ++j) a.u1[(size_t)(j + a.d) * a.ncol].x += g * u11[j];
-}
-
 // LU_buildP + LU_decP once at start (tridLU_3D_IB.f90:497-571, 643-684)
 struct BUILDP {
   double *dg1, *dg2, *dg3;      // [nyp][ncol]

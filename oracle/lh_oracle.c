@@ -656,8 +656,10 @@ static void meanflow_ctU(lho* o, int krk, double dt) {
   u11[0] = u11[0] * a[1];
   for (int j = 1; j <= n; ++j) u11[j] = (u11[j] - a[3 * j] * u11[j - 1]) * a[3 * j + 1];
   for (int j = n - 1; j >= 0; --j) u11[j] = u11[j] - a[3 * j + 2] * u11[j + 1];
+  /* flowrateRe(Qx1, u11) declares its dummy u(1:nn+2) while u11 is (0:nn+2)
+     (littleharsh.f90:443 vs 472-474): inside it u(j) is u11(j-1) */
   double qx1 = 0.0;
-  for (int j = 1; j <= n - 1; ++j) qx1 += (o->yv[UJ(o, j)] - o->yv[UJ(o, j - 1)]) * u11[j];
+  for (int j = 1; j <= n - 1; ++j) qx1 += (o->yv[UJ(o, j)] - o->yv[UJ(o, j - 1)]) * u11[j - 1];
   const double qx0 = flowrate_col(o, o->u1);
   const double g = (o->qxt - qx0) / qx1;
   o->mpgx = o->mpgx + g;
@@ -830,4 +832,86 @@ void lho_cfft(int n, const double* in, double* out, int sign) {
   plan_init(&pl, n);
   fft_exec(&pl, (const cplx*)in, 1, (cplx*)out, sign);
   plan_free(&pl);
+}
+
+/* ------------------------------------------------------------------------------------ */
+/* Routine-level hooks: used by the checks against vectors produced by the reference's   */
+/* own routines (tests/golden).                                                          */
+/* ------------------------------------------------------------------------------------ */
+/* RHS0_u1/u2/u3 alone: du and the y-Laplacians Lu for the current state and Nu */
+void lho_hook_rhs0(lho* o, int krk1) { rhs0(o, krk1 - 1); }
+
+/* which: 0..2 du1..3, 3..5 Lu1..3, 6 psi, 7 div (reference layout, full plane ranges) */
+void lho_get_work(const lho* o, int which, double* out) {
+  const cplx* src[8] = {o->du1, o->du2, o->du3, o->Lu1, o->Lu2, o->Lu3, o->psi, o->div};
+  const int nj[8] = {o->nyu, o->nyv, o->nyu, o->nyu, o->nyv, o->nyu, o->nyp, o->nyp};
+  memcpy(out, src[which], sizeof(cplx) * (size_t)o->ncol * nj[which]);
+}
+void lho_set_nonlinear_full(lho* o, const double* n1, const double* n2, const double* n3) {
+  memcpy(o->Nu1, n1, sizeof(cplx) * (size_t)o->ncol * o->nyu);
+  memcpy(o->Nu2, n2, sizeof(cplx) * (size_t)o->ncol * o->nyv);
+  memcpy(o->Nu3, n3, sizeof(cplx) * (size_t)o->ncol * o->nyu);
+}
+/* solveP followed by v_corr on the current state */
+void lho_hook_solvep_vcorr(lho* o, int krk1, double dt) {
+  solveP(o, krk1 - 1, dt);
+  v_corr(o, krk1 - 1, dt);
+}
+/* LU_build + LU_dec of a velocity grid: ay (3 x rows) and axz per column */
+void lho_hook_lu_vel(lho* o, int grid_u, int krk1, double dt, double* ay, double* axz) {
+  const double beta = o->bRK[krk1 - 1] * dt / o->re;
+  lu_build_dec(o, grid_u, beta, ay);
+  for (int c = 0; c < o->ncol; ++c)
+    axz[c] = 1.0 - beta * (o->k2x[c % o->mx] + o->k2z[c / o->mx]);
+}
+void lho_get_dg(const lho* o, double* out) {
+  memcpy(out, o->DG, sizeof(double) * 3 * (size_t)o->nyp * o->ncol);
+}
+/* flowrate_corr on column 0 of u1 with the given mpgx and QxT; returns g */
+double lho_hook_flowrate_corr(lho* o, int krk1, double dt, double qxt, double* mpgx) {
+  o->qxt = qxt;
+  o->mpgx = *mpgx;
+  meanflow_ctU(o, krk1 - 1, dt);
+  *mpgx = o->mpgx;
+  return o->dgx;
+}
+
+/* ------------------------------------------------------------------------------------ */
+/* spectra (spectra.f90:1-56): sp(j, column) += |f|^2 for u, v, w, p, and                */
+/* Re(u) Re(v_c) + Im(u) Im(v_c) with v averaged to the u centres (u2_itp2)              */
+/* Layout: 5 arrays of ncol x nyu (p in its first nyp rows, v in its first nyv rows).   */
+/* ------------------------------------------------------------------------------------ */
+void lho_spectra_accumulate(lho* o) {
+  const int nu = o->nyu;
+  double* spU = o->sp;
+  double* spV = o->sp + (size_t)o->ncol * nu;
+  double* spW = o->sp + 2 * (size_t)o->ncol * nu;
+  double* spUV = o->sp + 3 * (size_t)o->ncol * nu;
+  double* spP = o->sp + 4 * (size_t)o->ncol * nu;
+  for (int c = 0; c < o->ncol; ++c) {
+    for (int r = 0; r < o->nyu; ++r) {
+      cplx a = o->u1[(size_t)c * nu + r], w = o->u3[(size_t)c * nu + r];
+      spU[(size_t)c * nu + r] += a.re * a.re + a.im * a.im;
+      spW[(size_t)c * nu + r] += w.re * w.re + w.im * w.im;
+      /* v at the u centre r: average of the faces r-1 and r (interior centres) */
+      cplx vc = {0.0, 0.0};
+      if (r >= 1 && r <= o->nyu - 2) {
+        cplx b0 = o->u2[(size_t)c * o->nyv + r - 1], b1 = o->u2[(size_t)c * o->nyv + r];
+        vc.re = 0.5 * (b0.re + b1.re);
+        vc.im = 0.5 * (b0.im + b1.im);
+      }
+      spUV[(size_t)c * nu + r] += a.re * vc.re + a.im * vc.im;
+    }
+    for (int r = 0; r < o->nyv; ++r) {
+      cplx v = o->u2[(size_t)c * o->nyv + r];
+      spV[(size_t)c * nu + r] += v.re * v.re + v.im * v.im;
+    }
+    for (int r = 0; r < o->nyp; ++r) {
+      cplx p = o->p[(size_t)c * o->nyp + r];
+      spP[(size_t)c * nu + r] += p.re * p.re + p.im * p.im;
+    }
+  }
+}
+void lho_spectra_read(const lho* o, double* out) {
+  memcpy(out, o->sp, sizeof(double) * 5 * (size_t)o->ncol * o->nyu);
 }

@@ -58,6 +58,14 @@ def lib():
             L.lho_cond_stats_read.restype = ctypes.c_longlong
         if hasattr(L, "lho_spectra_read"):
             L.lho_spectra_read.argtypes = [vp, _dp]
+        L.lho_hook_rhs0.argtypes = [vp, ctypes.c_int]
+        L.lho_get_work.argtypes = [vp, ctypes.c_int, _dp]
+        L.lho_set_nonlinear_full.argtypes = [vp, _dp, _dp, _dp]
+        L.lho_hook_solvep_vcorr.argtypes = [vp, ctypes.c_int, ctypes.c_double]
+        L.lho_hook_lu_vel.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_double, _dp, _dp]
+        L.lho_get_dg.argtypes = [vp, _dp]
+        L.lho_hook_flowrate_corr.argtypes = [vp, ctypes.c_int, ctypes.c_double, ctypes.c_double, _dp]
+        L.lho_hook_flowrate_corr.restype = ctypes.c_double
         _lib = L
     return _lib
 
@@ -120,6 +128,54 @@ class Oracle:
         r = lib().lho_step(self._h, dt)
         if r:
             raise RuntimeError(f"lho_step failed ({r})")
+
+    # ---- routine-level hooks (checks against the reference's own routines)
+    def rhs0(self, krk, nu_full):
+        arrs = [np.ascontiguousarray(a, dtype=np.complex128) for a in nu_full]
+        lib().lho_set_nonlinear_full(self._h, *[_p(a) for a in arrs])
+        lib().lho_hook_rhs0(self._h, krk)
+        s = self.s
+        out = []
+        for which, n in zip(range(6), (s.nyu, s.nyv, s.nyu, s.nyu, s.nyv, s.nyu)):
+            a = np.zeros((s.ncol, n), complex)
+            lib().lho_get_work(self._h, which, _p(a))
+            out.append(a)
+        return out
+
+    def work(self, which):
+        s = self.s
+        n = {6: s.nyp, 7: s.nyp}[which]
+        a = np.zeros((s.ncol, n), complex)
+        lib().lho_get_work(self._h, which, _p(a))
+        return a
+
+    def solvep_vcorr(self, krk, dt):
+        lib().lho_hook_solvep_vcorr(self._h, krk, dt)
+
+    def lu_vel(self, grid_u, krk, dt):
+        nj = self.s.nyu if grid_u else self.s.nyv
+        ay, axz = np.zeros((nj, 3)), np.zeros(self.s.ncol)
+        lib().lho_hook_lu_vel(self._h, 1 if grid_u else 0, krk, dt, _p(ay), _p(axz))
+        return ay, axz
+
+    def dg(self):
+        a = np.zeros((self.s.ncol, self.s.nyp, 3))
+        lib().lho_get_dg(self._h, _p(a))
+        return a
+
+    def flowrate_corr(self, krk, dt, qxt, mpgx):
+        m = ctypes.c_double(mpgx)
+        g = lib().lho_hook_flowrate_corr(self._h, krk, dt, qxt, ctypes.byref(m))
+        return g, m.value
+
+    def spectra(self):
+        s = self.s
+        a = np.zeros((5, s.ncol, s.nyu))
+        lib().lho_spectra_read(self._h, _p(a))
+        return a
+
+    def spectra_accumulate(self):
+        lib().lho_spectra_accumulate(self._h)
 
     def max_div(self):
         return lib().lho_max_div(self._h)

@@ -1,0 +1,300 @@
+! refdrv.f90 -- test driver (project code) that calls the reference's own routines, compiled
+! from their sources in the reference tree, to produce golden vectors for the oracle.
+! It is checker infrastructure only: it sets the state of the reference module
+! `declaration` for one process and one resolution band, then calls the routines with plain
+! arrays handed over from C.  Nothing here is part of the solver.
+!
+! Every entry point is bind(C).  Complex arrays are interleaved (re, im) doubles; the
+! column layout is the reference's f(j, column).
+module refdrv
+  use iso_c_binding
+  use declaration
+  implicit none
+contains
+
+  ! Set up sizes, grid (y_grid_canopy), wavenumbers (def_k), RK tables, column maps and
+  ! band limits for one rank with every column in the middle band.
+  subroutine ref_setup(nx, nz, ny_in, dsty_in, posth_in, dyq_in, re_in, alp_in, bet_in, &
+                       mpgx_in, dirlen, dirc) bind(C, name="ref_setup")
+    integer(c_int), value :: nx, nz, ny_in, dsty_in, dirlen
+    real(c_double), value :: posth_in, dyq_in, re_in, alp_in, bet_in, mpgx_in
+    character(kind=c_char), intent(in) :: dirc(dirlen)
+    integer :: i, k, c, iband, ngx, ngz
+
+    np = 1
+    Re = re_in
+    alp = alp_in
+    bet = bet_in
+    mpgx = mpgx_in
+    Lx = 2d0*pi/alp
+    Lz = 2d0*pi/bet
+    Ly = 2d0
+    dsty = dsty_in
+    posth = posth_in
+    dyq = dyq_in
+    ppp = 5
+    dirout = ''
+    do i = 1, dirlen
+      dirout(i:i) = dirc(i)
+    end do
+    ngx = 3*nx/2
+    ngz = 3*nz/2
+    ! start.f90:93-118 with both bands at one resolution
+    allocate(Ngal(4,0:nband+1), N(4,0:nband+1), Ny(3,0:nband+1))
+    Ngal = 0
+    Ngal(1,1:nband) = ngx
+    Ngal(1,nband+1) = -2
+    Ngal(2,1:nband) = ngz
+    Ngal(3,nband) = ny_in - 2
+    N = Ngal
+    do iband = 1, nband
+      N(1,iband) = 2*(Ngal(1,iband)/3)
+      N(2,iband) = 2*(Ngal(2,iband)/3)
+    end do
+    allocate(h_ny(botband))
+    h_ny(1) = 1.5d0*posth
+    call y_grid_canopy
+    call def_k
+    ! RK coefficients as start.f90:413-427
+    aRK = (/ 4d0/15d0, 1d0/15d0, 1d0/6d0 /)
+    bRK = aRK
+    gRK = (/ 8d0/15d0, 5d0/12d0, 3d0/4d0 /)
+    cRK = (/ 0d0, -17d0/60d0, -5d0/12d0 /)
+    dRK = (/ 8d0/15d0, 2d0/15d0, 1d0/3d0 /)
+    ! columns: every column in the middle band, k outer (start.f90:1105-1119)
+    allocate(columns_num(nband,0:0))
+    columns_num = 0
+    columns_num(midband,0) = (N(1,midband)/2+1)*N(2,midband)
+    allocate(columns_i(columns_num(midband,0),nband,0:0), columns_k(columns_num(midband,0),nband,0:0))
+    c = 0
+    do k = 1, N(2,midband)
+      do i = 0, N(1,midband)/2
+        c = c + 1
+        columns_i(c,midband,0) = i
+        columns_k(c,midband,0) = k
+      end do
+    end do
+    allocate(jlim(2,3,nband))
+    jlim = 0
+    jlim(1,ugrid,2) = N(4,0);   jlim(2,ugrid,2) = N(4,3)+1
+    jlim(1,vgrid,2) = N(3,0);   jlim(2,vgrid,2) = N(3,3)+1
+    jlim(1,pgrid,2) = N(4,0)+1; jlim(2,pgrid,2) = N(4,3)
+    jlim(:,:,1) = jlim(:,:,2)
+    jlim(:,:,3) = jlim(:,:,2)
+    ! gridweighting, start.f90:497-503
+    allocate(gridweighting(nband,2))
+    gridweighting(:,1) = -(yv(N(3,0))-yu(N(4,0)))/(yv(N(3,0))-yu(N(4,0)+1))
+    gridweighting(:,2) = (yu(N(4,3)+1)-yv(N(3,3)+1))/(yv(N(3,3)+1)-yu(N(4,3)))
+    allocate(bandPL(0:0))
+    bandPL(0) = midband
+    jgal = 0
+    allocate(u11(0:nn+2))
+    allocate(DG(sband:eband))
+    do iband = sband, eband
+      allocate(DG(iband)%f_dg(3,jlim(1,pgrid,iband):jlim(2,pgrid,iband),columns_num(iband,0)))
+      DG(iband)%f_dg = 0d0
+    end do
+    call LU_buildP(jlim(1,pgrid,midband), jlim(2,pgrid,midband), 0, midband, DG)
+  end subroutine
+
+  subroutine ref_sizes(out) bind(C, name="ref_sizes")
+    integer(c_int), intent(out) :: out(8)
+    out(1) = N(4,0); out(2) = N(4,3); out(3) = N(3,0); out(4) = N(3,3)
+    out(5) = nn; out(6) = columns_num(midband,0); out(7) = N(3,1); out(8) = N(3,2)
+  end subroutine
+
+  ! metric arrays in the reference index ranges
+  subroutine ref_grid(yu_o, yv_o, du_o, dv_o, d2u_o, d2v_o, gw_o) bind(C, name="ref_grid")
+    real(c_double), intent(out) :: yu_o(*), yv_o(*), du_o(*), dv_o(*), d2u_o(*), d2v_o(*), gw_o(2)
+    integer :: nu, nv
+    nu = N(4,3) + 1 - N(4,0) + 1
+    nv = N(3,3) + 1 - N(3,0) + 1
+    yu_o(1:nu) = yu(N(4,0):N(4,3)+1)
+    du_o(1:nu) = dthdyu(N(4,0):N(4,3)+1)
+    yv_o(1:nv) = yv(N(3,0):N(3,3)+1)
+    dv_o(1:nv) = dthdyv(N(3,0):N(3,3)+1)
+    d2u_o(1:3*nu) = reshape(dyu2i(:,N(4,0):N(4,3)+1), (/3*nu/))
+    d2v_o(1:3*nv) = reshape(dyv2i(:,N(3,0):N(3,3)+1), (/3*nv/))
+    gw_o(1) = gridweighting(midband,1)
+    gw_o(2) = gridweighting(midband,2)
+  end subroutine
+
+  subroutine ref_wavenumbers(k1x, k2x, k1z, k2z) bind(C, name="ref_wavenumbers")
+    real(c_double), intent(out) :: k1x(*), k2x(*), k1z(*), k2z(*)
+    integer :: i, k
+    do i = 0, N(1,1)/2
+      k1x(i+1) = dimag(k1F_x(i))
+      k2x(i+1) = k2F_x(i)
+    end do
+    do k = 1, N(2,1)
+      k1z(k) = dimag(k1F_z(k))
+      k2z(k) = k2F_z(k)
+    end do
+  end subroutine
+
+  ! canopy lists: boundary_canopies writes them to a stream file in dirout
+  subroutine ref_canopies(ntx, ntz, dx, dz) bind(C, name="ref_canopies")
+    integer(c_int), value :: ntx, ntz, dx, dz
+    ntilex = ntx
+    ntilez = ntz
+    dstx = dx
+    dstz = dz
+    write(ext1,'(i4.4)') N(1,1)
+    write(ext2,'(i4.4)') N(2,1)
+    write(ext3,'(i4.4)') nn+2
+    call boundary_canopies
+  end subroutine
+
+  ! RHS0_u1/u2/u3 (littleharsh.f90:614-731) on state arrays; du and Lu returned
+  subroutine ref_rhs0(krk_in, u1, u2, u3, p, nu1, nu2, nu3, du1, du2, du3, lu1, lu2, lu3) &
+      bind(C, name="ref_rhs0")
+    integer(c_int), value :: krk_in
+    complex(c_double_complex), intent(in) :: u1(*), u2(*), u3(*), p(*), nu1(*), nu2(*), nu3(*)
+    complex(c_double_complex), intent(out) :: du1(*), du2(*), du3(*), lu1(*), lu2(*), lu3(*)
+    type(cfield) :: fu1(sband:eband), fu2(sband:eband), fu3(sband:eband), fp(sband:eband)
+    type(cfield) :: fn1(sband:eband), fn2(sband:eband), fn3(sband:eband)
+    type(cfield) :: fd1(sband:eband), fd2(sband:eband), fd3(sband:eband)
+    type(cfield) :: fl1(sband:eband), fl2(sband:eband), fl3(sband:eband)
+    kRK = krk_in
+    call mk(fu1, ugrid, 0, u1); call mk(fu2, vgrid, 0, u2); call mk(fu3, ugrid, 0, u3)
+    call mk(fp, pgrid, 0, p)
+    call mk(fn1, ugrid, 1, nu1); call mk(fn2, vgrid, 1, nu2); call mk(fn3, ugrid, 1, nu3)
+    call mkz(fd1, ugrid); call mkz(fd2, vgrid); call mkz(fd3, ugrid)
+    call mkz(fl1, ugrid); call mkz(fl2, vgrid); call mkz(fl3, ugrid)
+    call RHS0_u1(fd1, fu1, fn1, fp, fl1, 0)
+    call RHS0_u2(fd2, fu2, fn2, fp, fl2, 0)
+    call RHS0_u3(fd3, fu3, fn3, fp, fl3, 0)
+    call out(fd1, du1); call out(fd2, du2); call out(fd3, du3)
+    call out(fl1, lu1); call out(fl2, lu2); call out(fl3, lu3)
+  end subroutine
+
+  ! solveP + v_corr (littleharsh.f90:557-612, 826-937) with dt, kRK
+  subroutine ref_solvep_vcorr(krk_in, dt_in, u1, u2, u3, p, psi_o, div_o) bind(C, name="ref_solvep_vcorr")
+    integer(c_int), value :: krk_in
+    real(c_double), value :: dt_in
+    complex(c_double_complex), intent(inout) :: u1(*), u2(*), u3(*), p(*)
+    complex(c_double_complex), intent(out) :: psi_o(*), div_o(*)
+    type(cfield) :: fu1(sband:eband), fu2(sband:eband), fu3(sband:eband), fp(sband:eband)
+    type(cfield) :: fpsi(sband:eband), fdiv(sband:eband)
+    integer :: status(1), ierr
+    kRK = krk_in
+    dt = dt_in
+    dti = 1d0/dt
+    call mk(fu1, ugrid, 0, u1); call mk(fu2, vgrid, 0, u2); call mk(fu3, ugrid, 0, u3)
+    call mk(fp, pgrid, 0, p)
+    call mkz(fpsi, pgrid); call mkz(fdiv, pgrid)
+    call solveP(fp, fpsi, fu1, fu2, fu3, fdiv, 0)
+    call out(fpsi, psi_o)
+    call v_corr(fu1, fu2, fu3, fpsi, fdiv, 0, status, ierr)
+    call out(fu1, u1); call out(fu2, u2); call out(fu3, u3); call out(fp, p)
+    call out(fdiv, div_o)
+  end subroutine
+
+  ! LU_build + LU_dec of the velocity matrices (tridLU_3D_IB.f90:427-493, 601-640)
+  subroutine ref_lu_vel(krk_in, dt_in, grid, ay_o, axz_o) bind(C, name="ref_lu_vel")
+    integer(c_int), value :: krk_in, grid
+    real(c_double), value :: dt_in
+    real(c_double), intent(out) :: ay_o(*), axz_o(*)
+    real(8), allocatable :: ay(:,:,:), axz(:,:)
+    integer :: nj
+    kRK = krk_in
+    dt = dt_in
+    allocate(ay(3,jlim(1,grid,2):jlim(2,grid,2),nband))
+    allocate(axz(maxval(columns_num),nband))
+    ay = 0d0
+    axz = 0d0
+    call LU_build(jlim(1,grid,midband), jlim(2,grid,midband), grid, 0, midband, axz, ay)
+    call LU_dec(jlim(1,grid,midband), jlim(2,grid,midband), grid, 0, midband, ay)
+    nj = jlim(2,grid,2) - jlim(1,grid,2) + 1
+    ay_o(1:3*nj) = reshape(ay(:,:,midband), (/3*nj/))
+    axz_o(1:columns_num(midband,0)) = axz(1:columns_num(midband,0),midband)
+    deallocate(ay, axz)
+  end subroutine
+
+  ! DG factors after LU_buildP/LU_decP (tridLU_3D_IB.f90:497-571, 643-684)
+  subroutine ref_dg(dg_o) bind(C, name="ref_dg")
+    real(c_double), intent(out) :: dg_o(*)
+    integer :: n
+    n = size(DG(midband)%f_dg)
+    dg_o(1:n) = reshape(DG(midband)%f_dg, (/n/))
+  end subroutine
+
+  ! flowrate_corr + LUsol0 (littleharsh.f90:453-485) on the (0,0) column
+  subroutine ref_flowrate_corr(krk_in, dt_in, qxt_in, mpg, u, g) bind(C, name="ref_flowrate_corr")
+    integer(c_int), value :: krk_in
+    real(c_double), value :: dt_in, qxt_in
+    real(c_double), intent(inout) :: mpg
+    real(c_double), intent(out) :: g
+    complex(c_double_complex), intent(inout) :: u(*)
+    kRK = krk_in
+    dt = dt_in
+    QxT = qxt_in
+    call flowrate_corr(u, mpg, g)
+  end subroutine
+
+  ! flowrateIm (littleharsh.f90:412-431)
+  subroutine ref_flowrate(u, q) bind(C, name="ref_flowrate")
+    complex(c_double_complex), intent(in) :: u(*)
+    real(c_double), intent(out) :: q
+    call flowrateIm(q, u)
+  end subroutine
+
+  ! buildsp / buildspUV (spectra.f90:18-56) accumulated once from zero
+  subroutine ref_buildsp(grid, u, v, sp_o) bind(C, name="ref_buildsp")
+    integer(c_int), value :: grid
+    complex(c_double_complex), intent(in) :: u(*), v(*)
+    real(c_double), intent(out) :: sp_o(*)
+    real(8), allocatable :: sp(:,:)
+    integer :: n, g2
+    g2 = grid
+    if (grid == 0) g2 = ugrid
+    allocate(sp(jlim(1,g2,midband):jlim(2,g2,midband), columns_num(midband,0)))
+    sp = 0d0
+    if (grid == 0) then
+      call buildspUV(sp, u, v, 0)
+    else
+      call buildsp(sp, u, grid, 0)
+    end if
+    n = size(sp)
+    sp_o(1:n) = reshape(sp, (/n/))
+    deallocate(sp)
+  end subroutine
+
+  ! ---- helpers: cfield band arrays holding copies of plain arrays (middle band only)
+  subroutine mk(f, grid, inner, a)
+    type(cfield) :: f(sband:eband)
+    integer :: grid, inner
+    complex(c_double_complex), intent(in) :: a(*)
+    integer :: b, j0, j1, nj, nc
+    do b = sband, eband
+      j0 = jlim(1,grid,b) + inner
+      j1 = jlim(2,grid,b) - inner
+      nc = columns_num(b,0)
+      allocate(f(b)%f(j0:j1, nc))
+      f(b)%f = (0d0, 0d0)
+    end do
+    j0 = jlim(1,grid,midband) + inner
+    j1 = jlim(2,grid,midband) - inner
+    nj = j1 - j0 + 1
+    nc = columns_num(midband,0)
+    f(midband)%f = reshape(a(1:nj*nc), (/nj, nc/))
+  end subroutine
+
+  subroutine mkz(f, grid)
+    type(cfield) :: f(sband:eband)
+    integer :: grid, b
+    do b = sband, eband
+      allocate(f(b)%f(jlim(1,grid,b):jlim(2,grid,b), columns_num(b,0)))
+      f(b)%f = (0d0, 0d0)
+    end do
+  end subroutine
+
+  subroutine out(f, a)
+    type(cfield) :: f(sband:eband)
+    complex(c_double_complex), intent(out) :: a(*)
+    integer :: n
+    n = size(f(midband)%f)
+    a(1:n) = reshape(f(midband)%f, (/n/))
+  end subroutine
+
+end module refdrv
