@@ -915,3 +915,21 @@ void lho_spectra_accumulate(lho* o) {
 void lho_spectra_read(const lho* o, double* out) {
   memcpy(out, o->sp, sizeof(double) * 5 * (size_t)o->ncol * o->nyu);
 }
+
+/* nonlinear alone (du arrays are scratch here): Nu_new for the current state */
+void lho_hook_nonlinear(lho* o, int krk1) {
+  memset(o->du1, 0, sizeof(cplx) * (size_t)o->ncol * o->nyu);
+  memset(o->du2, 0, sizeof(cplx) * (size_t)o->ncol * o->nyv);
+  memset(o->du3, 0, sizeof(cplx) * (size_t)o->ncol * o->nyu);
+  nonlinear(o, krk1 - 1);
+}
+void lho_get_nonlinear_full(const lho* o, double* n1, double* n2, double* n3) {
+  memcpy(n1, o->Nu1, sizeof(cplx) * (size_t)o->ncol * o->nyu);
+  memcpy(n2, o->Nu2, sizeof(cplx) * (size_t)o->ncol * o->nyv);
+  memcpy(n3, o->Nu3, sizeof(cplx) * (size_t)o->ncol * o->nyu);
+}
+/* immersed-boundary forcing alone on x (reference layout, grid_u: u grid else v grid)
+   with the y-Laplacian Lu and beta */
+void lho_hook_ib(lho* o, int grid_u, double beta, double* x, const double* lu) {
+  immersed_boundaries(o, (cplx*)x, (const cplx*)lu, grid_u, beta);
+}

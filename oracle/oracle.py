@@ -66,6 +66,9 @@ def lib():
         L.lho_get_dg.argtypes = [vp, _dp]
         L.lho_hook_flowrate_corr.argtypes = [vp, ctypes.c_int, ctypes.c_double, ctypes.c_double, _dp]
         L.lho_hook_flowrate_corr.restype = ctypes.c_double
+        L.lho_hook_nonlinear.argtypes = [vp, ctypes.c_int]
+        L.lho_get_nonlinear_full.argtypes = [vp, _dp, _dp, _dp]
+        L.lho_hook_ib.argtypes = [vp, ctypes.c_int, ctypes.c_double, _dp, _dp]
         _lib = L
     return _lib
 
@@ -167,6 +170,22 @@ class Oracle:
         m = ctypes.c_double(mpgx)
         g = lib().lho_hook_flowrate_corr(self._h, krk, dt, qxt, ctypes.byref(m))
         return g, m.value
+
+    def nonlinear(self, krk=1):
+        """Nu_new (full plane ranges, reference layout) of the current state."""
+        s = self.s
+        lib().lho_hook_nonlinear(self._h, krk)
+        n1 = np.zeros((s.ncol, s.nyu), complex)
+        n2 = np.zeros((s.ncol, s.nyv), complex)
+        n3 = np.zeros((s.ncol, s.nyu), complex)
+        lib().lho_get_nonlinear_full(self._h, _p(n1), _p(n2), _p(n3))
+        return n1, n2, n3
+
+    def ib(self, grid_u, beta, x, lu):
+        x = np.array(x, dtype=np.complex128, order="C", copy=True)
+        lu = np.ascontiguousarray(lu, dtype=np.complex128)
+        lib().lho_hook_ib(self._h, 1 if grid_u else 0, beta, _p(x), _p(lu))
+        return x
 
     def spectra(self):
         s = self.s
