@@ -99,6 +99,10 @@ int lh_step(lh_handle* h, double dt);
  * of the state at the start of the step, evaluated on the device (no host round trip);
  * *dt_used (may be NULL) receives the step taken. */
 int lh_step_cfl(lh_handle* h, double cfl, double* dt_used);
+/* The CFL time step of the current state, dt = cfl / max(...) as above, without stepping;
+ * it also becomes the device time step used by lh_substep(h, krk, dt <= 0).  The CFL
+ * formula is project-defined: dtc/dti are declared (declaration.f90:51) but set by the
+ * absent nonlinear routine. */
 int lh_cfl_dt(lh_handle* h, double cfl, double* dt);
 
 /* Scalars: mean pressure gradient mpgx, flow rate Qx (flowrateIm, littleharsh.f90:412-431),
@@ -108,20 +112,32 @@ int lh_get_scalars(lh_handle* h, double* mpgx, double* qx, double* dt, double* c
 /* max |div u| over the p grid (error, littleharsh.f90:323-352). */
 int lh_max_div(lh_handle* h, double* err);
 
-/* Statistics (stats.f90:1-108): per-plane sums over the physical points of
- * u, u^2, v, v^2, w, w^2, p, p^2, u*v_itp, u*w, v*w_itp, wx, wx^2.  Output layout
- * (13 rows, each over the full plane range of its grid; rows 1-2, 5-6, 9-10 on the u
- * grid, 3-4, 11-13 on the v grid, 7-8 on the p grid) -- see lh_stats_read. */
+/* Statistics (stats.f90:1-108) of the current state: per-plane sums over the physical
+ * collocation points of 0 U, 1 U2, 2 V, 3 V2, 4 W, 5 W2, 6 P, 7 P2, 8 U*V_itp, 9 U*W,
+ * 10 V*W_itp, 11 wx, 12 wx^2 (the order of write_stats, stats.f90:337-349).  Rows 0-1, 4-5,
+ * 8-9 are on the u grid, 2-3 and 10-12 on the v grid, 6-7 on the p grid.  The planes the
+ * reference sums are made by its absent nonlinear routine; here v_itp is v averaged to the
+ * u centres (0 on the two ghost planes), w_itp is w averaged to the v faces and wx =
+ * dw/dy - dv/dz at the v faces, all on the 3/2 collocation grid.  The wx rows keep the
+ * reference's loop bounds (stats.f90:54-56: x < nx, z < nz of that grid).
+ * Each call adds one sample; lh_stats_read returns the sums. */
 int lh_stats_accumulate(lh_handle* h);
-/* out: 13 * (ny+2*dsty+1) doubles, row r at out[r*(ny+2*dsty+1) + (j+dsty)]; *nstat gets
- * the number of accumulated samples.  reset != 0 clears the accumulators. */
+/* out: 13 * (ny + 2*dsty + 1) doubles; row q, plane slot s (0 = first plane of the row's
+ * grid) at out[q*(ny + 2*dsty + 1) + s].  *nstat (may be NULL) gets the number of samples.
+ * reset != 0 clears the plain and conditional accumulators afterwards. */
 int lh_stats_read(lh_handle* h, double* out, int* nstat, int reset);
-/* Conditional statistics folded onto one canopy tile (stats.f90:110-253): out holds
- * 13 arrays of (dnx, dnz, nplanes_band) for the bottom band then the top band. */
+/* Conditional statistics (conddown1/2, condup1/2, stats.f90:110-253): the planes of the
+ * bottom canopy band and the mirrored planes of the top band (v sign flipped) folded onto
+ * one dnx x dnz tile (dnx = px/ntilex, dnz = pz/ntilez), bottom and top summed as recvXC
+ * writes them (stats.f90:474-508).  out holds the 13 quantities in the order above, each a
+ * Fortran array (dnx, dnz, nb) with nb = N(4,1)-N(4,0)+1 (u grid), N(3,1)-N(3,0)+1 (v grid)
+ * or N(4,1)-N(4,0)-1 (p grid); lh_cond_stats_size gives the total length. */
+int lh_cond_stats_size(lh_handle* h, long long* len);
 int lh_cond_stats_read(lh_handle* h, double* out, long long out_len, int reset);
 
-/* Spectra (spectra.f90:1-56): sp(j, column) += |f|^2 for u, v, w, p and Re(u conj v_itp).
- * out: 5 arrays in reference layout sp(j, column) over the u/v/u/u/p grids. */
+/* Spectra (spectra.f90:1-56): sp(j, column) += |f|^2 for U, V, W, then Re(u) Re(v_itp) +
+ * Im(u) Im(v_itp) (UV), then P.  out: 5 consecutive arrays of (ny+2*dsty+1) x ncol doubles,
+ * sp(j, column) with j fastest (v and p fill the first nyv / nyp rows of each column). */
 int lh_spectra_accumulate(lh_handle* h);
 int lh_spectra_read(lh_handle* h, double* out, int reset);
 

@@ -113,6 +113,7 @@ def load_library():
     lib.lh_max_div.argtypes = [vp, _dp]
     lib.lh_stats_accumulate.argtypes = [vp]
     lib.lh_stats_read.argtypes = [vp, _dp, _ip, ctypes.c_int]
+    lib.lh_cond_stats_size.argtypes = [vp, ctypes.POINTER(ctypes.c_longlong)]
     lib.lh_cond_stats_read.argtypes = [vp, _dp, ctypes.c_longlong, ctypes.c_int]
     lib.lh_spectra_accumulate.argtypes = [vp]
     lib.lh_spectra_read.argtypes = [vp, _dp, ctypes.c_int]
@@ -127,7 +128,7 @@ EXPORTED = [
     "lh_create", "lh_destroy", "lh_last_error", "lh_set_state", "lh_get_state",
     "lh_set_nonlinear", "lh_get_nonlinear", "lh_substep", "lh_step", "lh_step_cfl",
     "lh_cfl_dt", "lh_get_scalars", "lh_max_div", "lh_stats_accumulate", "lh_stats_read",
-    "lh_cond_stats_read", "lh_spectra_accumulate", "lh_spectra_read",
+    "lh_cond_stats_size", "lh_cond_stats_read", "lh_spectra_accumulate", "lh_spectra_read",
     "lh_timing_enable", "lh_timing_reset", "lh_timing_read",
 ]
 

@@ -2,11 +2,13 @@
 //
 // The handle owns one HIP stream, the device-resident state in [plane][column] layout and
 // all work buffers.  lh_substep issues the kernels of one RK3 substep:
-//   NL stage   : k_zinv (u,w,v) -> k_xnl (x transforms, products, CFL max, recombination)
-//                -> k_zasm (forward z transforms + RHS0 + nonlinear + rhs of solveU/V)
+//   NL stage   : k_zinv (u,w,v) -> k_xraw (x transforms, six products, CFL max)
+//                -> k_zasm (A/B recombination, forward z transforms, RHS0 + nonlinear +
+//                   rhs of solveU/V)
 //   IB stage   : k_zinv (rhs and Lu_y on canopy planes) -> k_xpass_ib -> k_zfwd
-//   column     : k_prep (LU of the shared velocity matrices) -> k_column (Thomas u,w,v,
-//                constant-flow-rate correction, solveP, v_corr)
+//   column     : k_prep (LU of the shared velocity matrices) -> k_vel (Thomas u, w, v)
+//                -> k_pres (constant-flow-rate correction, solveP, v_corr)
+// Statistics and spectra (lh_stats.hpp) run on request, outside the substep.
 // lh_step replays a captured HIP graph of three substeps (+ meanflow_ctP).
 #include <hip/hip_runtime.h>
 #include <math.h>
@@ -14,6 +16,7 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <cmath>
 #include <string>
 #include <vector>
 
@@ -21,6 +24,7 @@
 #include "lh_kernels.hpp"
 #include "lh_nl.hpp"
 #include "lh_col.hpp"
+#include "lh_stats.hpp"
 
 using namespace lh;
 
@@ -57,6 +61,10 @@ struct lh_handle {
   int nibu = 0, nibv = 0;
   IBEntry* ibent = nullptr;
   int nibent = 0;
+  // statistics and spectra (allocated on first use)
+  double *st_psum = nullptr, *st_pfold = nullptr, *st_acc = nullptr, *cst_acc = nullptr;
+  double* sp_acc = nullptr;
+  int nstat = 0;
   // host copies
   std::vector<int> h_ibu, h_ibv;
   std::vector<int> mask_of_u, mask_of_v;
@@ -157,15 +165,6 @@ static void launch_zfwd_t(lh_handle* h, const ZJobs& jobs, int njobs, int maxn, 
   hipLaunchKernelGGL((k_zfwd<PZ, L, TR>), grid, dim3(L * Plan<PZ>::T), sh, h->st, jobs, h->dm,
                      (const cx<TR>*)tw);
 }
-constexpr int kLevelsPerBlock = 16;
-template <int PX, typename TR>
-static void launch_xnl_t(lh_handle* h, const XNL2& a, const void* tw) {
-  const int nlev = h->dm.nyu - 2;
-  dim3 grid(h->dm.pz / 2, (nlev + a.jc - 1) / a.jc);
-  size_t sh = ((size_t)6 * (PX + 1) + Plan<PX>::template TwL<cx<TR>>::SIZE) * sizeof(cx<TR>);
-  hipLaunchKernelGGL((k_xnl<PX, TR>), grid, dim3(6 * Plan<PX>::T), sh, h->st, a, h->dm,
-                     (const cx<TR>*)tw);
-}
 template <int PX, typename TR>
 static void launch_xraw_t(lh_handle* h, const XRAW& a, const void* tw) {
   dim3 grid(h->dm.pz / 2, h->dm.nyu - 2);
@@ -197,6 +196,14 @@ static void launch_xib_t(lh_handle* h, const XIB& a, int nent, const void* tw) {
   size_t sh = (size_t)2 * (PX + 1) * sizeof(cx<TR>);
   hipLaunchKernelGGL((k_xpass_ib<PX, TR>), grid, dim3(2 * Plan<PX>::T), sh, h->st, a, h->dm,
                      (const cx<TR>*)tw);
+}
+
+template <int PX, typename TR>
+static void launch_stats_t(lh_handle* h, const STATS& a) {
+  dim3 grid(a.dnz, a.n);
+  size_t sh = (size_t)3 * (PX + 1) * sizeof(cx<double>) + (size_t)6 * a.dnx * sizeof(double);
+  hipLaunchKernelGGL((k_stats<PX, TR>), grid, dim3(3 * Plan<PX>::T), sh, h->st, a, h->dm,
+                     (const cx<double>*)h->twx64);
 }
 
 #define LH_DISPATCH_LEN(n, F, TR, ...)          \
@@ -563,7 +570,8 @@ int lh_destroy(lh_handle* h) {
                   h->k2z, h->scal, h->u1, h->u2, h->u3, h->p, h->nu1, h->nu2, h->nu3, h->r1,
                   h->r2, h->r3, h->psi, h->tipl, h->ph, h->zu, h->zw, h->zv, h->zp, h->dg1,
                   h->dg2, h->dg3, h->ayu, h->ayv, h->ctwork, h->masks, h->ibu_planes,
-                  h->ibv_planes, h->ibent};
+                  h->ibv_planes, h->ibent, h->st_psum, h->st_pfold, h->st_acc,
+                  h->cst_acc, h->sp_acc};
   for (void* q : ptrs)
     if (q) hipFree(q);
   if (h->twx && h->twx != h->twx64) hipFree(h->twx);
@@ -730,8 +738,30 @@ int lh_max_div(lh_handle* h, double* err) {
 
 int lh_cfl_dt(lh_handle* h, double cfl, double* dt) {
   if (!h || !dt) return LH_ERR_ARG;
-  (void)cfl;
-  return fail(h, LH_ERR_ARG, "lh_cfl_dt: not available yet");
+  if (!(cfl > 0.0)) return fail(h, LH_ERR_ARG, "cfl must be positive");
+  const Dims& dm = h->dm;
+  hipLaunchKernelGGL(k_set_scalar, dim3(1), dim3(1), 0, h->st, h->scal, (int)S_CFL, cfl);
+  hipLaunchKernelGGL(k_set_scalar, dim3(1), dim3(1), 0, h->st, h->scal, (int)S_CFLMAX, 0.0);
+  ZJobs jobs{};
+  jobs.j[0] = ZJob{h->u1, h->zu, nullptr, 0, dm.nyu, 0};
+  jobs.j[1] = ZJob{h->u3, h->zw, nullptr, 0, dm.nyu, 0};
+  jobs.j[2] = ZJob{h->u2, h->zv, nullptr, 0, dm.nyv, 0};
+  launch_zinv(h, jobs, 3, dm.nyu, false);
+  XRAW a{h->zu, h->zw, h->zv, h->zp, h->dthdyu, h->scal, 1};
+  if (h->prec == LH_PREC_FP64) {
+    LH_DISPATCH_LEN(dm.px, launch_xraw_t, double, h, a, h->twx)
+  } else {
+    LH_DISPATCH_LEN(dm.px, launch_xraw_t, float, h, a, h->twx)
+  }
+  hipLaunchKernelGGL(k_cfl_dt, dim3(1), dim3(1), 0, h->st, h->scal);
+  HIPCHK(h, hipGetLastError());
+  HIPCHK(h, hipStreamSynchronize(h->st));
+  double s[S_NSCAL];
+  HIPCHK(h, hipMemcpy(s, h->scal, sizeof(s), hipMemcpyDeviceToHost));
+  if (!(s[S_CFLMAX] > 0.0) || !std::isfinite(s[S_DT]))
+    return fail(h, LH_ERR_NUMERIC, "CFL maximum is zero or not finite");
+  *dt = s[S_DT];
+  return LH_OK;
 }
 
 int lh_timing_enable(lh_handle* h, int enable) {
@@ -759,10 +789,175 @@ int lh_timing_read(lh_handle* h, int kernel, double* total_ms, long long* launch
   return LH_OK;
 }
 
-int lh_stats_accumulate(lh_handle* h) { return fail(h, LH_ERR_ARG, "statistics not available yet"); }
-int lh_stats_read(lh_handle* h, double*, int*, int) { return fail(h, LH_ERR_ARG, "statistics not available yet"); }
-int lh_cond_stats_read(lh_handle* h, double*, long long, int) { return fail(h, LH_ERR_ARG, "statistics not available yet"); }
-int lh_spectra_accumulate(lh_handle* h) { return fail(h, LH_ERR_ARG, "spectra not available yet"); }
-int lh_spectra_read(lh_handle* h, double*, int) { return fail(h, LH_ERR_ARG, "spectra not available yet"); }
+// ---- statistics and spectra
+static const int kStatsGrid[13] = {0, 0, 1, 1, 0, 0, 2, 2, 0, 0, 1, 1, 1};
+
+struct StatsGeom {
+  int dnx, dnz, nb[3], toplo;
+  long long qoff[13], clen;
+};
+
+static int stats_geom(lh_handle* h, StatsGeom& g) {
+  const Dims& dm = h->dm;
+  if (dm.px % h->ntilex || dm.pz % h->ntilez)
+    return fail(h, LH_ERR_ARG, "statistics: px, pz must be multiples of ntilex, ntilez");
+  g.dnx = dm.px / h->ntilex;
+  g.dnz = dm.pz / h->ntilez;
+  const int n31 = h->band_v[0], n32 = h->band_v[1];
+  if (!(n31 >= -dm.d && n31 < n32 && n32 <= dm.ny + dm.d - 2))
+    return fail(h, LH_ERR_ARG, "statistics: band limits band_v must satisfy -dsty <= N(3,1) < N(3,2)");
+  g.nb[0] = n31 + dm.d + 2;   // u planes N(4,0) .. N(4,1)
+  g.nb[1] = n31 + dm.d + 1;   // v planes N(3,0) .. N(3,1)
+  g.nb[2] = n31 + dm.d;       // p planes N(4,0)+1 .. N(4,1)-1
+  g.toplo = n32 + dm.d + 1;
+  long long a = 0;
+  for (int q = 0; q < 13; ++q) {
+    g.qoff[q] = a;
+    a += (long long)g.dnx * g.dnz * g.nb[kStatsGrid[q]];
+  }
+  g.clen = a;
+  return LH_OK;
+}
+
+int lh_stats_accumulate(lh_handle* h) {
+  if (!h) return LH_ERR_ARG;
+  const Dims& dm = h->dm;
+  StatsGeom g;
+  int r = stats_geom(h, g);
+  if (r) return r;
+  if (!h->st_acc) {
+    const size_t nmax = dm.nyu;
+    HIPCHK(h, dalloc(&h->st_psum, nmax * g.dnz * 6));
+    HIPCHK(h, dalloc(&h->st_pfold, nmax * g.dnz * 6 * g.dnx));
+    HIPCHK(h, dalloc(&h->st_acc, 13 * nmax));
+    HIPCHK(h, dalloc(&h->cst_acc, (size_t)g.clen));
+    HIPCHK(h, hipMemset(h->st_acc, 0, 13 * nmax * sizeof(double)));
+    HIPCHK(h, hipMemset(h->cst_acc, 0, (size_t)g.clen * sizeof(double)));
+  }
+  // spectral v_itp -> r1, w_itp -> r2, wx -> r3 (the rhs buffers are free between substeps)
+  hipLaunchKernelGGL(k_stats_prep, dim3((dm.ncol + 255) / 256), dim3(256), 0, h->st, h->u2, h->u3,
+                     h->dthdyv, h->k1z, h->r1, h->r2, h->r3, dm);
+  static const int qmap[3][6] = {{0, 1, 4, 5, 8, 9}, {2, 3, 10, 11, 12, 0}, {6, 7, 0, 0, 0, 0}};
+  static const double tsg[3][6] = {{1, 1, 1, 1, -1, 1}, {-1, 1, -1, 1, 1, 1}, {1, 1, 1, 1, 1, 1}};
+  static const int nloc[3] = {6, 5, 2};
+  for (int type = 0; type < 3; ++type) {
+    // FP64 Z slots in zp: field f of plane p at slot f*n + p
+    const int n = type == 0 ? dm.nyu : (type == 1 ? dm.nyv : dm.nyp);
+    ZJobs jobs{};
+    int nf;
+    if (type == 0) {
+      jobs.j[0] = ZJob{h->u1, h->zp, nullptr, 0, n, 0};
+      jobs.j[1] = ZJob{h->u3, h->zp, nullptr, 0, n, n};
+      jobs.j[2] = ZJob{h->r1, h->zp, nullptr, 0, n, 2 * n};
+      nf = 3;
+    } else if (type == 1) {
+      jobs.j[0] = ZJob{h->u2, h->zp, nullptr, 0, n, 0};
+      jobs.j[1] = ZJob{h->r2, h->zp, nullptr, 0, n, n};
+      jobs.j[2] = ZJob{h->r3, h->zp, nullptr, 0, n, 2 * n};
+      nf = 3;
+    } else {
+      jobs.j[0] = ZJob{h->p, h->zp, nullptr, 0, n, 0};
+      nf = 1;
+    }
+    launch_zinv(h, jobs, nf, n, true);
+    STATS a{(const double2*)h->zp, nf, n, type, h->ntilex, h->ntilez, g.dnx, g.dnz,
+            h->st_psum, h->st_pfold};
+    LH_DISPATCH_LEN(dm.px, launch_stats_t, double, h, a)
+    STFIN f{};
+    f.psum = h->st_psum; f.pfold = h->st_pfold; f.st = h->st_acc; f.cst = h->cst_acc;
+    for (int q = 0; q < 13; ++q) f.qoff[q] = g.qoff[q];
+    for (int l = 0; l < 6; ++l) { f.qmap[l] = qmap[type][l]; f.topsign[l] = tsg[type][l]; }
+    f.nloc = nloc[type]; f.n = n; f.type = type; f.dnx = g.dnx; f.dnz = g.dnz; f.nyu = dm.nyu;
+    f.nb = g.nb[type]; f.toplo = g.toplo;
+    hipLaunchKernelGGL(k_stats_fin_plain, dim3((n * f.nloc + 255) / 256), dim3(256), 0, h->st, f);
+    const long long nc = (long long)f.nloc * f.nb * g.dnx * g.dnz;
+    hipLaunchKernelGGL(k_stats_fin_cond, dim3((unsigned)((nc + 255) / 256)), dim3(256), 0, h->st, f);
+  }
+  HIPCHK(h, hipGetLastError());
+  HIPCHK(h, hipStreamSynchronize(h->st));
+  h->nstat += 1;
+  return LH_OK;
+}
+
+int lh_stats_read(lh_handle* h, double* out, int* nstat, int reset) {
+  if (!h || !out) return LH_ERR_ARG;
+  const size_t n = 13 * (size_t)h->dm.nyu;
+  HIPCHK(h, hipStreamSynchronize(h->st));
+  if (h->st_acc) HIPCHK(h, hipMemcpy(out, h->st_acc, n * sizeof(double), hipMemcpyDeviceToHost));
+  else memset(out, 0, n * sizeof(double));
+  if (nstat) *nstat = h->nstat;
+  if (reset && h->st_acc) {
+    StatsGeom g;
+    int r = stats_geom(h, g);
+    if (r) return r;
+    HIPCHK(h, hipMemset(h->st_acc, 0, n * sizeof(double)));
+    HIPCHK(h, hipMemset(h->cst_acc, 0, (size_t)g.clen * sizeof(double)));
+    h->nstat = 0;
+  }
+  return LH_OK;
+}
+
+int lh_cond_stats_size(lh_handle* h, long long* len) {
+  if (!h || !len) return LH_ERR_ARG;
+  StatsGeom g;
+  int r = stats_geom(h, g);
+  if (r) return r;
+  *len = g.clen;
+  return LH_OK;
+}
+
+int lh_cond_stats_read(lh_handle* h, double* out, long long out_len, int reset) {
+  if (!h || !out) return LH_ERR_ARG;
+  StatsGeom g;
+  int r = stats_geom(h, g);
+  if (r) return r;
+  if (out_len < g.clen) return fail(h, LH_ERR_ARG, "lh_cond_stats_read: output too short");
+  HIPCHK(h, hipStreamSynchronize(h->st));
+  if (h->cst_acc)
+    HIPCHK(h, hipMemcpy(out, h->cst_acc, (size_t)g.clen * sizeof(double), hipMemcpyDeviceToHost));
+  else
+    memset(out, 0, (size_t)g.clen * sizeof(double));
+  if (reset && h->cst_acc) {
+    HIPCHK(h, hipMemset(h->cst_acc, 0, (size_t)g.clen * sizeof(double)));
+    HIPCHK(h, hipMemset(h->st_acc, 0, 13 * (size_t)h->dm.nyu * sizeof(double)));
+    h->nstat = 0;
+  }
+  return LH_OK;
+}
+
+int lh_spectra_accumulate(lh_handle* h) {
+  if (!h) return LH_ERR_ARG;
+  const Dims& dm = h->dm;
+  const size_t n = 5 * (size_t)dm.nyu * dm.ncol;
+  if (!h->sp_acc) {
+    HIPCHK(h, dalloc(&h->sp_acc, n));
+    HIPCHK(h, hipMemset(h->sp_acc, 0, n * sizeof(double)));
+  }
+  hipLaunchKernelGGL(k_spectra, dim3((dm.ncol + 255) / 256, dm.nyu), dim3(256), 0, h->st, h->u1,
+                     h->u2, h->u3, h->p, h->sp_acc, dm);
+  HIPCHK(h, hipGetLastError());
+  HIPCHK(h, hipStreamSynchronize(h->st));
+  return LH_OK;
+}
+
+int lh_spectra_read(lh_handle* h, double* out, int reset) {
+  if (!h || !out) return LH_ERR_ARG;
+  const Dims& dm = h->dm;
+  const size_t pl = (size_t)dm.nyu * dm.ncol;
+  HIPCHK(h, hipStreamSynchronize(h->st));
+  if (!h->sp_acc) {
+    memset(out, 0, 5 * pl * sizeof(double));
+    return LH_OK;
+  }
+  std::vector<double> t(5 * pl);
+  HIPCHK(h, hipMemcpy(t.data(), h->sp_acc, t.size() * sizeof(double), hipMemcpyDeviceToHost));
+  // device [f][slot][column] -> sp(j, column), j fastest, each array ncol x nyu
+  for (int f = 0; f < 5; ++f)
+    for (int c = 0; c < dm.ncol; ++c)
+      for (int s = 0; s < dm.nyu; ++s)
+        out[f * pl + (size_t)c * dm.nyu + s] = t[f * pl + (size_t)s * dm.ncol + c];
+  if (reset) HIPCHK(h, hipMemset(h->sp_acc, 0, 5 * pl * sizeof(double)));
+  return LH_OK;
+}
 
 }  // extern "C"

@@ -14,246 +14,15 @@
 //   u:  A1 = ikx uu + m_u (uv - uv_prev),   B1 = uw
 //   w:  A3 = ikx uw + m_u (vw - vw_prev),   B3 = ww
 //   v:  A2 = ikx uv_prev + m_v (vv - vv_prev), B2 = vw_prev   (face j-1 at level j)
-// The x pass (k_xnl) produces these six rows per level; the forward z pass (k_zasm)
-// transforms them and assembles RHS0 + nonlinear + the rhs of solveU/solveV in one go.
+// The x pass (k_xraw) produces the six raw products per level; the forward z pass
+// (k_zasm) forms A and B from them, transforms, and assembles RHS0 + nonlinear + the rhs
+// of solveU/solveV in one go.
 #pragma once
 #include <type_traits>
 #include "lh_kernels.hpp"
 
 namespace lh {
 
-struct XNL2 {
-  const void* zu;       // Z slots per u plane
-  const void* zw;
-  const void* zv;       // Z slots per v plane
-  void* zo;             // [6][nlev][pz][mx]
-  const double* dthdyu;
-  const double* dthdyv;
-  const double* k1x;
-  double* scal;
-  int want_cfl;
-  int jc;               // levels per block
-};
-
-// Block: one pair of z rows (z0, z0+1) and a chunk of consecutive levels.  Physical
-// u, w at the current centre and v at the face below stay in registers (each thread owns
-// fixed x positions); the Z-space products uv, vw, vv of the previous level stay in
-// registers of the thread that owns that kx index.
-template <int PX, typename TR>
-__global__ void __launch_bounds__(6 * Plan<PX>::T) __attribute__((amdgpu_waves_per_eu(3)))
-    k_xnl(XNL2 a, Dims dm, const cx<TR>* __restrict__ tw) {
-  using C = cx<TR>;
-  constexpr int T = Plan<PX>::T;
-  constexpr int NT = 6 * T;
-  constexpr int LS = PX + 1;
-  constexpr int XPT = (PX + NT - 1) / NT;       // x positions per thread
-  extern __shared__ char smem_raw[];
-  C* lds = reinterpret_cast<C*>(smem_raw);
-  __shared__ double red[NT / 64 + 1];
-  const int tid = threadIdx.x;
-  const int z0 = 2 * blockIdx.x;
-  const int nlev = dm.nyu - 2;
-  const int l0 = blockIdx.y * a.jc;
-  const int l1 = min(l0 + a.jc, nlev);
-  const int mx = dm.mx;
-  const size_t slot = (size_t)dm.pz * mx;
-  const C* zu = reinterpret_cast<const C*>(a.zu);
-  const C* zw = reinterpret_cast<const C*>(a.zw);
-  const C* zv = reinterpret_cast<const C*>(a.zv);
-  C* zo = reinterpret_cast<C*>(a.zo);
-  const int ln = tid / T, t = tid % T;
-  // twiddles in LDS after the six lines (filled by the threads of line 0)
-  typename Plan<PX>::template TwL<C> w;
-  w.init(lds + 6 * LS, t, tw, ln == 0);
-  const TR half = (TR)0.5;
-
-  // rows z0, z0+1 of up to six source slots: loads go to registers first (issue), then are
-  // packed into LDS lines (commit); the three per-level loads are issued one level ahead.
-  constexpr int MAXI = (6 * (PX / 3 + 1) + NT - 1) / NT;
-  constexpr int MAX3 = (3 * (PX / 3 + 1) + NT - 1) / NT;
-  auto issue = [&](const C* const* srcs, int nl, C* va, C* vb, auto cnt) {
-    constexpr int M = decltype(cnt)::value;
-#pragma unroll
-    for (int q = 0; q < M; ++q) {
-      const int e = tid + q * NT;
-      if (e < nl * mx) {
-        const int f = e / mx, n = e % mx;
-        const C* A = srcs[f] + (size_t)z0 * mx;
-        va[q] = A[n];
-        vb[q] = A[mx + n];
-      }
-    }
-  };
-  auto commit = [&](int nl, const C* va, const C* vb, auto cnt) {
-    constexpr int M = decltype(cnt)::value;
-#pragma unroll
-    for (int q = 0; q < M; ++q) {
-      const int e = tid + q * NT;
-      if (e < nl * mx) {
-        const int f = e / mx, n = e % mx;
-        C* line = lds + f * LS;
-        if (n == 0) {
-          line[0] = mk<C>(va[q].x, vb[q].x);
-        } else {
-          line[n] = mk<C>(va[q].x - vb[q].y, va[q].y + vb[q].x);
-          line[PX - n] = mk<C>(va[q].x + vb[q].y, -va[q].y + vb[q].x);
-        }
-      }
-    }
-    const int nzr = PX - 2 * mx + 1;   // positions mx .. PX-mx are zero
-    for (int e = tid; e < nl * nzr; e += NT) {
-      const int f = e / nzr, n = mx + e % nzr;
-      lds[f * LS + n] = mk<C>(0, 0);
-    }
-  };
-  using CntAll = std::integral_constant<int, MAXI>;
-  using Cnt3 = std::integral_constant<int, MAX3>;
-  // all six lines are transformed; when only nl < 6 carry data the others hold
-  // finite left-overs that are overwritten before they are read again
-  auto fft_lines = [&](int nl, auto inv_tag) {
-    constexpr bool INV = decltype(inv_tag)::value;
-    (void)nl;
-    Plan<PX>::template run<INV>(lds + ln * LS, t, w);
-  };
-  using Inv = std::true_type;
-  using Fwd = std::false_type;
-
-  // Z-space products of the previous level for this thread's kx index (rows z0, z0+1)
-  C puv0 = mk<C>(0, 0), puv1 = mk<C>(0, 0), pvw0 = mk<C>(0, 0), pvw1 = mk<C>(0, 0);
-  C pvv0 = mk<C>(0, 0), pvv1 = mk<C>(0, 0);
-  // physical u(j), w(j), v(j-1) at this thread's x positions (Re: row z0, Im: row z0+1)
-  C Uc[XPT], Wc[XPT], Vb[XPT];
-
-  auto unpack = [&](const C* line, int k, C& ra, C& rb) { unpack_rows<PX>(line, k, ra, rb); };
-
-  // ---------------- prologue ----------------
-  {
-    const int su0 = l0 + 1;
-    {
-      C va[MAXI], vb[MAXI];
-      const int sm = l0 > 0 ? su0 - 1 : su0;
-      const C* srcs[6] = {zu + sm * slot, zu + su0 * slot, zw + sm * slot, zw + su0 * slot,
-                          zv + (su0 - (l0 > 0 ? 2 : 1)) * slot, zv + (su0 - 1) * slot};
-      issue(srcs, 6, va, vb, CntAll{});
-      commit(6, va, vb, CntAll{});
-    }
-    lds_bar();
-    fft_lines(6, Inv{});
-#pragma unroll
-    for (int q = 0; q < XPT; ++q) {
-      const int x = tid + q * NT;
-      if (x < PX) {
-        const C Um = lds[0 * LS + x], U0 = lds[1 * LS + x], Wm = lds[2 * LS + x], W0 = lds[3 * LS + x];
-        const C Vm = lds[4 * LS + x], V0 = lds[5 * LS + x];
-        Uc[q] = U0; Wc[q] = W0; Vb[q] = V0;
-        if (l0 > 0) {
-          const C vc = mk<C>(half * (Vm.x + V0.x), half * (Vm.y + V0.y));
-          const C uf = mk<C>(half * (Um.x + U0.x), half * (Um.y + U0.y));
-          const C wf = mk<C>(half * (Wm.x + W0.x), half * (Wm.y + W0.y));
-          lds[0 * LS + x] = mk<C>(uf.x * V0.x, uf.y * V0.y);    // uv(j0-1)
-          lds[1 * LS + x] = mk<C>(V0.x * wf.x, V0.y * wf.y);    // vw(j0-1)
-          lds[2 * LS + x] = mk<C>(vc.x * vc.x, vc.y * vc.y);    // vv(j0-1)
-        }
-      }
-    }
-    if (l0 > 0) {
-      lds_bar();
-      fft_lines(3, Fwd{});
-      if (tid < mx) {
-        unpack(lds + 0 * LS, tid, puv0, puv1);
-        unpack(lds + 1 * LS, tid, pvw0, pvw1);
-        unpack(lds + 2 * LS, tid, pvv0, pvv1);
-      }
-    }
-    lds_bar();
-  }
-
-  double cmax = 0.0;
-  // ---------------- levels ----------------
-  for (int l = l0; l < l1; ++l) {
-    const int su = l + 1;
-    {
-      C pa[MAX3], pb[MAX3];
-      const C* srcs[3] = {zu + (su + 1) * slot, zw + (su + 1) * slot, zv + su * slot};
-      issue(srcs, 3, pa, pb, Cnt3{});
-      commit(3, pa, pb, Cnt3{});
-    }
-    lds_bar();
-    fft_lines(3, Inv{});
-    const double dyi = a.dthdyu[su];
-#pragma unroll
-    for (int q = 0; q < XPT; ++q) {
-      const int x = tid + q * NT;
-      if (x < PX) {
-        const C Ua = lds[0 * LS + x], Wa = lds[1 * LS + x], Vt = lds[2 * LS + x];
-        const C U = Uc[q], W = Wc[q], Vd = Vb[q];
-        const C vc = mk<C>(half * (Vd.x + Vt.x), half * (Vd.y + Vt.y));
-        const C uf = mk<C>(half * (U.x + Ua.x), half * (U.y + Ua.y));
-        const C wf = mk<C>(half * (W.x + Wa.x), half * (W.y + Wa.y));
-        lds[0 * LS + x] = mk<C>(U.x * U.x, U.y * U.y);          // uu(j)
-        lds[1 * LS + x] = mk<C>(U.x * W.x, U.y * W.y);          // uw(j)
-        lds[2 * LS + x] = mk<C>(W.x * W.x, W.y * W.y);          // ww(j)
-        lds[3 * LS + x] = mk<C>(vc.x * vc.x, vc.y * vc.y);      // vv(j)
-        lds[4 * LS + x] = mk<C>(uf.x * Vt.x, uf.y * Vt.y);      // uv(j)
-        lds[5 * LS + x] = mk<C>(Vt.x * wf.x, Vt.y * wf.y);      // vw(j)
-        if (a.want_cfl) {
-          const double s0 = fabs((double)U.x) * dm.dxi + fabs((double)vc.x) * dyi + fabs((double)W.x) * dm.dzi;
-          const double s1 = fabs((double)U.y) * dm.dxi + fabs((double)vc.y) * dyi + fabs((double)W.y) * dm.dzi;
-          cmax = fmax(cmax, fmax(s0, s1));
-        }
-        Uc[q] = Ua; Wc[q] = Wa; Vb[q] = Vt;
-      }
-    }
-    lds_bar();
-    fft_lines(6, Fwd{});
-    if (tid < mx) {
-      const int k = tid;
-      C uu0, uu1, uw0, uw1, ww0, ww1, vv0, vv1, uv0, uv1, vw0, vw1;
-      unpack(lds + 0 * LS, k, uu0, uu1);
-      unpack(lds + 1 * LS, k, uw0, uw1);
-      unpack(lds + 2 * LS, k, ww0, ww1);
-      unpack(lds + 3 * LS, k, vv0, vv1);
-      unpack(lds + 4 * LS, k, uv0, uv1);
-      unpack(lds + 5 * LS, k, vw0, vw1);
-      const TR kx = (TR)a.k1x[k];
-      const TR mu = (TR)(dm.ddthetavi * a.dthdyu[su]);
-      const TR mv = (TR)(dm.ddthetavi * a.dthdyv[su - 1]);
-      auto ik = [&](C v) { return mk<C>(-kx * v.y, kx * v.x); };
-      auto axpy = [&](C p, TR m, C d) { return mk<C>(p.x + m * d.x, p.y + m * d.y); };
-      const size_t base = (size_t)l * slot + (size_t)z0 * mx + k;
-      const size_t ps = (size_t)nlev * slot;   // stride between output kinds
-      // u centre j: A1, B1
-      zo[0 * ps + base] = axpy(ik(uu0), mu, csub(uv0, puv0));
-      zo[0 * ps + base + mx] = axpy(ik(uu1), mu, csub(uv1, puv1));
-      zo[1 * ps + base] = uw0;
-      zo[1 * ps + base + mx] = uw1;
-      // w centre j: A3, B3
-      zo[2 * ps + base] = axpy(ik(uw0), mu, csub(vw0, pvw0));
-      zo[2 * ps + base + mx] = axpy(ik(uw1), mu, csub(vw1, pvw1));
-      zo[3 * ps + base] = ww0;
-      zo[3 * ps + base + mx] = ww1;
-      // v face j-1: A2, B2
-      zo[4 * ps + base] = axpy(ik(puv0), mv, csub(vv0, pvv0));
-      zo[4 * ps + base + mx] = axpy(ik(puv1), mv, csub(vv1, pvv1));
-      zo[5 * ps + base] = pvw0;
-      zo[5 * ps + base + mx] = pvw1;
-      puv0 = uv0; puv1 = uv1; pvw0 = vw0; pvw1 = vw1; pvv0 = vv0; pvv1 = vv1;
-    }
-    lds_bar();
-  }
-  if (a.want_cfl) {
-    for (int off = 32; off > 0; off >>= 1) cmax = fmax(cmax, __shfl_down(cmax, off, 64));
-    if ((tid & 63) == 0) red[tid / 64] = cmax;
-    __syncthreads();
-    if (tid == 0) {
-      double m = 0.0;
-      for (int q = 0; q < (NT + 63) / 64; ++q) m = fmax(m, red[q]);
-      atomic_max_pos(a.scal + S_CFLMAX, m);
-    }
-  }
-}
-
-// One-level x pass (one row pair, one level per block): the six raw products
 // Nonlinear x pass: one row pair and one level j (centre j and face j) per block.
 //   lines 0..5 <- U(j), U(j+1), W(j), W(j+1), V(j-1), V(j)
 //   products   -> uu_c(j), uw_c(j), ww_c(j), vv_c(j), uv_f(j), vw_f(j)
@@ -363,8 +132,13 @@ __global__ void __launch_bounds__(6 * Plan<PX>::T)
   }
   if (a.want_cfl) {
     // block max (wave64 shuffles, then LDS across waves)
-    for (int off = 32; off > 0; off >>= 1) cmax = fmax(cmax, __shfl_down(cmax, off, 64));
+    // lanes past the end of a partial last wave hold no value: take only active sources
     const int wv = threadIdx.x / 64, lane = threadIdx.x % 64;
+    const int act = min(64, (int)blockDim.x - wv * 64);
+    for (int off = 32; off > 0; off >>= 1) {
+      const double o = __shfl_down(cmax, off, 64);
+      if (lane + off < act) cmax = fmax(cmax, o);
+    }
     if (lane == 0) red[wv] = cmax;
     __syncthreads();
     if (threadIdx.x == 0) {

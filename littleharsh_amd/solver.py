@@ -107,6 +107,40 @@ class Channel:
         self._chk(self._lib.lh_max_div(self._h, ctypes.byref(e)), "lh_max_div")
         return e.value
 
+    def cfl_dt(self, cfl: float) -> float:
+        dt = ctypes.c_double()
+        self._chk(self._lib.lh_cfl_dt(self._h, float(cfl), ctypes.byref(dt)), "lh_cfl_dt")
+        return dt.value
+
+    # ---- statistics (stats.f90) and spectra (spectra.f90)
+    def stats_accumulate(self):
+        self._chk(self._lib.lh_stats_accumulate(self._h), "lh_stats_accumulate")
+
+    def stats(self, reset: bool = False):
+        """(13, nyu) per-plane sums (write_stats row order) and the sample count."""
+        a = np.zeros((13, self.s.nyu))
+        n = ctypes.c_int()
+        self._chk(self._lib.lh_stats_read(self._h, _p(a), ctypes.byref(n), 1 if reset else 0),
+                  "lh_stats_read")
+        return a, n.value
+
+    def cond_stats_flat(self, reset: bool = False):
+        n = ctypes.c_longlong()
+        self._chk(self._lib.lh_cond_stats_size(self._h, ctypes.byref(n)), "lh_cond_stats_size")
+        a = np.zeros(n.value)
+        self._chk(self._lib.lh_cond_stats_read(self._h, _p(a), n.value, 1 if reset else 0),
+                  "lh_cond_stats_read")
+        return a
+
+    def spectra_accumulate(self):
+        self._chk(self._lib.lh_spectra_accumulate(self._h), "lh_spectra_accumulate")
+
+    def spectra(self, reset: bool = False):
+        """(5, ncol, nyu): U, V, W, UV, P in the reference layout sp(j, column)."""
+        a = np.zeros((5, self.s.ncol, self.s.nyu))
+        self._chk(self._lib.lh_spectra_read(self._h, _p(a), 1 if reset else 0), "lh_spectra_read")
+        return a
+
     # ---- per-kernel timing (HIP events on the handle's stream)
     def timing(self, enable: bool):
         self._chk(self._lib.lh_timing_enable(self._h, 1 if enable else 0), "lh_timing_enable")

@@ -916,6 +916,189 @@ void lho_spectra_read(const lho* o, double* out) {
   memcpy(out, o->sp, sizeof(double) * 5 * (size_t)o->ncol * o->nyu);
 }
 
+/* ------------------------------------------------------------------------------------ */
+/* statistics (stats.f90:1-108): per-plane sums over the physical collocation points of  */
+/*   0 U, 1 U2, 2 V, 3 V2, 4 W, 5 W2, 6 P, 7 P2, 8 UV, 9 UW, 10 VW, 11 wx, 12 wx2          */
+/* (the order of write_stats, stats.f90:337-349).  Rows 0,1,4,5,8,9 live on the u grid,  */
+/* 2,3,10,11,12 on the v grid, 6,7 on the p grid; row q, slot s at st[q*nyu + s].        */
+/* Fields: u, v, w, p; v_itp = v averaged to the u centres (0 on the two ghost planes);  */
+/* w_itp = w averaged to the v faces; wx = dw/dy - dv/dz at the v faces.  The producer   */
+/* of these planes (the nonlinear routine) is absent from the reference, so the          */
+/* interpolations and wx follow the stencils of v_corr (littleharsh.f90:880-893) on the  */
+/* collocation grid.  The plain wx rows keep the loop bounds of stats.f90:54-56: the     */
+/* points x < nx, z < nz of that px x pz plane; every other row sums all px x pz points. */
+/* Conditional statistics (conddown1/2, condup1/2, stats.f90:110-253; recvXC sums the    */
+/* two bands, stats.f90:474-508): the bottom band planes j in [N(g,0), N(g,1)] (p grid:  */
+/* [N(4,0)+1, N(4,1)-1]) and the top band planes mirrored, j1 = N(g,3)+1-j (p: N(4,3)-j),*/
+/* with the sign of v flipped, folded onto one dnx x dnz canopy tile:                    */
+/*   cst_q(i, k, j1) += sum_{ix, iz} x(i + dnx*ix, k + dnz*iz, j)                        */
+/* stored per quantity q as a Fortran (dnx, dnz, nb_q) array, quantities in order.       */
+/* ------------------------------------------------------------------------------------ */
+static const int st_grid[13] = {0, 0, 1, 1, 0, 0, 2, 2, 0, 0, 1, 1, 1};
+static const double st_topsign[13] = {1, 1, -1, 1, 1, 1, 1, 1, -1, 1, -1, 1, 1};
+
+/* planes per band on grid g (0 u, 1 v, 2 p) and the first top-band slot */
+static void stats_bands(const lho* o, int nb[3], int* toplo) {
+  const int n31 = o->band_v[0], n32 = o->band_v[1], d = o->d;
+  nb[0] = n31 + 1 + d + 1;   /* u: slots 0 .. N(4,1)+d */
+  nb[1] = n31 + d + 1;       /* v: slots 0 .. N(3,1)+d */
+  nb[2] = n31 + d;           /* p: slots 0 .. N(4,1)-1+d-1 */
+  *toplo = n32 + d + 1;      /* same slot on all three grids */
+}
+
+long long lho_cond_stats_len(const lho* o) {
+  int nb[3], tl;
+  stats_bands(o, nb, &tl);
+  const long long tile = (long long)(o->px / o->ntilex) * (o->pz / o->ntilez);
+  long long n = 0;
+  for (int q = 0; q < 13; ++q) n += tile * nb[st_grid[q]];
+  return n;
+}
+
+/* j1 of slot s on grid g, sign in *sg (0 = bottom, 1 = top); -1 outside both bands */
+static int stats_j1(const lho* o, int g, int s, int* top) {
+  int nb[3], tl;
+  stats_bands(o, nb, &tl);
+  const int n = g == 0 ? o->nyu : (g == 1 ? o->nyv : o->nyp);
+  if (s < nb[g]) { *top = 0; return s; }
+  if (s >= tl) {
+    const int j1 = n - 1 - s;
+    if (j1 >= 0 && j1 < nb[g]) { *top = 1; return j1; }
+  }
+  return -1;
+}
+
+/* spectral plane of field f at slot s, with v_itp / w_itp / wx built in Fourier space */
+enum { SF_U, SF_V, SF_W, SF_P, SF_VI, SF_WI, SF_WX };
+static void stats_plane(const lho* o, int f, int s, cplx* sp) {
+  const int nu = o->nyu, nv = o->nyv;
+  for (int c = 0; c < o->ncol; ++c) {
+    cplx r = {0.0, 0.0};
+    switch (f) {
+      case SF_U: r = o->u1[(size_t)c * nu + s]; break;
+      case SF_W: r = o->u3[(size_t)c * nu + s]; break;
+      case SF_V: r = o->u2[(size_t)c * nv + s]; break;
+      case SF_P: r = o->p[(size_t)c * o->nyp + s]; break;
+      case SF_VI:
+        if (s >= 1 && s <= nu - 2)
+          r = rmul(0.5, cadd(o->u2[(size_t)c * nv + s - 1], o->u2[(size_t)c * nv + s]));
+        break;
+      case SF_WI:
+        r = rmul(0.5, cadd(o->u3[(size_t)c * nu + s], o->u3[(size_t)c * nu + s + 1]));
+        break;
+      case SF_WX: {
+        const int k = c / o->mx;
+        const cplx dw = rmul(o->ddthetavi * o->dthdyv[s],
+                             csub(o->u3[(size_t)c * nu + s + 1], o->u3[(size_t)c * nu + s]));
+        r = csub(dw, imul(o->k1z[k], o->u2[(size_t)c * nv + s]));
+      } break;
+    }
+    sp[c] = r;
+  }
+}
+
+void lho_stats_accumulate(lho* o) {
+  const int px = o->px, pz = o->pz;
+  const int dnx = px / o->ntilex, dnz = pz / o->ntilez;
+  const size_t pp = (size_t)px * pz, tile = (size_t)dnx * dnz;
+  const long long clen = lho_cond_stats_len(o);
+  if (!o->cst) {
+    o->cst = (double*)calloc((size_t)clen, sizeof(double));
+    o->cst_len = clen;
+  }
+  long long qoff[13];
+  {
+    int nb[3], tl;
+    stats_bands(o, nb, &tl);
+    long long a = 0;
+    for (int q = 0; q < 13; ++q) { qoff[q] = a; a += (long long)tile * nb[st_grid[q]]; }
+  }
+  /* per plane task: grid, slot */
+  const int ntask = o->nyu + o->nyv + o->nyp;
+  double* psum = (double*)calloc((size_t)ntask * 13, sizeof(double));
+  double* pfold = (double*)calloc((size_t)ntask * 6 * tile, sizeof(double));
+#pragma omp parallel
+  {
+    cplx* sp = (cplx*)malloc(sizeof(cplx) * o->ncol);
+    cplx* work = (cplx*)malloc(sizeof(cplx) * work_len(o));
+    double* ph[3];
+    for (int q = 0; q < 3; ++q) ph[q] = (double*)malloc(sizeof(double) * pp);
+#pragma omp for schedule(dynamic)
+    for (int t = 0; t < ntask; ++t) {
+      int g, s, fl[3], nf;
+      if (t < o->nyu) { g = 0; s = t; fl[0] = SF_U; fl[1] = SF_W; fl[2] = SF_VI; nf = 3; }
+      else if (t < o->nyu + o->nyv) { g = 1; s = t - o->nyu; fl[0] = SF_V; fl[1] = SF_WI; fl[2] = SF_WX; nf = 3; }
+      else { g = 2; s = t - o->nyu - o->nyv; fl[0] = SF_P; nf = 1; }
+      for (int q = 0; q < nf; ++q) {
+        stats_plane(o, fl[q], s, sp);
+        four_to_phys(o, sp, ph[q], work);
+      }
+      /* local quantities: u grid (U, U2, W, W2, UV, UW), v grid (V, V2, VW, wx, wx2), p (P, P2) */
+      double* S = psum + (size_t)t * 13;
+      double* F = pfold + (size_t)t * 6 * tile;
+      for (size_t e = 0; e < pp; ++e) {
+        const size_t x = e % px, z = e / px;
+        const size_t ti = (z % dnz) * dnx + (x % dnx);
+        double v[6];
+        int nv6;
+        if (g == 0) {
+          const double a = ph[0][e], b = ph[1][e], c = ph[2][e];
+          v[0] = a; v[1] = a * a; v[2] = b; v[3] = b * b; v[4] = a * c; v[5] = a * b; nv6 = 6;
+        } else if (g == 1) {
+          const double a = ph[0][e], b = ph[1][e], c = ph[2][e];
+          v[0] = a; v[1] = a * a; v[2] = a * b; v[3] = c; v[4] = c * c; nv6 = 5;
+        } else {
+          const double a = ph[0][e];
+          v[0] = a; v[1] = a * a; nv6 = 2;
+        }
+        const int wxq = (g == 1 && ((int)x >= o->nx || (int)z >= o->nz)) ? 3 : nv6;
+        for (int q = 0; q < nv6; ++q) {
+          if (q < wxq) S[q] += v[q];
+          F[(size_t)q * tile + ti] += v[q];
+        }
+      }
+    }
+    free(sp); free(work);
+    for (int q = 0; q < 3; ++q) free(ph[q]);
+  }
+  /* local -> global quantity index per grid */
+  static const int qmap[3][6] = {{0, 1, 4, 5, 8, 9}, {2, 3, 10, 11, 12, -1}, {6, 7, -1, -1, -1, -1}};
+  static const int nloc[3] = {6, 5, 2};
+  for (int t = 0; t < ntask; ++t) {
+    int g, s;
+    if (t < o->nyu) { g = 0; s = t; }
+    else if (t < o->nyu + o->nyv) { g = 1; s = t - o->nyu; }
+    else { g = 2; s = t - o->nyu - o->nyv; }
+    int top;
+    const int j1 = stats_j1(o, g, s, &top);
+    for (int l = 0; l < nloc[g]; ++l) {
+      const int q = qmap[g][l];
+      o->st[(size_t)q * o->nyu + s] += psum[(size_t)t * 13 + l];
+      if (j1 < 0) continue;
+      const double sg = top ? st_topsign[q] : 1.0;
+      double* dst = o->cst + qoff[q] + (size_t)j1 * tile;
+      const double* src = pfold + ((size_t)t * 6 + l) * tile;
+      for (size_t e = 0; e < tile; ++e) dst[e] += sg * src[e];
+    }
+  }
+  free(psum);
+  free(pfold);
+  o->nstat += 1;
+}
+
+int lho_stats_read(lho* o, double* out) {
+  memcpy(out, o->st, sizeof(double) * 13 * (size_t)o->nyu);
+  return o->nstat;
+}
+long long lho_cond_stats_read(lho* o, double* out, long long len) {
+  const long long n = lho_cond_stats_len(o);
+  if (out && len >= n) {
+    if (o->cst) memcpy(out, o->cst, sizeof(double) * (size_t)n);
+    else memset(out, 0, sizeof(double) * (size_t)n);
+  }
+  return n;
+}
+
 /* nonlinear alone (du arrays are scratch here): Nu_new for the current state */
 void lho_hook_nonlinear(lho* o, int krk1) {
   memset(o->du1, 0, sizeof(cplx) * (size_t)o->ncol * o->nyu);

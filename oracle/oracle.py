@@ -47,17 +47,13 @@ def lib():
         L.lho_four_to_phys.argtypes = [vp, _dp, _dp]
         L.lho_phys_to_four.argtypes = [vp, _dp, _dp]
         L.lho_cfft.argtypes = [ctypes.c_int, _dp, _dp, ctypes.c_int]
-        for name in ("lho_stats_accumulate", "lho_spectra_accumulate"):
-            if hasattr(L, name):
-                getattr(L, name).argtypes = [vp]
-        if hasattr(L, "lho_stats_read"):
-            L.lho_stats_read.argtypes = [vp, _dp]
-            L.lho_stats_read.restype = ctypes.c_int
-        if hasattr(L, "lho_cond_stats_read"):
-            L.lho_cond_stats_read.argtypes = [vp, _dp, ctypes.c_longlong]
-            L.lho_cond_stats_read.restype = ctypes.c_longlong
-        if hasattr(L, "lho_spectra_read"):
-            L.lho_spectra_read.argtypes = [vp, _dp]
+        L.lho_stats_accumulate.argtypes = [vp]
+        L.lho_spectra_accumulate.argtypes = [vp]
+        L.lho_stats_read.argtypes = [vp, _dp]
+        L.lho_stats_read.restype = ctypes.c_int
+        L.lho_cond_stats_read.argtypes = [vp, _dp, ctypes.c_longlong]
+        L.lho_cond_stats_read.restype = ctypes.c_longlong
+        L.lho_spectra_read.argtypes = [vp, _dp]
         L.lho_hook_rhs0.argtypes = [vp, ctypes.c_int]
         L.lho_get_work.argtypes = [vp, ctypes.c_int, _dp]
         L.lho_set_nonlinear_full.argtypes = [vp, _dp, _dp, _dp]
@@ -196,6 +192,23 @@ class Oracle:
     def spectra_accumulate(self):
         lib().lho_spectra_accumulate(self._h)
 
+    def stats_accumulate(self):
+        lib().lho_stats_accumulate(self._h)
+
+    def stats(self):
+        """(13, nyu) per-plane sums (rows in write_stats order, slot-indexed per grid) and
+        the number of accumulated samples."""
+        a = np.zeros((13, self.s.nyu))
+        n = lib().lho_stats_read(self._h, _p(a))
+        return a, n
+
+    def cond_stats(self):
+        """13 conditional arrays, each (nb, dnz, dnx) = Fortran (dnx, dnz, nb)."""
+        n = lib().lho_cond_stats_read(self._h, None, 0)
+        flat = np.zeros(n)
+        lib().lho_cond_stats_read(self._h, _p(flat), n)
+        return split_cond(self.s, flat)
+
     def max_div(self):
         return lib().lho_max_div(self._h)
 
@@ -213,6 +226,26 @@ class Oracle:
         sp = np.zeros(self.s.ncol, complex)
         lib().lho_phys_to_four(self._h, _p(ph), _p(sp))
         return sp
+
+
+STATS_GRID = (0, 0, 1, 1, 0, 0, 2, 2, 0, 0, 1, 1, 1)   # u, v or p grid of each stats row
+
+
+def cond_bands(s):
+    """Planes per canopy band on the u, v and p grids."""
+    n31 = s.grid.band_v[0]
+    return (n31 + 2 + s.d, n31 + 1 + s.d, n31 + s.d)
+
+
+def split_cond(s, flat):
+    dnx, dnz = s.px // s.ntilex, s.pz // s.ntilez
+    nb = cond_bands(s)
+    out, a = [], 0
+    for q in range(13):
+        n = dnx * dnz * nb[STATS_GRID[q]]
+        out.append(flat[a:a + n].reshape(nb[STATS_GRID[q]], dnz, dnx))
+        a += n
+    return out
 
 
 def cfft(x: np.ndarray, sign: int) -> np.ndarray:

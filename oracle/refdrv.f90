@@ -260,6 +260,134 @@ contains
     deallocate(sp)
   end subroutine
 
+  ! stats (stats.f90:1-108) with one rank owning every plane, and the conditional folds
+  ! (conddown1/2, condup1/2) called as stats does for its bottom and top band ranks.
+  ! Physical planes arrive as (px, pz, nplanes) over the full plane range of each grid;
+  ! st_o gets the 13 rows in write_stats order (each over the full range of its grid,
+  ! padded to nyu); cb_o/ct_o the conditional arrays of the bottom and top bands.
+  subroutine ref_stats(ntx, ntz, pu, pv, pw, pp, pvi, pwi, pwx, st_o, cb_o, ct_o) &
+      bind(C, name="ref_stats")
+    integer(c_int), value :: ntx, ntz
+    real(c_double), intent(in) :: pu(*), pv(*), pw(*), pp(*), pvi(*), pwi(*), pwx(*)
+    real(c_double), intent(out) :: st_o(*), cb_o(*), ct_o(*)
+    integer :: status(64), ierr, gx, gz, lo(3), hi(3), nyu, q, a, g, nb(3)
+    real(8), allocatable :: c(:,:,:,:)
+    gx = Ngal(1,1)
+    gz = Ngal(2,1)
+    igal = gx + 2
+    kgal = gz
+    ntilex = ntx
+    ntilez = ntz
+    dnx = gx/ntilex
+    dnz = gz/ntilez
+    lo(ugrid) = N(4,0);   hi(ugrid) = N(4,3)+1
+    lo(vgrid) = N(3,0);   hi(vgrid) = N(3,3)+1
+    lo(pgrid) = N(4,0)+1; hi(pgrid) = N(4,3)
+    nyu = hi(ugrid) - lo(ugrid) + 1
+    call plane_in(u1PL, pu, lo(ugrid), hi(ugrid))
+    call plane_in(u3PL, pw, lo(ugrid), hi(ugrid))
+    call plane_in(u2PL_itp, pvi, lo(ugrid), hi(ugrid))
+    call plane_in(u2PL, pv, lo(vgrid), hi(vgrid))
+    call plane_in(u3PL_itp, pwi, lo(vgrid), hi(vgrid))
+    call plane_in(wx, pwx, lo(vgrid), hi(vgrid))
+    call plane_in(ppPL, pp, lo(pgrid), hi(pgrid))
+    if (associated(limPL_incw)) deallocate(limPL_incw)
+    allocate(limPL_incw(3,2,0:0))
+    limPL_incw(:,1,0) = lo
+    limPL_incw(:,2,0) = hi
+    bandPL(0) = midband
+    allocate(Um(lo(ugrid):hi(ugrid)), U2m(lo(ugrid):hi(ugrid)), Wm(lo(ugrid):hi(ugrid)), &
+             W2m(lo(ugrid):hi(ugrid)), UVm(lo(ugrid):hi(ugrid)), UWm(lo(ugrid):hi(ugrid)))
+    allocate(Vm(lo(vgrid):hi(vgrid)), V2m(lo(vgrid):hi(vgrid)), VWm(lo(vgrid):hi(vgrid)), &
+             wxm(lo(vgrid):hi(vgrid)), wx2m(lo(vgrid):hi(vgrid)))
+    allocate(Pm(lo(pgrid):hi(pgrid)), P2m(lo(pgrid):hi(pgrid)))
+    Um = 0d0; U2m = 0d0; Wm = 0d0; W2m = 0d0; UVm = 0d0; UWm = 0d0
+    Vm = 0d0; V2m = 0d0; VWm = 0d0; wxm = 0d0; wx2m = 0d0; Pm = 0d0; P2m = 0d0
+    istat = 0
+    call stats(0, status, ierr)
+    st_o(1:13*nyu) = 0d0
+    call row(1, Um); call row(2, U2m); call row(3, Vm); call row(4, V2m)
+    call row(5, Wm); call row(6, W2m); call row(7, Pm); call row(8, P2m)
+    call row(9, UVm); call row(10, UWm); call row(11, VWm); call row(12, wxm); call row(13, wx2m)
+    deallocate(Um, U2m, Wm, W2m, UVm, UWm, Vm, V2m, VWm, wxm, wx2m, Pm, P2m)
+    ! conditional: band planes as limPL_incw of the first band-1 rank / last band-3 rank
+    nb(ugrid) = N(4,1) - N(4,0) + 1
+    nb(vgrid) = N(3,1) - N(3,0) + 1
+    nb(pgrid) = N(4,1) - N(4,0) - 1
+    allocate(c(dnx, dnz, 0:maxval(nb)-1, 13))
+    c = 0d0
+    limPL_incw(ugrid,1,0) = N(4,0);   limPL_incw(ugrid,2,0) = N(4,1)
+    limPL_incw(vgrid,1,0) = N(3,0);   limPL_incw(vgrid,2,0) = N(3,1)
+    limPL_incw(pgrid,1,0) = N(4,0)+1; limPL_incw(pgrid,2,0) = N(4,1)-1
+    call conddown1(c(:,:,0:nb(ugrid)-1,1), c(:,:,0:nb(ugrid)-1,2), u1PL(1,1,limPL_incw(ugrid,1,0)), ugrid, 0)
+    call conddown1(c(:,:,0:nb(vgrid)-1,3), c(:,:,0:nb(vgrid)-1,4), u2PL(1,1,limPL_incw(vgrid,1,0)), vgrid, 0)
+    call conddown1(c(:,:,0:nb(ugrid)-1,5), c(:,:,0:nb(ugrid)-1,6), u3PL(1,1,limPL_incw(ugrid,1,0)), ugrid, 0)
+    call conddown1(c(:,:,0:nb(pgrid)-1,7), c(:,:,0:nb(pgrid)-1,8), ppPL(1,1,limPL_incw(pgrid,1,0)), pgrid, 0)
+    call conddown1(c(:,:,0:nb(vgrid)-1,12), c(:,:,0:nb(vgrid)-1,13), wx(1,1,limPL_incw(vgrid,1,0)), vgrid, 0)
+    call conddown2(c(:,:,0:nb(ugrid)-1,9), u1PL(1,1,limPL_incw(ugrid,1,0)), &
+                   u2PL_itp(1,1,limPL_incw(ugrid,1,0)), ugrid, 0)
+    call conddown2(c(:,:,0:nb(ugrid)-1,10), u1PL(1,1,limPL_incw(ugrid,1,0)), &
+                   u3PL(1,1,limPL_incw(ugrid,1,0)), ugrid, 0)
+    call conddown2(c(:,:,0:nb(vgrid)-1,11), u2PL(1,1,limPL_incw(vgrid,1,0)), &
+                   u3PL_itp(1,1,limPL_incw(vgrid,1,0)), vgrid, 0)
+    call cond_out(cb_o)
+    c = 0d0
+    limPL_incw(ugrid,1,0) = N(4,2)+1; limPL_incw(ugrid,2,0) = N(4,3)+1
+    limPL_incw(vgrid,1,0) = N(3,2)+1; limPL_incw(vgrid,2,0) = N(3,3)+1
+    limPL_incw(pgrid,1,0) = N(4,2)+2; limPL_incw(pgrid,2,0) = N(4,3)
+    call condup1(c(:,:,0:nb(ugrid)-1,1), c(:,:,0:nb(ugrid)-1,2), u1PL(1,1,limPL_incw(ugrid,1,0)), 1d0, ugrid, 0)
+    call condup1(c(:,:,0:nb(vgrid)-1,3), c(:,:,0:nb(vgrid)-1,4), u2PL(1,1,limPL_incw(vgrid,1,0)), -1d0, vgrid, 0)
+    call condup1(c(:,:,0:nb(ugrid)-1,5), c(:,:,0:nb(ugrid)-1,6), u3PL(1,1,limPL_incw(ugrid,1,0)), 1d0, ugrid, 0)
+    call condup1(c(:,:,0:nb(pgrid)-1,7), c(:,:,0:nb(pgrid)-1,8), ppPL(1,1,limPL_incw(pgrid,1,0)), 1d0, pgrid, 0)
+    call condup1(c(:,:,0:nb(vgrid)-1,12), c(:,:,0:nb(vgrid)-1,13), wx(1,1,limPL_incw(vgrid,1,0)), 1d0, vgrid, 0)
+    call condup2(c(:,:,0:nb(ugrid)-1,9), u1PL(1,1,limPL_incw(ugrid,1,0)), &
+                 u2PL_itp(1,1,limPL_incw(ugrid,1,0)), 1d0, -1d0, ugrid, 0)
+    call condup2(c(:,:,0:nb(ugrid)-1,10), u1PL(1,1,limPL_incw(ugrid,1,0)), &
+                 u3PL(1,1,limPL_incw(ugrid,1,0)), 1d0, 1d0, ugrid, 0)
+    call condup2(c(:,:,0:nb(vgrid)-1,11), u2PL(1,1,limPL_incw(vgrid,1,0)), &
+                 u3PL_itp(1,1,limPL_incw(vgrid,1,0)), -1d0, 1d0, vgrid, 0)
+    call cond_out(ct_o)
+    deallocate(c)
+    deallocate(u1PL, u3PL, u2PL_itp, u2PL, u3PL_itp, wx, ppPL)
+  contains
+    subroutine row(r, x)
+      integer :: r
+      real(8) :: x(:)
+      st_o((r-1)*nyu+1:(r-1)*nyu+size(x)) = x
+    end subroutine
+    ! the quantities in write_stats order, each (dnx, dnz, nb(grid of q))
+    subroutine cond_out(o)
+      real(c_double) :: o(*)
+      integer, parameter :: gq(13) = (/ ugrid, ugrid, vgrid, vgrid, ugrid, ugrid, pgrid, &
+                                        pgrid, ugrid, ugrid, vgrid, vgrid, vgrid /)
+      integer :: n
+      a = 0
+      do q = 1, 13
+        g = gq(q)
+        n = dnx*dnz*nb(g)
+        o(a+1:a+n) = reshape(c(:,:,0:nb(g)-1,q), (/n/))
+        a = a + n
+      end do
+    end subroutine
+  end subroutine
+
+  subroutine plane_in(x, src, lo, hi)
+    real(8), pointer :: x(:,:,:)
+    real(c_double), intent(in) :: src(*)
+    integer :: lo, hi, j, k, a, gx, gz
+    gx = igal - 2
+    gz = kgal
+    allocate(x(igal, kgal, lo:hi))
+    x = 0d0
+    a = 0
+    do j = lo, hi
+      do k = 1, gz
+        x(1:gx, k, j) = src(a+1:a+gx)
+        a = a + gx
+      end do
+    end do
+  end subroutine
+
   ! ---- helpers: cfield band arrays holding copies of plain arrays (middle band only)
   subroutine mk(f, grid, inner, a)
     type(cfield) :: f(sband:eband)

@@ -12,6 +12,8 @@ inputs and the reference routines' outputs for one small channel:
 * solveP + v_corr on a seeded state                                (littleharsh.f90:557-937)
 * flowrate_corr on the (0,0) column                                (littleharsh.f90:453-485)
 * buildsp / buildspUV                                              (spectra.f90:18-56)
+* stats and the conditional folds conddown1/2, condup1/2 on physical planes made by
+  rft/cft from the seeded state                                    (stats.f90:1-253)
 * FFTPACK rft/cft forward and inverse on seeded data               (rft_buff.f, cft_buff.f)
 * a physical <-> spectral plane transform built from rft/cft with the 3/2 zero padding
 
@@ -43,6 +45,29 @@ def _dp(a):
 
 def _ip(a):
     return a.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
+
+
+def plane_to_phys(fft, spec, nx, nz):
+    """(nz, mx) spectral plane -> (pz, px) physical plane: 3/2 zero padding, cft inverse
+    along z, rft inverse along x (rft_buff.f:77-84, cft_buff.f:48-57)."""
+    px, pz, mx = 3 * nx // 2, 3 * nz // 2, nx // 2 + 1
+    wsx = np.zeros(2 * px + 18)
+    fft.rfti_(ctypes.byref(ctypes.c_int(px)), _dp(wsx))
+    wsz = np.zeros(10 * pz + 19)
+    fft.cfti_(ctypes.byref(ctypes.c_int(pz)), _dp(wsz))
+    h = nz // 2
+    pad = np.zeros((pz, mx), complex)
+    pad[:h] = spec[:h]
+    pad[pz - h:] = spec[h:]
+    pad = np.ascontiguousarray(pad)
+    fft.cft_(_dp(pad), ctypes.byref(ctypes.c_int(2 * mx)), ctypes.byref(ctypes.c_int(2)),
+             ctypes.byref(ctypes.c_int(mx)), ctypes.byref(ctypes.c_int(0)), _dp(wsz))
+    bb = np.zeros((pz, px + 2))
+    bb[:, 0:2 * mx:2] = pad.real
+    bb[:, 1:2 * mx:2] = pad.imag
+    fft.rft_(_dp(bb), ctypes.byref(ctypes.c_int(px + 2)), ctypes.byref(ctypes.c_int(pz)),
+             ctypes.byref(ctypes.c_int(0)), _dp(wsx))
+    return bb[:, :px].copy()
 
 
 def main():
@@ -141,6 +166,31 @@ def main():
             spv = np.zeros(nj * ncol)
             ref.ref_buildsp(ctypes.c_int(grid), _dp(cp(a)), _dp(cp(b)), _dp(spv))
             out[f"spec_{name}"] = spv.reshape(ncol, nj)
+        # statistics: physical planes of u, w, v_itp (u grid), v, w_itp, wx (v grid), p,
+        # made from the seeded state with the reference's rft/cft
+        ddthetavi = 1.0
+        kz = np.repeat(k1z, mx)
+        vi = np.zeros_like(u1)
+        vi[:, 1:nyu - 1] = 0.5 * (u2[:, :nyu - 2] + u2[:, 1:nyu - 1])
+        wi = 0.5 * (u3[:, :nyv] + u3[:, 1:nyv + 1])
+        wxs = ddthetavi * dv[None, :] * (u3[:, 1:nyv + 1] - u3[:, :nyv]) - 1j * kz[:, None] * u2
+        nxc, nzc = c["nx"], c["nz"]
+        planes = {nm: np.stack([plane_to_phys(fft, a[:, j].reshape(nzc, mx), nxc, nzc)
+                                for j in range(a.shape[1])])
+                  for nm, a in (("u", u1), ("w", u3), ("vi", vi), ("v", u2), ("wi", wi),
+                                ("wx", wxs), ("p", p))}
+        pxc, pzc = 3 * nxc // 2, 3 * nzc // 2
+        dnx, dnz = pxc // c["ntilex"], pzc // c["ntilez"]
+        n41 = b1 + 1
+        nb = {"u": n41 - n40 + 1, "v": b1 - n30 + 1, "p": n41 - n40 - 1}
+        gq = "uuvvuuppuuvvv"
+        clen = sum(dnx * dnz * nb[g] for g in gq)
+        st = np.zeros(13 * nyu)
+        cb, ct = np.zeros(clen), np.zeros(clen)
+        ref.ref_stats(ctypes.c_int(c["ntilex"]), ctypes.c_int(c["ntilez"]),
+                      *[_dp(np.ascontiguousarray(planes[k])) for k in ("u", "v", "w", "p", "vi", "wi", "wx")],
+                      _dp(st), _dp(cb), _dp(ct))
+        out.update(stats=st.reshape(13, nyu), cstats_bot=cb, cstats_top=ct)
     # ---- FFTPACK
     rng = np.random.default_rng(7)
     for n in (12, 24, 48, 96):

@@ -103,3 +103,30 @@ def test_constant_flow_rate_tiny(require_gpu):
     qx1 = 2.0 * (8.0 / 15.0) * dt
     bound = 9 * 2.2e-16 * 120.0 / qx1
     assert abs(gm - om) < 1e3 * bound
+
+
+def test_one_substep_production(require_gpu):
+    """The 384 x 289 x 384 channel (Ngal 576, 48 x 24 posts): one substep of each kind."""
+    from littleharsh_amd.channel import config_production
+    s = config_production()
+    g, o, _ = make_pair(s, seed=5)
+    dt = 1e-4
+    for k in (1, 2):
+        g.substep(k, dt)
+        o.substep(k, dt)
+        errs = report(f"production substep{k}", g.get_state(), o.get_state())
+        assert max(errs.values()) < TOL
+
+
+def test_200_steps_small(require_gpu):
+    """64 x 97 x 64 canopy channel, 200 RK3 steps replayed from the HIP graph."""
+    s = small()
+    g, o, _ = make_pair(s, seed=11)
+    dt = 2e-4
+    for n in range(200):
+        g.step(dt)
+        o.step(dt)
+        if (n + 1) % 50 == 0:
+            errs = report(f"small step {n + 1}", g.get_state(), o.get_state())
+            assert max(errs.values()) < TOL
+    assert abs(g.scalars()["qx"] - o.flowrate()) < 1e-10 * abs(o.flowrate())

@@ -172,3 +172,27 @@ def test_plane_transforms_against_fftpack(g, setup):
     assert rel(sp, g["plane_spec"]) < 1e-14
     back = o.four_to_phys(g["plane_spec"].ravel())
     assert rel(back, g["plane_back"]) < 1e-13
+
+
+def test_statistics_against_stats_f90(g, setup):
+    """Plain and conditional statistics (stats.f90:1-253) of the seeded state, against the
+    reference's stats/conddown/condup run on planes built with rft/cft.  The conditional
+    arrays are the sum of the bottom and the mirrored top band, as recvXC writes them
+    (stats.f90:474-508)."""
+    from oracle.oracle import split_cond
+    o = Oracle(setup)
+    o.set_state(g["in_u1"], g["in_u2"], g["in_u3"], g["in_p"])
+    o.stats_accumulate()
+    st, n = o.stats()
+    assert n == 1
+    ref = g["stats"]
+    npts = setup.px * setup.pz
+    for q in range(13):
+        # sums of O(1) values over npts points: compare against the sum of magnitudes
+        scale = npts * max(np.abs(ref[q]).max() / npts, 1.0)
+        assert np.abs(st[q] - ref[q]).max() < 1e-12 * scale, q
+    cond = o.cond_stats()
+    both = split_cond(setup, g["cstats_bot"] + g["cstats_top"])
+    for q in range(13):
+        scale = setup.ntilex * setup.ntilez * max(np.abs(both[q]).max(), 1.0)
+        assert np.abs(cond[q] - both[q]).max() < 1e-12 * scale, q
