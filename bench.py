@@ -47,6 +47,27 @@ def load_traffic(name):
         return None
 
 
+def calibrate_copy(nbytes=1 << 30, reps=3):
+    """Device-to-device copies of a known size through the HIP runtime's blit kernel: the
+    reference point that scales the FETCH_SIZE / WRITE_SIZE counters to bytes."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    if hip.hipMalloc(ctypes.byref(a), ctypes.c_size_t(nbytes)) or \
+            hip.hipMalloc(ctypes.byref(b), ctypes.c_size_t(nbytes)):
+        raise RuntimeError("hipMalloc failed")
+    hip.hipMemset(a, 1, ctypes.c_size_t(nbytes))
+    hip.hipDeviceSynchronize()
+    for _ in range(reps):
+        if hip.hipMemcpy(b, a, ctypes.c_size_t(nbytes), 3):   # hipMemcpyDeviceToDevice
+            raise RuntimeError("hipMemcpy failed")
+    hip.hipDeviceSynchronize()
+    hip.hipFree(a)
+    hip.hipFree(b)
+    print(json.dumps({"calibration": "device copy", "bytes_read_per_copy": nbytes,
+                      "bytes_written_per_copy": nbytes, "copies": reps}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -56,7 +77,12 @@ def main():
     ap.add_argument("--workload", choices=["production", "small"], default="production")
     ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only run 1 GiB device copies (counter calibration for profiles)")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        calibrate_copy()
+        return
     if args.gpus != 1:
         print(f"bench.py: this build runs on exactly one GPU; --gpus {args.gpus} is not supported",
               file=sys.stderr)
