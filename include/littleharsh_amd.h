@@ -88,7 +88,9 @@ int lh_get_nonlinear(lh_handle* h, double* nu1, double* nu2, double* nu3);
 
 /* One RK3 substep kRK in 1..3 with time step dt: RHS0_u1/u2/u3, nonlinear, solveU/V/U,
  * meanflow_ctU (flag_ctpress == 0), solveP, v_corr (littleharsh.f90:166-188).
- * dt <= 0 uses the device CFL time step of the last lh_cfl_dt / lh_step_cfl. */
+ * dt <= 0 uses the device CFL time step of the last lh_cfl_dt / lh_step_cfl.  With
+ * kRK = 3 and flag_ctpress == 1 it also evaluates the flow rate (meanflow_ctP,
+ * littleharsh.f90:193-197), so three lh_substep calls do what one lh_step does. */
 int lh_substep(lh_handle* h, int krk, double dt);
 
 /* One full RK3 step (three substeps) followed by meanflow_ctP when flag_ctpress == 1

@@ -48,7 +48,6 @@ struct lh_handle {
   double2 *u1 = nullptr, *u2 = nullptr, *u3 = nullptr, *p = nullptr;
   double2 *nu1 = nullptr, *nu2 = nullptr, *nu3 = nullptr;
   double2 *r1 = nullptr, *r2 = nullptr, *r3 = nullptr, *psi = nullptr, *tipl = nullptr;
-  double2* ph = nullptr;                       // product spectra [6][nyu-2][ncol]
   void *zu = nullptr, *zw = nullptr, *zv = nullptr, *zp = nullptr;
   double *dg1 = nullptr, *dg2 = nullptr, *dg3 = nullptr;
   double *ayu = nullptr, *ayv = nullptr, *ctwork = nullptr;
@@ -430,7 +429,6 @@ int lh_create(const lh_config* c, lh_handle** out) {
   HIPCHK(h, dalloc(&h->nu1, nu)); HIPCHK(h, dalloc(&h->nu3, nu)); HIPCHK(h, dalloc(&h->nu2, nv));
   HIPCHK(h, dalloc(&h->r1, nu)); HIPCHK(h, dalloc(&h->r3, nu)); HIPCHK(h, dalloc(&h->r2, nv));
   HIPCHK(h, dalloc(&h->tipl, 6 * (size_t)dm.ncol));
-  HIPCHK(h, dalloc(&h->ph, 6 * (size_t)nlev * dm.ncol));
   HIPCHK(h, dalloc(&h->dg1, np)); HIPCHK(h, dalloc(&h->dg2, np)); HIPCHK(h, dalloc(&h->dg3, np));
   HIPCHK(h, dalloc(&h->ayu, 3 * dm.nyu)); HIPCHK(h, dalloc(&h->ayv, 3 * dm.nyv));
   HIPCHK(h, dalloc(&h->ctwork, 4 * (dm.ny + 2)));
@@ -568,7 +566,7 @@ int lh_destroy(lh_handle* h) {
   for (auto e : h->evpool) (void)hipEventDestroy(e);
   void* ptrs[] = {h->yu, h->yv, h->dthdyu, h->dthdyv, h->dyu2i, h->dyv2i, h->k1x, h->k2x, h->k1z,
                   h->k2z, h->scal, h->u1, h->u2, h->u3, h->p, h->nu1, h->nu2, h->nu3, h->r1,
-                  h->r2, h->r3, h->psi, h->tipl, h->ph, h->zu, h->zw, h->zv, h->zp, h->dg1,
+                  h->r2, h->r3, h->psi, h->tipl, h->zu, h->zw, h->zv, h->zp, h->dg1,
                   h->dg2, h->dg3, h->ayu, h->ayv, h->ctwork, h->masks, h->ibu_planes,
                   h->ibv_planes, h->ibent, h->st_psum, h->st_pfold, h->st_acc,
                   h->cst_acc, h->sp_acc};
@@ -665,6 +663,10 @@ int lh_substep(lh_handle* h, int krk, double dt) {
   if (krk < 1 || krk > 3) return fail(h, LH_ERR_ARG, "kRK must be 1, 2 or 3");
   if (dt > 0.0) hipLaunchKernelGGL(k_set_scalar, dim3(1), dim3(1), 0, h->st, h->scal, (int)S_DT, dt);
   issue_substep(h, krk - 1, false);
+  // meanflow_ctP follows the last substep of a step (littleharsh.f90:193-197)
+  if (krk == 3 && h->flag_ctpress == 1)
+    hipLaunchKernelGGL(k_flowrate, dim3(1), dim3(64), 0, h->st, h->u1, h->yv, h->scal, h->dm.ny,
+                       h->dm.d, h->dm.ncol);
   HIPCHK(h, hipGetLastError());
   HIPCHK(h, hipStreamSynchronize(h->st));
   tflush(h);

@@ -205,3 +205,25 @@ def test_step_is_divergence_free():
     o.set_state(*laminar_plus_noise(s, seed=4))
     o.step(2e-4)
     assert o.max_div() < 1e-10
+
+
+def test_cfl_maximum_matches_numpy():
+    """max over centres and collocation points of |u|/dx + |v_c| dthdyu + |w|/dz, v_c the
+    average of the faces below and above (the CFL restatement; dtc/dti are declared at
+    declaration.f90:51 but set by the absent nonlinear routine)."""
+    s = tiny()
+    u, v, w, p = random_state(s, seed=12)
+    o = Oracle(s)
+    o.set_state(u, v, w, p)
+    o.nonlinear(1)
+    P = (s.pz, s.px)
+    dxi = s.px * s.alp / (2 * np.pi)
+    dzi = s.pz * s.bet / (2 * np.pi)
+    m = 0.0
+    for r in range(1, s.nyu - 1):
+        U = to_phys(s, u[:, r].reshape(s.nz, s.mx), P)
+        W = to_phys(s, w[:, r].reshape(s.nz, s.mx), P)
+        V = 0.5 * (to_phys(s, v[:, r - 1].reshape(s.nz, s.mx), P)
+                   + to_phys(s, v[:, r].reshape(s.nz, s.mx), P))
+        m = max(m, float((np.abs(U) * dxi + np.abs(V) * s.grid.dthdyu[r] + np.abs(W) * dzi).max()))
+    assert abs(o.scalars()["cflmax"] - m) < 1e-12 * m
