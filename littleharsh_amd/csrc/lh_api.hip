@@ -146,7 +146,26 @@ static bool supported_len(int n) {
 // ---------------------------------------------------------------------------------------
 // templated launchers, dispatched on the transform lengths
 // ---------------------------------------------------------------------------------------
-template <int N> constexpr int zlines() { return N >= 384 ? 4 : (Plan<N>::T >= 8 ? 16 : 32); }
+#ifndef LH_ZLINES_LONG
+#define LH_ZLINES_LONG 4
+#endif
+#ifndef LH_ZASM_COLS
+#define LH_ZASM_COLS 8
+#endif
+#ifndef LH_VEL_SPLIT
+#define LH_VEL_SPLIT 0
+#endif
+#ifndef LH_COL_U
+#define LH_COL_U 8
+#endif
+// threads per block of the column kernels: one column per thread, so small blocks spread
+// the 74k columns of the production grid evenly over the 256 CUs
+#ifndef LH_COL_BS
+#define LH_COL_BS 64
+#endif
+template <int N> constexpr int zlines() {
+  return N >= 384 ? LH_ZLINES_LONG : (Plan<N>::T >= 8 ? 16 : 32);
+}
 
 template <int PZ, typename TR>
 static void launch_zinv_t(lh_handle* h, const ZJobs& jobs, int njobs, int maxn, const void* tw) {
@@ -164,14 +183,35 @@ static void launch_zfwd_t(lh_handle* h, const ZJobs& jobs, int njobs, int maxn, 
   hipLaunchKernelGGL((k_zfwd<PZ, L, TR>), grid, dim3(L * Plan<PZ>::T), sh, h->st, jobs, h->dm,
                      (const cx<TR>*)tw);
 }
+template <int PZ, typename TR>
+static void attr_z_t(int* dummy) {
+  constexpr int L = zlines<PZ>();
+  (void)dummy;
+  const int sh = (int)(L * (PZ + 1) * sizeof(cx<TR>));
+  (void)hipFuncSetAttribute((const void*)k_zinv<PZ, L, TR>,
+                            hipFuncAttributeMaxDynamicSharedMemorySize, sh);
+  (void)hipFuncSetAttribute((const void*)k_zfwd<PZ, L, TR>,
+                            hipFuncAttributeMaxDynamicSharedMemorySize, sh);
+}
+#ifndef LH_XRAW_LC
+#define LH_XRAW_LC 16
+#endif
 template <int PX, typename TR>
 static void launch_xraw_t(lh_handle* h, const XRAW& a, const void* tw) {
-  dim3 grid(h->dm.pz / 2, h->dm.nyu - 2);
   size_t sh = (size_t)6 * (PX + 1) * sizeof(cx<TR>);
-  hipLaunchKernelGGL((k_xraw<PX, TR>), grid, dim3(6 * Plan<PX>::T), sh, h->st, a, h->dm,
-                     (const cx<TR>*)tw);
+  if (LH_XRAW_LC > 1) {
+    sh += (size_t)Plan<PX>::template TwL<cx<TR>>::SIZE * sizeof(cx<TR>);
+    const int nlev = h->dm.nyu - 2, lc = LH_XRAW_LC;
+    dim3 grid(h->dm.pz / 2, (nlev + lc - 1) / lc);
+    hipLaunchKernelGGL((k_xraw2<PX, TR>), grid, dim3(6 * Plan<PX>::T), sh, h->st, a, h->dm,
+                       (const cx<TR>*)tw, lc);
+  } else {
+    dim3 grid(h->dm.pz / 2, h->dm.nyu - 2);
+    hipLaunchKernelGGL((k_xraw<PX, TR>), grid, dim3(6 * Plan<PX>::T), sh, h->st, a, h->dm,
+                       (const cx<TR>*)tw);
+  }
 }
-constexpr int kZasmCols = 4;
+constexpr int kZasmCols = LH_ZASM_COLS;
 template <int PZ, typename TR>
 static void launch_zasm_t(lh_handle* h, const ZASM& a, const void* tw) {
   constexpr int L = kZasmCols;
@@ -333,10 +373,23 @@ static void issue_substep(lh_handle* h, int krk /*0-based*/, bool cfl_mode) {
                dm.d, dm.ncol};
     // in: rhs u, w, v, p, LU factors (8 B each); out: u, w, v, p
     const double by = spl * (2.0 * nuvw + 2.0 * dm.nyp) + 3.0 * 8.0 * dm.ncol * (double)dm.nyp;
+#if LH_VEL_SPLIT
+    // one field per launch: the forward-sweep rows of one field (343 MB at 384x289x384)
+    // are re-read in reverse order while most of them still sit in the memory-side cache
+    for (int f = 0; f < 3; ++f)
+      TIMED(h, LH_K_VEL, 2.0 * (f == 2 ? dm.nyv : dm.nyu) * spl,
+            hipLaunchKernelGGL(k_vel<LH_COL_U>, dim3((dm.ncol + LH_COL_BS - 1) / LH_COL_BS, 1),
+                               dim3(LH_COL_BS), 0, h->st, a,
+                               dm, f));
+#else
     TIMED(h, LH_K_VEL, 2.0 * nuvw * spl,
-          hipLaunchKernelGGL(k_vel<8>, dim3((dm.ncol + 255) / 256, 3), dim3(256), 0, h->st, a, dm));
+          hipLaunchKernelGGL(k_vel<LH_COL_U>, dim3((dm.ncol + LH_COL_BS - 1) / LH_COL_BS, 3),
+                             dim3(LH_COL_BS), 0, h->st, a,
+                             dm, 0));
+#endif
     TIMED(h, LH_K_PRES, by - 2.0 * nuvw * spl + 2.0 * nuvw * spl,
-          hipLaunchKernelGGL(k_pres<8>, dim3((dm.ncol + 255) / 256), dim3(256), 0, h->st, a, dm));
+          hipLaunchKernelGGL(k_pres<LH_COL_U>, dim3((dm.ncol + LH_COL_BS - 1) / LH_COL_BS),
+                             dim3(LH_COL_BS), 0, h->st, a, dm));
   }
 }
 
@@ -397,6 +450,8 @@ int lh_create(const lh_config* c, lh_handle** out) {
     int dummy = 0;
     LH_DISPATCH_LEN(c->pz, attr_zasm_t, double, &dummy)
     LH_DISPATCH_LEN(c->pz, attr_zasm_t, float, &dummy)
+    LH_DISPATCH_LEN(c->pz, attr_z_t, double, &dummy)
+    LH_DISPATCH_LEN(c->pz, attr_z_t, float, &dummy)
   }
   const size_t nu = (size_t)dm.nyu * dm.ncol, nv = (size_t)dm.nyv * dm.ncol,
                np = (size_t)dm.nyp * dm.ncol;

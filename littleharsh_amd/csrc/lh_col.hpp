@@ -103,12 +103,14 @@ __device__ void ctu_column(const CTU& a) {
 }
 
 // velocity Thomas sweeps, one (column, component) per thread
+// (blockIdx.y + f0 selects u1, u3 or u2)
 template <int U>
-__global__ void __launch_bounds__(256) k_vel(COL a, Dims dm) {
+__global__ void __launch_bounds__(256) k_vel(COL a, Dims dm, int f0) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= dm.ncol) return;
-  if (blockIdx.y == 0) thomas_u<U>(a.r1, a.u1, a.ayu, dm.nyu, dm.ncol, c);
-  else if (blockIdx.y == 1) thomas_u<U>(a.r3, a.u3, a.ayu, dm.nyu, dm.ncol, c);
+  const int f = blockIdx.y + f0;
+  if (f == 0) thomas_u<U>(a.r1, a.u1, a.ayu, dm.nyu, dm.ncol, c);
+  else if (f == 1) thomas_u<U>(a.r3, a.u3, a.ayu, dm.nyu, dm.ncol, c);
   else thomas_u<U>(a.r2, a.u2, a.ayv, dm.nyv, dm.ncol, c);
 }
 

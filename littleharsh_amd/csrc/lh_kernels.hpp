@@ -241,6 +241,9 @@ struct PREP {
   int nyu, nyv;
 };
 
+// LU_build + LU_dec of the two shared y matrices (tridLU_3D_IB.f90:427-493, 601-640), one
+// thread per grid.  The recurrence runs in registers (the previous row's scaled upper
+// entry is the only carried value) with the operations in the order of LU_dec.
 __global__ void k_prep(PREP a) {
   const int t = threadIdx.x;
   if (t > 1) return;
@@ -248,23 +251,30 @@ __global__ void k_prep(PREP a) {
   const double* dd = t == 0 ? a.dyu2i : a.dyv2i;
   double* ay = t == 0 ? a.ayu : a.ayv;
   const double beta = a.bre * a.scal[S_DT];
+  // row 0: (0, 1, gw(1)) on the u grid, Dirichlet (0, 1, 0) on the v grid
+  double up = t == 0 ? a.gw0 : 0.0;
+  double di = 1.0 / 1.0;
   ay[0] = 0.0;
-  ay[1] = 1.0;
-  ay[2] = t == 0 ? a.gw0 : 0.0;
-  for (int r = 1; r < nj - 1; ++r) {
-    ay[3 * r + 0] = -beta * dd[3 * r + 0];
-    ay[3 * r + 1] = 1.0 - beta * dd[3 * r + 1];
-    ay[3 * r + 2] = -beta * dd[3 * r + 2];
+  ay[1] = di;
+  for (int r = 1; r < nj; ++r) {
+    double lo, dg, upr;
+    if (r < nj - 1) {
+      lo = -beta * dd[3 * r + 0];
+      dg = 1.0 - beta * dd[3 * r + 1];
+      upr = -beta * dd[3 * r + 2];
+    } else {                      // last row: (gw(2), 1, 0) on the u grid, (0, 1, 0) on v
+      lo = t == 0 ? a.gw1 : 0.0;
+      dg = 1.0;
+      upr = 0.0;
+    }
+    const double a3 = up * di;    // upper of row r-1 scaled by its 1/diag
+    ay[3 * (r - 1) + 2] = a3;
+    di = 1.0 / (dg - lo * a3);
+    ay[3 * r + 0] = lo;
+    ay[3 * r + 1] = di;
+    up = upr;
   }
-  const int r = nj - 1;
-  ay[3 * r + 0] = t == 0 ? a.gw1 : 0.0;
-  ay[3 * r + 1] = 1.0;
-  ay[3 * r + 2] = 0.0;
-  ay[1] = 1.0 / ay[1];
-  for (int q = 1; q < nj; ++q) {
-    ay[3 * (q - 1) + 2] = ay[3 * (q - 1) + 2] * ay[3 * (q - 1) + 1];
-    ay[3 * q + 1] = 1.0 / (ay[3 * q + 1] - ay[3 * q + 0] * ay[3 * (q - 1) + 2]);
-  }
+  ay[3 * (nj - 1) + 2] = up;
 }
 
 // flowrate_corr + LUsol0 (littleharsh.f90:453-485; tridLU_3D_IB.f90:398-424, 573-599,

@@ -21,6 +21,10 @@
 #include <type_traits>
 #include "lh_kernels.hpp"
 
+#ifndef LH_ZASM_ORDER
+#define LH_ZASM_ORDER 0
+#endif
+
 namespace lh {
 
 // Nonlinear x pass: one row pair and one level j (centre j and face j) per block.
@@ -149,6 +153,176 @@ __global__ void __launch_bounds__(6 * Plan<PX>::T)
   }
 }
 
+// Sliding-window x pass: one row pair and a chunk of `lc` consecutive levels per block.
+// Each thread owns fixed x positions of every line, so the physical U(j), W(j), V(j-1) of
+// those positions stay in registers from one level to the next and a level costs three
+// inverse transforms instead of six.  Levels are taken two at a time, so that all six
+// line groups of the block transform together: six new lines (U, W at centres j+1, j+2 and
+// V at faces j, j+1) are inverse-transformed, then the six products of level j and those
+// of level j+1 are forward-transformed in turn.
+#ifndef LH_XRAW2_WPE
+#define LH_XRAW2_WPE 2
+#endif
+template <int PX, typename TR>
+__global__ void __launch_bounds__(6 * Plan<PX>::T) __attribute__((amdgpu_waves_per_eu(LH_XRAW2_WPE)))
+    k_xraw2(XRAW a, Dims dm, const cx<TR>* __restrict__ tw, int lc) {
+  using C = cx<TR>;
+  constexpr int T = Plan<PX>::T;
+  constexpr int LS = PX + 1;
+  constexpr int NT = 6 * T;
+  constexpr int NX = (PX + NT - 1) / NT;      // x positions per thread
+  extern __shared__ char smem_raw[];
+  C* lds = reinterpret_cast<C*>(smem_raw);
+  __shared__ double red[16];
+  const int tid = threadIdx.x;
+  const int z0 = 2 * blockIdx.x;
+  const int nlev = dm.nyu - 2;
+  const int l0 = blockIdx.y * lc;
+  const int l1 = min(l0 + lc, nlev);
+  const int mx = dm.mx;
+  const size_t slot = (size_t)dm.pz * mx;
+  const C* zu = reinterpret_cast<const C*>(a.zu);
+  const C* zw = reinterpret_cast<const C*>(a.zw);
+  const C* zv = reinterpret_cast<const C*>(a.zv);
+  C* zp = reinterpret_cast<C*>(a.zp);
+  const int ln = tid / T, t = tid % T;
+  // twiddles in an LDS table after the six lines (registers hold the level window)
+  typename Plan<PX>::template TwL<C> w;
+  w.init(lds + 6 * LS, t, tw, ln == 0);
+
+  // packed rows (z0, z0+1) of n Z slots -> line slots 0..n-1
+  auto load_lines = [&](const C* const* src, int n) {
+    constexpr int M = (6 * (PX / 3 + 1) + NT - 1) / NT;
+    C va[M], vb[M];
+#pragma unroll
+    for (int q = 0; q < M; ++q) {
+      const int e = tid + q * NT;
+      if (e < n * mx) {
+        const C* A = src[e / mx] + (size_t)z0 * mx;
+        va[q] = A[e % mx];
+        vb[q] = A[mx + e % mx];
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < M; ++q) {
+      const int e = tid + q * NT;
+      if (e < n * mx) {
+        const int f = e / mx, k = e % mx;
+        C* line = lds + f * LS;
+        if (k == 0) {
+          line[0] = mk<C>(va[q].x, vb[q].x);
+        } else {
+          line[k] = mk<C>(va[q].x - vb[q].y, va[q].y + vb[q].x);
+          line[PX - k] = mk<C>(va[q].x + vb[q].y, -va[q].y + vb[q].x);
+        }
+      }
+    }
+    const int nzr = PX - 2 * mx + 1;
+    for (int e = tid; e < n * nzr; e += NT) lds[(e / nzr) * LS + mx + e % nzr] = mk<C>(0, 0);
+    lds_bar();
+  };
+  auto read_lines = [&](int s0, C (*dst)[NX]) {
+#pragma unroll
+    for (int f = 0; f < 3; ++f)
+#pragma unroll
+      for (int m = 0; m < NX; ++m) {
+        const int x = tid + m * NT;
+        if (x < PX) dst[f][m] = lds[(s0 + f) * LS + x];
+      }
+  };
+  double cmax = 0.0;
+  const TR half = (TR)0.5;
+  // products of level l from old (U(j), W(j), V(j-1)) and new (U(j+1), W(j+1), V(j)) values,
+  // forward transforms, and the six product rows out
+  auto level_out = [&](int l, const C (*o)[NX], const C (*nw)[NX]) {
+    const double dyi = a.dthdyu[l + 1];
+#pragma unroll
+    for (int m = 0; m < NX; ++m) {
+      const int x = tid + m * NT;
+      if (x < PX) {
+        const C U = o[0][m], W = o[1][m], Vb = o[2][m];
+        const C Ua = nw[0][m], Wa = nw[1][m], Vt = nw[2][m];
+        const TR vc0 = half * (Vb.x + Vt.x), vc1 = half * (Vb.y + Vt.y);
+        const TR uf0 = half * (U.x + Ua.x), uf1 = half * (U.y + Ua.y);
+        const TR wf0 = half * (W.x + Wa.x), wf1 = half * (W.y + Wa.y);
+        lds[0 * LS + x] = mk<C>(U.x * U.x, U.y * U.y);
+        lds[1 * LS + x] = mk<C>(U.x * W.x, U.y * W.y);
+        lds[2 * LS + x] = mk<C>(W.x * W.x, W.y * W.y);
+        lds[3 * LS + x] = mk<C>(vc0 * vc0, vc1 * vc1);
+        lds[4 * LS + x] = mk<C>(uf0 * Vt.x, uf1 * Vt.y);
+        lds[5 * LS + x] = mk<C>(Vt.x * wf0, Vt.y * wf1);
+        if (a.want_cfl) {
+          const double s0 = fabs((double)U.x) * dm.dxi + fabs((double)vc0) * dyi + fabs((double)W.x) * dm.dzi;
+          const double s1 = fabs((double)U.y) * dm.dxi + fabs((double)vc1) * dyi + fabs((double)W.y) * dm.dzi;
+          cmax = fmax(cmax, fmax(s0, s1));
+        }
+      }
+    }
+    lds_bar();
+    Plan<PX>::template run<false>(lds + ln * LS, t, w);
+    for (int e = tid; e < 6 * mx; e += NT) {
+      const int q = e / mx, k = e % mx;
+      C va, vb;
+      unpack_rows<PX>(lds + q * LS, k, va, vb);
+      C* dst = zp + ((size_t)q * nlev + l) * slot + (size_t)z0 * mx;
+      dst[k] = va;
+      dst[mx + k] = vb;
+    }
+    lds_bar();
+  };
+
+  C c0[3][NX], n1[3][NX], n2[3][NX];
+  {
+    const int su = l0 + 1;
+    const C* s[3] = {zu + su * slot, zw + su * slot, zv + (su - 1) * slot};
+    load_lines(s, 3);
+    Plan<PX>::template run<true>(lds + ln * LS, t, w);
+    read_lines(0, c0);
+    lds_bar();
+  }
+  int l = l0;
+  for (; l + 1 < l1; l += 2) {
+    const int su = l + 1;
+    const C* s[6] = {zu + (su + 1) * slot, zw + (su + 1) * slot, zv + su * slot,
+                     zu + (su + 2) * slot, zw + (su + 2) * slot, zv + (su + 1) * slot};
+    load_lines(s, 6);
+    Plan<PX>::template run<true>(lds + ln * LS, t, w);
+    read_lines(0, n1);
+    read_lines(3, n2);
+    lds_bar();
+    level_out(l, c0, n1);
+    level_out(l + 1, n1, n2);
+#pragma unroll
+    for (int f = 0; f < 3; ++f)
+#pragma unroll
+      for (int m = 0; m < NX; ++m) c0[f][m] = n2[f][m];
+  }
+  if (l < l1) {
+    const int su = l + 1;
+    const C* s[3] = {zu + (su + 1) * slot, zw + (su + 1) * slot, zv + su * slot};
+    load_lines(s, 3);
+    Plan<PX>::template run<true>(lds + ln * LS, t, w);
+    read_lines(0, n1);
+    lds_bar();
+    level_out(l, c0, n1);
+  }
+  if (a.want_cfl) {
+    const int wv = tid / 64, lane = tid % 64;
+    const int act = min(64, (int)blockDim.x - wv * 64);
+    for (int off = 32; off > 0; off >>= 1) {
+      const double o = __shfl_down(cmax, off, 64);
+      if (lane + off < act) cmax = fmax(cmax, o);
+    }
+    if (lane == 0) red[wv] = cmax;
+    __syncthreads();
+    if (tid == 0) {
+      double m = 0.0;
+      for (int q = 0; q < (int)((blockDim.x + 63) / 64); ++q) m = fmax(m, red[q]);
+      atomic_max_pos(a.scal + S_CFLMAX, m);
+    }
+  }
+}
+
 // Forward z transform of the combined rows + RHS0 + nonlinear + rhs of solveU/solveV:
 // one block per (L consecutive kx columns, level, field).  Block ids are remapped so that
 // the two blocks sharing a 128-byte line of every [plane][column] row run on one XCD.
@@ -179,10 +353,19 @@ __global__ void __launch_bounds__(2 * L * Plan<PZ>::T)
   int bid = blockIdx.x;
   const int per = nblk / 8;
   if (bid < per * 8) bid = (bid % 8) * per + bid / 8;
+#if LH_ZASM_ORDER == 1
+  // fields of one (columns, level) and neighbouring levels adjacent: shared product and
+  // state rows are re-read from L2
+  const int f = bid % 3;                      // 0: u1, 1: u3, 2: v
+  const int rest = bid / 3;
+  const int l = rest % nlev;
+  const int ib = rest / nlev;
+#else
   const int ib = bid % a.nib;
   const int rest = bid / a.nib;
   const int l = rest % nlev;
   const int f = rest / nlev;                  // 0: u1, 1: u3, 2: v
+#endif
   const int i0 = ib * L;
   const int mx = dm.mx, nz = dm.nz, h = nz / 2, ncol = dm.ncol;
   const size_t slot = (size_t)PZ * mx;
