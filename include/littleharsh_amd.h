@@ -152,13 +152,14 @@ int lh_spectra_read(lh_handle* h, double* out, int reset);
 #define LH_K_ZINV 0   /* inverse z transforms of u, w, v                  */
 #define LH_K_XNL 1    /* x transforms + products                          */
 #define LH_K_ZFWD 2   /* forward z transforms of the products             */
-#define LH_K_ASM 3    /* RHS0 + nonlinear assembly + rhs of the solves    */
+#define LH_K_ASM 3    /* forward z transforms of the products -> Nu       */
 #define LH_K_IBZ 4    /* canopy-plane z transforms                         */
 #define LH_K_XIB 5    /* canopy-plane x transforms + forcing               */
 #define LH_K_VEL 6    /* velocity Thomas sweeps                            */
 #define LH_K_PRES 7   /* pressure solve + v_corr                          */
 #define LH_K_MISC 8   /* small kernels: LU factors, mean flow, scalars     */
-#define LH_K_COUNT 9
+#define LH_K_RHS 9    /* RHS0 + nonlinear + rhs of the solves, per column  */
+#define LH_K_COUNT 10
 int lh_timing_enable(lh_handle* h, int enable);
 int lh_timing_read(lh_handle* h, int kernel, double* total_ms, long long* launches,
                    double* bytes_per_launch);

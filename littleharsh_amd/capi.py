@@ -132,4 +132,5 @@ EXPORTED = [
     "lh_timing_enable", "lh_timing_reset", "lh_timing_read",
 ]
 
-KERNEL_NAMES = ["zinv", "xpass_nl", "zfwd", "assemble", "ib_z", "ib_x", "vel", "pressure", "misc"]
+KERNEL_NAMES = ["zinv", "xpass_nl", "zfwd", "assemble", "ib_z", "ib_x", "vel", "pressure", "misc",
+                "rhs"]

@@ -2,11 +2,11 @@
 //
 // The handle owns one HIP stream, the device-resident state in [plane][column] layout and
 // all work buffers.  lh_substep issues the kernels of one RK3 substep:
-//   NL stage   : k_zinv (u,w,v) -> k_xraw (x transforms, six products, CFL max)
-//                -> k_zasm (A/B recombination, forward z transforms, RHS0 + nonlinear +
-//                   rhs of solveU/V)
-//   IB stage   : k_zinv (rhs and Lu_y on canopy planes) -> k_xpass_ib -> k_zfwd
-//   column     : k_prep (LU of the shared velocity matrices) -> k_vel (Thomas u, w, v)
+//   NL stage   : k_zinv (u,w,v) -> k_xraw2 (x transforms, six products, CFL max)
+//                -> k_zasm (A/B recombination, forward z transforms -> Nu_new)
+//   IB stage   : k_rhs_ib (rhs and Lu_y on canopy planes) -> k_zinv -> k_xpass_ib -> k_zfwd
+//   column     : k_prep (LU of the shared velocity matrices) -> k_vel_rhs (RHS0 + rhs of
+//                solveU/V on the fly, Thomas u, w, v)
 //                -> k_pres (constant-flow-rate correction, solveP, v_corr)
 // Statistics and spectra (lh_stats.hpp) run on request, outside the substep.
 // lh_step replays a captured HIP graph of three substeps (+ meanflow_ctP).
@@ -46,7 +46,10 @@ struct lh_handle {
   double* scal = nullptr;
   // state
   double2 *u1 = nullptr, *u2 = nullptr, *u3 = nullptr, *p = nullptr;
-  double2 *nu1 = nullptr, *nu2 = nullptr, *nu3 = nullptr;
+  // nonlinear terms, two buffers: substeps 1 and 3 write A and read B, substep 2 the reverse
+  double2 *nu1 = nullptr, *nu2 = nullptr, *nu3 = nullptr;     // A
+  double2 *nb1 = nullptr, *nb2 = nullptr, *nb3 = nullptr;     // B
+  int nu_last = 0;                                             // buffer written last (0: A)
   double2 *r1 = nullptr, *r2 = nullptr, *r3 = nullptr, *psi = nullptr, *tipl = nullptr;
   void *zu = nullptr, *zw = nullptr, *zv = nullptr, *zp = nullptr;
   double *dg1 = nullptr, *dg2 = nullptr, *dg3 = nullptr;
@@ -57,6 +60,7 @@ struct lh_handle {
   unsigned char* masks = nullptr;
   int nmask = 0;
   int *ibu_planes = nullptr, *ibv_planes = nullptr;
+  unsigned char *ibflag_u = nullptr, *ibflag_v = nullptr;   // per slot: canopy plane
   int nibu = 0, nibv = 0;
   IBEntry* ibent = nullptr;
   int nibent = 0;
@@ -152,11 +156,11 @@ static bool supported_len(int n) {
 #ifndef LH_ZASM_COLS
 #define LH_ZASM_COLS 8
 #endif
-#ifndef LH_VEL_SPLIT
-#define LH_VEL_SPLIT 0
-#endif
 #ifndef LH_COL_U
 #define LH_COL_U 8
+#endif
+#ifndef LH_VEL_U
+#define LH_VEL_U 2
 #endif
 // threads per block of the column kernels: one column per thread, so small blocks spread
 // the 74k columns of the production grid evenly over the 256 CUs
@@ -212,12 +216,20 @@ static void launch_xraw_t(lh_handle* h, const XRAW& a, const void* tw) {
   }
 }
 constexpr int kZasmCols = LH_ZASM_COLS;
+#ifndef LH_ZASM_LC
+#define LH_ZASM_LC 4
+#endif
+template <int PZ, typename TR>
+static size_t zasm_lds() {
+  return ((size_t)2 * kZasmCols * (PZ + 1) + Plan<PZ>::template TwL<cx<TR>>::SIZE) *
+         sizeof(cx<TR>);
+}
 template <int PZ, typename TR>
 static void launch_zasm_t(lh_handle* h, const ZASM& a, const void* tw) {
   constexpr int L = kZasmCols;
   const int nlev = h->dm.nyu - 2;
-  dim3 grid(a.nib * nlev * 3);
-  size_t sh = (size_t)2 * L * (PZ + 1) * sizeof(cx<TR>);
+  dim3 grid(a.nib * ((nlev + a.lc - 1) / a.lc) * 3);
+  const size_t sh = zasm_lds<PZ, TR>();
   hipLaunchKernelGGL((k_zasm<PZ, L, TR>), grid, dim3(2 * L * Plan<PZ>::T), sh, h->st, a, h->dm,
                      (const cx<TR>*)tw);
 }
@@ -226,8 +238,7 @@ static void attr_zasm_t(int* dummy) {
   constexpr int L = kZasmCols;
   (void)dummy;
   (void)hipFuncSetAttribute((const void*)k_zasm<PZ, L, TR>,
-                            hipFuncAttributeMaxDynamicSharedMemorySize,
-                            (int)(2 * L * (PZ + 1) * sizeof(cx<TR>)));
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)zasm_lds<PZ, TR>());
 }
 template <int PX, typename TR>
 static void launch_xib_t(lh_handle* h, const XIB& a, int nent, const void* tw) {
@@ -310,20 +321,20 @@ static void issue_substep(lh_handle* h, int krk /*0-based*/, bool cfl_mode) {
   }
   if (krk == 0 && cfl_mode)
     TIMED(h, LH_K_MISC, 0.0, hipLaunchKernelGGL(k_cfl_dt, dim3(1), dim3(1), 0, h->st, h->scal));
+  // Nu_new into the buffer of this substep; RHS0 + rhs per column reads both
+  const bool wb = (krk % 2) == 1;
+  double2* nw[3] = {wb ? h->nb1 : h->nu1, wb ? h->nb2 : h->nu2, wb ? h->nb3 : h->nu3};
+  double2* ol[3] = {wb ? h->nu1 : h->nb1, wb ? h->nu2 : h->nb2, wb ? h->nu3 : h->nb3};
+  h->nu_last = wb ? 1 : 0;
   {
     ZASM a{};
     a.zo = h->zp;
-    a.u1 = h->u1; a.u2 = h->u2; a.u3 = h->u3; a.p = h->p;
-    a.nu1 = h->nu1; a.nu2 = h->nu2; a.nu3 = h->nu3;
-    a.r1 = h->r1; a.r2 = h->r2; a.r3 = h->r3; a.tipl = h->tipl;
-    a.dyu2i = h->dyu2i; a.dyv2i = h->dyv2i; a.dthdyu = h->dthdyu; a.dthdyv = h->dthdyv;
-    a.k1x = h->k1x; a.k2x = h->k2x; a.k1z = h->k1z; a.k2z = h->k2z; a.scal = h->scal;
-    a.tip_u0 = h->tip_u[0] + dm.d; a.tip_u1 = h->tip_u[1] + dm.d;
-    a.tip_v0 = h->tip_v[0] + dm.d; a.tip_v1 = h->tip_v[1] + dm.d;
+    a.nu1 = nw[0]; a.nu2 = nw[1]; a.nu3 = nw[2];
+    a.dthdyu = h->dthdyu; a.dthdyv = h->dthdyv; a.k1x = h->k1x; a.k1z = h->k1z;
     a.nib = (dm.mx + kZasmCols - 1) / kZasmCols;
-    a.C1 = aR / dm.re; a.C2 = -cR; a.dR = dR; a.g = gR; a.bre = bre;
-    // in: six product Z rows per level, u/v/w, p, Nu_old; out: Nu_new, rhs
-    const double by = 6.0 * nlev * zsl + spl * (nuvw + dm.nyp + 3.0 * nuvw);
+    a.lc = LH_ZASM_LC;
+    // in: six product Z rows per level; out: Nu_new
+    const double by = 6.0 * nlev * zsl + spl * (2.0 * nlev + (nlev - 1));
     TIMED(h, LH_K_ASM, by, {
       if (f64) {
         LH_DISPATCH_LEN(dm.pz, launch_zasm_t, double, h, a, h->twz)
@@ -332,8 +343,26 @@ static void issue_substep(lh_handle* h, int krk /*0-based*/, bool cfl_mode) {
       }
     });
   }
+  RHS ra{};
+  ra.u1 = h->u1; ra.u2 = h->u2; ra.u3 = h->u3; ra.p = h->p;
+  ra.no1 = ol[0]; ra.no2 = ol[1]; ra.no3 = ol[2];
+  ra.nn1 = nw[0]; ra.nn2 = nw[1]; ra.nn3 = nw[2];
+  ra.r1 = h->r1; ra.r2 = h->r2; ra.r3 = h->r3; ra.tipl = h->tipl;
+  ra.dyu2i = h->dyu2i; ra.dyv2i = h->dyv2i; ra.dthdyv = h->dthdyv;
+  ra.k1x = h->k1x; ra.k2x = h->k2x; ra.k1z = h->k1z; ra.k2z = h->k2z; ra.scal = h->scal;
+  ra.planes_u = h->ibu_planes; ra.planes_v = h->ibv_planes;
+  ra.ib_u = h->ibflag_u; ra.ib_v = h->ibflag_v;
+  ra.np_u = h->nibu; ra.np_v = h->nibv;
+  ra.tip_u0 = h->tip_u[0] + dm.d; ra.tip_u1 = h->tip_u[1] + dm.d;
+  ra.tip_v0 = h->tip_v[0] + dm.d; ra.tip_v1 = h->tip_v[1] + dm.d;
+  ra.C1 = aR / dm.re; ra.C2 = -cR; ra.dR = dR; ra.g = gR; ra.bre = bre;
   // ---- IB stage (FP64 transforms): canopy planes of rhs u, w, v and Lu_y on tip planes
   if (h->nibent > 0) {
+    const int npl = h->nibu > h->nibv ? h->nibu : h->nibv;
+    // in: u (3 rows), p, Nu_old, Nu_new per canopy plane; out: rhs (+ Lu_y of the tips)
+    TIMED(h, LH_K_RHS, spl * 7.0 * (2.0 * h->nibu + h->nibv),
+          hipLaunchKernelGGL(k_rhs_ib, dim3((dm.ncol + LH_COL_BS - 1) / LH_COL_BS, npl, 3),
+                             dim3(LH_COL_BS), 0, h->st, ra, dm));
     const double z64 = 16.0 * dm.pz * dm.mx;
     const double nib = 2.0 * h->nibu + h->nibv;
     ZJobs jobs{};
@@ -373,20 +402,11 @@ static void issue_substep(lh_handle* h, int krk /*0-based*/, bool cfl_mode) {
                dm.d, dm.ncol};
     // in: rhs u, w, v, p, LU factors (8 B each); out: u, w, v, p
     const double by = spl * (2.0 * nuvw + 2.0 * dm.nyp) + 3.0 * 8.0 * dm.ncol * (double)dm.nyp;
-#if LH_VEL_SPLIT
-    // one field per launch: the forward-sweep rows of one field (343 MB at 384x289x384)
-    // are re-read in reverse order while most of them still sit in the memory-side cache
-    for (int f = 0; f < 3; ++f)
-      TIMED(h, LH_K_VEL, 2.0 * (f == 2 ? dm.nyv : dm.nyu) * spl,
-            hipLaunchKernelGGL(k_vel<LH_COL_U>, dim3((dm.ncol + LH_COL_BS - 1) / LH_COL_BS, 1),
-                               dim3(LH_COL_BS), 0, h->st, a,
-                               dm, f));
-#else
-    TIMED(h, LH_K_VEL, 2.0 * nuvw * spl,
-          hipLaunchKernelGGL(k_vel<LH_COL_U>, dim3((dm.ncol + LH_COL_BS - 1) / LH_COL_BS, 3),
-                             dim3(LH_COL_BS), 0, h->st, a,
-                             dm, 0));
-#endif
+    // in: u, p, Nu_old, Nu_new (rhs on canopy planes); eliminated rows out and back; out: u
+    TIMED(h, LH_K_VEL, spl * (7.0 * nuvw),
+          hipLaunchKernelGGL(k_vel_rhs<LH_VEL_U>, dim3((dm.ncol + LH_COL_BS - 1) / LH_COL_BS, 3),
+                             dim3(LH_COL_BS), 0, h->st, ra, dm, (const double*)h->ayu,
+                             (const double*)h->ayv));
     TIMED(h, LH_K_PRES, by - 2.0 * nuvw * spl + 2.0 * nuvw * spl,
           hipLaunchKernelGGL(k_pres<LH_COL_U>, dim3((dm.ncol + LH_COL_BS - 1) / LH_COL_BS),
                              dim3(LH_COL_BS), 0, h->st, a, dm));
@@ -482,6 +502,7 @@ int lh_create(const lh_config* c, lh_handle** out) {
   HIPCHK(h, dalloc(&h->u1, nu)); HIPCHK(h, dalloc(&h->u3, nu)); HIPCHK(h, dalloc(&h->u2, nv));
   HIPCHK(h, dalloc(&h->p, np)); HIPCHK(h, dalloc(&h->psi, np));
   HIPCHK(h, dalloc(&h->nu1, nu)); HIPCHK(h, dalloc(&h->nu3, nu)); HIPCHK(h, dalloc(&h->nu2, nv));
+  HIPCHK(h, dalloc(&h->nb1, nu)); HIPCHK(h, dalloc(&h->nb3, nu)); HIPCHK(h, dalloc(&h->nb2, nv));
   HIPCHK(h, dalloc(&h->r1, nu)); HIPCHK(h, dalloc(&h->r3, nu)); HIPCHK(h, dalloc(&h->r2, nv));
   HIPCHK(h, dalloc(&h->tipl, 6 * (size_t)dm.ncol));
   HIPCHK(h, dalloc(&h->dg1, np)); HIPCHK(h, dalloc(&h->dg2, np)); HIPCHK(h, dalloc(&h->dg3, np));
@@ -502,6 +523,8 @@ int lh_create(const lh_config* c, lh_handle** out) {
   HIPCHK(h, hipMemset(h->u2, 0, nv * 16)); HIPCHK(h, hipMemset(h->p, 0, np * 16));
   HIPCHK(h, hipMemset(h->nu1, 0, nu * 16)); HIPCHK(h, hipMemset(h->nu3, 0, nu * 16));
   HIPCHK(h, hipMemset(h->nu2, 0, nv * 16));
+  HIPCHK(h, hipMemset(h->nb1, 0, nu * 16)); HIPCHK(h, hipMemset(h->nb3, 0, nu * 16));
+  HIPCHK(h, hipMemset(h->nb2, 0, nv * 16));
   // twiddles exp(-2 pi i m/N)
   auto mk_tw = [&](int n, void** d64, void** dnl) -> int {
     std::vector<double2> t(n);
@@ -602,6 +625,15 @@ int lh_create(const lh_config* c, lh_handle** out) {
       HIPCHK(h, hipMalloc((void**)&h->ibv_planes, sizeof(int) * h->nibv));
       HIPCHK(h, hipMemcpy(h->ibv_planes, h->h_ibv.data(), sizeof(int) * h->nibv, hipMemcpyHostToDevice));
     }
+    std::vector<unsigned char> fu(dm.nyu, 0), fv(dm.nyv, 0);
+    if (h->nibent > 0) {
+      for (int s : h->h_ibu) fu[s] = 1;
+      for (int s : h->h_ibv) fv[s] = 1;
+    }
+    HIPCHK(h, hipMalloc((void**)&h->ibflag_u, dm.nyu));
+    HIPCHK(h, hipMalloc((void**)&h->ibflag_v, dm.nyv));
+    HIPCHK(h, hipMemcpy(h->ibflag_u, fu.data(), dm.nyu, hipMemcpyHostToDevice));
+    HIPCHK(h, hipMemcpy(h->ibflag_v, fv.data(), dm.nyv, hipMemcpyHostToDevice));
   }
   // pressure LU factors, once (start.f90:628-630)
   {
@@ -620,10 +652,10 @@ int lh_destroy(lh_handle* h) {
     if (g) (void)hipGraphExecDestroy(g);
   for (auto e : h->evpool) (void)hipEventDestroy(e);
   void* ptrs[] = {h->yu, h->yv, h->dthdyu, h->dthdyv, h->dyu2i, h->dyv2i, h->k1x, h->k2x, h->k1z,
-                  h->k2z, h->scal, h->u1, h->u2, h->u3, h->p, h->nu1, h->nu2, h->nu3, h->r1,
+                  h->k2z, h->scal, h->u1, h->u2, h->u3, h->p, h->nu1, h->nu2, h->nu3, h->nb1, h->nb2, h->nb3, h->r1,
                   h->r2, h->r3, h->psi, h->tipl, h->zu, h->zw, h->zv, h->zp, h->dg1,
                   h->dg2, h->dg3, h->ayu, h->ayv, h->ctwork, h->masks, h->ibu_planes,
-                  h->ibv_planes, h->ibent, h->st_psum, h->st_pfold, h->st_acc,
+                  h->ibv_planes, h->ibflag_u, h->ibflag_v, h->ibent, h->st_psum, h->st_pfold, h->st_acc,
                   h->cst_acc, h->sp_acc};
   for (void* q : ptrs)
     if (q) hipFree(q);
@@ -697,9 +729,12 @@ int lh_set_nonlinear(lh_handle* h, const double* n1, const double* n2, const dou
   if (!h) return LH_ERR_ARG;
   HIPCHK(h, hipStreamSynchronize(h->st));
   int r;
-  if (n1 && (r = xfer_nl(h, h->nu1, (double*)n1, h->dm.nyu, true))) return r;
-  if (n2 && (r = xfer_nl(h, h->nu2, (double*)n2, h->dm.nyv, true))) return r;
-  if (n3 && (r = xfer_nl(h, h->nu3, (double*)n3, h->dm.nyu, true))) return r;
+  // both buffers: the next substep reads whichever its parity selects
+  for (int b = 0; b < 2; ++b) {
+    if (n1 && (r = xfer_nl(h, b ? h->nb1 : h->nu1, (double*)n1, h->dm.nyu, true))) return r;
+    if (n2 && (r = xfer_nl(h, b ? h->nb2 : h->nu2, (double*)n2, h->dm.nyv, true))) return r;
+    if (n3 && (r = xfer_nl(h, b ? h->nb3 : h->nu3, (double*)n3, h->dm.nyu, true))) return r;
+  }
   return LH_OK;
 }
 
@@ -707,9 +742,10 @@ int lh_get_nonlinear(lh_handle* h, double* n1, double* n2, double* n3) {
   if (!h) return LH_ERR_ARG;
   HIPCHK(h, hipStreamSynchronize(h->st));
   int r;
-  if (n1 && (r = xfer_nl(h, h->nu1, n1, h->dm.nyu, false))) return r;
-  if (n2 && (r = xfer_nl(h, h->nu2, n2, h->dm.nyv, false))) return r;
-  if (n3 && (r = xfer_nl(h, h->nu3, n3, h->dm.nyu, false))) return r;
+  const bool b = h->nu_last == 1;
+  if (n1 && (r = xfer_nl(h, b ? h->nb1 : h->nu1, n1, h->dm.nyu, false))) return r;
+  if (n2 && (r = xfer_nl(h, b ? h->nb2 : h->nu2, n2, h->dm.nyv, false))) return r;
+  if (n3 && (r = xfer_nl(h, b ? h->nb3 : h->nu3, n3, h->dm.nyu, false))) return r;
   return LH_OK;
 }
 

@@ -1,5 +1,6 @@
 // lh_col.hpp -- column stage of the RK3 substep: one thread per spectral column (kx, kz)
-// runs the velocity Thomas sweeps of LUsolU/LUsolV (tridLU_3D_IB.f90:106-116, 179-189),
+// builds the rhs of solveU/solveV and runs the velocity Thomas sweeps of LUsolU/LUsolV
+// (tridLU_3D_IB.f90:106-116, 179-189),
 // the constant-flow-rate correction of the (0,0) column when enabled (flowrate_corr,
 // littleharsh.f90:453-485), solveP with LUsolP (littleharsh.f90:557-612,
 // tridLU_3D_IB.f90:362-396) and v_corr (littleharsh.f90:826-929).  Consecutive columns
@@ -22,44 +23,6 @@ struct COL {
   int ctu;              // flag_ctpress == 0
   CTU ct;
 };
-
-template <int U>
-__device__ __forceinline__ void thomas_u(double2* __restrict__ x, double2* __restrict__ out,
-                                         const double* __restrict__ ay, int nj, int ncol, int c) {
-  double2 prev = drm(ay[1], x[c]);
-  x[c] = prev;
-  int r = 1;
-  for (; r + U <= nj; r += U) {
-    double2 v[U];
-#pragma unroll
-    for (int q = 0; q < U; ++q) v[q] = x[(size_t)(r + q) * ncol + c];
-#pragma unroll
-    for (int q = 0; q < U; ++q) {
-      prev = drm(ay[3 * (r + q) + 1], dsub(v[q], drm(ay[3 * (r + q)], prev)));
-      x[(size_t)(r + q) * ncol + c] = prev;
-    }
-  }
-  for (; r < nj; ++r) {
-    prev = drm(ay[3 * r + 1], dsub(x[(size_t)r * ncol + c], drm(ay[3 * r], prev)));
-    x[(size_t)r * ncol + c] = prev;
-  }
-  out[(size_t)(nj - 1) * ncol + c] = prev;
-  r = nj - 2;
-  for (; r - U + 1 >= 0; r -= U) {
-    double2 v[U];
-#pragma unroll
-    for (int q = 0; q < U; ++q) v[q] = x[(size_t)(r - q) * ncol + c];
-#pragma unroll
-    for (int q = 0; q < U; ++q) {
-      prev = dsub(v[q], drm(ay[3 * (r - q) + 2], prev));
-      out[(size_t)(r - q) * ncol + c] = prev;
-    }
-  }
-  for (; r >= 0; --r) {
-    prev = dsub(x[(size_t)r * ncol + c], drm(ay[3 * r + 2], prev));
-    out[(size_t)r * ncol + c] = prev;
-  }
-}
 
 // flowrate_corr body on the (0,0) column (thread of column 0 only)
 __device__ void ctu_column(const CTU& a) {
@@ -102,17 +65,174 @@ __device__ void ctu_column(const CTU& a) {
   for (int j = 0; j <= n; ++j) a.u1[(size_t)(j + a.d) * a.ncol].x += g * u11[j];
 }
 
-// velocity Thomas sweeps, one (column, component) per thread
-// (blockIdx.y + f0 selects u1, u3 or u2)
+// RHS0 + nonlinear + rhs of solveU/solveV (littleharsh.f90:614-731, 733-824) at plane s of
+// component f for column c:
+//   Lu  = dy2i(1,s) u(s-1) + (k2x + k2z + dy2i(2,s)) u(s) + dy2i(3,s) u(s+1)
+//   du  = aRK/Re Lu - cRK Nu_old - dRK grad p [- dRK mpgx on the (0,0) u mode] - gRK Nu_new
+//   rhs = (u + dt du) / axz
+// The velocity sweeps (k_vel) evaluate it on the fly; the canopy planes, which the
+// immersed-boundary stage rewrites in physical space first, get it from k_rhs_ib together
+// with Lu_y on the tip planes.
+struct RHS {
+  double2 *u1, *u2, *u3;
+  const double2* p;
+  const double2 *no1, *no2, *no3;            // Nu_old
+  const double2 *nn1, *nn2, *nn3;            // Nu_new
+  double2 *r1, *r2, *r3, *tipl;
+  const double *dyu2i, *dyv2i, *dthdyv, *k1x, *k2x, *k1z, *k2z, *scal;
+  const int *planes_u, *planes_v;            // canopy planes (slots) of the u and v grids
+  const unsigned char *ib_u, *ib_v;          // per slot: 1 on canopy planes
+  int np_u, np_v;
+  int tip_u0, tip_u1, tip_v0, tip_v1;        // slots
+  double C1, C2, dR, g, bre;
+};
+
+struct RhsCol {
+  double kx, kz, k2, axzi, kp, mp, dt;      // axzi = 1/axz
+};
+
+__device__ __forceinline__ RhsCol rhs_col(const RHS& a, const Dims& dm, int c, int f) {
+  RhsCol r;
+  const int i = c % dm.mx, k = c / dm.mx;
+  r.dt = a.scal[S_DT];
+  const double beta = a.bre * r.dt;
+  r.kx = a.k1x[i];
+  r.kz = a.k1z[k];
+  r.k2 = a.k2x[i] + a.k2z[k];
+  r.axzi = 1.0 / (1.0 - beta * (a.k2x[i] + a.k2z[k]));
+  r.kp = -a.dR * (f == 0 ? r.kx : r.kz);
+  r.mp = (f == 0 && c == 0) ? -a.dR * a.scal[S_MPGX] : 0.0;
+  return r;
+}
+
+// rhs at slot s from u(s-1), u(s), u(s+1), the pressure rows (p(s-1) for u/w; p(s-1), p(s)
+// for v), Nu_old(s) and Nu_new(s)
+__device__ __forceinline__ double2 rhs_at(const RHS& a, const Dims& dm, const RhsCol& k, int f,
+                                          int s, double2 um, double2 u0, double2 up, double2 pl,
+                                          double2 pr, double2 vo, double2 vn) {
+  const double* wy = (f < 2 ? a.dyu2i : a.dyv2i) + 3 * s;
+  const double2 Lu = dadd(dadd(drm(wy[0], um), drm(k.k2 + wy[1], u0)), drm(wy[2], up));
+  double2 du;
+  if (f < 2) {
+    du = dadd(dadd(drm(a.C1, Lu), drm(a.C2, vo)), dim(k.kp, pl));
+    du.x += k.mp;
+  } else {
+    const double C3v = -a.dR * dm.ddthetavi * a.dthdyv[s];
+    du = dadd(dadd(drm(a.C1, Lu), drm(a.C2, vo)), drm(C3v, dsub(pr, pl)));
+  }
+  du = dsub(du, drm(a.g, vn));
+  return drm(k.axzi, dadd(u0, drm(k.dt, du)));
+}
+
+// rhs on the canopy planes (grid: columns x planes x component) and Lu_y on the tip planes
+__global__ void __launch_bounds__(256) k_rhs_ib(RHS a, Dims dm) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  const int f = blockIdx.z;
+  const int q = blockIdx.y;
+  const int ncol = dm.ncol;
+  if (c >= ncol || q >= (f < 2 ? a.np_u : a.np_v)) return;
+  const int s = f < 2 ? a.planes_u[q] : a.planes_v[q];
+  const double2* __restrict__ uu = f == 0 ? a.u1 : (f == 1 ? a.u3 : a.u2);
+  const double2* __restrict__ no = f == 0 ? a.no1 : (f == 1 ? a.no3 : a.no2);
+  const double2* __restrict__ nn = f == 0 ? a.nn1 : (f == 1 ? a.nn3 : a.nn2);
+  double2* __restrict__ rr = f == 0 ? a.r1 : (f == 1 ? a.r3 : a.r2);
+  const RhsCol k = rhs_col(a, dm, c, f);
+  const size_t o = (size_t)s * ncol + c;
+  const double2 um = uu[o - ncol], u0 = uu[o], up = uu[o + ncol];
+  const double2 pl = a.p[o - ncol], pr = f == 2 ? a.p[o] : d2(0.0, 0.0);
+  rr[o] = rhs_at(a, dm, k, f, s, um, u0, up, pl, pr, no[o], nn[o]);
+  const bool tip = f < 2 ? (s == a.tip_u0 || s == a.tip_u1) : (s == a.tip_v0 || s == a.tip_v1);
+  if (tip) {
+    const double* wy = (f < 2 ? a.dyu2i : a.dyv2i) + 3 * s;
+    const int tipr = f < 2 ? 2 * f + (s == a.tip_u0 ? 0 : 1) : 4 + (s == a.tip_v0 ? 0 : 1);
+    a.tipl[(size_t)tipr * ncol + c] = dadd(dadd(drm(wy[0], um), drm(wy[1], u0)), drm(wy[2], up));
+  }
+}
+
+// solveU/solveV with the rhs built on the fly: forward elimination of LUsolU/LUsolV
+// (tridLU_3D_IB.f90:106-116) walking up the planes with the rhs of every non-canopy plane
+// computed from u(s-1..s+1), p, Nu_old, Nu_new in registers (canopy planes read the rhs the
+// immersed-boundary stage left), the eliminated rows parked in the rhs array, then the
+// back substitution into u.  F: 0 u1, 1 u3, 2 u2.
+template <int U, int F>
+__device__ __forceinline__ void solve_rhs(const RHS& a, const Dims& dm, int c,
+                                          const double* __restrict__ ay) {
+  const int ncol = dm.ncol;
+  const double2* __restrict__ no = F == 0 ? a.no1 : (F == 1 ? a.no3 : a.no2);
+  const double2* __restrict__ nn = F == 0 ? a.nn1 : (F == 1 ? a.nn3 : a.nn2);
+  double2* __restrict__ uu = F == 0 ? a.u1 : (F == 1 ? a.u3 : a.u2);
+  double2* __restrict__ x = F == 0 ? a.r1 : (F == 1 ? a.r3 : a.r2);
+  const unsigned char* __restrict__ ib = F < 2 ? a.ib_u : a.ib_v;
+  const double2* __restrict__ pp = a.p;
+  const int nj = F < 2 ? dm.nyu : dm.nyv;
+  const RhsCol k = rhs_col(a, dm, c, F);
+  const double2 zero = d2(0.0, 0.0);
+  // row 0: rhs = 0 (boundary row)
+  double2 prev = drm(ay[1], zero);
+  x[c] = prev;
+  double2 um = uu[c], u0 = uu[(size_t)ncol + c];
+  double2 pl = pp[c];                          // v: p(s-1) of plane s = 1
+  int r = 1;
+  for (; r < nj - 1; r += U) {
+    double2 up[U], pr[U], vo[U], vn[U];
+#pragma unroll
+    for (int q = 0; q < U; ++q) {
+      const int s = r + q;
+      if (s < nj - 1) {
+        const size_t o = (size_t)s * ncol + c;
+        up[q] = uu[o + ncol];
+        pr[q] = F == 2 ? pp[o] : pp[o - ncol];
+        vo[q] = no[o];
+        vn[q] = nn[o];
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < U; ++q) {
+      const int s = r + q;
+      if (s < nj - 1) {
+        const size_t o = (size_t)s * ncol + c;
+        double2 rhs;
+        if (ib[s]) rhs = x[o];
+        else if (F == 2) rhs = rhs_at(a, dm, k, F, s, um, u0, up[q], pl, pr[q], vo[q], vn[q]);
+        else rhs = rhs_at(a, dm, k, F, s, um, u0, up[q], pr[q], zero, vo[q], vn[q]);
+        if (F == 2) pl = pr[q];
+        prev = drm(ay[3 * s + 1], dsub(rhs, drm(ay[3 * s], prev)));
+        x[o] = prev;
+        um = u0;
+        u0 = up[q];
+      }
+    }
+  }
+  // last row: rhs = 0
+  prev = drm(ay[3 * (nj - 1) + 1], dsub(zero, drm(ay[3 * (nj - 1)], prev)));
+  uu[(size_t)(nj - 1) * ncol + c] = prev;
+  r = nj - 2;
+  for (; r - U + 1 >= 0; r -= U) {
+    double2 v[U];
+#pragma unroll
+    for (int q = 0; q < U; ++q) v[q] = x[(size_t)(r - q) * ncol + c];
+#pragma unroll
+    for (int q = 0; q < U; ++q) {
+      prev = dsub(v[q], drm(ay[3 * (r - q) + 2], prev));
+      uu[(size_t)(r - q) * ncol + c] = prev;
+    }
+  }
+  for (; r >= 0; --r) {
+    prev = dsub(x[(size_t)r * ncol + c], drm(ay[3 * r + 2], prev));
+    uu[(size_t)r * ncol + c] = prev;
+  }
+}
+
 template <int U>
-__global__ void __launch_bounds__(256) k_vel(COL a, Dims dm, int f0) {
+__global__ void __launch_bounds__(256) k_vel_rhs(RHS a, Dims dm, const double* __restrict__ ayu,
+                                                 const double* __restrict__ ayv) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= dm.ncol) return;
-  const int f = blockIdx.y + f0;
-  if (f == 0) thomas_u<U>(a.r1, a.u1, a.ayu, dm.nyu, dm.ncol, c);
-  else if (f == 1) thomas_u<U>(a.r3, a.u3, a.ayu, dm.nyu, dm.ncol, c);
-  else thomas_u<U>(a.r2, a.u2, a.ayv, dm.nyv, dm.ncol, c);
+  if (blockIdx.y == 0) solve_rhs<U, 0>(a, dm, c, ayu);
+  else if (blockIdx.y == 1) solve_rhs<U, 1>(a, dm, c, ayu);
+  else solve_rhs<U, 2>(a, dm, c, ayv);
 }
+
 
 // constant-flow-rate correction (column 0), solveP and v_corr, one column per thread
 template <int U>

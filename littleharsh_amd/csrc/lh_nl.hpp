@@ -14,9 +14,9 @@
 //   u:  A1 = ikx uu + m_u (uv - uv_prev),   B1 = uw
 //   w:  A3 = ikx uw + m_u (vw - vw_prev),   B3 = ww
 //   v:  A2 = ikx uv_prev + m_v (vv - vv_prev), B2 = vw_prev   (face j-1 at level j)
-// The x pass (k_xraw) produces the six raw products per level; the forward z pass
-// (k_zasm) forms A and B from them, transforms, and assembles RHS0 + nonlinear + the rhs
-// of solveU/solveV in one go.
+// The x pass (k_xraw2) produces the six raw products per level; the forward z pass
+// (k_zasm) forms A and B from them and transforms them into Nu_new.  RHS0 and the rhs of
+// solveU/solveV are assembled per column afterwards (k_rhs, lh_col.hpp).
 #pragma once
 #include <type_traits>
 #include "lh_kernels.hpp"
@@ -323,20 +323,22 @@ __global__ void __launch_bounds__(6 * Plan<PX>::T) __attribute__((amdgpu_waves_p
   }
 }
 
-// Forward z transform of the combined rows + RHS0 + nonlinear + rhs of solveU/solveV:
-// one block per (L consecutive kx columns, level, field).  Block ids are remapped so that
-// the two blocks sharing a 128-byte line of every [plane][column] row run on one XCD.
+// Forward z transform of the combined rows -> Nu_new: one block per (L consecutive kx
+// columns, field, chunk of lc levels).  Walking up the levels, the upper product row of the
+// y difference of level l is the lower one of level l+1 and stays in registers, so a level
+// reads three product rows instead of four.  Block ids are remapped so that consecutive
+// column groups run on one XCD.
 struct ZASM {
   const void* zo;
-  const double2 *u1, *u2, *u3, *p;
-  double2 *nu1, *nu2, *nu3;
-  double2 *r1, *r2, *r3, *tipl;
-  const double *dyu2i, *dyv2i, *dthdyu, *dthdyv, *k1x, *k2x, *k1z, *k2z, *scal;
-  int tip_u0, tip_u1, tip_v0, tip_v1;
-  int nib;        // column blocks
-  double C1, C2, dR, g, bre;
+  double2 *nu1, *nu2, *nu3;                  // Nu_new of this substep
+  const double *dthdyu, *dthdyv, *k1x, *k1z;
+  int nib;                                   // column blocks
+  int lc;                                    // levels per block
 };
 
+#ifndef LH_ZASM_ZB
+#define LH_ZASM_ZB 3
+#endif
 template <int PZ, int L, typename TR>
 __global__ void __launch_bounds__(2 * L * Plan<PZ>::T)
     k_zasm(ZASM a, Dims dm, const cx<TR>* __restrict__ tw) {
@@ -344,6 +346,8 @@ __global__ void __launch_bounds__(2 * L * Plan<PZ>::T)
   constexpr int T = Plan<PZ>::T;
   constexpr int NT = 2 * L * T;
   constexpr int LS = PZ + 1;
+  constexpr int ITZ = (L * PZ + NT - 1) / NT;
+  constexpr int NZ = 2 * PZ / 3;
   extern __shared__ char smem_raw[];
   C* lds = reinterpret_cast<C*>(smem_raw);
   const int tid = threadIdx.x;
@@ -353,19 +357,15 @@ __global__ void __launch_bounds__(2 * L * Plan<PZ>::T)
   int bid = blockIdx.x;
   const int per = nblk / 8;
   if (bid < per * 8) bid = (bid % 8) * per + bid / 8;
-#if LH_ZASM_ORDER == 1
-  // fields of one (columns, level) and neighbouring levels adjacent: shared product and
-  // state rows are re-read from L2
-  const int f = bid % 3;                      // 0: u1, 1: u3, 2: v
-  const int rest = bid / 3;
-  const int l = rest % nlev;
-  const int ib = rest / nlev;
-#else
   const int ib = bid % a.nib;
   const int rest = bid / a.nib;
-  const int l = rest % nlev;
-  const int f = rest / nlev;                  // 0: u1, 1: u3, 2: v
-#endif
+  const int nch = (nlev + a.lc - 1) / a.lc;
+  const int ch = rest % nch;
+  const int f = rest / nch;                   // 0: u1, 1: u3, 2: v
+  const int l0 = ch * a.lc;
+  // face ny+d-1 is the wall: no Nu2 there
+  const int l1 = min(l0 + a.lc, f == 2 ? nlev - 1 : nlev);
+  if (l0 >= l1) return;
   const int i0 = ib * L;
   const int mx = dm.mx, nz = dm.nz, h = nz / 2, ncol = dm.ncol;
   const size_t slot = (size_t)PZ * mx;
@@ -375,126 +375,79 @@ __global__ void __launch_bounds__(2 * L * Plan<PZ>::T)
   // v face j   (level l):  A = ikx uv(l) + m_v (vv(l+1) - vv(l)),   B = vw(l)
   const C* zo = reinterpret_cast<const C*>(a.zo);
   const size_t ps = (size_t)nlev * slot;
-  const C* zl = zo + (size_t)l * slot;
-  const int l_lo = f < 2 ? l - 1 : l;         // second operand of the y difference
-  const int l_hi = f < 2 ? l : l + 1;
-  const bool has_lo = l_lo >= 0 && l_hi < nlev;
   const int q0 = f == 0 ? 0 : (f == 1 ? 1 : 4);       // ikx term
   const int qd = f == 0 ? 4 : (f == 1 ? 5 : 3);       // y-difference term
   const int qb = f == 0 ? 1 : (f == 1 ? 2 : 5);       // B
-  const TR mlev = (TR)(dm.ddthetavi * (f < 2 ? a.dthdyu[l + 1] : a.dthdyv[l + 1]));
-  {
-    constexpr int ITZ = (L * PZ + NT - 1) / NT;
-    C r0[ITZ], rh[ITZ], rl[ITZ], rb[ITZ];
-#pragma unroll
-    for (int q = 0; q < ITZ; ++q) {
-      const int e = tid + q * NT;
-      const int l2 = e % L, z = e / L, i = i0 + l2;
-      if (e < L * PZ && i < mx) {
-        const size_t o = (size_t)z * mx + i;
-        r0[q] = zl[q0 * ps + o];
-        rb[q] = zl[qb * ps + o];
-        rh[q] = zo[qd * ps + (size_t)l_hi * slot + o];
-        rl[q] = has_lo ? zo[qd * ps + (size_t)l_lo * slot + o] : mk<C>(0, 0);
-      }
-    }
-#pragma unroll
-    for (int q = 0; q < ITZ; ++q) {
-      const int e = tid + q * NT;
-      const int l2 = e % L, z = e / L, i = i0 + l2;
-      if (e < L * PZ) {
-        C A = mk<C>(0, 0), B = mk<C>(0, 0);
-        if (i < mx) {
-          const TR kx = (TR)a.k1x[i];
-          A = mk<C>(-kx * r0[q].y + mlev * (rh[q].x - rl[q].x), kx * r0[q].x + mlev * (rh[q].y - rl[q].y));
-          B = rb[q];
-        }
-        lds[(2 * l2) * LS + z] = A;
-        lds[(2 * l2 + 1) * LS + z] = B;
-      }
-    }
-  }
-  __syncthreads();
-  {
-    const int ln = tid / T, t = tid % T;
-    typename Plan<PZ>::template Tw<C> w;
-    w.load(t, tw);
-    Plan<PZ>::template run<false>(lds + ln * LS, t, w);
-  }
-  // assembly, one (column, plane) output per item; loads of all items are issued first
-  const double dt = a.scal[S_DT];
-  const double beta = a.bre * dt;
-  const double sz = 1.0 / (double)PZ;
-  const double2 zero = d2(0.0, 0.0);
-  const int s = l + 1;                        // output row: u slot of centre j / v slot of face j
-  double2* __restrict__ rr = f == 0 ? a.r1 : (f == 1 ? a.r3 : a.r2);
-  const double2* __restrict__ uu = f == 0 ? a.u1 : (f == 1 ? a.u3 : a.u2);
   double2* __restrict__ nu = f == 0 ? a.nu1 : (f == 1 ? a.nu3 : a.nu2);
-  const double2* __restrict__ pp = a.p;
-  constexpr int NZ = 2 * PZ / 3;
-  constexpr int ITEMS = (L * NZ + NT - 1) / NT;
-  const double mpgx = a.scal[S_MPGX];
-  const double* wy = (f < 2 ? a.dyu2i : a.dyv2i) + 3 * s;
-  const double w0 = wy[0], w1 = wy[1], w2 = wy[2];
-  const double C3v = f == 2 ? -a.dR * dm.ddthetavi * a.dthdyv[s] : 0.0;
-  const bool tip = f < 2 ? (s == a.tip_u0 || s == a.tip_u1) : (s == a.tip_v0 || s == a.tip_v1);
-  const int tipr = f < 2 ? 2 * f + (s == a.tip_u0 ? 0 : 1) : 4 + (s == a.tip_v0 ? 0 : 1);
-  const bool active = !(f == 2 && l == nlev - 1);   // face ny+d-1 is the wall
-  double2 vm[ITEMS], v0[ITEMS], vp[ITEMS], pa[ITEMS], pb[ITEMS], no[ITEMS];
+  const int ln = tid / T, t = tid % T;
+  // twiddles in an LDS table behind the 2L lines (registers hold the carried rows)
+  typename Plan<PZ>::template TwL<C> w;
+  w.init(lds + 2 * L * LS, t, tw, ln == 0);
+  // lower operand of the y difference at the first level
+  C rl[ITZ];
+  {
+    const int l_lo = f < 2 ? l0 - 1 : l0;
 #pragma unroll
-  for (int q = 0; q < ITEMS; ++q) {
-    const int e = tid + q * NT;
-    const int l2 = e % L, k = e / L, i = i0 + l2;
-    if (e < L * NZ && i < mx && active) {
-      const int c = k * mx + i;
-      const size_t o = (size_t)s * ncol + c;
-      vm[q] = uu[o - ncol];
-      v0[q] = uu[o];
-      vp[q] = uu[o + ncol];
-      pa[q] = pp[(size_t)(s - 1) * ncol + c];
-      pb[q] = f == 2 ? pp[o] : zero;
-      no[q] = nu[o];
+    for (int q = 0; q < ITZ; ++q) {
+      const int e = tid + q * NT;
+      const int l2 = e % L, z = e / L, i = i0 + l2;
+      rl[q] = mk<C>(0, 0);
+      if (e < L * PZ && i < mx && l_lo >= 0)
+        rl[q] = zo[qd * ps + (size_t)l_lo * slot + (size_t)z * mx + i];
     }
   }
+  const double sz = 1.0 / (double)PZ;
+  for (int l = l0; l < l1; ++l) {
+    const int l_hi = f < 2 ? l : l + 1;
+    const TR mlev = (TR)(dm.ddthetavi * (f < 2 ? a.dthdyu[l + 1] : a.dthdyv[l + 1]));
+    // loads batched ZB elements at a time: the carried rows and the twiddles hold registers
 #pragma unroll
-  for (int q = 0; q < ITEMS; ++q) {
-    const int e = tid + q * NT;
-    const int l2 = e % L, k = e / L, i = i0 + l2;
-    if (e >= L * NZ || i >= mx) continue;
-    const int c = k * mx + i;
-    // BC rows handled by the first/last level blocks
-    if (f < 2) {
-      if (l == 0) rr[c] = zero;
-      if (l == nlev - 1) rr[(size_t)(nlev + 1) * ncol + c] = zero;
-    } else {
-      if (l == 0) { rr[c] = zero; rr[(size_t)nlev * ncol + c] = zero; }
+    for (int q1 = 0; q1 < ITZ; q1 += LH_ZASM_ZB) {
+      C r0[LH_ZASM_ZB], rh[LH_ZASM_ZB], rb[LH_ZASM_ZB];
+#pragma unroll
+      for (int u = 0; u < LH_ZASM_ZB; ++u) {
+        const int q = q1 + u;
+        const int e = tid + q * NT;
+        const int l2 = e % L, z = e / L, i = i0 + l2;
+        if (q < ITZ && e < L * PZ && i < mx) {
+          const size_t o = (size_t)z * mx + i;
+          r0[u] = zo[q0 * ps + (size_t)l * slot + o];
+          rb[u] = zo[qb * ps + (size_t)l * slot + o];
+          rh[u] = zo[qd * ps + (size_t)l_hi * slot + o];
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < LH_ZASM_ZB; ++u) {
+        const int q = q1 + u;
+        const int e = tid + q * NT;
+        const int l2 = e % L, z = e / L, i = i0 + l2;
+        if (q < ITZ && e < L * PZ) {
+          C A = mk<C>(0, 0), B = mk<C>(0, 0);
+          if (i < mx) {
+            const TR kx = (TR)a.k1x[i];
+            A = mk<C>(-kx * r0[u].y + mlev * (rh[u].x - rl[q].x), kx * r0[u].x + mlev * (rh[u].y - rl[q].y));
+            B = rb[u];
+            rl[q] = rh[u];
+          }
+          lds[(2 * l2) * LS + z] = A;
+          lds[(2 * l2 + 1) * LS + z] = B;
+        }
+      }
     }
-    if (!active) continue;
-    const int kk = k < h ? k : k + (PZ - nz);
-    const C A = lds[(2 * l2) * LS + kk], B = lds[(2 * l2 + 1) * LS + kk];
-    const double kx = a.k1x[i], kz = a.k1z[k], k2 = a.k2x[i] + a.k2z[k];
-    const double axz = 1.0 - beta * (a.k2x[i] + a.k2z[k]);
-    // Nu = (F(A) + ikz F(B)) / pz
-    const double2 nn = d2(sz * ((double)A.x - kz * (double)B.y), sz * ((double)A.y + kz * (double)B.x));
-    const size_t o = (size_t)s * ncol + c;
-    const double2 Lu = dadd(dadd(drm(w0, vm[q]), drm(k2 + w1, v0[q])), drm(w2, vp[q]));
-    double2 du;
-    if (f < 2) {
-      du = dadd(dadd(drm(a.C1, Lu), drm(a.C2, no[q])), dim(-a.dR * (f == 0 ? kx : kz), pa[q]));
-      if (f == 0 && c == 0) du.x += -a.dR * mpgx;
-    } else {
-      du = dadd(dadd(drm(a.C1, Lu), drm(a.C2, no[q])), drm(C3v, dsub(pb[q], pa[q])));
+    __syncthreads();
+    Plan<PZ>::template run<false>(lds + ln * LS, t, w);
+    // Nu = (F(A) + ikz F(B)) / pz on the retained modes
+    const int s = l + 1;                      // u slot of centre j / v slot of face j
+    for (int e = tid; e < L * NZ; e += NT) {
+      const int l2 = e % L, k = e / L, i = i0 + l2;
+      if (i >= mx) continue;
+      const int kk = k < h ? k : k + (PZ - nz);
+      const C A = lds[(2 * l2) * LS + kk], B = lds[(2 * l2 + 1) * LS + kk];
+      const double kz = a.k1z[k];
+      nu[(size_t)s * ncol + k * mx + i] =
+          d2(sz * ((double)A.x - kz * (double)B.y), sz * ((double)A.y + kz * (double)B.x));
     }
-    nu[o] = nn;
-    du = dsub(du, drm(a.g, nn));
-    double2 r = dadd(v0[q], drm(dt, du));
-    r.x /= axz;
-    r.y /= axz;
-    rr[o] = r;
-    if (tip) {
-      const double2 Luy = dadd(dadd(drm(w0, vm[q]), drm(w1, v0[q])), drm(w2, vp[q]));
-      a.tipl[(size_t)tipr * ncol + c] = Luy;
-    }
+    __syncthreads();
   }
 }
 
