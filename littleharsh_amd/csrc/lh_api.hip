@@ -204,7 +204,7 @@ template <int PX, typename TR>
 static void launch_xraw_t(lh_handle* h, const XRAW& a, const void* tw) {
   size_t sh = (size_t)6 * (PX + 1) * sizeof(cx<TR>);
   if (LH_XRAW_LC > 1) {
-    sh += (size_t)Plan<PX>::template TwL<cx<TR>>::SIZE * sizeof(cx<TR>);
+    sh = ((size_t)12 * (PX + 1) + Plan<PX>::template TwL<cx<TR>>::SIZE) * sizeof(cx<TR>);
     const int nlev = h->dm.nyu - 2, lc = LH_XRAW_LC;
     dim3 grid(h->dm.pz / 2, (nlev + lc - 1) / lc);
     hipLaunchKernelGGL((k_xraw2<PX, TR>), grid, dim3(6 * Plan<PX>::T), sh, h->st, a, h->dm,

@@ -221,9 +221,39 @@ __device__ __forceinline__ void stage(C* line, int t, const W& w) {
   lds_bar();
 }
 
+// The same stage out of place: read the line from src, write the autosorted result to dst.
+// Only the barrier after the writes is needed (the next stage reads dst and overwrites src,
+// whose reads all precede that barrier).
+template <int N, int T, int R, int NS, bool INV, typename C, typename W>
+__device__ __forceinline__ void stage_pp(const C* src, C* dst, int t, const W& w) {
+  constexpr int NB = N / R;
+  constexpr int PER = (NB + T - 1) / T;
+#pragma unroll
+  for (int q = 0; q < PER; ++q) {
+    const int b = t + q * T;
+    if (PER * T == NB || b < NB) {
+      C v[R];
+#pragma unroll
+      for (int r = 0; r < R; ++r) v[r] = src[b + r * NB];
+      const int j = b % NS;
+      if constexpr (NS > 1) {
+#pragma unroll
+        for (int r = 1; r < R; ++r) v[r] = INV ? cmulc(v[r], w.get(q, r)) : cmul(v[r], w.get(q, r));
+      }
+      DFT<R, INV, C>::run(v);
+      const int idx = (b / NS) * NS * R + j;
+#pragma unroll
+      for (int r = 0; r < R; ++r) dst[idx + r * NS] = v[r];
+    }
+  }
+  lds_bar();
+}
+
 struct NoTw {};
 
 // Plans: threads per line T, the radix sequence, and the per-thread twiddle set Tw<C>.
+// run() works in place; run_pp() ping-pongs between two lines and leaves the result in
+// the second one when NSTAGE is odd, in the first one when it is even.
 #define LH_PLAN2(NN, TT, R1, R2)                                                        \
   template <> struct Plan<NN> {                                                         \
     static constexpr int T = TT;                                                        \
@@ -241,6 +271,12 @@ struct NoTw {};
     template <bool INV, typename C, typename W> static __device__ void run(C* l, int t, const W& w) { \
       stage<NN, TT, R1, 1, INV>(l, t, NoTw{});                                          \
       stage<NN, TT, R2, R1, INV>(l, t, w.s2);                                           \
+    }                                                                                   \
+    static constexpr int NSTAGE = 2;                                                    \
+    template <bool INV, typename C, typename W>                                         \
+    static __device__ void run_pp(C* a, C* b, int t, const W& w) {                      \
+      stage_pp<NN, TT, R1, 1, INV>(a, b, t, NoTw{});                                    \
+      stage_pp<NN, TT, R2, R1, INV>(b, a, t, w.s2);                                     \
     }                                                                                   \
   };
 #define LH_PLAN3(NN, TT, R1, R2, R3)                                                    \
@@ -265,6 +301,13 @@ struct NoTw {};
       stage<NN, TT, R1, 1, INV>(l, t, NoTw{});                                          \
       stage<NN, TT, R2, R1, INV>(l, t, w.s2);                                           \
       stage<NN, TT, R3, R1 * R2, INV>(l, t, w.s3);                                      \
+    }                                                                                   \
+    static constexpr int NSTAGE = 3;                                                    \
+    template <bool INV, typename C, typename W>                                         \
+    static __device__ void run_pp(C* a, C* b, int t, const W& w) {                      \
+      stage_pp<NN, TT, R1, 1, INV>(a, b, t, NoTw{});                                    \
+      stage_pp<NN, TT, R2, R1, INV>(b, a, t, w.s2);                                     \
+      stage_pp<NN, TT, R3, R1 * R2, INV>(a, b, t, w.s3);                                \
     }                                                                                   \
   };
 

@@ -186,9 +186,12 @@ __global__ void __launch_bounds__(6 * Plan<PX>::T) __attribute__((amdgpu_waves_p
   const C* zv = reinterpret_cast<const C*>(a.zv);
   C* zp = reinterpret_cast<C*>(a.zp);
   const int ln = tid / T, t = tid % T;
-  // twiddles in an LDS table after the six lines (registers hold the level window)
+  // two sets of six lines (out-of-place transforms), then the twiddle table
+  C* const b0 = lds;
+  C* const b1 = lds + 6 * LS;
+  C* const bres = (Plan<PX>::NSTAGE % 2) ? b1 : b0;   // where a transform of b0 ends
   typename Plan<PX>::template TwL<C> w;
-  w.init(lds + 6 * LS, t, tw, ln == 0);
+  w.init(lds + 12 * LS, t, tw, ln == 0);
 
   // packed rows (z0, z0+1) of n Z slots -> line slots 0..n-1
   auto load_lines = [&](const C* const* src, int n) {
@@ -208,7 +211,7 @@ __global__ void __launch_bounds__(6 * Plan<PX>::T) __attribute__((amdgpu_waves_p
       const int e = tid + q * NT;
       if (e < n * mx) {
         const int f = e / mx, k = e % mx;
-        C* line = lds + f * LS;
+        C* line = b0 + f * LS;
         if (k == 0) {
           line[0] = mk<C>(va[q].x, vb[q].x);
         } else {
@@ -218,7 +221,7 @@ __global__ void __launch_bounds__(6 * Plan<PX>::T) __attribute__((amdgpu_waves_p
       }
     }
     const int nzr = PX - 2 * mx + 1;
-    for (int e = tid; e < n * nzr; e += NT) lds[(e / nzr) * LS + mx + e % nzr] = mk<C>(0, 0);
+    for (int e = tid; e < n * nzr; e += NT) b0[(e / nzr) * LS + mx + e % nzr] = mk<C>(0, 0);
     lds_bar();
   };
   auto read_lines = [&](int s0, C (*dst)[NX]) {
@@ -227,7 +230,7 @@ __global__ void __launch_bounds__(6 * Plan<PX>::T) __attribute__((amdgpu_waves_p
 #pragma unroll
       for (int m = 0; m < NX; ++m) {
         const int x = tid + m * NT;
-        if (x < PX) dst[f][m] = lds[(s0 + f) * LS + x];
+        if (x < PX) dst[f][m] = bres[(s0 + f) * LS + x];
       }
   };
   double cmax = 0.0;
@@ -245,12 +248,12 @@ __global__ void __launch_bounds__(6 * Plan<PX>::T) __attribute__((amdgpu_waves_p
         const TR vc0 = half * (Vb.x + Vt.x), vc1 = half * (Vb.y + Vt.y);
         const TR uf0 = half * (U.x + Ua.x), uf1 = half * (U.y + Ua.y);
         const TR wf0 = half * (W.x + Wa.x), wf1 = half * (W.y + Wa.y);
-        lds[0 * LS + x] = mk<C>(U.x * U.x, U.y * U.y);
-        lds[1 * LS + x] = mk<C>(U.x * W.x, U.y * W.y);
-        lds[2 * LS + x] = mk<C>(W.x * W.x, W.y * W.y);
-        lds[3 * LS + x] = mk<C>(vc0 * vc0, vc1 * vc1);
-        lds[4 * LS + x] = mk<C>(uf0 * Vt.x, uf1 * Vt.y);
-        lds[5 * LS + x] = mk<C>(Vt.x * wf0, Vt.y * wf1);
+        b0[0 * LS + x] = mk<C>(U.x * U.x, U.y * U.y);
+        b0[1 * LS + x] = mk<C>(U.x * W.x, U.y * W.y);
+        b0[2 * LS + x] = mk<C>(W.x * W.x, W.y * W.y);
+        b0[3 * LS + x] = mk<C>(vc0 * vc0, vc1 * vc1);
+        b0[4 * LS + x] = mk<C>(uf0 * Vt.x, uf1 * Vt.y);
+        b0[5 * LS + x] = mk<C>(Vt.x * wf0, Vt.y * wf1);
         if (a.want_cfl) {
           const double s0 = fabs((double)U.x) * dm.dxi + fabs((double)vc0) * dyi + fabs((double)W.x) * dm.dzi;
           const double s1 = fabs((double)U.y) * dm.dxi + fabs((double)vc1) * dyi + fabs((double)W.y) * dm.dzi;
@@ -259,11 +262,11 @@ __global__ void __launch_bounds__(6 * Plan<PX>::T) __attribute__((amdgpu_waves_p
       }
     }
     lds_bar();
-    Plan<PX>::template run<false>(lds + ln * LS, t, w);
+    Plan<PX>::template run_pp<false>(b0 + ln * LS, b1 + ln * LS, t, w);
     for (int e = tid; e < 6 * mx; e += NT) {
       const int q = e / mx, k = e % mx;
       C va, vb;
-      unpack_rows<PX>(lds + q * LS, k, va, vb);
+      unpack_rows<PX>(bres + q * LS, k, va, vb);
       C* dst = zp + ((size_t)q * nlev + l) * slot + (size_t)z0 * mx;
       dst[k] = va;
       dst[mx + k] = vb;
@@ -276,7 +279,7 @@ __global__ void __launch_bounds__(6 * Plan<PX>::T) __attribute__((amdgpu_waves_p
     const int su = l0 + 1;
     const C* s[3] = {zu + su * slot, zw + su * slot, zv + (su - 1) * slot};
     load_lines(s, 3);
-    Plan<PX>::template run<true>(lds + ln * LS, t, w);
+    Plan<PX>::template run_pp<true>(b0 + ln * LS, b1 + ln * LS, t, w);
     read_lines(0, c0);
     lds_bar();
   }
@@ -286,7 +289,7 @@ __global__ void __launch_bounds__(6 * Plan<PX>::T) __attribute__((amdgpu_waves_p
     const C* s[6] = {zu + (su + 1) * slot, zw + (su + 1) * slot, zv + su * slot,
                      zu + (su + 2) * slot, zw + (su + 2) * slot, zv + (su + 1) * slot};
     load_lines(s, 6);
-    Plan<PX>::template run<true>(lds + ln * LS, t, w);
+    Plan<PX>::template run_pp<true>(b0 + ln * LS, b1 + ln * LS, t, w);
     read_lines(0, n1);
     read_lines(3, n2);
     lds_bar();
@@ -301,7 +304,7 @@ __global__ void __launch_bounds__(6 * Plan<PX>::T) __attribute__((amdgpu_waves_p
     const int su = l + 1;
     const C* s[3] = {zu + (su + 1) * slot, zw + (su + 1) * slot, zv + su * slot};
     load_lines(s, 3);
-    Plan<PX>::template run<true>(lds + ln * LS, t, w);
+    Plan<PX>::template run_pp<true>(b0 + ln * LS, b1 + ln * LS, t, w);
     read_lines(0, n1);
     lds_bar();
     level_out(l, c0, n1);
