@@ -143,6 +143,27 @@ int lh_cond_stats_read(lh_handle* h, double* out, long long out_len, int reset);
 int lh_spectra_accumulate(lh_handle* h);
 int lh_spectra_read(lh_handle* h, double* out, int reset);
 
+/* Launch parameters and the autotuner (no counterpart in the reference).  The results do
+ * not depend on them: each only changes how work is split into blocks.
+ *   xraw_lc  levels per block of the sliding x pass        (>= 1)
+ *   zasm_lc  levels per block of the z assembly            (>= 1)
+ *   col_bs   threads per block of the column kernels       (32..256, multiple of 32)
+ * lh_autotune times `steps` CFL steps per candidate and keeps the fastest value of each
+ * parameter in turn; the state is saved before and restored after. */
+int lh_set_launch(lh_handle* h, int xraw_lc, int zasm_lc, int col_bs);
+int lh_get_launch(lh_handle* h, int* xraw_lc, int* zasm_lc, int* col_bs);
+int lh_autotune(lh_handle* h, int steps);
+
+/* Many independent channels (handles of any sizes on one device) advanced together: one
+ * CFL step of every instance per call, issued as one captured HIP graph in which the
+ * instances are parallel branches, each on its handle's stream (no counterpart in the
+ * reference; the instances share nothing). */
+typedef struct lh_batch lh_batch;
+int lh_batch_create(int n, lh_handle* const* handles, lh_batch** out);
+int lh_batch_step_cfl(lh_batch* b, double cfl);
+const char* lh_batch_last_error(const lh_batch* b);
+int lh_batch_destroy(lh_batch* b);
+
 /* Per-kernel timing (no counterpart in the reference; measurement support).  While
  * enabled, lh_step/lh_step_cfl issue their kernels directly (no graph) with a HIP event
  * pair around every launch on the handle's stream; durations accumulate per kernel

@@ -117,6 +117,14 @@ def load_library():
     lib.lh_cond_stats_read.argtypes = [vp, _dp, ctypes.c_longlong, ctypes.c_int]
     lib.lh_spectra_accumulate.argtypes = [vp]
     lib.lh_spectra_read.argtypes = [vp, _dp, ctypes.c_int]
+    lib.lh_set_launch.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+    lib.lh_get_launch.argtypes = [vp, _ip, _ip, _ip]
+    lib.lh_autotune.argtypes = [vp, ctypes.c_int]
+    lib.lh_batch_create.argtypes = [ctypes.c_int, ctypes.POINTER(vp), ctypes.POINTER(vp)]
+    lib.lh_batch_step_cfl.argtypes = [vp, ctypes.c_double]
+    lib.lh_batch_last_error.argtypes = [vp]
+    lib.lh_batch_last_error.restype = ctypes.c_char_p
+    lib.lh_batch_destroy.argtypes = [vp]
     lib.lh_timing_enable.argtypes = [vp, ctypes.c_int]
     lib.lh_timing_reset.argtypes = [vp]
     lib.lh_timing_read.argtypes = [vp, ctypes.c_int, _dp, ctypes.POINTER(ctypes.c_longlong), _dp]
@@ -130,6 +138,8 @@ EXPORTED = [
     "lh_cfl_dt", "lh_get_scalars", "lh_max_div", "lh_stats_accumulate", "lh_stats_read",
     "lh_cond_stats_size", "lh_cond_stats_read", "lh_spectra_accumulate", "lh_spectra_read",
     "lh_timing_enable", "lh_timing_reset", "lh_timing_read",
+    "lh_set_launch", "lh_get_launch", "lh_autotune",
+    "lh_batch_create", "lh_batch_step_cfl", "lh_batch_last_error", "lh_batch_destroy",
 ]
 
 KERNEL_NAMES = ["zinv", "xpass_nl", "zfwd", "assemble", "ib_z", "ib_x", "vel", "pressure", "misc",

@@ -343,3 +343,45 @@ def config_production() -> ChannelSetup:
     return make_setup(nx=384, nz=384, ny_total=289, dsty=24, re=550.0, alp=1.0, bet=2.0,
                       mpgx=-1.0, flag_ctpress=1, posth=0.15, dyq=10.0, ntilex=48,
                       ntilez=24, dstx=6, dstz=12)
+
+
+def _nearest_divisor(n: int, target: float, lo: int, hi: int, also: int | None = None):
+    """Divisor of n (and of `also`, when given) in [lo, hi] nearest to target, or None."""
+    best = None
+    for d in range(max(1, lo), hi + 1):
+        if n % d or (also is not None and also % d):
+            continue
+        if best is None or abs(d - target) < abs(best - target):
+            best = d
+    return best
+
+
+def random_instance(seed: int) -> ChannelSetup:
+    """One channel of the batched multi-instance workload: nx, nz from {32, 48, 64, 96}, ny
+    (v planes wall to wall) from {65, 97, 129}, Re_tau from {180, 300, 395}, canopy post
+    spacing uniform in [0.03pi, 0.1pi] (rounded to a tile count that divides the collocation
+    grid), post height uniform in [0.1h, 0.25h], post side a third of the spacing."""
+    rng = np.random.default_rng(seed)
+    for _ in range(100):
+        nx = int(rng.choice([32, 48, 64, 96]))
+        nz = int(rng.choice([32, 48, 64, 96]))
+        ny_total = int(rng.choice([65, 97, 129]))
+        re = float(rng.choice([180.0, 300.0, 395.0]))
+        posth = float(rng.uniform(0.1, 0.25))
+        sx = float(rng.uniform(0.03, 0.1)) * np.pi
+        sz = float(rng.uniform(0.03, 0.1)) * np.pi
+        px, pz = 3 * nx // 2, 3 * nz // 2
+        ntx = _nearest_divisor(px, 2.0 * np.pi / sx, 2, px // 2)
+        ntz = _nearest_divisor(pz, np.pi / sz, 2, pz // 2, also=px)
+        if ntx is None or ntz is None:
+            continue
+        dnx, dnz = px // ntx, pz // ntz
+        dstx = max(1, min(dnx - 1, round(dnx / 3)))
+        dstz = max(1, min(dnz - 1, round(dnz / 3)))
+        dsty = max(3, int(round(0.6 * posth * ny_total)))
+        try:
+            return make_setup(nx=nx, nz=nz, ny_total=ny_total, dsty=dsty, re=re, posth=posth,
+                              dyq=6.0, ntilex=ntx, ntilez=ntz, dstx=dstx, dstz=dstz)
+        except ValueError:
+            continue
+    raise ValueError(f"no valid instance for seed {seed}")

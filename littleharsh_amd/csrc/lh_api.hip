@@ -28,6 +28,21 @@
 
 using namespace lh;
 
+// Default launch parameters (per handle; lh_autotune may change them):
+//   LH_XRAW_LC  levels per block of the sliding x pass
+//   LH_ZASM_LC  levels per block of the z assembly
+//   LH_COL_BS   threads per block of the column kernels (one column per thread: small blocks
+//               spread the 74k columns of the production grid evenly over the 256 CUs)
+#ifndef LH_XRAW_LC
+#define LH_XRAW_LC 16
+#endif
+#ifndef LH_ZASM_LC
+#define LH_ZASM_LC 4
+#endif
+#ifndef LH_COL_BS
+#define LH_COL_BS 64
+#endif
+
 struct lh_handle {
   std::string err;
   int dev = 0;
@@ -74,6 +89,10 @@ struct lh_handle {
   // graphs
   hipGraphExec_t gexec[2] = {nullptr, nullptr};   // [0] fixed dt, [1] device CFL dt
   double graph_dt = -1.0;
+  // launch parameters (lh_autotune picks them per instance)
+  int xraw_lc = LH_XRAW_LC;
+  int zasm_lc = LH_ZASM_LC;
+  int col_bs = LH_COL_BS;
   // per-kernel timing
   bool timing = false;
   std::vector<hipEvent_t> evpool;
@@ -162,11 +181,6 @@ static bool supported_len(int n) {
 #ifndef LH_VEL_U
 #define LH_VEL_U 2
 #endif
-// threads per block of the column kernels: one column per thread, so small blocks spread
-// the 74k columns of the production grid evenly over the 256 CUs
-#ifndef LH_COL_BS
-#define LH_COL_BS 64
-#endif
 template <int N> constexpr int zlines() {
   return N >= 384 ? LH_ZLINES_LONG : (Plan<N>::T >= 8 ? 16 : 32);
 }
@@ -197,15 +211,12 @@ static void attr_z_t(int* dummy) {
   (void)hipFuncSetAttribute((const void*)k_zfwd<PZ, L, TR>,
                             hipFuncAttributeMaxDynamicSharedMemorySize, sh);
 }
-#ifndef LH_XRAW_LC
-#define LH_XRAW_LC 16
-#endif
 template <int PX, typename TR>
 static void launch_xraw_t(lh_handle* h, const XRAW& a, const void* tw) {
   size_t sh = (size_t)6 * (PX + 1) * sizeof(cx<TR>);
-  if (LH_XRAW_LC > 1) {
+  if (h->xraw_lc > 1) {
     sh = ((size_t)12 * (PX + 1) + Plan<PX>::template TwL<cx<TR>>::SIZE) * sizeof(cx<TR>);
-    const int nlev = h->dm.nyu - 2, lc = LH_XRAW_LC;
+    const int nlev = h->dm.nyu - 2, lc = h->xraw_lc;
     dim3 grid(h->dm.pz / 2, (nlev + lc - 1) / lc);
     hipLaunchKernelGGL((k_xraw2<PX, TR>), grid, dim3(6 * Plan<PX>::T), sh, h->st, a, h->dm,
                        (const cx<TR>*)tw, lc);
@@ -216,9 +227,6 @@ static void launch_xraw_t(lh_handle* h, const XRAW& a, const void* tw) {
   }
 }
 constexpr int kZasmCols = LH_ZASM_COLS;
-#ifndef LH_ZASM_LC
-#define LH_ZASM_LC 4
-#endif
 template <int PZ, typename TR>
 static size_t zasm_lds() {
   return ((size_t)2 * kZasmCols * (PZ + 1) + Plan<PZ>::template TwL<cx<TR>>::SIZE) *
@@ -332,7 +340,7 @@ static void issue_substep(lh_handle* h, int krk /*0-based*/, bool cfl_mode) {
     a.nu1 = nw[0]; a.nu2 = nw[1]; a.nu3 = nw[2];
     a.dthdyu = h->dthdyu; a.dthdyv = h->dthdyv; a.k1x = h->k1x; a.k1z = h->k1z;
     a.nib = (dm.mx + kZasmCols - 1) / kZasmCols;
-    a.lc = LH_ZASM_LC;
+    a.lc = h->zasm_lc;
     // in: six product Z rows per level; out: Nu_new
     const double by = 6.0 * nlev * zsl + spl * (2.0 * nlev + (nlev - 1));
     TIMED(h, LH_K_ASM, by, {
@@ -361,8 +369,8 @@ static void issue_substep(lh_handle* h, int krk /*0-based*/, bool cfl_mode) {
     const int npl = h->nibu > h->nibv ? h->nibu : h->nibv;
     // in: u (3 rows), p, Nu_old, Nu_new per canopy plane; out: rhs (+ Lu_y of the tips)
     TIMED(h, LH_K_RHS, spl * 7.0 * (2.0 * h->nibu + h->nibv),
-          hipLaunchKernelGGL(k_rhs_ib, dim3((dm.ncol + LH_COL_BS - 1) / LH_COL_BS, npl, 3),
-                             dim3(LH_COL_BS), 0, h->st, ra, dm));
+          hipLaunchKernelGGL(k_rhs_ib, dim3((dm.ncol + h->col_bs - 1) / h->col_bs, npl, 3),
+                             dim3(h->col_bs), 0, h->st, ra, dm));
     const double z64 = 16.0 * dm.pz * dm.mx;
     const double nib = 2.0 * h->nibu + h->nibv;
     ZJobs jobs{};
@@ -404,12 +412,12 @@ static void issue_substep(lh_handle* h, int krk /*0-based*/, bool cfl_mode) {
     const double by = spl * (2.0 * nuvw + 2.0 * dm.nyp) + 3.0 * 8.0 * dm.ncol * (double)dm.nyp;
     // in: u, p, Nu_old, Nu_new (rhs on canopy planes); eliminated rows out and back; out: u
     TIMED(h, LH_K_VEL, spl * (7.0 * nuvw),
-          hipLaunchKernelGGL(k_vel_rhs<LH_VEL_U>, dim3((dm.ncol + LH_COL_BS - 1) / LH_COL_BS, 3),
-                             dim3(LH_COL_BS), 0, h->st, ra, dm, (const double*)h->ayu,
+          hipLaunchKernelGGL(k_vel_rhs<LH_VEL_U>, dim3((dm.ncol + h->col_bs - 1) / h->col_bs, 3),
+                             dim3(h->col_bs), 0, h->st, ra, dm, (const double*)h->ayu,
                              (const double*)h->ayv));
     TIMED(h, LH_K_PRES, by - 2.0 * nuvw * spl + 2.0 * nuvw * spl,
-          hipLaunchKernelGGL(k_pres<LH_COL_U>, dim3((dm.ncol + LH_COL_BS - 1) / LH_COL_BS),
-                             dim3(LH_COL_BS), 0, h->st, a, dm));
+          hipLaunchKernelGGL(k_pres<LH_COL_U>, dim3((dm.ncol + h->col_bs - 1) / h->col_bs),
+                             dim3(h->col_bs), 0, h->st, a, dm));
   }
 }
 
@@ -879,6 +887,181 @@ int lh_timing_read(lh_handle* h, int kernel, double* total_ms, long long* launch
   if (total_ms) *total_ms = h->tms[kernel];
   if (launches) *launches = h->tcnt[kernel];
   if (bytes_per_launch) *bytes_per_launch = h->tcnt[kernel] ? h->tbytes[kernel] / h->tcnt[kernel] : 0.0;
+  return LH_OK;
+}
+
+// ---- launch parameters and the autotuner
+static void drop_graphs(lh_handle* h) {
+  for (auto& g : h->gexec)
+    if (g) { (void)hipGraphExecDestroy(g); g = nullptr; }
+}
+
+int lh_set_launch(lh_handle* h, int xraw_lc, int zasm_lc, int col_bs) {
+  if (!h) return LH_ERR_ARG;
+  if (xraw_lc < 1 || zasm_lc < 1 || col_bs < 32 || col_bs > 256 || col_bs % 32)
+    return fail(h, LH_ERR_ARG, "lh_set_launch: xraw_lc, zasm_lc >= 1, col_bs in 32..256 (multiple of 32)");
+  HIPCHK(h, hipStreamSynchronize(h->st));
+  h->xraw_lc = xraw_lc;
+  h->zasm_lc = zasm_lc;
+  h->col_bs = col_bs;
+  drop_graphs(h);
+  return LH_OK;
+}
+
+int lh_get_launch(lh_handle* h, int* xraw_lc, int* zasm_lc, int* col_bs) {
+  if (!h) return LH_ERR_ARG;
+  if (xraw_lc) *xraw_lc = h->xraw_lc;
+  if (zasm_lc) *zasm_lc = h->zasm_lc;
+  if (col_bs) *col_bs = h->col_bs;
+  return LH_OK;
+}
+
+// Coordinate search over the launch parameters: every candidate runs `steps` CFL steps
+// (after one warm-up step) timed with events on the handle's stream; the state, the
+// nonlinear terms and the device scalars are saved first and restored afterwards.
+int lh_autotune(lh_handle* h, int steps) {
+  if (!h) return LH_ERR_ARG;
+  if (steps < 1) steps = 2;
+  if (h->timing) return fail(h, LH_ERR_ARG, "lh_autotune: disable timing first");
+  const Dims& dm = h->dm;
+  HIPCHK(h, hipStreamSynchronize(h->st));
+  const size_t nu = (size_t)dm.nyu * dm.ncol, nv = (size_t)dm.nyv * dm.ncol,
+               np = (size_t)dm.nyp * dm.ncol;
+  double2* bufs[10] = {h->u1, h->u2, h->u3, h->p, h->nu1, h->nu2, h->nu3, h->nb1, h->nb2, h->nb3};
+  const size_t lens[10] = {nu, nv, nu, np, nu, nv, nu, nu, nv, nu};
+  size_t tot = 0;
+  for (size_t l : lens) tot += l;
+  double2* save = nullptr;
+  HIPCHK(h, dalloc(&save, tot + S_NSCAL));
+  size_t off = 0;
+  for (int q = 0; q < 10; ++q) {
+    HIPCHK(h, hipMemcpyAsync(save + off, bufs[q], lens[q] * 16, hipMemcpyDeviceToDevice, h->st));
+    off += lens[q];
+  }
+  HIPCHK(h, hipMemcpyAsync(save + off, h->scal, S_NSCAL * 8, hipMemcpyDeviceToDevice, h->st));
+  hipEvent_t e0, e1;
+  HIPCHK(h, hipEventCreate(&e0));
+  HIPCHK(h, hipEventCreate(&e1));
+  hipLaunchKernelGGL(k_set_scalar, dim3(1), dim3(1), 0, h->st, h->scal, (int)S_CFL, 0.5);
+  auto trial = [&](float* ms) -> int {
+    issue_step(h, true);
+    HIPCHK(h, hipEventRecord(e0, h->st));
+    for (int s = 0; s < steps; ++s) issue_step(h, true);
+    HIPCHK(h, hipEventRecord(e1, h->st));
+    HIPCHK(h, hipEventSynchronize(e1));
+    HIPCHK(h, hipGetLastError());
+    HIPCHK(h, hipEventElapsedTime(ms, e0, e1));
+    return LH_OK;
+  };
+  const int cand_bs[] = {64, 128, 256};
+  const int cand_x[] = {4, 8, 16, 32};
+  const int cand_z[] = {1, 2, 4, 8};
+  int r = LH_OK;
+  for (int pass = 0; pass < 3 && r == LH_OK; ++pass) {
+    const int* cand = pass == 0 ? cand_bs : (pass == 1 ? cand_x : cand_z);
+    const int nc = pass == 0 ? 3 : 4;
+    int* knob = pass == 0 ? &h->col_bs : (pass == 1 ? &h->xraw_lc : &h->zasm_lc);
+    int best = *knob;
+    float best_ms = 1e30f;
+    for (int c = 0; c < nc && r == LH_OK; ++c) {
+      *knob = cand[c];
+      float ms = 0.f;
+      r = trial(&ms);
+      if (ms < best_ms) { best_ms = ms; best = cand[c]; }
+    }
+    *knob = best;
+  }
+  off = 0;
+  for (int q = 0; q < 10; ++q) {
+    (void)hipMemcpyAsync(bufs[q], save + off, lens[q] * 16, hipMemcpyDeviceToDevice, h->st);
+    off += lens[q];
+  }
+  (void)hipMemcpyAsync(h->scal, save + off, S_NSCAL * 8, hipMemcpyDeviceToDevice, h->st);
+  (void)hipStreamSynchronize(h->st);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  (void)hipFree(save);
+  drop_graphs(h);
+  return r;
+}
+
+// ---- many independent channels advanced by one graph launch per step
+struct lh_batch {
+  std::vector<lh_handle*> hs;
+  hipStream_t master = nullptr;
+  hipGraphExec_t g = nullptr;
+  double cfl = -1.0;
+  std::vector<hipEvent_t> ev;
+  std::string err;
+};
+
+int lh_batch_create(int n, lh_handle* const* hs, lh_batch** out) {
+  if (!out || n < 1 || !hs) return LH_ERR_ARG;
+  *out = nullptr;
+  lh_batch* b = new lh_batch();
+  for (int i = 0; i < n; ++i) {
+    if (!hs[i]) { delete b; return LH_ERR_ARG; }
+    b->hs.push_back(hs[i]);
+  }
+  if (hipStreamCreate(&b->master) != hipSuccess) { delete b; return LH_ERR_HIP; }
+  b->ev.assign(n + 1, nullptr);
+  for (auto& e : b->ev)
+    if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) {
+      lh_batch_destroy(b);
+      return LH_ERR_HIP;
+    }
+  *out = b;
+  return LH_OK;
+}
+
+const char* lh_batch_last_error(const lh_batch* b) { return b ? b->err.c_str() : "null batch"; }
+
+int lh_batch_step_cfl(lh_batch* b, double cfl) {
+  if (!b) return LH_ERR_ARG;
+  if (!(cfl > 0.0)) { b->err = "cfl must be positive"; return LH_ERR_ARG; }
+  for (auto* h : b->hs)
+    if (h->timing) { b->err = "disable timing on every handle of a batch"; return LH_ERR_ARG; }
+  if (!b->g || b->cfl != cfl) {
+    if (b->g) { (void)hipGraphExecDestroy(b->g); b->g = nullptr; }
+    hipGraph_t g;
+    auto chk = [&](hipError_t e, const char* what) {
+      if (e != hipSuccess) b->err = std::string(what) + ": " + hipGetErrorString(e);
+      return e == hipSuccess;
+    };
+    if (!chk(hipStreamBeginCapture(b->master, hipStreamCaptureModeGlobal), "capture")) return LH_ERR_HIP;
+    bool ok = chk(hipEventRecord(b->ev[0], b->master), "fork");
+    for (size_t i = 0; i < b->hs.size() && ok; ++i) {
+      lh_handle* h = b->hs[i];
+      ok = chk(hipStreamWaitEvent(h->st, b->ev[0], 0), "fork wait");
+      hipLaunchKernelGGL(k_set_scalar, dim3(1), dim3(1), 0, h->st, h->scal, (int)S_CFL, cfl);
+      issue_step(h, true);
+      ok = ok && chk(hipEventRecord(b->ev[i + 1], h->st), "join");
+      ok = ok && chk(hipStreamWaitEvent(b->master, b->ev[i + 1], 0), "join wait");
+    }
+    const bool ended = chk(hipStreamEndCapture(b->master, &g), "end capture");
+    if (!ok || !ended) return LH_ERR_HIP;
+    if (!chk(hipGraphInstantiate(&b->g, g, nullptr, nullptr, 0), "instantiate")) {
+      (void)hipGraphDestroy(g);
+      return LH_ERR_HIP;
+    }
+    (void)hipGraphDestroy(g);
+    b->cfl = cfl;
+  }
+  if (hipGraphLaunch(b->g, b->master) != hipSuccess || hipStreamSynchronize(b->master) != hipSuccess) {
+    b->err = "batch graph launch failed";
+    return LH_ERR_HIP;
+  }
+  return LH_OK;
+}
+
+int lh_batch_destroy(lh_batch* b) {
+  if (!b) return LH_OK;
+  if (b->master) (void)hipStreamSynchronize(b->master);
+  if (b->g) (void)hipGraphExecDestroy(b->g);
+  for (auto e : b->ev)
+    if (e) (void)hipEventDestroy(e);
+  if (b->master) (void)hipStreamDestroy(b->master);
+  delete b;
   return LH_OK;
 }
 

@@ -141,6 +141,21 @@ class Channel:
         self._chk(self._lib.lh_spectra_read(self._h, _p(a), 1 if reset else 0), "lh_spectra_read")
         return a
 
+    # ---- launch parameters (results do not depend on them)
+    def set_launch(self, xraw_lc: int, zasm_lc: int, col_bs: int):
+        self._chk(self._lib.lh_set_launch(self._h, int(xraw_lc), int(zasm_lc), int(col_bs)),
+                  "lh_set_launch")
+
+    def get_launch(self):
+        a, b, c = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        self._chk(self._lib.lh_get_launch(self._h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c)),
+                  "lh_get_launch")
+        return dict(xraw_lc=a.value, zasm_lc=b.value, col_bs=c.value)
+
+    def autotune(self, steps: int = 2):
+        self._chk(self._lib.lh_autotune(self._h, int(steps)), "lh_autotune")
+        return self.get_launch()
+
     # ---- per-kernel timing (HIP events on the handle's stream)
     def timing(self, enable: bool):
         self._chk(self._lib.lh_timing_enable(self._h, 1 if enable else 0), "lh_timing_enable")
@@ -156,3 +171,32 @@ class Channel:
                                                ctypes.byref(by)), "lh_timing_read")
             out[name] = dict(ms=ms.value, launches=n.value, bytes_per_launch=by.value)
         return out
+
+
+class Batch:
+    """Independent channels advanced together: one CFL step of every instance per call, issued
+    as one HIP graph whose branches are the instances (lh_batch_step_cfl)."""
+
+    def __init__(self, channels):
+        self.channels = list(channels)
+        self._lib = capi.load_library()
+        arr = (ctypes.c_void_p * len(self.channels))(*[c._h.value for c in self.channels])
+        b = ctypes.c_void_p()
+        r = self._lib.lh_batch_create(len(self.channels), arr, ctypes.byref(b))
+        if r != capi.LH_OK:
+            raise LHError(f"lh_batch_create failed (status {r})")
+        self._b = b
+
+    def step_cfl(self, cfl: float):
+        r = self._lib.lh_batch_step_cfl(self._b, float(cfl))
+        if r != capi.LH_OK:
+            raise LHError(f"lh_batch_step_cfl: {self._lib.lh_batch_last_error(self._b).decode()} "
+                          f"(status {r})")
+
+    def close(self):
+        if getattr(self, "_b", None):
+            self._lib.lh_batch_destroy(self._b)
+            self._b = None
+
+    def __del__(self):
+        self.close()

@@ -1,0 +1,62 @@
+"""Launch parameters, the autotuner and the multi-instance batch (one graph launch per step
+for channels of different sizes) give the same results as the plain per-handle steps."""
+import numpy as np
+import pytest
+
+from littleharsh_amd.channel import random_instance
+from littleharsh_amd.initial import laminar_plus_noise
+
+from cases import tiny
+
+pytestmark = pytest.mark.gpu
+
+
+def test_launch_parameters_do_not_change_results(require_gpu):
+    from littleharsh_amd.solver import Channel
+    s = tiny()
+    st = laminar_plus_noise(s, seed=41)
+    a, b = Channel(s), Channel(s)
+    a.set_state(*st)
+    b.set_state(*st)
+    b.set_launch(4, 1, 128)
+    assert b.get_launch() == dict(xraw_lc=4, zasm_lc=1, col_bs=128)
+    for _ in range(2):
+        a.step(2e-4)
+        b.step(2e-4)
+    for x, y in zip(a.get_state(), b.get_state()):
+        assert np.array_equal(x, y)
+
+
+def test_autotune_keeps_the_state(require_gpu):
+    from littleharsh_amd.solver import Channel
+    s = tiny()
+    st = laminar_plus_noise(s, seed=42)
+    g = Channel(s)
+    g.set_state(*st)
+    p = g.autotune(1)
+    print("tuned", p, flush=True)
+    assert p["col_bs"] in (64, 128, 256) and p["xraw_lc"] in (4, 8, 16, 32) and p["zasm_lc"] in (1, 2, 4, 8)
+    for x, y in zip(g.get_state(), st):
+        assert np.array_equal(x, y)
+
+
+def test_batch_matches_single_instances(require_gpu):
+    from littleharsh_amd.solver import Batch, Channel
+    setups = [random_instance(i) for i in (0, 1, 2)]
+    states = [laminar_plus_noise(s, seed=50 + i) for i, s in enumerate(setups)]
+    chans, refs = [], []
+    for s, st in zip(setups, states):
+        c, r = Channel(s), Channel(s)
+        c.set_state(*st)
+        r.set_state(*st)
+        chans.append(c)
+        refs.append(r)
+    b = Batch(chans)
+    for _ in range(2):
+        b.step_cfl(0.5)
+        for r in refs:
+            r.step_cfl(0.5)
+    for c, r in zip(chans, refs):
+        for x, y in zip(c.get_state(), r.get_state()):
+            assert np.array_equal(x, y)
+    b.close()
