@@ -119,7 +119,8 @@ def main():
     dname, dk = dom
     avg_s = dk["ms"] / 1e3 / max(dk["launches"], 1)
     ach = dk["bytes_per_launch"] / avg_s / 1e9
-    traffic = load_traffic(f"traffic_{dname}_{args.workload}_{args.precision}.json")
+    tr = load_traffic(f"traffic_{args.workload}_{args.precision}.json")
+    traffic = (tr or {}).get("kernels", {}).get(dname)
     roof = {"bound": "hbm", "kernel": dname, "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
             "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
             "bytes_per_launch": dk["bytes_per_launch"], "avg_launch_ms": avg_s * 1e3,
