@@ -371,6 +371,99 @@ contains
     end subroutine
   end subroutine
 
+  ! spectra writer recvwrspec (spectra.f90:125-182) for one rank: sp(j, column) over the
+  ! jlim range of `grid`, written to the file named by fname (length n)
+  subroutine ref_write_spectra(grid, sp_in, istat_in, t_in, iter_in, mpgx_in, flen, fname) &
+      bind(C, name="ref_write_spectra")
+    integer(c_int), value :: grid, istat_in, iter_in, flen
+    real(c_double), value :: t_in, mpgx_in
+    real(c_double), intent(in) :: sp_in(*)
+    character(kind=c_char), intent(in) :: fname(flen)
+    real(8), allocatable :: sp(:,:), buff(:,:,:)
+    integer :: status(64), ierr, i, nj, nc
+    t = t_in; iter = iter_in; istat = istat_in; mpgx = mpgx_in
+    fnameimb = ''
+    do i = 1, flen
+      fnameimb(i:i) = fname(i)
+    end do
+    nj = jlim(2,grid,midband) - jlim(1,grid,midband) + 1
+    nc = columns_num(midband,0)
+    allocate(sp(jlim(1,grid,midband):jlim(2,grid,midband), nc))
+    sp = reshape(sp_in(1:nj*nc), (/nj, nc/))
+    allocate(buff(0:N(1,midband)/2, 1:N(2,midband)/2+1, jlim(1,grid,midband):jlim(2,grid,midband)))
+    call recvwrspec(sp, buff, grid, 0, status, ierr)
+    deallocate(sp, buff)
+  end subroutine
+
+  ! write_stats (stats.f90:255-416) for one rank owning the bottom-band planes: the plain
+  ! rows over those planes, the conditional arrays of that band.  Inputs: rows(13, nyu)
+  ! (slot-indexed over the full range of each grid) and the 13 conditional arrays one after
+  ! another in the order of write_stats, each (dnx, dnz, nb).
+  subroutine ref_write_stats(ntx, ntz, rows, cond, istat_in, t_in, iter_in, mpgx_in, dlen, dirc) &
+      bind(C, name="ref_write_stats")
+    integer(c_int), value :: ntx, ntz, istat_in, iter_in, dlen
+    real(c_double), value :: t_in, mpgx_in
+    real(c_double), intent(in) :: rows(*), cond(*)
+    character(kind=c_char), intent(in) :: dirc(dlen)
+    integer :: status(64), ierr, i, lo(3), hi(3), nyu, a, q, g, nb(3)
+    integer, parameter :: gq(13) = (/ ugrid, ugrid, vgrid, vgrid, ugrid, ugrid, pgrid, &
+                                      pgrid, ugrid, ugrid, vgrid, vgrid, vgrid /)
+    t = t_in; iter = iter_in; istat = istat_in; mpgx = mpgx_in
+    dirout = ''
+    do i = 1, dlen
+      dirout(i:i) = dirc(i)
+    end do
+    ntilex = ntx; ntilez = ntz
+    dnx = Ngal(1,1)/ntilex
+    dnz = Ngal(2,1)/ntilez
+    write(ext1,'(i4.4)') N(1,1)
+    write(ext2,'(i4.4)') N(2,1)
+    write(ext3,'(i4.4)') nn+2
+    write(ext4,'(i5.5)') int(10d0*t)
+    lo(ugrid) = N(4,0);   hi(ugrid) = N(4,1)
+    lo(vgrid) = N(3,0);   hi(vgrid) = N(3,1)
+    lo(pgrid) = N(4,0)+1; hi(pgrid) = N(4,1)-1
+    nb = hi - lo + 1
+    nyu = N(4,3) + 1 - N(4,0) + 1
+    if (associated(limPL_incw)) deallocate(limPL_incw)
+    allocate(limPL_incw(3,2,0:0))
+    limPL_incw(:,1,0) = lo
+    limPL_incw(:,2,0) = hi
+    if (associated(procs)) deallocate(procs)
+    allocate(procs(0:nband))
+    procs = (/ 0, 0, 1, 1 /)
+    call rowin(Um, 1, ugrid);  call rowin(U2m, 2, ugrid)
+    call rowin(Vm, 3, vgrid);  call rowin(V2m, 4, vgrid)
+    call rowin(Wm, 5, ugrid);  call rowin(W2m, 6, ugrid)
+    call rowin(Pm, 7, pgrid);  call rowin(P2m, 8, pgrid)
+    call rowin(UVm, 9, ugrid); call rowin(UWm, 10, ugrid); call rowin(VWm, 11, vgrid)
+    call rowin(wxm, 12, vgrid); call rowin(wx2m, 13, vgrid)
+    a = 0
+    call cin(UmC, 1); call cin(U2mC, 2); call cin(VmC, 3); call cin(V2mC, 4)
+    call cin(WmC, 5); call cin(W2mC, 6); call cin(PmC, 7); call cin(P2mC, 8)
+    call cin(UVmC, 9); call cin(UWmC, 10); call cin(VWmC, 11); call cin(wxmC, 12); call cin(wx2mC, 13)
+    call write_stats(0, status, ierr)
+  contains
+    subroutine rowin(x, r, grid)
+      real(8), pointer :: x(:)
+      integer :: r, grid, j
+      if (associated(x)) deallocate(x)
+      allocate(x(lo(grid):hi(grid)))
+      do j = lo(grid), hi(grid)
+        x(j) = rows((r-1)*nyu + (j - lo(grid)) + 1)
+      end do
+    end subroutine
+    subroutine cin(x, q1)
+      real(8), pointer :: x(:,:,:)
+      integer :: q1, m
+      if (associated(x)) deallocate(x)
+      allocate(x(dnx, dnz, 0:nb(gq(q1))-1))
+      m = dnx*dnz*nb(gq(q1))
+      x = reshape(cond(a+1:a+m), (/dnx, dnz, nb(gq(q1))/))
+      a = a + m
+    end subroutine
+  end subroutine
+
   subroutine plane_in(x, src, lo, hi)
     real(8), pointer :: x(:,:,:)
     real(c_double), intent(in) :: src(*)

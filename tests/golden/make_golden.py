@@ -14,6 +14,8 @@ inputs and the reference routines' outputs for one small channel:
 * buildsp / buildspUV                                              (spectra.f90:18-56)
 * stats and the conditional folds conddown1/2, condup1/2 on physical planes made by
   rft/cft from the seeded state                                    (stats.f90:1-253)
+* the files the reference writes: y_grid_canopy.dat, boundary_*.dat, stats_*.dat
+  (write_stats) and the spectra files (recvwrspec), as raw bytes
 * FFTPACK rft/cft forward and inverse on seeded data               (rft_buff.f, cft_buff.f)
 * a physical <-> spectral plane transform built from rft/cft with the 3/2 zero padding
 
@@ -191,6 +193,28 @@ def main():
                       *[_dp(np.ascontiguousarray(planes[k])) for k in ("u", "v", "w", "p", "vi", "wi", "wx")],
                       _dp(st), _dp(cb), _dp(ct))
         out.update(stats=st.reshape(13, nyu), cstats_bot=cb, cstats_top=ct)
+        # files written by the reference's own writers
+        out["file_y_grid"] = np.frombuffer(open(os.path.join(td, "y_grid_canopy.dat"), "rb").read(), np.uint8)
+        out["file_boundary"] = np.frombuffer(raw, np.uint8)
+        spw = np.zeros(nyu * ncol)
+        ref.ref_buildsp(ctypes.c_int(2), _dp(cp(u3)), _dp(cp(u3)), _dp(spw))
+        out["spec_w"] = spw.reshape(ncol, nyu)
+        t_w, it_w, is_w, mp_w = 1.25, 17, 3, -1.0
+        out["wr_meta"] = np.array([t_w, it_w, is_w, mp_w])
+        for tagn, grid, arr in (("U", 2, out["spec_u"]), ("V", 1, out["spec_v"]), ("W", 2, out["spec_w"]),
+                                ("UV", 2, out["spec_uv"]), ("P", 3, out["spec_p"])):
+            fn = os.path.join(td, f"sp{tagn}.dat").encode()
+            spc = np.ascontiguousarray(arr, dtype=np.float64)
+            ref.ref_write_spectra(ctypes.c_int(grid), _dp(spc), ctypes.c_int(is_w),
+                                  ctypes.c_double(t_w), ctypes.c_int(it_w), ctypes.c_double(mp_w),
+                                  ctypes.c_int(len(fn)), ctypes.c_char_p(fn))
+            out[f"file_sp{tagn}"] = np.frombuffer(open(fn, "rb").read(), np.uint8)
+        ref.ref_write_stats(ctypes.c_int(c["ntilex"]), ctypes.c_int(c["ntilez"]), _dp(st), _dp(cb),
+                            ctypes.c_int(is_w), ctypes.c_double(t_w), ctypes.c_int(it_w),
+                            ctypes.c_double(mp_w), ctypes.c_int(len(d)), ctypes.c_char_p(d))
+        fst = [f for f in os.listdir(td) if f.startswith("stats_")][0]
+        out["file_stats"] = np.frombuffer(open(os.path.join(td, fst), "rb").read(), np.uint8)
+        out["file_stats_name"] = np.frombuffer(fst.encode(), np.uint8)
     # ---- FFTPACK
     rng = np.random.default_rng(7)
     for n in (12, 24, 48, 96):
