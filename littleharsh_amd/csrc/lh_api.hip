@@ -189,7 +189,7 @@ template <int PZ, typename TR>
 static void launch_zinv_t(lh_handle* h, const ZJobs& jobs, int njobs, int maxn, const void* tw) {
   constexpr int L = zlines<PZ>();
   dim3 grid((h->dm.mx + L - 1) / L, maxn, njobs);
-  size_t sh = (size_t)L * (PZ + 1) * sizeof(cx<TR>);
+  size_t sh = (size_t)L * zls<PZ>() * sizeof(cx<TR>);
   hipLaunchKernelGGL((k_zinv<PZ, L, TR>), grid, dim3(L * Plan<PZ>::T), sh, h->st, jobs, h->dm,
                      (const cx<TR>*)tw);
 }
@@ -197,7 +197,7 @@ template <int PZ, typename TR>
 static void launch_zfwd_t(lh_handle* h, const ZJobs& jobs, int njobs, int maxn, const void* tw) {
   constexpr int L = zlines<PZ>();
   dim3 grid((h->dm.mx + L - 1) / L, maxn, njobs);
-  size_t sh = (size_t)L * (PZ + 1) * sizeof(cx<TR>);
+  size_t sh = (size_t)L * zls<PZ>() * sizeof(cx<TR>);
   hipLaunchKernelGGL((k_zfwd<PZ, L, TR>), grid, dim3(L * Plan<PZ>::T), sh, h->st, jobs, h->dm,
                      (const cx<TR>*)tw);
 }
@@ -205,7 +205,7 @@ template <int PZ, typename TR>
 static void attr_z_t(int* dummy) {
   constexpr int L = zlines<PZ>();
   (void)dummy;
-  const int sh = (int)(L * (PZ + 1) * sizeof(cx<TR>));
+  const int sh = (int)(L * zls<PZ>() * sizeof(cx<TR>));
   (void)hipFuncSetAttribute((const void*)k_zinv<PZ, L, TR>,
                             hipFuncAttributeMaxDynamicSharedMemorySize, sh);
   (void)hipFuncSetAttribute((const void*)k_zfwd<PZ, L, TR>,
@@ -215,7 +215,8 @@ template <int PX, typename TR>
 static void launch_xraw_t(lh_handle* h, const XRAW& a, const void* tw) {
   size_t sh = (size_t)6 * (PX + 1) * sizeof(cx<TR>);
   if (h->xraw_lc > 1) {
-    sh = ((size_t)12 * (PX + 1) + Plan<PX>::template TwL<cx<TR>>::SIZE) * sizeof(cx<TR>);
+    sh = ((size_t)12 * (PadLen<PX, LH_XRAW_PAD>::value + LH_XRAW_LSP) + Plan<PX>::template TwL<cx<TR>>::SIZE) *
+         sizeof(cx<TR>);
     const int nlev = h->dm.nyu - 2, lc = h->xraw_lc;
     dim3 grid(h->dm.pz / 2, (nlev + lc - 1) / lc);
     hipLaunchKernelGGL((k_xraw2<PX, TR>), grid, dim3(6 * Plan<PX>::T), sh, h->st, a, h->dm,

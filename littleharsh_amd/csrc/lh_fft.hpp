@@ -186,9 +186,14 @@ struct StageTwL {
   __device__ __forceinline__ C get(int q, int r) const { return p[q * (R - 1) + r - 1]; }
 };
 
+// Position of element e of a line stored with one pad slot after every 2^S elements
+// (S = 0: no padding); spreads the strided stage writes over the LDS banks.
+template <int S> __device__ __forceinline__ constexpr int lpad(int e) { return S ? e + (e >> S) : e; }
+template <int N, int S> struct PadLen { static constexpr int value = S ? N + (N >> S) : N; };
+
 // One Stockham stage of radix R on a line of N values; NS = product of earlier radices.
 // w: the thread's twiddles for this stage (unused when NS == 1).
-template <int N, int T, int R, int NS, bool INV, typename C, typename W>
+template <int N, int T, int R, int NS, bool INV, int S, typename C, typename W>
 __device__ __forceinline__ void stage(C* line, int t, const W& w) {
   constexpr int NB = N / R;
   constexpr int PER = (NB + T - 1) / T;
@@ -198,7 +203,7 @@ __device__ __forceinline__ void stage(C* line, int t, const W& w) {
     const int b = t + q * T;
     if (PER * T == NB || b < NB) {
 #pragma unroll
-      for (int r = 0; r < R; ++r) v[q][r] = line[b + r * NB];
+      for (int r = 0; r < R; ++r) v[q][r] = line[lpad<S>(b + r * NB)];
     }
   }
   lds_bar();
@@ -215,7 +220,7 @@ __device__ __forceinline__ void stage(C* line, int t, const W& w) {
       DFT<R, INV, C>::run(v[q]);
       const int idx = (b / NS) * NS * R + j;
 #pragma unroll
-      for (int r = 0; r < R; ++r) line[idx + r * NS] = v[q][r];
+      for (int r = 0; r < R; ++r) line[lpad<S>(idx + r * NS)] = v[q][r];
     }
   }
   lds_bar();
@@ -224,7 +229,7 @@ __device__ __forceinline__ void stage(C* line, int t, const W& w) {
 // The same stage out of place: read the line from src, write the autosorted result to dst.
 // Only the barrier after the writes is needed (the next stage reads dst and overwrites src,
 // whose reads all precede that barrier).
-template <int N, int T, int R, int NS, bool INV, typename C, typename W>
+template <int N, int T, int R, int NS, bool INV, int S, typename C, typename W>
 __device__ __forceinline__ void stage_pp(const C* src, C* dst, int t, const W& w) {
   constexpr int NB = N / R;
   constexpr int PER = (NB + T - 1) / T;
@@ -234,7 +239,7 @@ __device__ __forceinline__ void stage_pp(const C* src, C* dst, int t, const W& w
     if (PER * T == NB || b < NB) {
       C v[R];
 #pragma unroll
-      for (int r = 0; r < R; ++r) v[r] = src[b + r * NB];
+      for (int r = 0; r < R; ++r) v[r] = src[lpad<S>(b + r * NB)];
       const int j = b % NS;
       if constexpr (NS > 1) {
 #pragma unroll
@@ -243,7 +248,7 @@ __device__ __forceinline__ void stage_pp(const C* src, C* dst, int t, const W& w
       DFT<R, INV, C>::run(v);
       const int idx = (b / NS) * NS * R + j;
 #pragma unroll
-      for (int r = 0; r < R; ++r) dst[idx + r * NS] = v[r];
+      for (int r = 0; r < R; ++r) dst[lpad<S>(idx + r * NS)] = v[r];
     }
   }
   lds_bar();
@@ -268,15 +273,16 @@ struct NoTw {};
         s2.init(tab, t, tw, fill);                                                      \
       }                                                                                 \
     };                                                                                  \
-    template <bool INV, typename C, typename W> static __device__ void run(C* l, int t, const W& w) { \
-      stage<NN, TT, R1, 1, INV>(l, t, NoTw{});                                          \
-      stage<NN, TT, R2, R1, INV>(l, t, w.s2);                                           \
+    template <bool INV, int S = 0, typename C, typename W>                              \
+    static __device__ void run(C* l, int t, const W& w) {                               \
+      stage<NN, TT, R1, 1, INV, S>(l, t, NoTw{});                                       \
+      stage<NN, TT, R2, R1, INV, S>(l, t, w.s2);                                        \
     }                                                                                   \
     static constexpr int NSTAGE = 2;                                                    \
-    template <bool INV, typename C, typename W>                                         \
+    template <bool INV, int S = 0, typename C, typename W>                              \
     static __device__ void run_pp(C* a, C* b, int t, const W& w) {                      \
-      stage_pp<NN, TT, R1, 1, INV>(a, b, t, NoTw{});                                    \
-      stage_pp<NN, TT, R2, R1, INV>(b, a, t, w.s2);                                     \
+      stage_pp<NN, TT, R1, 1, INV, S>(a, b, t, NoTw{});                                 \
+      stage_pp<NN, TT, R2, R1, INV, S>(b, a, t, w.s2);                                  \
     }                                                                                   \
   };
 #define LH_PLAN3(NN, TT, R1, R2, R3)                                                    \
@@ -297,17 +303,18 @@ struct NoTw {};
         s3.init(tab + StageTwL<NN, TT, R2, R1, C>::SIZE, t, tw, fill);                  \
       }                                                                                 \
     };                                                                                  \
-    template <bool INV, typename C, typename W> static __device__ void run(C* l, int t, const W& w) { \
-      stage<NN, TT, R1, 1, INV>(l, t, NoTw{});                                          \
-      stage<NN, TT, R2, R1, INV>(l, t, w.s2);                                           \
-      stage<NN, TT, R3, R1 * R2, INV>(l, t, w.s3);                                      \
+    template <bool INV, int S = 0, typename C, typename W>                              \
+    static __device__ void run(C* l, int t, const W& w) {                               \
+      stage<NN, TT, R1, 1, INV, S>(l, t, NoTw{});                                       \
+      stage<NN, TT, R2, R1, INV, S>(l, t, w.s2);                                        \
+      stage<NN, TT, R3, R1 * R2, INV, S>(l, t, w.s3);                                   \
     }                                                                                   \
     static constexpr int NSTAGE = 3;                                                    \
-    template <bool INV, typename C, typename W>                                         \
+    template <bool INV, int S = 0, typename C, typename W>                              \
     static __device__ void run_pp(C* a, C* b, int t, const W& w) {                      \
-      stage_pp<NN, TT, R1, 1, INV>(a, b, t, NoTw{});                                    \
-      stage_pp<NN, TT, R2, R1, INV>(b, a, t, w.s2);                                     \
-      stage_pp<NN, TT, R3, R1 * R2, INV>(a, b, t, w.s3);                                \
+      stage_pp<NN, TT, R1, 1, INV, S>(a, b, t, NoTw{});                                 \
+      stage_pp<NN, TT, R2, R1, INV, S>(b, a, t, w.s2);                                  \
+      stage_pp<NN, TT, R3, R1 * R2, INV, S>(a, b, t, w.s3);                             \
     }                                                                                   \
   };
 

@@ -44,12 +44,19 @@ struct ZJobs {
   ZJob j[4];
 };
 
+#ifndef LH_Z_PAD
+#define LH_Z_PAD 0
+#endif
+// LDS line stride of the z transforms (lines padded by one slot per 2^LH_Z_PAD elements)
+template <int PZ> constexpr int zls() { return PadLen<PZ, LH_Z_PAD>::value + 1; }
+
 // inverse: spectral plane (nz x mx), zero-padded to pz in k, inverse cft along z
 template <int PZ, int L, typename TR>
 __global__ void __launch_bounds__(L* Plan<PZ>::T)
     k_zinv(ZJobs jobs, Dims dm, const cx<TR>* __restrict__ tw) {
   using C = cx<TR>;
-  constexpr int LS = PZ + 1;
+  constexpr int LS = zls<PZ>();
+  constexpr int PS = LH_Z_PAD;
   extern __shared__ char smem_raw[];
   C* lds = reinterpret_cast<C*>(smem_raw);
   const ZJob jb = jobs.j[blockIdx.z];
@@ -66,18 +73,18 @@ __global__ void __launch_bounds__(L* Plan<PZ>::T)
     if (i < mx) {
       const double2 v = src[(size_t)k * mx + i];
       const int kk = k < h ? k : k + (PZ - nz);
-      lds[l * LS + kk] = mk<C>((TR)v.x, (TR)v.y);
+      lds[l * LS + lpad<PS>(kk)] = mk<C>((TR)v.x, (TR)v.y);
     }
   }
   __syncthreads();
   const int l = threadIdx.x / Plan<PZ>::T, t = threadIdx.x % Plan<PZ>::T;
   typename Plan<PZ>::template Tw<C> w;
   w.load(t, tw);
-  Plan<PZ>::template run<true>(lds + l * LS, t, w);
+  Plan<PZ>::template run<true, PS>(lds + l * LS, t, w);
   C* dst = reinterpret_cast<C*>(jb.dst) + (size_t)(jb.slot0 + p) * PZ * mx;
   for (int e = threadIdx.x; e < L * PZ; e += blockDim.x) {
     const int l2 = e % L, z = e / L, i = i0 + l2;
-    if (i < mx) dst[(size_t)z * mx + i] = lds[l2 * LS + z];
+    if (i < mx) dst[(size_t)z * mx + i] = lds[l2 * LS + lpad<PS>(z)];
   }
 }
 
@@ -86,7 +93,8 @@ template <int PZ, int L, typename TR>
 __global__ void __launch_bounds__(L* Plan<PZ>::T)
     k_zfwd(ZJobs jobs, Dims dm, const cx<TR>* __restrict__ tw) {
   using C = cx<TR>;
-  constexpr int LS = PZ + 1;
+  constexpr int LS = zls<PZ>();
+  constexpr int PS = LH_Z_PAD;
   extern __shared__ char smem_raw[];
   C* lds = reinterpret_cast<C*>(smem_raw);
   const ZJob jb = jobs.j[blockIdx.z];
@@ -98,20 +106,20 @@ __global__ void __launch_bounds__(L* Plan<PZ>::T)
   const C* src = reinterpret_cast<const C*>(jb.src) + (size_t)(jb.slot0 + p) * PZ * mx;
   for (int e = threadIdx.x; e < L * PZ; e += blockDim.x) {
     const int l = e % L, z = e / L, i = i0 + l;
-    lds[l * LS + z] = i < mx ? src[(size_t)z * mx + i] : mk<C>(0, 0);
+    lds[l * LS + lpad<PS>(z)] = i < mx ? src[(size_t)z * mx + i] : mk<C>(0, 0);
   }
   __syncthreads();
   const int l = threadIdx.x / Plan<PZ>::T, t = threadIdx.x % Plan<PZ>::T;
   typename Plan<PZ>::template Tw<C> w;
   w.load(t, tw);
-  Plan<PZ>::template run<false>(lds + l * LS, t, w);
+  Plan<PZ>::template run<false, PS>(lds + l * LS, t, w);
   double2* dst = reinterpret_cast<double2*>(jb.dst) + (size_t)plane * dm.ncol;
   const double sz = 1.0 / (double)PZ;
   for (int e = threadIdx.x; e < L * nz; e += blockDim.x) {
     const int l2 = e % L, k = e / L, i = i0 + l2;
     if (i < mx) {
       const int kk = k < h ? k : k + (PZ - nz);
-      const C v = lds[l2 * LS + kk];
+      const C v = lds[l2 * LS + lpad<PS>(kk)];
       double2 o;
       o.x = sz * (double)v.x;
       o.y = sz * (double)v.y;
@@ -144,6 +152,19 @@ template <int PX, typename C>
 __device__ __forceinline__ void unpack_rows(const C* line, int k, C& a, C& b) {
   using T = decltype(C::x);
   const C zk = line[k], zn = line[k == 0 ? 0 : PX - k];
+  const T sx = (T)1 / (T)PX;
+  a = mk<C>((T)0.5 * (zk.x + zn.x), (T)0.5 * (zk.y - zn.y));
+  b = mk<C>((T)0.5 * (zk.y + zn.y), -(T)0.5 * (zk.x - zn.x));
+  a = mk<C>(sx * a.x, sx * a.y);
+  b = mk<C>(sx * b.x, sx * b.y);
+  if (k == 0) { a.y = 0; b.y = 0; }
+}
+
+// unpack_rows on a line stored with lpad<S> positions
+template <int PX, int S, typename C>
+__device__ __forceinline__ void unpack_rows_p(const C* line, int k, C& a, C& b) {
+  using T = decltype(C::x);
+  const C zk = line[lpad<S>(k)], zn = line[lpad<S>(k == 0 ? 0 : PX - k)];
   const T sx = (T)1 / (T)PX;
   a = mk<C>((T)0.5 * (zk.x + zn.x), (T)0.5 * (zk.y - zn.y));
   b = mk<C>((T)0.5 * (zk.y + zn.y), -(T)0.5 * (zk.x - zn.x));

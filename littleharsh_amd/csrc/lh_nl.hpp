@@ -160,6 +160,12 @@ __global__ void __launch_bounds__(6 * Plan<PX>::T)
 // line groups of the block transform together: six new lines (U, W at centres j+1, j+2 and
 // V at faces j, j+1) are inverse-transformed, then the six products of level j and those
 // of level j+1 are forward-transformed in turn.
+#ifndef LH_XRAW_PAD
+#define LH_XRAW_PAD 5
+#endif
+#ifndef LH_XRAW_LSP
+#define LH_XRAW_LSP 1
+#endif
 #ifndef LH_XRAW2_WPE
 #define LH_XRAW2_WPE 2
 #endif
@@ -168,7 +174,8 @@ __global__ void __launch_bounds__(6 * Plan<PX>::T) __attribute__((amdgpu_waves_p
     k_xraw2(XRAW a, Dims dm, const cx<TR>* __restrict__ tw, int lc) {
   using C = cx<TR>;
   constexpr int T = Plan<PX>::T;
-  constexpr int LS = PX + 1;
+  constexpr int PS = LH_XRAW_PAD;                  // line padding (one slot per 2^PS)
+  constexpr int LS = PadLen<PX, PS>::value + LH_XRAW_LSP;
   constexpr int NT = 6 * T;
   constexpr int NX = (PX + NT - 1) / NT;      // x positions per thread
   extern __shared__ char smem_raw[];
@@ -215,13 +222,13 @@ __global__ void __launch_bounds__(6 * Plan<PX>::T) __attribute__((amdgpu_waves_p
         if (k == 0) {
           line[0] = mk<C>(va[q].x, vb[q].x);
         } else {
-          line[k] = mk<C>(va[q].x - vb[q].y, va[q].y + vb[q].x);
-          line[PX - k] = mk<C>(va[q].x + vb[q].y, -va[q].y + vb[q].x);
+          line[lpad<PS>(k)] = mk<C>(va[q].x - vb[q].y, va[q].y + vb[q].x);
+          line[lpad<PS>(PX - k)] = mk<C>(va[q].x + vb[q].y, -va[q].y + vb[q].x);
         }
       }
     }
     const int nzr = PX - 2 * mx + 1;
-    for (int e = tid; e < n * nzr; e += NT) b0[(e / nzr) * LS + mx + e % nzr] = mk<C>(0, 0);
+    for (int e = tid; e < n * nzr; e += NT) b0[(e / nzr) * LS + lpad<PS>(mx + e % nzr)] = mk<C>(0, 0);
     lds_bar();
   };
   auto read_lines = [&](int s0, C (*dst)[NX]) {
@@ -230,7 +237,7 @@ __global__ void __launch_bounds__(6 * Plan<PX>::T) __attribute__((amdgpu_waves_p
 #pragma unroll
       for (int m = 0; m < NX; ++m) {
         const int x = tid + m * NT;
-        if (x < PX) dst[f][m] = bres[(s0 + f) * LS + x];
+        if (x < PX) dst[f][m] = bres[(s0 + f) * LS + lpad<PS>(x)];
       }
   };
   double cmax = 0.0;
@@ -248,12 +255,12 @@ __global__ void __launch_bounds__(6 * Plan<PX>::T) __attribute__((amdgpu_waves_p
         const TR vc0 = half * (Vb.x + Vt.x), vc1 = half * (Vb.y + Vt.y);
         const TR uf0 = half * (U.x + Ua.x), uf1 = half * (U.y + Ua.y);
         const TR wf0 = half * (W.x + Wa.x), wf1 = half * (W.y + Wa.y);
-        b0[0 * LS + x] = mk<C>(U.x * U.x, U.y * U.y);
-        b0[1 * LS + x] = mk<C>(U.x * W.x, U.y * W.y);
-        b0[2 * LS + x] = mk<C>(W.x * W.x, W.y * W.y);
-        b0[3 * LS + x] = mk<C>(vc0 * vc0, vc1 * vc1);
-        b0[4 * LS + x] = mk<C>(uf0 * Vt.x, uf1 * Vt.y);
-        b0[5 * LS + x] = mk<C>(Vt.x * wf0, Vt.y * wf1);
+        b0[0 * LS + lpad<PS>(x)] = mk<C>(U.x * U.x, U.y * U.y);
+        b0[1 * LS + lpad<PS>(x)] = mk<C>(U.x * W.x, U.y * W.y);
+        b0[2 * LS + lpad<PS>(x)] = mk<C>(W.x * W.x, W.y * W.y);
+        b0[3 * LS + lpad<PS>(x)] = mk<C>(vc0 * vc0, vc1 * vc1);
+        b0[4 * LS + lpad<PS>(x)] = mk<C>(uf0 * Vt.x, uf1 * Vt.y);
+        b0[5 * LS + lpad<PS>(x)] = mk<C>(Vt.x * wf0, Vt.y * wf1);
         if (a.want_cfl) {
           const double s0 = fabs((double)U.x) * dm.dxi + fabs((double)vc0) * dyi + fabs((double)W.x) * dm.dzi;
           const double s1 = fabs((double)U.y) * dm.dxi + fabs((double)vc1) * dyi + fabs((double)W.y) * dm.dzi;
@@ -262,11 +269,11 @@ __global__ void __launch_bounds__(6 * Plan<PX>::T) __attribute__((amdgpu_waves_p
       }
     }
     lds_bar();
-    Plan<PX>::template run_pp<false>(b0 + ln * LS, b1 + ln * LS, t, w);
+    Plan<PX>::template run_pp<false, PS>(b0 + ln * LS, b1 + ln * LS, t, w);
     for (int e = tid; e < 6 * mx; e += NT) {
       const int q = e / mx, k = e % mx;
       C va, vb;
-      unpack_rows<PX>(bres + q * LS, k, va, vb);
+      unpack_rows_p<PX, PS>(bres + q * LS, k, va, vb);
       C* dst = zp + ((size_t)q * nlev + l) * slot + (size_t)z0 * mx;
       dst[k] = va;
       dst[mx + k] = vb;
@@ -279,7 +286,7 @@ __global__ void __launch_bounds__(6 * Plan<PX>::T) __attribute__((amdgpu_waves_p
     const int su = l0 + 1;
     const C* s[3] = {zu + su * slot, zw + su * slot, zv + (su - 1) * slot};
     load_lines(s, 3);
-    Plan<PX>::template run_pp<true>(b0 + ln * LS, b1 + ln * LS, t, w);
+    Plan<PX>::template run_pp<true, PS>(b0 + ln * LS, b1 + ln * LS, t, w);
     read_lines(0, c0);
     lds_bar();
   }
@@ -289,7 +296,7 @@ __global__ void __launch_bounds__(6 * Plan<PX>::T) __attribute__((amdgpu_waves_p
     const C* s[6] = {zu + (su + 1) * slot, zw + (su + 1) * slot, zv + su * slot,
                      zu + (su + 2) * slot, zw + (su + 2) * slot, zv + (su + 1) * slot};
     load_lines(s, 6);
-    Plan<PX>::template run_pp<true>(b0 + ln * LS, b1 + ln * LS, t, w);
+    Plan<PX>::template run_pp<true, PS>(b0 + ln * LS, b1 + ln * LS, t, w);
     read_lines(0, n1);
     read_lines(3, n2);
     lds_bar();
@@ -304,7 +311,7 @@ __global__ void __launch_bounds__(6 * Plan<PX>::T) __attribute__((amdgpu_waves_p
     const int su = l + 1;
     const C* s[3] = {zu + (su + 1) * slot, zw + (su + 1) * slot, zv + su * slot};
     load_lines(s, 3);
-    Plan<PX>::template run_pp<true>(b0 + ln * LS, b1 + ln * LS, t, w);
+    Plan<PX>::template run_pp<true, PS>(b0 + ln * LS, b1 + ln * LS, t, w);
     read_lines(0, n1);
     lds_bar();
     level_out(l, c0, n1);
