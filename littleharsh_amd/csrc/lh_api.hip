@@ -215,7 +215,7 @@ template <int PX, typename TR>
 static void launch_xraw_t(lh_handle* h, const XRAW& a, const void* tw) {
   size_t sh = (size_t)6 * (PX + 1) * sizeof(cx<TR>);
   if (h->xraw_lc > 1) {
-    sh = ((size_t)12 * (PadLen<PX, LH_XRAW_PAD>::value + LH_XRAW_LSP) + Plan<PX>::template TwL<cx<TR>>::SIZE) *
+    sh = ((size_t)15 * (PadLen<PX, LH_XRAW_PAD>::value + LH_XRAW_LSP) + tw_lds<PX, cx<TR>, LH_XRAW_TWR>()) *
          sizeof(cx<TR>);
     const int nlev = h->dm.nyu - 2, lc = h->xraw_lc;
     dim3 grid(h->dm.pz / 2, (nlev + lc - 1) / lc);
@@ -230,8 +230,7 @@ static void launch_xraw_t(lh_handle* h, const XRAW& a, const void* tw) {
 constexpr int kZasmCols = LH_ZASM_COLS;
 template <int PZ, typename TR>
 static size_t zasm_lds() {
-  return ((size_t)2 * kZasmCols * (PZ + 1) + Plan<PZ>::template TwL<cx<TR>>::SIZE) *
-         sizeof(cx<TR>);
+  return ((size_t)2 * kZasmCols * (PZ + 1) + tw_lds<PZ, cx<TR>, LH_ZASM_TWR>()) * sizeof(cx<TR>);
 }
 template <int PZ, typename TR>
 static void launch_zasm_t(lh_handle* h, const ZASM& a, const void* tw) {

@@ -12,6 +12,7 @@
 // by the callers.
 #pragma once
 #include <hip/hip_runtime.h>
+#include <type_traits>
 
 namespace lh {
 
@@ -162,6 +163,10 @@ struct StageTw {
     }
   }
   __device__ __forceinline__ C get(int q, int r) const { return w[q][r - 1]; }
+  template <bool INV, int RR> __device__ __forceinline__ void apply(int q, C (&v)[RR]) const {
+#pragma unroll
+    for (int r = 1; r < R; ++r) v[r] = INV ? cmulc(v[r], w[q][r - 1]) : cmul(v[r], w[q][r - 1]);
+  }
 };
 
 // The same twiddles kept in an LDS table (one row of PER*(R-1) values per line position t)
@@ -184,6 +189,42 @@ struct StageTwL {
     }
   }
   __device__ __forceinline__ C get(int q, int r) const { return p[q * (R - 1) + r - 1]; }
+  template <bool INV, int RR> __device__ __forceinline__ void apply(int q, C (&v)[RR]) const {
+#pragma unroll
+    for (int r = 1; r < R; ++r) v[r] = INV ? cmulc(v[r], get(q, r)) : cmul(v[r], get(q, r));
+  }
+};
+
+// The same twiddles from one base value per butterfly: W^{j r} for r = 2..R-1 is formed by
+// repeated multiplication with W^j (a rotation recurrence, as FFTPACK builds its tables),
+// so a stage keeps PER complex values in registers and reads no table.  The rounding error
+// of the r-th power grows like r ulp (R <= 16 here).
+template <int N, int T, int R, int NS, typename C>
+struct StageTwR {
+  static constexpr int NB = N / R;
+  static constexpr int PER = (NB + T - 1) / T;
+  C w[PER];
+  __device__ __forceinline__ void load(int t, const C* __restrict__ tw) {
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+      const int b = t + q * T;
+      const int j = (b < NB ? b : 0) % NS;
+      w[q] = tw[j * (N / (NS * R))];
+    }
+  }
+  template <bool INV, int RR> __device__ __forceinline__ void apply(int q, C (&v)[RR]) const {
+    C b = w[q];
+    // an empty register constraint: the powers must not be hoisted out of the caller's
+    // loops, where they would hold the full table in registers again
+    asm volatile("" : "+v"(b.x), "+v"(b.y));
+    C p = b;
+    v[1] = INV ? cmulc(v[1], p) : cmul(v[1], p);
+#pragma unroll
+    for (int r = 2; r < R; ++r) {
+      p = cmul(p, b);
+      v[r] = INV ? cmulc(v[r], p) : cmul(v[r], p);
+    }
+  }
 };
 
 // Position of element e of a line stored with one pad slot after every 2^S elements
@@ -212,11 +253,7 @@ __device__ __forceinline__ void stage(C* line, int t, const W& w) {
     const int b = t + q * T;
     if (PER * T == NB || b < NB) {
       const int j = b % NS;
-      if constexpr (NS > 1) {
-#pragma unroll
-        for (int r = 1; r < R; ++r)
-          v[q][r] = INV ? cmulc(v[q][r], w.get(q, r)) : cmul(v[q][r], w.get(q, r));
-      }
+      if constexpr (NS > 1) w.template apply<INV>(q, v[q]);
       DFT<R, INV, C>::run(v[q]);
       const int idx = (b / NS) * NS * R + j;
 #pragma unroll
@@ -241,10 +278,7 @@ __device__ __forceinline__ void stage_pp(const C* src, C* dst, int t, const W& w
 #pragma unroll
       for (int r = 0; r < R; ++r) v[r] = src[lpad<S>(b + r * NB)];
       const int j = b % NS;
-      if constexpr (NS > 1) {
-#pragma unroll
-        for (int r = 1; r < R; ++r) v[r] = INV ? cmulc(v[r], w.get(q, r)) : cmul(v[r], w.get(q, r));
-      }
+      if constexpr (NS > 1) w.template apply<INV>(q, v);
       DFT<R, INV, C>::run(v);
       const int idx = (b / NS) * NS * R + j;
 #pragma unroll
@@ -272,6 +306,10 @@ struct NoTw {};
       __device__ __forceinline__ void init(C* tab, int t, const C* tw, bool fill) {     \
         s2.init(tab, t, tw, fill);                                                      \
       }                                                                                 \
+    };                                                                                  \
+    template <typename C> struct TwR {                                                  \
+      StageTwR<NN, TT, R2, R1, C> s2;                                                   \
+      __device__ __forceinline__ void load(int t, const C* tw) { s2.load(t, tw); }      \
     };                                                                                  \
     template <bool INV, int S = 0, typename C, typename W>                              \
     static __device__ void run(C* l, int t, const W& w) {                               \
@@ -303,6 +341,11 @@ struct NoTw {};
         s3.init(tab + StageTwL<NN, TT, R2, R1, C>::SIZE, t, tw, fill);                  \
       }                                                                                 \
     };                                                                                  \
+    template <typename C> struct TwR {                                                  \
+      StageTwR<NN, TT, R2, R1, C> s2;                                                   \
+      StageTwR<NN, TT, R3, R1 * R2, C> s3;                                              \
+      __device__ __forceinline__ void load(int t, const C* tw) { s2.load(t, tw); s3.load(t, tw); } \
+    };                                                                                  \
     template <bool INV, int S = 0, typename C, typename W>                              \
     static __device__ void run(C* l, int t, const W& w) {                               \
       stage<NN, TT, R1, 1, INV, S>(l, t, NoTw{});                                       \
@@ -319,6 +362,21 @@ struct NoTw {};
   };
 
 template <int N> struct Plan;
+
+// Twiddle source of a kernel: the LDS table (REC false) or per-butterfly bases in
+// registers with the powers formed on the fly (REC true); LDS entries the choice needs.
+template <int N, typename C, bool REC>
+using TwSel = typename std::conditional<REC, typename Plan<N>::template TwR<C>,
+                                        typename Plan<N>::template TwL<C>>::type;
+template <int N, typename C, bool REC> constexpr int tw_lds() {
+  return REC ? 0 : Plan<N>::template TwL<C>::SIZE;
+}
+template <bool REC, typename W, typename C>
+__device__ __forceinline__ void tw_setup(W& w, C* tab, int t, const C* tw, bool fill) {
+  if constexpr (REC) w.load(t, tw);
+  else w.init(tab, t, tw, fill);
+}
+
 LH_PLAN3(576, 48, 12, 12, 4)
 LH_PLAN3(384, 48, 8, 12, 4)
 LH_PLAN2(144, 12, 12, 12)

@@ -47,6 +47,13 @@ struct ZJobs {
 #ifndef LH_Z_PAD
 #define LH_Z_PAD 0
 #endif
+#ifndef LH_Z_TWR
+#define LH_Z_TWR 0
+#endif
+// register twiddles of the z transforms: the full per-thread set, or bases with powers
+template <int PZ, typename C>
+using ZTw = typename std::conditional<LH_Z_TWR, typename Plan<PZ>::template TwR<C>,
+                                      typename Plan<PZ>::template Tw<C>>::type;
 // LDS line stride of the z transforms (lines padded by one slot per 2^LH_Z_PAD elements)
 template <int PZ> constexpr int zls() { return PadLen<PZ, LH_Z_PAD>::value + 1; }
 
@@ -78,7 +85,7 @@ __global__ void __launch_bounds__(L* Plan<PZ>::T)
   }
   __syncthreads();
   const int l = threadIdx.x / Plan<PZ>::T, t = threadIdx.x % Plan<PZ>::T;
-  typename Plan<PZ>::template Tw<C> w;
+  ZTw<PZ, C> w;
   w.load(t, tw);
   Plan<PZ>::template run<true, PS>(lds + l * LS, t, w);
   C* dst = reinterpret_cast<C*>(jb.dst) + (size_t)(jb.slot0 + p) * PZ * mx;
@@ -110,7 +117,7 @@ __global__ void __launch_bounds__(L* Plan<PZ>::T)
   }
   __syncthreads();
   const int l = threadIdx.x / Plan<PZ>::T, t = threadIdx.x % Plan<PZ>::T;
-  typename Plan<PZ>::template Tw<C> w;
+  ZTw<PZ, C> w;
   w.load(t, tw);
   Plan<PZ>::template run<false, PS>(lds + l * LS, t, w);
   double2* dst = reinterpret_cast<double2*>(jb.dst) + (size_t)plane * dm.ncol;

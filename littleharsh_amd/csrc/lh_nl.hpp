@@ -169,6 +169,9 @@ __global__ void __launch_bounds__(6 * Plan<PX>::T)
 #ifndef LH_XRAW2_WPE
 #define LH_XRAW2_WPE 2
 #endif
+#ifndef LH_XRAW_TWR
+#define LH_XRAW_TWR 0
+#endif
 template <int PX, typename TR>
 __global__ void __launch_bounds__(6 * Plan<PX>::T) __attribute__((amdgpu_waves_per_eu(LH_XRAW2_WPE)))
     k_xraw2(XRAW a, Dims dm, const cx<TR>* __restrict__ tw, int lc) {
@@ -193,12 +196,21 @@ __global__ void __launch_bounds__(6 * Plan<PX>::T) __attribute__((amdgpu_waves_p
   const C* zv = reinterpret_cast<const C*>(a.zv);
   C* zp = reinterpret_cast<C*>(a.zp);
   const int ln = tid / T, t = tid % T;
-  // two sets of six lines (out-of-place transforms), then the twiddle table
+  // two sets of six lines (out-of-place transforms), three lines that keep the upper level
+  // of a level pair until its turn, then the twiddle table
   C* const b0 = lds;
   C* const b1 = lds + 6 * LS;
-  C* const bres = (Plan<PX>::NSTAGE % 2) ? b1 : b0;   // where a transform of b0 ends
-  typename Plan<PX>::template TwL<C> w;
-  w.init(lds + 12 * LS, t, tw, ln == 0);
+  C* const keep = lds + 12 * LS;
+  // An inverse transform of line f starts in in_line(f) and ends in out_line(f): lines 3..5
+  // (the upper level of a pair) end in `keep` whatever the parity of the stage count.
+  constexpr bool ODD = Plan<PX>::NSTAGE % 2 == 1;
+  auto in_line = [&](int f) { return ODD || f < 3 ? b0 + f * LS : keep + (f - 3) * LS; };
+  auto other_line = [&](int f) { return !ODD || f < 3 ? b1 + f * LS : keep + (f - 3) * LS; };
+  C* const bres = ODD ? b1 : b0;               // where the transforms of lines 0..2 end
+  C* const mine_a = in_line(ln);
+  C* const mine_b = other_line(ln);
+  TwSel<PX, C, LH_XRAW_TWR> w;
+  tw_setup<LH_XRAW_TWR>(w, lds + 15 * LS, t, tw, ln == 0);
 
   // packed rows (z0, z0+1) of n Z slots -> line slots 0..n-1
   auto load_lines = [&](const C* const* src, int n) {
@@ -218,7 +230,7 @@ __global__ void __launch_bounds__(6 * Plan<PX>::T) __attribute__((amdgpu_waves_p
       const int e = tid + q * NT;
       if (e < n * mx) {
         const int f = e / mx, k = e % mx;
-        C* line = b0 + f * LS;
+        C* line = in_line(f);
         if (k == 0) {
           line[0] = mk<C>(va[q].x, vb[q].x);
         } else {
@@ -228,16 +240,16 @@ __global__ void __launch_bounds__(6 * Plan<PX>::T) __attribute__((amdgpu_waves_p
       }
     }
     const int nzr = PX - 2 * mx + 1;
-    for (int e = tid; e < n * nzr; e += NT) b0[(e / nzr) * LS + lpad<PS>(mx + e % nzr)] = mk<C>(0, 0);
+    for (int e = tid; e < n * nzr; e += NT) in_line(e / nzr)[lpad<PS>(mx + e % nzr)] = mk<C>(0, 0);
     lds_bar();
   };
-  auto read_lines = [&](int s0, C (*dst)[NX]) {
+  auto read_lines = [&](const C* src, C (*dst)[NX]) {
 #pragma unroll
     for (int f = 0; f < 3; ++f)
 #pragma unroll
       for (int m = 0; m < NX; ++m) {
         const int x = tid + m * NT;
-        if (x < PX) dst[f][m] = bres[(s0 + f) * LS + lpad<PS>(x)];
+        if (x < PX) dst[f][m] = src[f * LS + lpad<PS>(x)];
       }
   };
   double cmax = 0.0;
@@ -278,17 +290,20 @@ __global__ void __launch_bounds__(6 * Plan<PX>::T) __attribute__((amdgpu_waves_p
       dst[k] = va;
       dst[mx + k] = vb;
     }
-    lds_bar();
+    // With an odd stage count the unpack reads b1 and whatever follows writes b0 (products,
+    // loads) before a barrier, so only an even count needs one here.
+    if constexpr (!ODD) lds_bar();
   };
 
-  C c0[3][NX], n1[3][NX], n2[3][NX];
+  // c0: physical U(j), W(j), V(j-1) of the current level; n1: those of the next one
+  C c0[3][NX], n1[3][NX];
   {
     const int su = l0 + 1;
     const C* s[3] = {zu + su * slot, zw + su * slot, zv + (su - 1) * slot};
     load_lines(s, 3);
-    Plan<PX>::template run_pp<true, PS>(b0 + ln * LS, b1 + ln * LS, t, w);
-    read_lines(0, c0);
-    lds_bar();
+    Plan<PX>::template run_pp<true, PS>(mine_a, mine_b, t, w);
+    read_lines(bres, c0);
+    if constexpr (!ODD) lds_bar();   // odd: read b1 while the next writes go to b0
   }
   int l = l0;
   for (; l + 1 < l1; l += 2) {
@@ -296,24 +311,21 @@ __global__ void __launch_bounds__(6 * Plan<PX>::T) __attribute__((amdgpu_waves_p
     const C* s[6] = {zu + (su + 1) * slot, zw + (su + 1) * slot, zv + su * slot,
                      zu + (su + 2) * slot, zw + (su + 2) * slot, zv + (su + 1) * slot};
     load_lines(s, 6);
-    Plan<PX>::template run_pp<true, PS>(b0 + ln * LS, b1 + ln * LS, t, w);
-    read_lines(0, n1);
-    read_lines(3, n2);
-    lds_bar();
+    Plan<PX>::template run_pp<true, PS>(mine_a, mine_b, t, w);
+    read_lines(bres, n1);
+    if constexpr (!ODD) lds_bar();
     level_out(l, c0, n1);
-    level_out(l + 1, n1, n2);
-#pragma unroll
-    for (int f = 0; f < 3; ++f)
-#pragma unroll
-      for (int m = 0; m < NX; ++m) c0[f][m] = n2[f][m];
+    // the upper level waited in `keep`; it becomes the current level of the next pair
+    read_lines(keep, c0);
+    level_out(l + 1, n1, c0);
   }
   if (l < l1) {
     const int su = l + 1;
     const C* s[3] = {zu + (su + 1) * slot, zw + (su + 1) * slot, zv + su * slot};
     load_lines(s, 3);
-    Plan<PX>::template run_pp<true, PS>(b0 + ln * LS, b1 + ln * LS, t, w);
-    read_lines(0, n1);
-    lds_bar();
+    Plan<PX>::template run_pp<true, PS>(mine_a, mine_b, t, w);
+    read_lines(bres, n1);
+    if constexpr (!ODD) lds_bar();
     level_out(l, c0, n1);
   }
   if (a.want_cfl) {
@@ -348,6 +360,9 @@ struct ZASM {
 
 #ifndef LH_ZASM_ZB
 #define LH_ZASM_ZB 3
+#endif
+#ifndef LH_ZASM_TWR
+#define LH_ZASM_TWR 0
 #endif
 template <int PZ, int L, typename TR>
 __global__ void __launch_bounds__(2 * L * Plan<PZ>::T)
@@ -390,9 +405,10 @@ __global__ void __launch_bounds__(2 * L * Plan<PZ>::T)
   const int qb = f == 0 ? 1 : (f == 1 ? 2 : 5);       // B
   double2* __restrict__ nu = f == 0 ? a.nu1 : (f == 1 ? a.nu3 : a.nu2);
   const int ln = tid / T, t = tid % T;
-  // twiddles in an LDS table behind the 2L lines (registers hold the carried rows)
-  typename Plan<PZ>::template TwL<C> w;
-  w.init(lds + 2 * L * LS, t, tw, ln == 0);
+  // twiddles: an LDS table behind the 2L lines, or register bases (registers also hold the
+  // carried rows)
+  TwSel<PZ, C, LH_ZASM_TWR> w;
+  tw_setup<LH_ZASM_TWR>(w, lds + 2 * L * LS, t, tw, ln == 0);
   // lower operand of the y difference at the first level
   C rl[ITZ];
   {
