@@ -170,7 +170,7 @@ static bool supported_len(int n) {
 // templated launchers, dispatched on the transform lengths
 // ---------------------------------------------------------------------------------------
 #ifndef LH_ZLINES_LONG
-#define LH_ZLINES_LONG 4
+#define LH_ZLINES_LONG 16
 #endif
 #ifndef LH_ZASM_COLS
 #define LH_ZASM_COLS 8
@@ -188,17 +188,21 @@ template <int N> constexpr int zlines() {
 template <int PZ, typename TR>
 static void launch_zinv_t(lh_handle* h, const ZJobs& jobs, int njobs, int maxn, const void* tw) {
   constexpr int L = zlines<PZ>();
-  dim3 grid((h->dm.mx + L - 1) / L, maxn, njobs);
+  ZJobs jb = jobs;
+  jb.maxn = maxn;
+  dim3 grid(((h->dm.mx + L - 1) / L) * maxn * njobs);
   size_t sh = (size_t)L * zls<PZ>() * sizeof(cx<TR>);
-  hipLaunchKernelGGL((k_zinv<PZ, L, TR>), grid, dim3(L * Plan<PZ>::T), sh, h->st, jobs, h->dm,
+  hipLaunchKernelGGL((k_zinv<PZ, L, TR>), grid, dim3(L * Plan<PZ>::T), sh, h->st, jb, h->dm,
                      (const cx<TR>*)tw);
 }
 template <int PZ, typename TR>
 static void launch_zfwd_t(lh_handle* h, const ZJobs& jobs, int njobs, int maxn, const void* tw) {
   constexpr int L = zlines<PZ>();
-  dim3 grid((h->dm.mx + L - 1) / L, maxn, njobs);
+  ZJobs jb = jobs;
+  jb.maxn = maxn;
+  dim3 grid(((h->dm.mx + L - 1) / L) * maxn * njobs);
   size_t sh = (size_t)L * zls<PZ>() * sizeof(cx<TR>);
-  hipLaunchKernelGGL((k_zfwd<PZ, L, TR>), grid, dim3(L * Plan<PZ>::T), sh, h->st, jobs, h->dm,
+  hipLaunchKernelGGL((k_zfwd<PZ, L, TR>), grid, dim3(L * Plan<PZ>::T), sh, h->st, jb, h->dm,
                      (const cx<TR>*)tw);
 }
 template <int PZ, typename TR>

@@ -290,12 +290,37 @@ __device__ __forceinline__ void stage_pp(const C* src, C* dst, int t, const W& w
 
 struct NoTw {};
 
+// Stage pieces for transforms whose first stage takes its inputs from registers (loaded
+// from global memory by the caller) or whose last stage leaves its outputs in registers
+// (stored by the caller).  Butterfly b of a stage of radix R reads positions b + r*N/R and
+// writes the autosorted positions stage_pos(b, r).
+template <int N, int R, int NS> __device__ __forceinline__ int stage_pos(int b, int r) {
+  return (b / NS) * NS * R + b % NS + r * NS;
+}
+// twiddle (butterfly q of this thread), DFT and store to dst
+template <int N, int R, int NS, bool INV, int S, typename C, typename W>
+__device__ __forceinline__ void stage_regs_store(C (&v)[R], int b, int q, const W& w, C* dst) {
+  if constexpr (NS > 1) w.template apply<INV>(q, v);
+  DFT<R, INV, C>::run(v);
+#pragma unroll
+  for (int r = 0; r < R; ++r) dst[lpad<S>(stage_pos<N, R, NS>(b, r))] = v[r];
+}
+// load from src, twiddle and DFT; the outputs stay in v (positions stage_pos(b, r))
+template <int N, int R, int NS, bool INV, int S, typename C, typename W>
+__device__ __forceinline__ void stage_load_regs(const C* src, int b, int q, const W& w, C (&v)[R]) {
+#pragma unroll
+  for (int r = 0; r < R; ++r) v[r] = src[lpad<S>(b + r * (N / R))];
+  if constexpr (NS > 1) w.template apply<INV>(q, v);
+  DFT<R, INV, C>::run(v);
+}
+
 // Plans: threads per line T, the radix sequence, and the per-thread twiddle set Tw<C>.
 // run() works in place; run_pp() ping-pongs between two lines and leaves the result in
 // the second one when NSTAGE is odd, in the first one when it is even.
 #define LH_PLAN2(NN, TT, R1, R2)                                                        \
   template <> struct Plan<NN> {                                                         \
     static constexpr int T = TT;                                                        \
+    static constexpr int RA = R1, RB = R2, RC = 1;                                      \
     template <typename C> struct Tw {                                                   \
       StageTw<NN, TT, R2, R1, C> s2;                                                    \
       __device__ __forceinline__ void load(int t, const C* tw) { s2.load(t, tw); }      \
@@ -326,6 +351,7 @@ struct NoTw {};
 #define LH_PLAN3(NN, TT, R1, R2, R3)                                                    \
   template <> struct Plan<NN> {                                                         \
     static constexpr int T = TT;                                                        \
+    static constexpr int RA = R1, RB = R2, RC = R3;                                     \
     template <typename C> struct Tw {                                                   \
       StageTw<NN, TT, R2, R1, C> s2;                                                    \
       StageTw<NN, TT, R3, R1 * R2, C> s3;                                               \

@@ -42,6 +42,7 @@ struct ZJob {
 };
 struct ZJobs {
   ZJob j[4];
+  int maxn;             // planes of the largest job (the grid covers njobs x maxn planes)
 };
 
 #ifndef LH_Z_PAD
@@ -54,83 +55,195 @@ struct ZJobs {
 template <int PZ, typename C>
 using ZTw = typename std::conditional<LH_Z_TWR, typename Plan<PZ>::template TwR<C>,
                                       typename Plan<PZ>::template Tw<C>>::type;
-// LDS line stride of the z transforms (lines padded by one slot per 2^LH_Z_PAD elements)
-template <int PZ> constexpr int zls() { return PadLen<PZ, LH_Z_PAD>::value + 1; }
+#ifndef LH_Z_LSP
+#define LH_Z_LSP 2
+#endif
+#ifndef LH_Z_WPE
+#define LH_Z_WPE 3
+#endif
+// LDS line stride of the z transforms (lines padded by one slot per 2^LH_Z_PAD elements);
+// LH_Z_LSP = 2 puts the 8 columns of a 16-lane group on distinct banks.
+template <int PZ> constexpr int zls() { return PadLen<PZ, LH_Z_PAD>::value + LH_Z_LSP; }
+#ifndef LH_Z_XCD
+#define LH_Z_XCD 1
+#endif
+// Block -> (column group, plane, job).  The grid is linear: with LH_Z_XCD the linear id is
+// remapped so that consecutive column groups of a plane run on one XCD (blocks are dealt
+// round-robin over the 8 XCDs).
+struct ZBlock { int cg, p, job; };
+__device__ __forceinline__ ZBlock z_block(int ncg, int maxn) {
+  int bid = blockIdx.x;
+  if (LH_Z_XCD) {
+    const int nblk = gridDim.x, per = nblk / 8;
+    if (bid < per * 8) bid = (bid % 8) * per + bid / 8;
+  }
+  ZBlock r;
+  r.cg = bid % ncg;
+  r.p = (bid / ncg) % maxn;
+  r.job = bid / (ncg * maxn);
+  return r;
+}
+
+// z transforms of L kx columns per block.  Threads run column-fastest (l = tid % L), so
+// every global access of a wave covers L consecutive columns of a z row.  The first stage
+// takes its inputs straight from global memory into registers and the last stage stores
+// its outputs straight to global memory; only the middle stage (three-stage plans) reads
+// and writes the LDS line in place.
+template <int PZ> struct ZShape {
+  using P = Plan<PZ>;
+  static constexpr int T = P::T, R1 = P::RA, R2 = P::RB, R3 = P::RC;
+  static constexpr bool THREE = R3 > 1;
+  static constexpr int RL = THREE ? R3 : R2, NSL = PZ / RL;   // last stage
+  static constexpr int NB1 = PZ / R1, PER1 = (NB1 + T - 1) / T;
+  static constexpr int NB2 = PZ / R2, PER2 = (NB2 + T - 1) / T;
+  static constexpr int PERL = (NSL + T - 1) / T;
+};
+
+// middle stage of a three-stage plan, in place on the line
+template <int PZ, bool INV, int PS, typename C, typename W>
+__device__ __forceinline__ void z_middle(C* line, int t, const W& w) {
+  using Z = ZShape<PZ>;
+  C v[Z::PER2][Z::R2];
+#pragma unroll
+  for (int q = 0; q < Z::PER2; ++q) {
+    const int b = t + q * Z::T;
+    if (Z::PER2 * Z::T == Z::NB2 || b < Z::NB2) {
+#pragma unroll
+      for (int r = 0; r < Z::R2; ++r) v[q][r] = line[lpad<PS>(b + r * Z::NB2)];
+    }
+  }
+  lds_bar();
+#pragma unroll
+  for (int q = 0; q < Z::PER2; ++q) {
+    const int b = t + q * Z::T;
+    if (Z::PER2 * Z::T == Z::NB2 || b < Z::NB2)
+      stage_regs_store<PZ, Z::R2, Z::R1, INV, PS>(v[q], b, q, w.s2, line);
+  }
+  lds_bar();
+}
+// last stage: butterfly q of this thread, outputs at positions b + r * NSL
+template <int PZ, bool INV, int PS, typename C, typename W>
+__device__ __forceinline__ void z_last(const C* line, int b, int q, const W& w, C (&v)[ZShape<PZ>::RL]) {
+  using Z = ZShape<PZ>;
+  if constexpr (Z::THREE) stage_load_regs<PZ, Z::RL, Z::R1 * Z::R2, INV, PS>(line, b, q, w.s3, v);
+  else stage_load_regs<PZ, Z::RL, Z::R1, INV, PS>(line, b, q, w.s2, v);
+}
 
 // inverse: spectral plane (nz x mx), zero-padded to pz in k, inverse cft along z
 template <int PZ, int L, typename TR>
-__global__ void __launch_bounds__(L* Plan<PZ>::T)
+__global__ void __launch_bounds__(L* Plan<PZ>::T) __attribute__((amdgpu_waves_per_eu(LH_Z_WPE)))
     k_zinv(ZJobs jobs, Dims dm, const cx<TR>* __restrict__ tw) {
   using C = cx<TR>;
+  using Z = ZShape<PZ>;
   constexpr int LS = zls<PZ>();
   constexpr int PS = LH_Z_PAD;
   extern __shared__ char smem_raw[];
   C* lds = reinterpret_cast<C*>(smem_raw);
-  const ZJob jb = jobs.j[blockIdx.z];
-  const int p = blockIdx.y;
+  const ZBlock zb = z_block((dm.mx + L - 1) / L, jobs.maxn);
+  const ZJob jb = jobs.j[zb.job];
+  const int p = zb.p;
   if (p >= jb.n) return;
   const int plane = jb.planes ? jb.planes[p] : jb.first + p;
-  const int i0 = blockIdx.x * L;
-  const double2* src = jb.src + (size_t)plane * dm.ncol;
   const int nz = dm.nz, mx = dm.mx, h = nz / 2;
-  for (int e = threadIdx.x; e < L * LS; e += blockDim.x) lds[e] = mk<C>(0, 0);
-  __syncthreads();
-  for (int e = threadIdx.x; e < L * nz; e += blockDim.x) {
-    const int l = e % L, k = e / L, i = i0 + l;
-    if (i < mx) {
-      const double2 v = src[(size_t)k * mx + i];
-      const int kk = k < h ? k : k + (PZ - nz);
-      lds[l * LS + lpad<PS>(kk)] = mk<C>((TR)v.x, (TR)v.y);
-    }
-  }
-  __syncthreads();
-  const int l = threadIdx.x / Plan<PZ>::T, t = threadIdx.x % Plan<PZ>::T;
+  const int l = threadIdx.x % L, t = threadIdx.x / L;
+  const int i = zb.cg * L + l;
+  const bool ok = i < mx;
+  const double2* src = jb.src + (size_t)plane * dm.ncol + i;
+  C* line = lds + l * LS;
   ZTw<PZ, C> w;
   w.load(t, tw);
-  Plan<PZ>::template run<true, PS>(lds + l * LS, t, w);
-  C* dst = reinterpret_cast<C*>(jb.dst) + (size_t)(jb.slot0 + p) * PZ * mx;
-  for (int e = threadIdx.x; e < L * PZ; e += blockDim.x) {
-    const int l2 = e % L, z = e / L, i = i0 + l2;
-    if (i < mx) dst[(size_t)z * mx + i] = lds[l2 * LS + lpad<PS>(z)];
+  // position n of the padded line holds mode n (n < h), the zero padding, or mode n-(pz-nz)
+#pragma unroll
+  for (int q = 0; q < Z::PER1; ++q) {
+    const int b = t + q * Z::T;
+    if (Z::PER1 * Z::T == Z::NB1 || b < Z::NB1) {
+      C v[Z::R1];
+#pragma unroll
+      for (int r = 0; r < Z::R1; ++r) {
+        const int n = b + r * Z::NB1;
+        const int k = n < h ? n : n - (PZ - nz);
+        v[r] = mk<C>(0, 0);
+        if (ok && (n < h || k >= h)) {
+          const double2 s = src[(size_t)k * mx];
+          v[r] = mk<C>((TR)s.x, (TR)s.y);
+        }
+      }
+      stage_regs_store<PZ, Z::R1, 1, true, PS>(v, b, q, NoTw{}, line);
+    }
+  }
+  lds_bar();
+  if constexpr (Z::THREE) z_middle<PZ, true, PS>(line, t, w);
+  C* dst = reinterpret_cast<C*>(jb.dst) + (size_t)(jb.slot0 + p) * PZ * mx + i;
+#pragma unroll
+  for (int q = 0; q < Z::PERL; ++q) {
+    const int b = t + q * Z::T;
+    if (Z::PERL * Z::T == Z::NSL || b < Z::NSL) {
+      C v[Z::RL];
+      z_last<PZ, true, PS>(line, b, q, w, v);
+      if (ok) {
+#pragma unroll
+        for (int r = 0; r < Z::RL; ++r) dst[(size_t)(b + r * Z::NSL) * mx] = v[r];
+      }
+    }
   }
 }
 
 // forward: Z slot -> forward cft along z, x 1/pz, truncation to nz modes -> spectral FP64
 template <int PZ, int L, typename TR>
-__global__ void __launch_bounds__(L* Plan<PZ>::T)
+__global__ void __launch_bounds__(L* Plan<PZ>::T) __attribute__((amdgpu_waves_per_eu(LH_Z_WPE)))
     k_zfwd(ZJobs jobs, Dims dm, const cx<TR>* __restrict__ tw) {
   using C = cx<TR>;
+  using Z = ZShape<PZ>;
   constexpr int LS = zls<PZ>();
   constexpr int PS = LH_Z_PAD;
   extern __shared__ char smem_raw[];
   C* lds = reinterpret_cast<C*>(smem_raw);
-  const ZJob jb = jobs.j[blockIdx.z];
-  const int p = blockIdx.y;
+  const ZBlock zb = z_block((dm.mx + L - 1) / L, jobs.maxn);
+  const ZJob jb = jobs.j[zb.job];
+  const int p = zb.p;
   if (p >= jb.n) return;
   const int plane = jb.planes ? jb.planes[p] : jb.first + p;
-  const int i0 = blockIdx.x * L;
   const int nz = dm.nz, mx = dm.mx, h = nz / 2;
-  const C* src = reinterpret_cast<const C*>(jb.src) + (size_t)(jb.slot0 + p) * PZ * mx;
-  for (int e = threadIdx.x; e < L * PZ; e += blockDim.x) {
-    const int l = e % L, z = e / L, i = i0 + l;
-    lds[l * LS + lpad<PS>(z)] = i < mx ? src[(size_t)z * mx + i] : mk<C>(0, 0);
-  }
-  __syncthreads();
-  const int l = threadIdx.x / Plan<PZ>::T, t = threadIdx.x % Plan<PZ>::T;
+  const int l = threadIdx.x % L, t = threadIdx.x / L;
+  const int i = zb.cg * L + l;
+  const bool ok = i < mx;
+  const C* src = reinterpret_cast<const C*>(jb.src) + (size_t)(jb.slot0 + p) * PZ * mx + i;
+  C* line = lds + l * LS;
   ZTw<PZ, C> w;
   w.load(t, tw);
-  Plan<PZ>::template run<false, PS>(lds + l * LS, t, w);
-  double2* dst = reinterpret_cast<double2*>(jb.dst) + (size_t)plane * dm.ncol;
+#pragma unroll
+  for (int q = 0; q < Z::PER1; ++q) {
+    const int b = t + q * Z::T;
+    if (Z::PER1 * Z::T == Z::NB1 || b < Z::NB1) {
+      C v[Z::R1];
+#pragma unroll
+      for (int r = 0; r < Z::R1; ++r) v[r] = ok ? src[(size_t)(b + r * Z::NB1) * mx] : mk<C>(0, 0);
+      stage_regs_store<PZ, Z::R1, 1, false, PS>(v, b, q, NoTw{}, line);
+    }
+  }
+  lds_bar();
+  if constexpr (Z::THREE) z_middle<PZ, false, PS>(line, t, w);
+  double2* dst = reinterpret_cast<double2*>(jb.dst) + (size_t)plane * dm.ncol + i;
   const double sz = 1.0 / (double)PZ;
-  for (int e = threadIdx.x; e < L * nz; e += blockDim.x) {
-    const int l2 = e % L, k = e / L, i = i0 + l2;
-    if (i < mx) {
-      const int kk = k < h ? k : k + (PZ - nz);
-      const C v = lds[l2 * LS + lpad<PS>(kk)];
-      double2 o;
-      o.x = sz * (double)v.x;
-      o.y = sz * (double)v.y;
-      dst[(size_t)k * mx + i] = o;
+#pragma unroll
+  for (int q = 0; q < Z::PERL; ++q) {
+    const int b = t + q * Z::T;
+    if (Z::PERL * Z::T == Z::NSL || b < Z::NSL) {
+      C v[Z::RL];
+      z_last<PZ, false, PS>(line, b, q, w, v);
+      if (ok) {
+#pragma unroll
+        for (int r = 0; r < Z::RL; ++r) {
+          const int kk = b + r * Z::NSL;          // retained: kk < h or kk >= pz - (nz - h)
+          const int k = kk < h ? kk : kk - (PZ - nz);
+          if (kk < h || k >= h) {
+            double2 o;
+            o.x = sz * (double)v[r].x;
+            o.y = sz * (double)v[r].y;
+            dst[(size_t)k * mx] = o;
+          }
+        }
+      }
     }
   }
 }

@@ -172,6 +172,9 @@ __global__ void __launch_bounds__(6 * Plan<PX>::T)
 #ifndef LH_XRAW_TWR
 #define LH_XRAW_TWR 0
 #endif
+#ifndef LH_XRAW_XCD
+#define LH_XRAW_XCD 0
+#endif
 template <int PX, typename TR>
 __global__ void __launch_bounds__(6 * Plan<PX>::T) __attribute__((amdgpu_waves_per_eu(LH_XRAW2_WPE)))
     k_xraw2(XRAW a, Dims dm, const cx<TR>* __restrict__ tw, int lc) {
@@ -185,9 +188,16 @@ __global__ void __launch_bounds__(6 * Plan<PX>::T) __attribute__((amdgpu_waves_p
   C* lds = reinterpret_cast<C*>(smem_raw);
   __shared__ double red[16];
   const int tid = threadIdx.x;
-  const int z0 = 2 * blockIdx.x;
+  // block -> (row pair, level chunk); with LH_XRAW_XCD consecutive row pairs of a chunk run
+  // on one XCD (the 8 XCDs take linear block ids round-robin)
+  int bid = blockIdx.x + gridDim.x * blockIdx.y;
+  if (LH_XRAW_XCD) {
+    const int nblk = gridDim.x * gridDim.y, per = nblk / 8;
+    if (bid < per * 8) bid = (bid % 8) * per + bid / 8;
+  }
+  const int z0 = 2 * (bid % gridDim.x);
   const int nlev = dm.nyu - 2;
-  const int l0 = blockIdx.y * lc;
+  const int l0 = (bid / gridDim.x) * lc;
   const int l1 = min(l0 + lc, nlev);
   const int mx = dm.mx;
   const size_t slot = (size_t)dm.pz * mx;
