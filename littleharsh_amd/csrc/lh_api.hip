@@ -217,19 +217,13 @@ static void attr_z_t(int* dummy) {
 }
 template <int PX, typename TR>
 static void launch_xraw_t(lh_handle* h, const XRAW& a, const void* tw) {
-  size_t sh = (size_t)6 * (PX + 1) * sizeof(cx<TR>);
-  if (h->xraw_lc > 1) {
-    sh = ((size_t)15 * (PadLen<PX, LH_XRAW_PAD>::value + LH_XRAW_LSP) + tw_lds<PX, cx<TR>, LH_XRAW_TWR>()) *
-         sizeof(cx<TR>);
-    const int nlev = h->dm.nyu - 2, lc = h->xraw_lc;
-    dim3 grid(h->dm.pz / 2, (nlev + lc - 1) / lc);
-    hipLaunchKernelGGL((k_xraw2<PX, TR>), grid, dim3(6 * Plan<PX>::T), sh, h->st, a, h->dm,
-                       (const cx<TR>*)tw, lc);
-  } else {
-    dim3 grid(h->dm.pz / 2, h->dm.nyu - 2);
-    hipLaunchKernelGGL((k_xraw<PX, TR>), grid, dim3(6 * Plan<PX>::T), sh, h->st, a, h->dm,
-                       (const cx<TR>*)tw);
-  }
+  const size_t sh =
+      ((size_t)15 * (PadLen<PX, LH_XRAW_PAD>::value + LH_XRAW_LSP) + tw_lds<PX, cx<TR>, LH_XRAW_TWR>()) *
+      sizeof(cx<TR>);
+  const int nlev = h->dm.nyu - 2, lc = h->xraw_lc;
+  dim3 grid(h->dm.pz / 2, (nlev + lc - 1) / lc);
+  hipLaunchKernelGGL((k_xraw2<PX, TR>), grid, dim3(6 * Plan<PX>::T), sh, h->st, a, h->dm,
+                     (const cx<TR>*)tw, lc);
 }
 constexpr int kZasmCols = LH_ZASM_COLS;
 template <int PZ, typename TR>

@@ -27,131 +27,19 @@
 
 namespace lh {
 
-// Nonlinear x pass: one row pair and one level j (centre j and face j) per block.
-//   lines 0..5 <- U(j), U(j+1), W(j), W(j+1), V(j-1), V(j)
-//   products   -> uu_c(j), uw_c(j), ww_c(j), vv_c(j), uv_f(j), vw_f(j)
+// Nonlinear x pass: products uu_c(j), uw_c(j), ww_c(j), vv_c(j), uv_f(j), vw_f(j) of each
+// level j, forward-transformed in x.  Rows z0, z0+1 (z0 even) share one complex transform;
+// their row of the product buffer holds that packed spectrum P unscaled: P[k] at k and
+// P[px-k] at mx + k for k = 0..mx-1 (the z assembly separates the two rows).
 struct XRAW {
   const void* zu;
   const void* zw;
   const void* zv;
-  void* zp;                  // [6][level][pz][mx]
+  void* zp;                  // [6][level][pz/2][2 mx] packed row pairs
   const double* dthdyu;      // per u plane
   double* scal;
   int want_cfl;
 };
-
-template <int PX, typename TR>
-__global__ void __launch_bounds__(6 * Plan<PX>::T)
-    k_xraw(XRAW a, Dims dm, const cx<TR>* __restrict__ tw) {
-  using C = cx<TR>;
-  constexpr int T = Plan<PX>::T;
-  constexpr int LS = PX + 1;
-  extern __shared__ char smem_raw[];
-  C* lds = reinterpret_cast<C*>(smem_raw);
-  __shared__ double red[16];
-  const int z0 = 2 * blockIdx.x;
-  const int level = blockIdx.y;            // centre / face j = level - d + 1
-  const int su = level + 1;                // u slot of centre j
-  const int mx = dm.mx, pz = dm.pz;
-  const size_t slot = (size_t)pz * mx;
-  const C* zu = reinterpret_cast<const C*>(a.zu);
-  const C* zw = reinterpret_cast<const C*>(a.zw);
-  const C* zv = reinterpret_cast<const C*>(a.zv);
-  const C* src[6] = {zu + su * slot, zu + (su + 1) * slot, zw + su * slot, zw + (su + 1) * slot,
-                     zv + (su - 1) * slot, zv + su * slot};
-  {
-    // loads of the packed half spectra first, then the LDS writes
-    constexpr int NT = 6 * T;
-    constexpr int M = (6 * (PX / 3 + 1) + NT - 1) / NT;
-    C va[M], vb[M];
-#pragma unroll
-    for (int q = 0; q < M; ++q) {
-      const int e = threadIdx.x + q * NT;
-      if (e < 6 * mx) {
-        const C* A = src[e / mx] + (size_t)z0 * mx;
-        va[q] = A[e % mx];
-        vb[q] = A[mx + e % mx];
-      }
-    }
-#pragma unroll
-    for (int q = 0; q < M; ++q) {
-      const int e = threadIdx.x + q * NT;
-      if (e < 6 * mx) {
-        const int f = e / mx, n = e % mx;
-        C* line = lds + f * LS;
-        if (n == 0) {
-          line[0] = mk<C>(va[q].x, vb[q].x);
-        } else {
-          line[n] = mk<C>(va[q].x - vb[q].y, va[q].y + vb[q].x);
-          line[PX - n] = mk<C>(va[q].x + vb[q].y, -va[q].y + vb[q].x);
-        }
-      }
-    }
-    const int nzr = PX - 2 * mx + 1;
-    for (int e = threadIdx.x; e < 6 * nzr; e += NT) lds[(e / nzr) * LS + mx + e % nzr] = mk<C>(0, 0);
-  }
-  lds_bar();
-  const int ln = threadIdx.x / T, t = threadIdx.x % T;
-  typename Plan<PX>::template Tw<C> w;
-  w.load(t, tw);
-  Plan<PX>::template run<true>(lds + ln * LS, t, w);
-  // products (each thread owns its x positions in all six lines)
-  const TR half = (TR)0.5;
-  const double dyi = a.dthdyu[su];
-  double cmax = 0.0;
-  for (int x = threadIdx.x; x < PX; x += blockDim.x) {
-    const C U = lds[0 * LS + x], Ua = lds[1 * LS + x], W = lds[2 * LS + x], Wa = lds[3 * LS + x];
-    const C Vb = lds[4 * LS + x], Vt = lds[5 * LS + x];
-    C pr[6];
-    {
-      const TR vc0 = half * (Vb.x + Vt.x), vc1 = half * (Vb.y + Vt.y);
-      const TR uf0 = half * (U.x + Ua.x), uf1 = half * (U.y + Ua.y);
-      const TR wf0 = half * (W.x + Wa.x), wf1 = half * (W.y + Wa.y);
-      pr[0] = mk<C>(U.x * U.x, U.y * U.y);
-      pr[1] = mk<C>(U.x * W.x, U.y * W.y);
-      pr[2] = mk<C>(W.x * W.x, W.y * W.y);
-      pr[3] = mk<C>(vc0 * vc0, vc1 * vc1);
-      pr[4] = mk<C>(uf0 * Vt.x, uf1 * Vt.y);
-      pr[5] = mk<C>(Vt.x * wf0, Vt.y * wf1);
-      if (a.want_cfl) {
-        const double s0 = fabs((double)U.x) * dm.dxi + fabs((double)vc0) * dyi + fabs((double)W.x) * dm.dzi;
-        const double s1 = fabs((double)U.y) * dm.dxi + fabs((double)vc1) * dyi + fabs((double)W.y) * dm.dzi;
-        cmax = fmax(cmax, fmax(s0, s1));
-      }
-    }
-#pragma unroll
-    for (int q = 0; q < 6; ++q) lds[q * LS + x] = pr[q];
-  }
-  lds_bar();
-  Plan<PX>::template run<false>(lds + ln * LS, t, w);
-  C* zp = reinterpret_cast<C*>(a.zp);
-  const size_t nlev = dm.nyu - 2;
-  for (int e = threadIdx.x; e < 6 * mx; e += blockDim.x) {
-    const int q = e / mx, k = e % mx;
-    C va, vb;
-    unpack_rows<PX>(lds + q * LS, k, va, vb);
-    C* dst = zp + ((size_t)q * nlev + level) * slot + (size_t)z0 * mx;
-    dst[k] = va;
-    dst[mx + k] = vb;
-  }
-  if (a.want_cfl) {
-    // block max (wave64 shuffles, then LDS across waves)
-    // lanes past the end of a partial last wave hold no value: take only active sources
-    const int wv = threadIdx.x / 64, lane = threadIdx.x % 64;
-    const int act = min(64, (int)blockDim.x - wv * 64);
-    for (int off = 32; off > 0; off >>= 1) {
-      const double o = __shfl_down(cmax, off, 64);
-      if (lane + off < act) cmax = fmax(cmax, o);
-    }
-    if (lane == 0) red[wv] = cmax;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      double m = 0.0;
-      for (int w = 0; w < (int)((blockDim.x + 63) / 64); ++w) m = fmax(m, red[w]);
-      atomic_max_pos(a.scal + S_CFLMAX, m);
-    }
-  }
-}
 
 // Sliding-window x pass: one row pair and a chunk of `lc` consecutive levels per block.
 // Each thread owns fixed x positions of every line, so the physical U(j), W(j), V(j-1) of
@@ -266,7 +154,7 @@ __global__ void __launch_bounds__(6 * Plan<PX>::T) __attribute__((amdgpu_waves_p
   const TR half = (TR)0.5;
   // products of level l from old (U(j), W(j), V(j-1)) and new (U(j+1), W(j+1), V(j)) values,
   // forward transforms, and the six product rows out
-  auto level_out = [&](int l, const C (*o)[NX], const C (*nw)[NX]) {
+  auto level_out = [&](int l, const C (*o)[NX], const C (*nw)[NX], C* pa, C* pb) {
     const double dyi = a.dthdyu[l + 1];
 #pragma unroll
     for (int m = 0; m < NX; ++m) {
@@ -277,12 +165,12 @@ __global__ void __launch_bounds__(6 * Plan<PX>::T) __attribute__((amdgpu_waves_p
         const TR vc0 = half * (Vb.x + Vt.x), vc1 = half * (Vb.y + Vt.y);
         const TR uf0 = half * (U.x + Ua.x), uf1 = half * (U.y + Ua.y);
         const TR wf0 = half * (W.x + Wa.x), wf1 = half * (W.y + Wa.y);
-        b0[0 * LS + lpad<PS>(x)] = mk<C>(U.x * U.x, U.y * U.y);
-        b0[1 * LS + lpad<PS>(x)] = mk<C>(U.x * W.x, U.y * W.y);
-        b0[2 * LS + lpad<PS>(x)] = mk<C>(W.x * W.x, W.y * W.y);
-        b0[3 * LS + lpad<PS>(x)] = mk<C>(vc0 * vc0, vc1 * vc1);
-        b0[4 * LS + lpad<PS>(x)] = mk<C>(uf0 * Vt.x, uf1 * Vt.y);
-        b0[5 * LS + lpad<PS>(x)] = mk<C>(Vt.x * wf0, Vt.y * wf1);
+        pa[0 * LS + lpad<PS>(x)] = mk<C>(U.x * U.x, U.y * U.y);
+        pa[1 * LS + lpad<PS>(x)] = mk<C>(U.x * W.x, U.y * W.y);
+        pa[2 * LS + lpad<PS>(x)] = mk<C>(W.x * W.x, W.y * W.y);
+        pa[3 * LS + lpad<PS>(x)] = mk<C>(vc0 * vc0, vc1 * vc1);
+        pa[4 * LS + lpad<PS>(x)] = mk<C>(uf0 * Vt.x, uf1 * Vt.y);
+        pa[5 * LS + lpad<PS>(x)] = mk<C>(Vt.x * wf0, Vt.y * wf1);
         if (a.want_cfl) {
           const double s0 = fabs((double)U.x) * dm.dxi + fabs((double)vc0) * dyi + fabs((double)W.x) * dm.dzi;
           const double s1 = fabs((double)U.y) * dm.dxi + fabs((double)vc1) * dyi + fabs((double)W.y) * dm.dzi;
@@ -291,18 +179,27 @@ __global__ void __launch_bounds__(6 * Plan<PX>::T) __attribute__((amdgpu_waves_p
       }
     }
     lds_bar();
-    Plan<PX>::template run_pp<false, PS>(b0 + ln * LS, b1 + ln * LS, t, w);
-    for (int e = tid; e < 6 * mx; e += NT) {
-      const int q = e / mx, k = e % mx;
-      C va, vb;
-      unpack_rows_p<PX, PS>(bres + q * LS, k, va, vb);
-      C* dst = zp + ((size_t)q * nlev + l) * slot + (size_t)z0 * mx;
-      dst[k] = va;
-      dst[mx + k] = vb;
+    // all stages but the last ping-pong between pa and pb; the last one leaves P in registers
+    C* const ha = pa + ln * LS;
+    C* const hb = pb + ln * LS;
+    stage_pp<PX, T, ZShape<PX>::R1, 1, false, PS>(ha, hb, t, NoTw{});
+    if constexpr (ZShape<PX>::THREE) stage_pp<PX, T, ZShape<PX>::R2, ZShape<PX>::R1, false, PS>(hb, ha, t, w.s2);
+    const C* const hl = ZShape<PX>::THREE ? ha : hb;
+    C* dst = zp + ((size_t)ln * nlev + l) * slot + (size_t)z0 * mx;
+#pragma unroll
+    for (int q = 0; q < ZShape<PX>::PERL; ++q) {
+      const int b = t + q * T;
+      if (ZShape<PX>::PERL * T == ZShape<PX>::NSL || b < ZShape<PX>::NSL) {
+        C v[ZShape<PX>::RL];
+        z_last<PX, false, PS>(hl, b, q, w, v);
+#pragma unroll
+        for (int r = 0; r < ZShape<PX>::RL; ++r) {
+          const int k = b + r * ZShape<PX>::NSL;
+          if (k < mx) dst[k] = v[r];
+          else if (k > PX - mx) dst[mx + (PX - k)] = v[r];
+        }
+      }
     }
-    // With an odd stage count the unpack reads b1 and whatever follows writes b0 (products,
-    // loads) before a barrier, so only an even count needs one here.
-    if constexpr (!ODD) lds_bar();
   };
 
   // c0: physical U(j), W(j), V(j-1) of the current level; n1: those of the next one
@@ -324,10 +221,11 @@ __global__ void __launch_bounds__(6 * Plan<PX>::T) __attribute__((amdgpu_waves_p
     Plan<PX>::template run_pp<true, PS>(mine_a, mine_b, t, w);
     read_lines(bres, n1);
     if constexpr (!ODD) lds_bar();
-    level_out(l, c0, n1);
+    level_out(l, c0, n1, b0, b1);
     // the upper level waited in `keep`; it becomes the current level of the next pair
     read_lines(keep, c0);
-    level_out(l + 1, n1, c0);
+    // with an odd stage count the last stage of level l reads b0: level l+1 starts in b1
+    level_out(l + 1, n1, c0, ODD ? b1 : b0, ODD ? b0 : b1);
   }
   if (l < l1) {
     const int su = l + 1;
@@ -336,7 +234,7 @@ __global__ void __launch_bounds__(6 * Plan<PX>::T) __attribute__((amdgpu_waves_p
     Plan<PX>::template run_pp<true, PS>(mine_a, mine_b, t, w);
     read_lines(bres, n1);
     if constexpr (!ODD) lds_bar();
-    level_out(l, c0, n1);
+    level_out(l, c0, n1, b0, b1);
   }
   if (a.want_cfl) {
     const int wv = tid / 64, lane = tid % 64;
@@ -369,7 +267,7 @@ struct ZASM {
 };
 
 #ifndef LH_ZASM_ZB
-#define LH_ZASM_ZB 3
+#define LH_ZASM_ZB 1
 #endif
 #ifndef LH_ZASM_TWR
 #define LH_ZASM_TWR 0
@@ -381,7 +279,8 @@ __global__ void __launch_bounds__(2 * L * Plan<PZ>::T)
   constexpr int T = Plan<PZ>::T;
   constexpr int NT = 2 * L * T;
   constexpr int LS = PZ + 1;
-  constexpr int ITZ = (L * PZ + NT - 1) / NT;
+  constexpr int NPR = PZ / 2;                          // row pairs
+  constexpr int ITZ = (L * NPR + NT - 1) / NT;         // (column, row pair) items per thread
   constexpr int NZ = 2 * PZ / 3;
   extern __shared__ char smem_raw[];
   C* lds = reinterpret_cast<C*>(smem_raw);
@@ -419,54 +318,68 @@ __global__ void __launch_bounds__(2 * L * Plan<PZ>::T)
   // carried rows)
   TwSel<PZ, C, LH_ZASM_TWR> w;
   tw_setup<LH_ZASM_TWR>(w, lds + 2 * L * LS, t, tw, ln == 0);
-  // lower operand of the y difference at the first level
-  C rl[ITZ];
+  // The product rows hold the packed x spectrum P of each row pair (z, z+1): rows z and z+1
+  // of column k are (P[k] + conj P[px-k]) / 2 and (P[k] - conj P[px-k]) / 2i, times 1/px.
+  const TR sx = (TR)1 / (TR)dm.px, hf = (TR)0.5;
+  auto row_pair = [&](const C* row, int k, C& ra, C& rb) {
+    const C zk = row[k], zn = k == 0 ? zk : row[mx + k];
+    ra = mk<C>(sx * (hf * (zk.x + zn.x)), sx * (hf * (zk.y - zn.y)));
+    rb = mk<C>(sx * (hf * (zk.y + zn.y)), sx * (-hf * (zk.x - zn.x)));
+    if (k == 0) { ra.y = 0; rb.y = 0; }
+  };
+  // lower operand of the y difference at the first level (rows 2p, 2p+1 of item q)
+  C rl[ITZ][2];
   {
     const int l_lo = f < 2 ? l0 - 1 : l0;
 #pragma unroll
     for (int q = 0; q < ITZ; ++q) {
       const int e = tid + q * NT;
-      const int l2 = e % L, z = e / L, i = i0 + l2;
-      rl[q] = mk<C>(0, 0);
-      if (e < L * PZ && i < mx && l_lo >= 0)
-        rl[q] = zo[qd * ps + (size_t)l_lo * slot + (size_t)z * mx + i];
+      const int l2 = e % L, pr = e / L, i = i0 + l2;
+      rl[q][0] = mk<C>(0, 0);
+      rl[q][1] = mk<C>(0, 0);
+      if (e < L * NPR && i < mx && l_lo >= 0)
+        row_pair(zo + qd * ps + (size_t)l_lo * slot + (size_t)pr * 2 * mx, i, rl[q][0], rl[q][1]);
     }
   }
   const double sz = 1.0 / (double)PZ;
   for (int l = l0; l < l1; ++l) {
     const int l_hi = f < 2 ? l : l + 1;
     const TR mlev = (TR)(dm.ddthetavi * (f < 2 ? a.dthdyu[l + 1] : a.dthdyv[l + 1]));
-    // loads batched ZB elements at a time: the carried rows and the twiddles hold registers
+    // loads batched LH_ZASM_ZB items at a time: the carried rows and the twiddles hold registers
 #pragma unroll
     for (int q1 = 0; q1 < ITZ; q1 += LH_ZASM_ZB) {
-      C r0[LH_ZASM_ZB], rh[LH_ZASM_ZB], rb[LH_ZASM_ZB];
+      C r0[LH_ZASM_ZB][2], rh[LH_ZASM_ZB][2], rb[LH_ZASM_ZB][2];
 #pragma unroll
       for (int u = 0; u < LH_ZASM_ZB; ++u) {
         const int q = q1 + u;
         const int e = tid + q * NT;
-        const int l2 = e % L, z = e / L, i = i0 + l2;
-        if (q < ITZ && e < L * PZ && i < mx) {
-          const size_t o = (size_t)z * mx + i;
-          r0[u] = zo[q0 * ps + (size_t)l * slot + o];
-          rb[u] = zo[qb * ps + (size_t)l * slot + o];
-          rh[u] = zo[qd * ps + (size_t)l_hi * slot + o];
+        const int l2 = e % L, pr = e / L, i = i0 + l2;
+        if (q < ITZ && e < L * NPR && i < mx) {
+          const size_t o = (size_t)pr * 2 * mx;
+          row_pair(zo + q0 * ps + (size_t)l * slot + o, i, r0[u][0], r0[u][1]);
+          row_pair(zo + qb * ps + (size_t)l * slot + o, i, rb[u][0], rb[u][1]);
+          row_pair(zo + qd * ps + (size_t)l_hi * slot + o, i, rh[u][0], rh[u][1]);
         }
       }
 #pragma unroll
       for (int u = 0; u < LH_ZASM_ZB; ++u) {
         const int q = q1 + u;
         const int e = tid + q * NT;
-        const int l2 = e % L, z = e / L, i = i0 + l2;
-        if (q < ITZ && e < L * PZ) {
-          C A = mk<C>(0, 0), B = mk<C>(0, 0);
-          if (i < mx) {
-            const TR kx = (TR)a.k1x[i];
-            A = mk<C>(-kx * r0[u].y + mlev * (rh[u].x - rl[q].x), kx * r0[u].x + mlev * (rh[u].y - rl[q].y));
-            B = rb[u];
-            rl[q] = rh[u];
+        const int l2 = e % L, pr = e / L, i = i0 + l2;
+        if (q < ITZ && e < L * NPR) {
+#pragma unroll
+          for (int h2 = 0; h2 < 2; ++h2) {
+            C A = mk<C>(0, 0), B = mk<C>(0, 0);
+            if (i < mx) {
+              const C x0 = r0[u][h2], xh = rh[u][h2], xl = rl[q][h2];
+              const TR kx = (TR)a.k1x[i];
+              A = mk<C>(-kx * x0.y + mlev * (xh.x - xl.x), kx * x0.x + mlev * (xh.y - xl.y));
+              B = rb[u][h2];
+              rl[q][h2] = xh;
+            }
+            lds[(2 * l2) * LS + 2 * pr + h2] = A;
+            lds[(2 * l2 + 1) * LS + 2 * pr + h2] = B;
           }
-          lds[(2 * l2) * LS + z] = A;
-          lds[(2 * l2 + 1) * LS + z] = B;
         }
       }
     }
