@@ -67,7 +67,7 @@ struct lh_handle {
   int nu_last = 0;                                             // buffer written last (0: A)
   double2 *r1 = nullptr, *r2 = nullptr, *r3 = nullptr, *psi = nullptr, *tipl = nullptr;
   void *zu = nullptr, *zw = nullptr, *zv = nullptr, *zp = nullptr;
-  double *dg1 = nullptr, *dg2 = nullptr, *dg3 = nullptr;
+  double* dg3 = nullptr;
   double *ayu = nullptr, *ayv = nullptr, *ctwork = nullptr;
   void *twx = nullptr, *twz = nullptr;         // twiddles in the NL precision
   void *twx64 = nullptr, *twz64 = nullptr;     // FP64 twiddles (IB, statistics)
@@ -400,7 +400,8 @@ static void issue_substep(lh_handle* h, int krk /*0-based*/, bool cfl_mode) {
     COL a{};
     a.r1 = h->r1; a.r2 = h->r2; a.r3 = h->r3;
     a.u1 = h->u1; a.u2 = h->u2; a.u3 = h->u3; a.p = h->p; a.psi = h->psi;
-    a.ayu = h->ayu; a.ayv = h->ayv; a.dg1 = h->dg1; a.dg2 = h->dg2; a.dg3 = h->dg3;
+    a.ayu = h->ayu; a.ayv = h->ayv; a.dg3 = h->dg3;
+    a.k2x = h->k2x; a.k2z = h->k2z; a.yu = h->yu;
     a.dthdyu = h->dthdyu; a.dthdyv = h->dthdyv; a.k1x = h->k1x; a.k1z = h->k1z;
     a.scal = h->scal; a.ab = aR + bR; a.gw0 = h->gw[0]; a.gw1 = h->gw[1];
     a.ctu = h->flag_ctpress == 0 ? 1 : 0;
@@ -511,7 +512,7 @@ int lh_create(const lh_config* c, lh_handle** out) {
   HIPCHK(h, dalloc(&h->nb1, nu)); HIPCHK(h, dalloc(&h->nb3, nu)); HIPCHK(h, dalloc(&h->nb2, nv));
   HIPCHK(h, dalloc(&h->r1, nu)); HIPCHK(h, dalloc(&h->r3, nu)); HIPCHK(h, dalloc(&h->r2, nv));
   HIPCHK(h, dalloc(&h->tipl, 6 * (size_t)dm.ncol));
-  HIPCHK(h, dalloc(&h->dg1, np)); HIPCHK(h, dalloc(&h->dg2, np)); HIPCHK(h, dalloc(&h->dg3, np));
+  HIPCHK(h, dalloc(&h->dg3, np));
   HIPCHK(h, dalloc(&h->ayu, 3 * dm.nyu)); HIPCHK(h, dalloc(&h->ayv, 3 * dm.nyv));
   HIPCHK(h, dalloc(&h->ctwork, 4 * (dm.ny + 2)));
   const size_t zel = h->prec == LH_PREC_FP64 ? sizeof(double2) : sizeof(float2);
@@ -643,7 +644,7 @@ int lh_create(const lh_config* c, lh_handle** out) {
   }
   // pressure LU factors, once (start.f90:628-630)
   {
-    BUILDP a{h->dg1, h->dg2, h->dg3, h->dthdyu, h->dthdyv, h->yu, h->k2x, h->k2z};
+    BUILDP a{h->dg3, h->dthdyu, h->dthdyv, h->yu, h->k2x, h->k2z};
     hipLaunchKernelGGL(k_buildP, dim3((dm.ncol + 255) / 256), dim3(256), 0, h->st, a, dm);
     HIPCHK(h, hipGetLastError());
   }
@@ -659,8 +660,7 @@ int lh_destroy(lh_handle* h) {
   for (auto e : h->evpool) (void)hipEventDestroy(e);
   void* ptrs[] = {h->yu, h->yv, h->dthdyu, h->dthdyv, h->dyu2i, h->dyv2i, h->k1x, h->k2x, h->k1z,
                   h->k2z, h->scal, h->u1, h->u2, h->u3, h->p, h->nu1, h->nu2, h->nu3, h->nb1, h->nb2, h->nb3, h->r1,
-                  h->r2, h->r3, h->psi, h->tipl, h->zu, h->zw, h->zv, h->zp, h->dg1,
-                  h->dg2, h->dg3, h->ayu, h->ayv, h->ctwork, h->masks, h->ibu_planes,
+                  h->r2, h->r3, h->psi, h->tipl, h->zu, h->zw, h->zv, h->zp, h->dg3, h->ayu, h->ayv, h->ctwork, h->masks, h->ibu_planes,
                   h->ibv_planes, h->ibflag_u, h->ibflag_v, h->ibent, h->st_psum, h->st_pfold, h->st_acc,
                   h->cst_acc, h->sp_acc};
   for (void* q : ptrs)

@@ -15,8 +15,8 @@ struct COL {
   double2 *r1, *r2, *r3;
   double2 *u1, *u2, *u3, *p, *psi;
   const double *ayu, *ayv;
-  const double *dg1, *dg2, *dg3;
-  const double *dthdyu, *dthdyv, *k1x, *k1z;
+  const double* dg3;                       // modified uppers of the Poisson LU
+  const double *dthdyu, *dthdyv, *k1x, *k1z, *k2x, *k2z, *yu;
   double* scal;
   double ab;            // aRK + bRK
   double gw0, gw1;
@@ -255,10 +255,13 @@ __global__ void __launch_bounds__(256) k_pres(COL a, Dims dm) {
   double2* __restrict__ pp = a.p;
   double2 prev = d2(0.0, 0.0);
   double2 vlow = u2[c];   // v face below p row 0
+  // LU_decP on the fly: the lower and the inverse pivot of each row come from the operator
+  // row and the running modified upper a3p (the same operations k_buildP used for dg3)
+  const double k2x = a.k2x[i], k2z = a.k2z[k];
+  double a3p = 0.0;
   int r = 0;
   for (; r < np; r += U) {
     double2 a1[U], a3[U], a2[U];
-    double g1[U], g2[U];
 #pragma unroll
     for (int q = 0; q < U; ++q) {
       if (r + q < np) {
@@ -266,21 +269,23 @@ __global__ void __launch_bounds__(256) k_pres(COL a, Dims dm) {
         a1[q] = u1[ou];
         a3[q] = u3[ou];
         a2[q] = u2[ou];
-        g1[q] = a.dg1[(size_t)(r + q) * ncol + c];
-        g2[q] = a.dg2[(size_t)(r + q) * ncol + c];
       }
     }
 #pragma unroll
     for (int q = 0; q < U; ++q) {
       const int rr = r + q;
       if (rr < np) {
+        const PoisRow w = poisson_row(rr, c == 0, k2x, k2z, np, dv, a.dthdyu, a.dthdyv, a.yu, dm.d);
+        const double g1 = w.lo;
+        const double g2 = 1.0 / (w.dg - w.lo * a3p);
+        a3p = w.up * g2;
         const double2 dvg = dadd(dadd(dim(kx, a1[q]), dim(kz, a3[q])),
                                  drm(dv * a.dthdyu[rr + 1], dsub(a2[q], vlow)));
         vlow = a2[q];
         double2 ps = drm(dtirk, dvg);
         if (c == 0 && rr == 0) ps = d2(0.0, 0.0);
-        if (rr == 0) prev = drm(g2[q], ps);
-        else prev = drm(g2[q], dsub(ps, drm(g1[q], prev)));
+        if (rr == 0) prev = drm(g2, ps);
+        else prev = drm(g2, dsub(ps, drm(g1, prev)));
         psi[(size_t)rr * ncol + c] = prev;
       }
     }

@@ -429,9 +429,51 @@ struct CTU {
   int ny, d, ncol;
 };
 
-// LU_buildP + LU_decP once at start (tridLU_3D_IB.f90:497-571, 643-684)
+// Row r of the Poisson operator D G + k^2 of one column (LU_buildP, tridLU_3D_IB.f90:497-571):
+// lower lo, diagonal dg and upper up; p row r <-> plane j = r - d + 1, u/v slot r + 1.  Row 0
+// of column 0 is the pin (0, 1/(yu(2)-yu(1))^2, 0).
+struct PoisRow {
+  double lo, dg, up;
+};
+__device__ __forceinline__ PoisRow poisson_row(int r, bool pin, double k2x, double k2z, int np, double dv,
+                                               const double* __restrict__ dthdyu,
+                                               const double* __restrict__ dthdyv,
+                                               const double* __restrict__ yu, int d) {
+  // D_vec_b(j) = -dv*dthdyu(j+1), G_vec_b(j) = -dv*dthdyv(j), D_vec_t(j) = dv*dthdyu(j),
+  // G_vec_t(j) = dv*dthdyv(j-1)
+  auto DB = [&](int q) { return -dv * dthdyu[q + 2]; };
+  auto GB = [&](int q) { return -dv * dthdyv[q + 1]; };
+  auto DT = [&](int q) { return dv * dthdyu[q + 1]; };
+  auto GT = [&](int q) { return dv * dthdyv[q]; };
+  PoisRow w;
+  if (r == 0) {
+    w.lo = 0.0;
+    if (pin) {
+      const double dy = yu[2 + d] - yu[1 + d];
+      w.dg = 1.0 / (dy * dy);
+      w.up = 0.0;
+    } else {
+      w.dg = (DT(0) * GB(0)) + k2x + k2z;
+      w.up = (DT(0) * GT(1));
+    }
+  } else {
+    w.lo = (DB(r - 1) * GB(r - 1));
+    if (r < np - 1) {
+      w.dg = (DB(r - 1) * GT(r)) + (DT(r) * GB(r)) + k2x + k2z;
+      w.up = (DT(r) * GT(r + 1));
+    } else {
+      w.dg = (DB(r - 1) * GT(r)) + k2x + k2z;
+      w.up = 0.0;
+    }
+  }
+  return w;
+}
+
+// LU_decP once at start (tridLU_3D_IB.f90:643-684): the modified uppers a3 = up / pivot,
+// which the back substitution of every solve reads; the lower and the inverse pivots are
+// formed again inside the forward sweep (k_pres) with the same operations.
 struct BUILDP {
-  double *dg1, *dg2, *dg3;      // [nyp][ncol]
+  double* dg3;                  // [nyp][ncol]
   const double *dthdyu, *dthdyv, *yu, *k2x, *k2z;
 };
 
@@ -441,49 +483,13 @@ __global__ void __launch_bounds__(256) k_buildP(BUILDP a, Dims dm) {
   const int i = c % dm.mx, k = c / dm.mx;
   const double k2x = a.k2x[i], k2z = a.k2z[k];
   const int np = dm.nyp;
-  const double dv = dm.ddthetavi;
-  const int ncol = dm.ncol;
-  // p row r <-> plane j = r - d + 1, u/v slot r + 1
-#define DB(r) (-dv * a.dthdyu[(r) + 2])   /* D_vec_b(j) = -dv*dthdyu(j+1) */
-#define GB(r) (-dv * a.dthdyv[(r) + 1])   /* G_vec_b(j) */
-#define DT(r) (dv * a.dthdyu[(r) + 1])    /* D_vec_t(j) */
-#define GT(r) (dv * a.dthdyv[(r) + 0])    /* G_vec_t(j) = dv*dthdyv(j-1) */
-  double a1, a2, a3p;   // current row: lower, inverse pivot; a3p = modified upper of previous row
-  // row 0
-  double d0, u0;
-  if (c == 0) {
-    const double dy = a.yu[2 + dm.d] - a.yu[1 + dm.d];
-    d0 = 1.0 / (dy * dy);
-    u0 = 0.0;
-  } else {
-    d0 = (DT(0) * GB(0)) + k2x + k2z;
-    u0 = (DT(0) * GT(1));
+  double a3p = 0.0;
+  for (int r = 0; r < np; ++r) {
+    const PoisRow w = poisson_row(r, c == 0, k2x, k2z, np, dm.ddthetavi, a.dthdyu, a.dthdyv, a.yu, dm.d);
+    const double a2 = 1.0 / (w.dg - w.lo * a3p);
+    a3p = w.up * a2;
+    a.dg3[(size_t)r * dm.ncol + c] = a3p;
   }
-  a2 = 1.0 / d0;
-  a.dg1[c] = 0.0;
-  a.dg2[c] = a2;
-  a3p = u0 * a2;
-  a.dg3[c] = a3p;
-  for (int r = 1; r < np; ++r) {
-    double diag, up;
-    a1 = (DB(r - 1) * GB(r - 1));
-    if (r < np - 1) {
-      diag = (DB(r - 1) * GT(r)) + (DT(r) * GB(r)) + k2x + k2z;
-      up = (DT(r) * GT(r + 1));
-    } else {
-      diag = (DB(r - 1) * GT(r)) + k2x + k2z;
-      up = 0.0;
-    }
-    a2 = 1.0 / (diag - a1 * a3p);
-    a3p = up * a2;
-    a.dg1[(size_t)r * ncol + c] = a1;
-    a.dg2[(size_t)r * ncol + c] = a2;
-    a.dg3[(size_t)r * ncol + c] = a3p;
-  }
-#undef DB
-#undef GB
-#undef DT
-#undef GT
 }
 
 // meanflow_ctP (littleharsh.f90:505-521): Qx of the (0,0) column
