@@ -140,7 +140,8 @@ __global__ void __launch_bounds__(256) k_rhs_ib(RHS a, Dims dm) {
   const size_t o = (size_t)s * ncol + c;
   const double2 um = uu[o - ncol], u0 = uu[o], up = uu[o + ncol];
   const double2 pl = a.p[o - ncol], pr = f == 2 ? a.p[o] : d2(0.0, 0.0);
-  rr[o] = rhs_at(a, dm, k, f, s, um, u0, up, pl, pr, no[o], nn[o]);
+  const double2 vo = a.C2 != 0.0 ? no[o] : d2(0.0, 0.0);   // C2 = -cRK is 0 in substep 1
+  rr[o] = rhs_at(a, dm, k, f, s, um, u0, up, pl, pr, vo, nn[o]);
   const bool tip = f < 2 ? (s == a.tip_u0 || s == a.tip_u1) : (s == a.tip_v0 || s == a.tip_v1);
   if (tip) {
     const double* wy = (f < 2 ? a.dyu2i : a.dyv2i) + 3 * s;
@@ -167,6 +168,8 @@ __device__ __forceinline__ void solve_rhs(const RHS& a, const Dims& dm, int c,
   const int nj = F < 2 ? dm.nyu : dm.nyv;
   const RhsCol k = rhs_col(a, dm, c, F);
   const double2 zero = d2(0.0, 0.0);
+  // Nu_old enters with C2 = -cRK, which is 0 in the first substep: its rows are not read then
+  const bool use_old = a.C2 != 0.0;
   // row 0: rhs = 0 (boundary row)
   double2 prev = drm(ay[1], zero);
   x[c] = prev;
@@ -182,7 +185,8 @@ __device__ __forceinline__ void solve_rhs(const RHS& a, const Dims& dm, int c,
         const size_t o = (size_t)s * ncol + c;
         up[q] = uu[o + ncol];
         pr[q] = F == 2 ? pp[o] : pp[o - ncol];
-        vo[q] = no[o];
+        if (use_old) vo[q] = no[o];
+        else vo[q] = zero;
         vn[q] = nn[o];
       }
     }

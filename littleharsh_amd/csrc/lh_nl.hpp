@@ -320,11 +320,12 @@ __global__ void __launch_bounds__(2 * L * Plan<PZ>::T)
   tw_setup<LH_ZASM_TWR>(w, lds + 2 * L * LS, t, tw, ln == 0);
   // The product rows hold the packed x spectrum P of each row pair (z, z+1): rows z and z+1
   // of column k are (P[k] + conj P[px-k]) / 2 and (P[k] - conj P[px-k]) / 2i, times 1/px.
-  const TR sx = (TR)1 / (TR)dm.px, hf = (TR)0.5;
+  // (1/px)(x/2) = (1/(2px)) x exactly: halving is exact
+  const TR sxh = (TR)0.5 * ((TR)1 / (TR)dm.px);
   auto row_pair = [&](const C* row, int k, C& ra, C& rb) {
     const C zk = row[k], zn = k == 0 ? zk : row[mx + k];
-    ra = mk<C>(sx * (hf * (zk.x + zn.x)), sx * (hf * (zk.y - zn.y)));
-    rb = mk<C>(sx * (hf * (zk.y + zn.y)), sx * (-hf * (zk.x - zn.x)));
+    ra = mk<C>(sxh * (zk.x + zn.x), sxh * (zk.y - zn.y));
+    rb = mk<C>(sxh * (zk.y + zn.y), -(sxh * (zk.x - zn.x)));
     if (k == 0) { ra.y = 0; rb.y = 0; }
   };
   // lower operand of the y difference at the first level (rows 2p, 2p+1 of item q)
