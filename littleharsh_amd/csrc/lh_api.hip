@@ -16,6 +16,7 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <algorithm>
 #include <cmath>
 #include <string>
 #include <vector>
@@ -79,6 +80,7 @@ struct lh_handle {
   int nibu = 0, nibv = 0;
   IBEntry* ibent = nullptr;
   int nibent = 0;
+  int nibtip = 0;                 // entries with a Lu_y slot (tip planes), stored first
   // statistics and spectra (allocated on first use)
   double *st_psum = nullptr, *st_pfold = nullptr, *st_acc = nullptr, *cst_acc = nullptr;
   double* sp_acc = nullptr;
@@ -248,7 +250,7 @@ static void attr_zasm_t(int* dummy) {
 }
 template <int PX, typename TR>
 static void launch_xib_t(lh_handle* h, const XIB& a, int nent, const void* tw) {
-  dim3 grid(h->dm.pz / 2, nent);
+  dim3 grid(h->dm.pz / (a.paired ? 4 : 2), nent);
   size_t sh = (size_t)2 * (PX + 1) * sizeof(cx<TR>);
   hipLaunchKernelGGL((k_xpass_ib<PX, TR>), grid, dim3(2 * Plan<PX>::T), sh, h->st, a, h->dm,
                      (const cx<TR>*)tw);
@@ -380,9 +382,14 @@ static void issue_substep(lh_handle* h, int krk /*0-based*/, bool cfl_mode) {
     int maxn = h->nibu > h->nibv ? h->nibu : h->nibv;
     if (maxn < 6) maxn = 6;
     TIMED(h, LH_K_IBZ, (nib + 6.0) * (spl + z64), launch_zinv(h, jobs, 4, maxn, true));
-    XIB a{h->zp, h->ibent, h->masks, h->scal, bre};
-    TIMED(h, LH_K_XIB, (2.0 * nib + 6.0) * z64,
-          { LH_DISPATCH_LEN(dm.px, launch_xib_t, double, h, a, h->nibent, h->twx64) });
+    // entries are sorted tip first: the tips take one row pair per block, the rest two
+    XIB a{h->zp, h->ibent, h->masks, h->scal, bre, 0};
+    XIB b{h->zp, h->ibent + h->nibtip, h->masks, h->scal, bre, 1};
+    TIMED(h, LH_K_XIB, (2.0 * nib + 6.0) * z64, {
+      if (h->nibtip > 0) LH_DISPATCH_LEN(dm.px, launch_xib_t, double, h, a, h->nibtip, h->twx64)
+      if (h->nibent > h->nibtip)
+        LH_DISPATCH_LEN(dm.px, launch_xib_t, double, h, b, h->nibent - h->nibtip, h->twx64)
+    });
     ZJobs fj{};
     slot = 0;
     fj.j[0] = ZJob{(const double2*)h->zp, h->r1, h->ibu_planes, 0, h->nibu, slot}; slot += h->nibu;
@@ -614,7 +621,10 @@ int lh_create(const lh_config* c, lh_handle** out) {
       ent.push_back(IBEntry{sv0 + q, ts >= 0 ? st0 + ts : -1, h->mask_of_v[s], 0});
     }
     (void)slot;
+    std::stable_partition(ent.begin(), ent.end(), [](const IBEntry& x) { return x.lslot >= 0; });
     h->nibent = (int)ent.size();
+    h->nibtip = 0;
+    for (const IBEntry& x : ent) h->nibtip += x.lslot >= 0 ? 1 : 0;
     if (h->nmask > 0) {
       HIPCHK(h, hipMalloc((void**)&h->masks, pp * h->nmask));
       for (int q = 0; q < h->nmask; ++q)

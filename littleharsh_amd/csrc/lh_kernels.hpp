@@ -308,10 +308,12 @@ struct IBEntry {
 };
 struct XIB {
   void* z;
-  const IBEntry* ent;
+  const IBEntry* ent;           // this launch's entries
   const unsigned char* masks;   // [nmask][pz][px]
   const double* scal;
   double bre;                   // bRK(kRK)/Re
+  int paired;                   // 0: tip entries (rhs and Lu_y of one row pair per block)
+                                // 1: other entries (two row pairs z0, z0+2 per block)
 };
 
 template <int PX, typename TR>
@@ -322,14 +324,16 @@ __global__ void __launch_bounds__(2 * Plan<PX>::T)
   constexpr int LS = PX + 1;
   extern __shared__ char smem_raw[];
   C* lds = reinterpret_cast<C*>(smem_raw);
-  const int z0 = 2 * blockIdx.x;
+  const bool paired = a.paired != 0;
+  const int z0 = (paired ? 4 : 2) * blockIdx.x;
   const IBEntry en = a.ent[blockIdx.y];
   const int mx = dm.mx, pz = dm.pz;
   const size_t slot = (size_t)pz * mx;
   C* z = reinterpret_cast<C*>(a.z);
+  // line 0: rhs rows (z0, z0+1); line 1: Lu_y rows (z0, z0+1) of a tip entry, or the rhs
+  // rows (z0+2, z0+3)
   const C* A = z + en.slot * slot + (size_t)z0 * mx;
-  const C* B = en.lslot >= 0 ? z + en.lslot * slot + (size_t)z0 * mx : A;
-  const int nl = en.lslot >= 0 ? 2 : 1;
+  const C* B = paired ? A + 2 * mx : z + en.lslot * slot + (size_t)z0 * mx;
   for (int e = threadIdx.x; e < 2 * PX; e += blockDim.x) {
     const int ln = e / PX, n = e % PX;
     lds[ln * LS + n] = pack_rows<PX>(ln ? B : A, (ln ? B : A) + mx, n, mx);
@@ -341,27 +345,38 @@ __global__ void __launch_bounds__(2 * Plan<PX>::T)
   Plan<PX>::template run<true>(lds + ln * LS, t, w);
   const double beta = a.bre * a.scal[S_DT];
   const unsigned char* m0 = a.masks + ((size_t)en.mask * pz + z0) * dm.px;
-  const unsigned char* m1 = m0 + dm.px;
-  for (int x = threadIdx.x; x < PX; x += blockDim.x) {
-    C v = lds[x];
-    if (nl == 2) {
+  if (paired) {
+    // rhs = 0 on the canopy points of the four rows
+    for (int e = threadIdx.x; e < 2 * PX; e += blockDim.x) {
+      const int l2 = e / PX, x = e % PX;
+      const unsigned char* m = m0 + (size_t)(2 * l2) * dm.px;
+      C v = lds[l2 * LS + x];
+      if (m[x]) v.x = 0;
+      if (m[dm.px + x]) v.y = 0;
+      lds[l2 * LS + x] = v;
+    }
+  } else {
+    // tip plane: rhs = -beta Lu_y on the canopy points
+    const unsigned char* m1 = m0 + dm.px;
+    for (int x = threadIdx.x; x < PX; x += blockDim.x) {
+      C v = lds[x];
       const C l = lds[LS + x];
       if (m0[x]) v.x = (TR)(-beta * (double)l.x);
       if (m1[x]) v.y = (TR)(-beta * (double)l.y);
-    } else {
-      if (m0[x]) v.x = 0;
-      if (m1[x]) v.y = 0;
+      lds[x] = v;
     }
-    lds[x] = v;
   }
   __syncthreads();
   Plan<PX>::template run<false>(lds + ln * LS, t, w);
   C* dst = z + en.slot * slot + (size_t)z0 * mx;
-  for (int k = threadIdx.x; k < mx; k += blockDim.x) {
+  const int nout = paired ? 2 : 1;
+  for (int e = threadIdx.x; e < nout * mx; e += blockDim.x) {
+    const int l2 = e / mx, k = e % mx;
     C va, vb;
-    unpack_rows<PX>(lds, k, va, vb);
-    dst[k] = va;
-    dst[mx + k] = vb;
+    unpack_rows<PX>(lds + l2 * LS, k, va, vb);
+    C* d = dst + (size_t)(2 * l2) * mx;
+    d[k] = va;
+    d[mx + k] = vb;
   }
 }
 

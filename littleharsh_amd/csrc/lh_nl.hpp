@@ -21,8 +21,10 @@
 #include <type_traits>
 #include "lh_kernels.hpp"
 
+// z assembly block order: 0 = field slowest, 1 = field fastest (the three blocks of one
+// column group and level chunk, which share product rows, run on one XCD together)
 #ifndef LH_ZASM_ORDER
-#define LH_ZASM_ORDER 0
+#define LH_ZASM_ORDER 1
 #endif
 
 namespace lh {
@@ -291,11 +293,17 @@ __global__ void __launch_bounds__(2 * L * Plan<PZ>::T)
   int bid = blockIdx.x;
   const int per = nblk / 8;
   if (bid < per * 8) bid = (bid % 8) * per + bid / 8;
-  const int ib = bid % a.nib;
-  const int rest = bid / a.nib;
   const int nch = (nlev + a.lc - 1) / a.lc;
-  const int ch = rest % nch;
-  const int f = rest / nch;                   // 0: u1, 1: u3, 2: v
+  int ib, ch, f;                              // f: 0 u1, 1 u3, 2 v
+  if (LH_ZASM_ORDER) {
+    f = bid % 3;
+    ib = (bid / 3) % a.nib;
+    ch = bid / (3 * a.nib);
+  } else {
+    ib = bid % a.nib;
+    ch = (bid / a.nib) % nch;
+    f = bid / (a.nib * nch);
+  }
   const int l0 = ch * a.lc;
   // face ny+d-1 is the wall: no Nu2 there
   const int l1 = min(l0 + a.lc, f == 2 ? nlev - 1 : nlev);
