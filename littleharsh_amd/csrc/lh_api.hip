@@ -30,16 +30,34 @@
 using namespace lh;
 
 // Default launch parameters (per handle; lh_autotune may change them):
-//   LH_XRAW_LC  levels per block of the sliding x pass
-//   LH_ZASM_LC  levels per block of the z assembly
+//   LH_XRAW_LC  levels per block of the sliding x pass (0: chosen per grid, see chunk_levels)
+//   LH_ZASM_LC  levels per block of the z assembly (0: chosen per grid)
 //   LH_COL_BS   threads per block of the column kernels (one column per thread: small blocks
 //               spread the 74k columns of the production grid evenly over the 256 CUs)
 #ifndef LH_XRAW_LC
-#define LH_XRAW_LC 16
+#define LH_XRAW_LC 0
 #endif
 #ifndef LH_ZASM_LC
-#define LH_ZASM_LC 4
+#define LH_ZASM_LC 0
 #endif
+
+// Levels per block for a kernel whose blocks each fill a CU (LDS-bound, one resident block):
+// it runs in ceil(blocks / CUs) rounds of (levels + start-up) level times; the start-up of a
+// level chunk costs about `start` levels.  Returns the level count in [lo, hi] of least cost.
+static int chunk_levels(int nlev, long colblocks, int ncu, int lo, int hi, double start) {
+  int best = lo;
+  double bc = 1e300;
+  for (int lc = lo; lc <= hi; ++lc) {
+    const long nblk = colblocks * ((nlev + lc - 1) / lc);
+    const double rounds = (double)((nblk + ncu - 1) / ncu);
+    const double cost = rounds * (lc + start);
+    if (cost < bc - 1e-9) {
+      bc = cost;
+      best = lc;
+    }
+  }
+  return best;
+}
 #ifndef LH_COL_BS
 #define LH_COL_BS 64
 #endif
@@ -480,6 +498,14 @@ int lh_create(const lh_config* c, lh_handle** out) {
   *out = h;
   HIPCHK(h, hipGetDevice(&h->dev));
   HIPCHK(h, hipStreamCreate(&h->st));
+  {
+    int ncu = 0;
+    HIPCHK(h, hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, h->dev));
+    if (ncu < 1) ncu = 1;
+    if (h->xraw_lc <= 0) h->xraw_lc = chunk_levels(dm.nyu - 2, dm.pz / 2, ncu, 4, 64, 1.0);
+    if (h->zasm_lc <= 0)
+      h->zasm_lc = chunk_levels(dm.nyu - 2, 3L * ((dm.mx + LH_ZASM_COLS - 1) / LH_ZASM_COLS), ncu, 2, 12, 0.25);
+  }
   {
     int dummy = 0;
     LH_DISPATCH_LEN(c->pz, attr_zasm_t, double, &dummy)
@@ -961,21 +987,25 @@ int lh_autotune(lh_handle* h, int steps) {
     HIPCHK(h, hipEventElapsedTime(ms, e0, e1));
     return LH_OK;
   };
-  const int cand_bs[] = {64, 128, 256};
-  const int cand_x[] = {4, 8, 16, 32};
-  const int cand_z[] = {1, 2, 4, 8};
+  // candidates of each knob, the current value first (the x pass default comes from the
+  // dispatch-round model, its neighbours bracket it)
+  const int x0 = h->xraw_lc, z0 = h->zasm_lc;
+  const std::vector<int> cands[3] = {{h->col_bs, 64, 128, 256},
+                                     {x0, x0 / 2 > 0 ? x0 / 2 : 1, 2 * x0, 16},
+                                     {z0, 1, 2, 4, 8}};
   int r = LH_OK;
   for (int pass = 0; pass < 3 && r == LH_OK; ++pass) {
-    const int* cand = pass == 0 ? cand_bs : (pass == 1 ? cand_x : cand_z);
-    const int nc = pass == 0 ? 3 : 4;
     int* knob = pass == 0 ? &h->col_bs : (pass == 1 ? &h->xraw_lc : &h->zasm_lc);
     int best = *knob;
     float best_ms = 1e30f;
-    for (int c = 0; c < nc && r == LH_OK; ++c) {
-      *knob = cand[c];
+    for (size_t c = 0; c < cands[pass].size() && r == LH_OK; ++c) {
+      const int v = cands[pass][c];
+      if (c > 0 && std::find(cands[pass].begin(), cands[pass].begin() + c, v) != cands[pass].begin() + c)
+        continue;                               // already tried
+      *knob = v;
       float ms = 0.f;
       r = trial(&ms);
-      if (ms < best_ms) { best_ms = ms; best = cand[c]; }
+      if (ms < best_ms) { best_ms = ms; best = v; }
     }
     *knob = best;
   }

@@ -35,7 +35,8 @@ def test_autotune_keeps_the_state(require_gpu):
     g.set_state(*st)
     p = g.autotune(1)
     print("tuned", p, flush=True)
-    assert p["col_bs"] in (64, 128, 256) and p["xraw_lc"] in (4, 8, 16, 32) and p["zasm_lc"] in (1, 2, 4, 8)
+    # candidates: the modelled defaults, their halves and doubles, and a few fixed sizes
+    assert p["col_bs"] in (64, 128, 256) and 1 <= p["xraw_lc"] <= 128 and 1 <= p["zasm_lc"] <= 24
     for x, y in zip(g.get_state(), st):
         assert np.array_equal(x, y)
 
