@@ -236,10 +236,22 @@ static void attr_z_t(int* dummy) {
                             hipFuncAttributeMaxDynamicSharedMemorySize, sh);
 }
 template <int PX, typename TR>
+static size_t xraw_lds() {
+  return ((size_t)15 * (PadLen<PX, LH_XRAW_PAD>::value + LH_XRAW_LSP) + tw_lds<PX, cx<TR>, LH_XRAW_TWR>()) *
+         sizeof(cx<TR>);
+}
+// resident blocks per CU of the x pass (LDS and registers)
+template <int PX, typename TR>
+static void xraw_occ_t(int* nb) {
+  (void)hipFuncSetAttribute((const void*)k_xraw2<PX, TR>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)xraw_lds<PX, TR>());
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(nb, k_xraw2<PX, TR>, 6 * Plan<PX>::T, xraw_lds<PX, TR>()) !=
+      hipSuccess)
+    *nb = 1;
+}
+template <int PX, typename TR>
 static void launch_xraw_t(lh_handle* h, const XRAW& a, const void* tw) {
-  const size_t sh =
-      ((size_t)15 * (PadLen<PX, LH_XRAW_PAD>::value + LH_XRAW_LSP) + tw_lds<PX, cx<TR>, LH_XRAW_TWR>()) *
-      sizeof(cx<TR>);
+  const size_t sh = xraw_lds<PX, TR>();
   const int nlev = h->dm.nyu - 2, lc = h->xraw_lc;
   dim3 grid(h->dm.pz / 2, (nlev + lc - 1) / lc);
   hipLaunchKernelGGL((k_xraw2<PX, TR>), grid, dim3(6 * Plan<PX>::T), sh, h->st, a, h->dm,
@@ -258,6 +270,13 @@ static void launch_zasm_t(lh_handle* h, const ZASM& a, const void* tw) {
   const size_t sh = zasm_lds<PZ, TR>();
   hipLaunchKernelGGL((k_zasm<PZ, L, TR>), grid, dim3(2 * L * Plan<PZ>::T), sh, h->st, a, h->dm,
                      (const cx<TR>*)tw);
+}
+template <int PZ, typename TR>
+static void zasm_occ_t(int* nb) {
+  constexpr int L = kZasmCols;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(nb, k_zasm<PZ, L, TR>, 2 * L * Plan<PZ>::T, zasm_lds<PZ, TR>()) !=
+      hipSuccess)
+    *nb = 1;
 }
 template <int PZ, typename TR>
 static void attr_zasm_t(int* dummy) {
@@ -499,19 +518,30 @@ int lh_create(const lh_config* c, lh_handle** out) {
   HIPCHK(h, hipGetDevice(&h->dev));
   HIPCHK(h, hipStreamCreate(&h->st));
   {
-    int ncu = 0;
-    HIPCHK(h, hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, h->dev));
-    if (ncu < 1) ncu = 1;
-    if (h->xraw_lc <= 0) h->xraw_lc = chunk_levels(dm.nyu - 2, dm.pz / 2, ncu, 4, 64, 1.0);
-    if (h->zasm_lc <= 0)
-      h->zasm_lc = chunk_levels(dm.nyu - 2, 3L * ((dm.mx + LH_ZASM_COLS - 1) / LH_ZASM_COLS), ncu, 2, 12, 0.25);
-  }
-  {
     int dummy = 0;
     LH_DISPATCH_LEN(c->pz, attr_zasm_t, double, &dummy)
     LH_DISPATCH_LEN(c->pz, attr_zasm_t, float, &dummy)
     LH_DISPATCH_LEN(c->pz, attr_z_t, double, &dummy)
     LH_DISPATCH_LEN(c->pz, attr_z_t, float, &dummy)
+  }
+  {
+    int ncu = 0;
+    HIPCHK(h, hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, h->dev));
+    if (ncu < 1) ncu = 1;
+    // slots = CUs x resident blocks per CU of the kernel at its precision
+    int bx = 1, bz = 1;
+    if (h->prec == LH_PREC_MIXED) {
+      LH_DISPATCH_LEN(dm.px, xraw_occ_t, float, &bx)
+      LH_DISPATCH_LEN(dm.pz, zasm_occ_t, float, &bz)
+    } else {
+      LH_DISPATCH_LEN(dm.px, xraw_occ_t, double, &bx)
+      LH_DISPATCH_LEN(dm.pz, zasm_occ_t, double, &bz)
+    }
+    if (bx < 1) bx = 1;
+    if (bz < 1) bz = 1;
+    if (h->xraw_lc <= 0) h->xraw_lc = chunk_levels(dm.nyu - 2, dm.pz / 2, ncu * bx, 4, 64, 1.0);
+    if (h->zasm_lc <= 0)
+      h->zasm_lc = chunk_levels(dm.nyu - 2, 3L * ((dm.mx + LH_ZASM_COLS - 1) / LH_ZASM_COLS), ncu * bz, 2, 12, 0.25);
   }
   const size_t nu = (size_t)dm.nyu * dm.ncol, nv = (size_t)dm.nyv * dm.ncol,
                np = (size_t)dm.nyp * dm.ncol;

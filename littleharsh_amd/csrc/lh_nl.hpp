@@ -59,6 +59,11 @@ struct XRAW {
 #ifndef LH_XRAW2_WPE
 #define LH_XRAW2_WPE 2
 #endif
+// FP32 lines take half the LDS, so two blocks would fit per CU with three waves per SIMD;
+// measured, that is no faster than one block (the chunk model counts resident blocks)
+#ifndef LH_XRAW2_WPE32
+#define LH_XRAW2_WPE32 2
+#endif
 #ifndef LH_XRAW_TWR
 #define LH_XRAW_TWR 0
 #endif
@@ -66,7 +71,8 @@ struct XRAW {
 #define LH_XRAW_XCD 0
 #endif
 template <int PX, typename TR>
-__global__ void __launch_bounds__(6 * Plan<PX>::T) __attribute__((amdgpu_waves_per_eu(LH_XRAW2_WPE)))
+__global__ void __launch_bounds__(6 * Plan<PX>::T)
+    __attribute__((amdgpu_waves_per_eu(sizeof(TR) == 4 ? LH_XRAW2_WPE32 : LH_XRAW2_WPE)))
     k_xraw2(XRAW a, Dims dm, const cx<TR>* __restrict__ tw, int lc) {
   using C = cx<TR>;
   constexpr int T = Plan<PX>::T;
