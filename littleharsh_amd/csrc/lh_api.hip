@@ -199,7 +199,7 @@ static bool supported_len(int n) {
 #define LH_COL_U 8
 #endif
 #ifndef LH_VEL_U
-#define LH_VEL_U 2
+#define LH_VEL_U 1
 #endif
 template <int N> constexpr int zlines() {
   return N >= 384 ? LH_ZLINES_LONG : (Plan<N>::T >= 8 ? 16 : 32);
