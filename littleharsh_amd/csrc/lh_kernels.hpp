@@ -280,18 +280,6 @@ __device__ __forceinline__ void unpack_rows(const C* line, int k, C& a, C& b) {
   if (k == 0) { a.y = 0; b.y = 0; }
 }
 
-// unpack_rows on a line stored with lpad<S> positions
-template <int PX, int S, typename C>
-__device__ __forceinline__ void unpack_rows_p(const C* line, int k, C& a, C& b) {
-  using T = decltype(C::x);
-  const C zk = line[lpad<S>(k)], zn = line[lpad<S>(k == 0 ? 0 : PX - k)];
-  const T sx = (T)1 / (T)PX;
-  a = mk<C>((T)0.5 * (zk.x + zn.x), (T)0.5 * (zk.y - zn.y));
-  b = mk<C>((T)0.5 * (zk.y + zn.y), -(T)0.5 * (zk.x - zn.x));
-  a = mk<C>(sx * a.x, sx * a.y);
-  b = mk<C>(sx * b.x, sx * b.y);
-  if (k == 0) { a.y = 0; b.y = 0; }
-}
 
 __device__ __forceinline__ void atomic_max_pos(double* addr, double v) {
   // non-negative doubles order like their bit patterns

@@ -14,9 +14,10 @@
 //   u:  A1 = ikx uu + m_u (uv - uv_prev),   B1 = uw
 //   w:  A3 = ikx uw + m_u (vw - vw_prev),   B3 = ww
 //   v:  A2 = ikx uv_prev + m_v (vv - vv_prev), B2 = vw_prev   (face j-1 at level j)
-// The x pass (k_xraw2) produces the six raw products per level; the forward z pass
-// (k_zasm) forms A and B from them and transforms them into Nu_new.  RHS0 and the rhs of
-// solveU/solveV are assembled per column afterwards (k_rhs, lh_col.hpp).
+// The x pass (k_xraw2) produces the six raw products per level (their x spectra, two rows
+// packed per complex transform); the forward z pass (k_zasm) separates the rows, forms A
+// and B and transforms them into Nu_new.  RHS0 and the rhs of solveU/solveV are built
+// inside the velocity sweeps (k_vel_rhs, lh_col.hpp).
 #pragma once
 #include <type_traits>
 #include "lh_kernels.hpp"
