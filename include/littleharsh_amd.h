@@ -148,8 +148,12 @@ int lh_spectra_read(lh_handle* h, double* out, int reset);
  *   xraw_lc  levels per block of the sliding x pass        (>= 1)
  *   zasm_lc  levels per block of the z assembly            (>= 1)
  *   col_bs   threads per block of the column kernels       (32..256, multiple of 32)
- * lh_autotune times `steps` CFL steps per candidate and keeps the fastest value of each
- * parameter in turn; the state is saved before and restored after. */
+ * lh_create sets the two level counts from the grid and the device: each block of those
+ * kernels holds a CU's LDS, so the launch runs in ceil(blocks / resident slots) rounds,
+ * and the count minimising rounds x (levels + start-up) is taken.
+ * lh_autotune times `steps` CFL steps per candidate (the current values, their halves
+ * and doubles, a few fixed sizes) and keeps the fastest value of each parameter in turn;
+ * the state is saved before and restored after. */
 int lh_set_launch(lh_handle* h, int xraw_lc, int zasm_lc, int col_bs);
 int lh_get_launch(lh_handle* h, int* xraw_lc, int* zasm_lc, int* col_bs);
 int lh_autotune(lh_handle* h, int steps);

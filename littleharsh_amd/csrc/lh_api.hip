@@ -66,6 +66,10 @@ struct lh_handle {
   std::string err;
   int dev = 0;
   hipStream_t st = nullptr;
+  // side stream for the LU of the velocity matrices (k_prep), which needs only dt: it runs
+  // beside the nonlinear stage, forked and joined by the two events
+  hipStream_t side = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   Dims dm{};
   int prec = LH_PREC_FP64;
   int flag_ctpress = 1;
@@ -343,6 +347,19 @@ static void issue_substep(lh_handle* h, int krk /*0-based*/, bool cfl_mode) {
   const double zsl = zel * dm.pz * dm.mx;                    // bytes per Z slot
   const double spl = 16.0 * dm.ncol;                         // bytes per spectral plane
   const double nuvw = 2.0 * dm.nyu + dm.nyv;                 // u, w, v planes
+  // k_prep needs only dt: in graph and direct issue it runs on the side stream beside the
+  // nonlinear stage (timing mode keeps it in line); at kRK = 1 of a CFL step dt comes from
+  // the first x pass, so it forks after k_cfl_dt
+  const PREP prep{h->dyu2i, h->dyv2i, h->scal, h->ayu, h->ayv, h->gw[0], h->gw[1], bre, dm.nyu, dm.nyv};
+  const bool side = !h->timing;
+  auto fork_prep = [&]() {
+    (void)hipEventRecord(h->ev_fork, h->st);
+    (void)hipStreamWaitEvent(h->side, h->ev_fork, 0);
+    hipLaunchKernelGGL(k_prep, dim3(1), dim3(64), 0, h->side, prep);
+    (void)hipEventRecord(h->ev_join, h->side);
+  };
+  const bool late = krk == 0 && cfl_mode;
+  if (side && !late) fork_prep();
   // ---- NL stage: inverse z transforms of u, w (all u planes) and v (all v planes)
   {
     ZJobs jobs{};
@@ -366,6 +383,7 @@ static void issue_substep(lh_handle* h, int krk /*0-based*/, bool cfl_mode) {
   }
   if (krk == 0 && cfl_mode)
     TIMED(h, LH_K_MISC, 0.0, hipLaunchKernelGGL(k_cfl_dt, dim3(1), dim3(1), 0, h->st, h->scal));
+  if (side && late) fork_prep();
   // Nu_new into the buffer of this substep; RHS0 + rhs per column reads both
   const bool wb = (krk % 2) == 1;
   double2* nw[3] = {wb ? h->nb1 : h->nu1, wb ? h->nb2 : h->nu2, wb ? h->nb3 : h->nu3};
@@ -436,10 +454,10 @@ static void issue_substep(lh_handle* h, int krk /*0-based*/, bool cfl_mode) {
           launch_zfwd(h, fj, 3, h->nibu > h->nibv ? h->nibu : h->nibv, true));
   }
   // ---- column stage
-  {
-    PREP a{h->dyu2i, h->dyv2i, h->scal, h->ayu, h->ayv, h->gw[0], h->gw[1], bre, dm.nyu, dm.nyv};
-    TIMED(h, LH_K_MISC, 0.0, hipLaunchKernelGGL(k_prep, dim3(1), dim3(64), 0, h->st, a));
-  }
+  if (side)
+    (void)hipStreamWaitEvent(h->st, h->ev_join, 0);
+  else
+    TIMED(h, LH_K_MISC, 0.0, hipLaunchKernelGGL(k_prep, dim3(1), dim3(64), 0, h->st, prep));
   {
     COL a{};
     a.r1 = h->r1; a.r2 = h->r2; a.r3 = h->r3;
@@ -468,7 +486,7 @@ static void issue_step(lh_handle* h, bool cfl_mode) {
   for (int k = 0; k < 3; ++k) issue_substep(h, k, cfl_mode);
   if (h->flag_ctpress == 1)
     TIMED(h, LH_K_MISC, 0.0,
-          hipLaunchKernelGGL(k_flowrate, dim3(1), dim3(64), 0, h->st, h->u1, h->yv, h->scal,
+          hipLaunchKernelGGL(k_flowrate, dim3(1), dim3(256), 0, h->st, h->u1, h->yv, h->scal,
                              h->dm.ny, h->dm.d, h->dm.ncol));
 }
 
@@ -517,6 +535,9 @@ int lh_create(const lh_config* c, lh_handle** out) {
   *out = h;
   HIPCHK(h, hipGetDevice(&h->dev));
   HIPCHK(h, hipStreamCreate(&h->st));
+  HIPCHK(h, hipStreamCreate(&h->side));
+  HIPCHK(h, hipEventCreateWithFlags(&h->ev_fork, hipEventDisableTiming));
+  HIPCHK(h, hipEventCreateWithFlags(&h->ev_join, hipEventDisableTiming));
   {
     int dummy = 0;
     LH_DISPATCH_LEN(c->pz, attr_zasm_t, double, &dummy)
@@ -721,6 +742,7 @@ int lh_create(const lh_config* c, lh_handle** out) {
 int lh_destroy(lh_handle* h) {
   if (!h) return LH_OK;
   if (h->st) hipStreamSynchronize(h->st);
+  if (h->side) hipStreamSynchronize(h->side);
   for (auto& g : h->gexec)
     if (g) (void)hipGraphExecDestroy(g);
   for (auto e : h->evpool) (void)hipEventDestroy(e);
@@ -736,6 +758,9 @@ int lh_destroy(lh_handle* h) {
   if (h->twx64) hipFree(h->twx64);
   if (h->twz64) hipFree(h->twz64);
   if (h->st) hipStreamDestroy(h->st);
+  if (h->side) hipStreamDestroy(h->side);
+  if (h->ev_fork) (void)hipEventDestroy(h->ev_fork);
+  if (h->ev_join) (void)hipEventDestroy(h->ev_join);
   delete h;
   return LH_OK;
 }
@@ -828,7 +853,7 @@ int lh_substep(lh_handle* h, int krk, double dt) {
   issue_substep(h, krk - 1, false);
   // meanflow_ctP follows the last substep of a step (littleharsh.f90:193-197)
   if (krk == 3 && h->flag_ctpress == 1)
-    hipLaunchKernelGGL(k_flowrate, dim3(1), dim3(64), 0, h->st, h->u1, h->yv, h->scal, h->dm.ny,
+    hipLaunchKernelGGL(k_flowrate, dim3(1), dim3(256), 0, h->st, h->u1, h->yv, h->scal, h->dm.ny,
                        h->dm.d, h->dm.ncol);
   HIPCHK(h, hipGetLastError());
   HIPCHK(h, hipStreamSynchronize(h->st));

@@ -495,12 +495,21 @@ __global__ void __launch_bounds__(256) k_buildP(BUILDP a, Dims dm) {
   }
 }
 
-// meanflow_ctP (littleharsh.f90:505-521): Qx of the (0,0) column
-__global__ void k_flowrate(const double2* u1, const double* yv, double* scal, int ny, int d, int ncol) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+// meanflow_ctP (littleharsh.f90:505-521): Qx of the (0,0) column, one block of 256 threads
+// (strided partial sums, then a fixed-order tree: the same sum on every run)
+__global__ void __launch_bounds__(256) k_flowrate(const double2* u1, const double* yv, double* scal, int ny,
+                                                  int d, int ncol) {
+  __shared__ double part[256];
   double q = 0.0;
-  for (int j = 1; j <= ny - 1; ++j) q += (yv[j + d] - yv[j - 1 + d]) * u1[(size_t)(j + d) * ncol].x;
-  scal[S_QX] = q;
+  for (int j = 1 + (int)threadIdx.x; j <= ny - 1; j += 256)
+    q += (yv[j + d] - yv[j - 1 + d]) * u1[(size_t)(j + d) * ncol].x;
+  part[threadIdx.x] = q;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) part[threadIdx.x] += part[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) scal[S_QX] = part[0];
 }
 
 // error (littleharsh.f90:323-352): max |div| over the p grid
