@@ -241,7 +241,7 @@ static void attr_z_t(int* dummy) {
 }
 template <int PX, typename TR>
 static size_t xraw_lds() {
-  return ((size_t)15 * (PadLen<PX, LH_XRAW_PAD>::value + LH_XRAW_LSP) + tw_lds<PX, cx<TR>, LH_XRAW_TWR>()) *
+  return ((size_t)15 * (PadLen<PX, LH_XRAW_PAD>::value + LH_XRAW_LSP) + Plan<PX>::template TwL<cx<TR>>::SIZE) *
          sizeof(cx<TR>);
 }
 // resident blocks per CU of the x pass (LDS and registers)
@@ -264,7 +264,7 @@ static void launch_xraw_t(lh_handle* h, const XRAW& a, const void* tw) {
 constexpr int kZasmCols = LH_ZASM_COLS;
 template <int PZ, typename TR>
 static size_t zasm_lds() {
-  return ((size_t)2 * kZasmCols * (PZ + 1) + tw_lds<PZ, cx<TR>, LH_ZASM_TWR>()) * sizeof(cx<TR>);
+  return ((size_t)2 * kZasmCols * (PZ + 1) + Plan<PZ>::template TwL<cx<TR>>::SIZE) * sizeof(cx<TR>);
 }
 template <int PZ, typename TR>
 static void launch_zasm_t(lh_handle* h, const ZASM& a, const void* tw) {

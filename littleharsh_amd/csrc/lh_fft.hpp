@@ -12,7 +12,6 @@
 // by the callers.
 #pragma once
 #include <hip/hip_runtime.h>
-#include <type_traits>
 
 namespace lh {
 
@@ -195,38 +194,6 @@ struct StageTwL {
   }
 };
 
-// The same twiddles from one base value per butterfly: W^{j r} for r = 2..R-1 is formed by
-// repeated multiplication with W^j (a rotation recurrence, as FFTPACK builds its tables),
-// so a stage keeps PER complex values in registers and reads no table.  The rounding error
-// of the r-th power grows like r ulp (R <= 16 here).
-template <int N, int T, int R, int NS, typename C>
-struct StageTwR {
-  static constexpr int NB = N / R;
-  static constexpr int PER = (NB + T - 1) / T;
-  C w[PER];
-  __device__ __forceinline__ void load(int t, const C* __restrict__ tw) {
-#pragma unroll
-    for (int q = 0; q < PER; ++q) {
-      const int b = t + q * T;
-      const int j = (b < NB ? b : 0) % NS;
-      w[q] = tw[j * (N / (NS * R))];
-    }
-  }
-  template <bool INV, int RR> __device__ __forceinline__ void apply(int q, C (&v)[RR]) const {
-    C b = w[q];
-    // an empty register constraint: the powers must not be hoisted out of the caller's
-    // loops, where they would hold the full table in registers again
-    asm volatile("" : "+v"(b.x), "+v"(b.y));
-    C p = b;
-    v[1] = INV ? cmulc(v[1], p) : cmul(v[1], p);
-#pragma unroll
-    for (int r = 2; r < R; ++r) {
-      p = cmul(p, b);
-      v[r] = INV ? cmulc(v[r], p) : cmul(v[r], p);
-    }
-  }
-};
-
 // Position of element e of a line stored with one pad slot after every 2^S elements
 // (S = 0: no padding); spreads the strided stage writes over the LDS banks.
 template <int S> __device__ __forceinline__ constexpr int lpad(int e) { return S ? e + (e >> S) : e; }
@@ -332,10 +299,6 @@ __device__ __forceinline__ void stage_load_regs(const C* src, int b, int q, cons
         s2.init(tab, t, tw, fill);                                                      \
       }                                                                                 \
     };                                                                                  \
-    template <typename C> struct TwR {                                                  \
-      StageTwR<NN, TT, R2, R1, C> s2;                                                   \
-      __device__ __forceinline__ void load(int t, const C* tw) { s2.load(t, tw); }      \
-    };                                                                                  \
     template <bool INV, int S = 0, typename C, typename W>                              \
     static __device__ void run(C* l, int t, const W& w) {                               \
       stage<NN, TT, R1, 1, INV, S>(l, t, NoTw{});                                       \
@@ -367,11 +330,6 @@ __device__ __forceinline__ void stage_load_regs(const C* src, int b, int q, cons
         s3.init(tab + StageTwL<NN, TT, R2, R1, C>::SIZE, t, tw, fill);                  \
       }                                                                                 \
     };                                                                                  \
-    template <typename C> struct TwR {                                                  \
-      StageTwR<NN, TT, R2, R1, C> s2;                                                   \
-      StageTwR<NN, TT, R3, R1 * R2, C> s3;                                              \
-      __device__ __forceinline__ void load(int t, const C* tw) { s2.load(t, tw); s3.load(t, tw); } \
-    };                                                                                  \
     template <bool INV, int S = 0, typename C, typename W>                              \
     static __device__ void run(C* l, int t, const W& w) {                               \
       stage<NN, TT, R1, 1, INV, S>(l, t, NoTw{});                                       \
@@ -388,20 +346,6 @@ __device__ __forceinline__ void stage_load_regs(const C* src, int b, int q, cons
   };
 
 template <int N> struct Plan;
-
-// Twiddle source of a kernel: the LDS table (REC false) or per-butterfly bases in
-// registers with the powers formed on the fly (REC true); LDS entries the choice needs.
-template <int N, typename C, bool REC>
-using TwSel = typename std::conditional<REC, typename Plan<N>::template TwR<C>,
-                                        typename Plan<N>::template TwL<C>>::type;
-template <int N, typename C, bool REC> constexpr int tw_lds() {
-  return REC ? 0 : Plan<N>::template TwL<C>::SIZE;
-}
-template <bool REC, typename W, typename C>
-__device__ __forceinline__ void tw_setup(W& w, C* tab, int t, const C* tw, bool fill) {
-  if constexpr (REC) w.load(t, tw);
-  else w.init(tab, t, tw, fill);
-}
 
 LH_PLAN3(576, 48, 12, 12, 4)
 LH_PLAN3(384, 48, 8, 12, 4)

@@ -48,13 +48,6 @@ struct ZJobs {
 #ifndef LH_Z_PAD
 #define LH_Z_PAD 0
 #endif
-#ifndef LH_Z_TWR
-#define LH_Z_TWR 0
-#endif
-// register twiddles of the z transforms: the full per-thread set, or bases with powers
-template <int PZ, typename C>
-using ZTw = typename std::conditional<LH_Z_TWR, typename Plan<PZ>::template TwR<C>,
-                                      typename Plan<PZ>::template Tw<C>>::type;
 #ifndef LH_Z_LSP
 #define LH_Z_LSP 2
 #endif
@@ -150,7 +143,7 @@ __global__ void __launch_bounds__(L* Plan<PZ>::T) __attribute__((amdgpu_waves_pe
   const bool ok = i < mx;
   const double2* src = jb.src + (size_t)plane * dm.ncol + i;
   C* line = lds + l * LS;
-  ZTw<PZ, C> w;
+  typename Plan<PZ>::template Tw<C> w;
   w.load(t, tw);
   // position n of the padded line holds mode n (n < h), the zero padding, or mode n-(pz-nz)
 #pragma unroll
@@ -209,7 +202,7 @@ __global__ void __launch_bounds__(L* Plan<PZ>::T) __attribute__((amdgpu_waves_pe
   const bool ok = i < mx;
   const C* src = reinterpret_cast<const C*>(jb.src) + (size_t)(jb.slot0 + p) * PZ * mx + i;
   C* line = lds + l * LS;
-  ZTw<PZ, C> w;
+  typename Plan<PZ>::template Tw<C> w;
   w.load(t, tw);
 #pragma unroll
   for (int q = 0; q < Z::PER1; ++q) {

@@ -65,12 +65,6 @@ struct XRAW {
 #ifndef LH_XRAW2_WPE32
 #define LH_XRAW2_WPE32 2
 #endif
-#ifndef LH_XRAW_TWR
-#define LH_XRAW_TWR 0
-#endif
-#ifndef LH_XRAW_XCD
-#define LH_XRAW_XCD 0
-#endif
 template <int PX, typename TR>
 __global__ void __launch_bounds__(6 * Plan<PX>::T)
     __attribute__((amdgpu_waves_per_eu(sizeof(TR) == 4 ? LH_XRAW2_WPE32 : LH_XRAW2_WPE)))
@@ -85,16 +79,9 @@ __global__ void __launch_bounds__(6 * Plan<PX>::T)
   C* lds = reinterpret_cast<C*>(smem_raw);
   __shared__ double red[16];
   const int tid = threadIdx.x;
-  // block -> (row pair, level chunk); with LH_XRAW_XCD consecutive row pairs of a chunk run
-  // on one XCD (the 8 XCDs take linear block ids round-robin)
-  int bid = blockIdx.x + gridDim.x * blockIdx.y;
-  if (LH_XRAW_XCD) {
-    const int nblk = gridDim.x * gridDim.y, per = nblk / 8;
-    if (bid < per * 8) bid = (bid % 8) * per + bid / 8;
-  }
-  const int z0 = 2 * (bid % gridDim.x);
+  const int z0 = 2 * blockIdx.x;               // row pair
   const int nlev = dm.nyu - 2;
-  const int l0 = (bid / gridDim.x) * lc;
+  const int l0 = blockIdx.y * lc;               // level chunk
   const int l1 = min(l0 + lc, nlev);
   const int mx = dm.mx;
   const size_t slot = (size_t)dm.pz * mx;
@@ -116,8 +103,8 @@ __global__ void __launch_bounds__(6 * Plan<PX>::T)
   C* const bres = ODD ? b1 : b0;               // where the transforms of lines 0..2 end
   C* const mine_a = in_line(ln);
   C* const mine_b = other_line(ln);
-  TwSel<PX, C, LH_XRAW_TWR> w;
-  tw_setup<LH_XRAW_TWR>(w, lds + 15 * LS, t, tw, ln == 0);
+  typename Plan<PX>::template TwL<C> w;
+  w.init(lds + 15 * LS, t, tw, ln == 0);
 
   // packed rows (z0, z0+1) of n Z slots -> line slots 0..n-1
   auto load_lines = [&](const C* const* src, int n) {
@@ -278,9 +265,6 @@ struct ZASM {
 #ifndef LH_ZASM_ZB
 #define LH_ZASM_ZB 1
 #endif
-#ifndef LH_ZASM_TWR
-#define LH_ZASM_TWR 0
-#endif
 template <int PZ, int L, typename TR>
 __global__ void __launch_bounds__(2 * L * Plan<PZ>::T)
     k_zasm(ZASM a, Dims dm, const cx<TR>* __restrict__ tw) {
@@ -329,10 +313,9 @@ __global__ void __launch_bounds__(2 * L * Plan<PZ>::T)
   const int qb = f == 0 ? 1 : (f == 1 ? 2 : 5);       // B
   double2* __restrict__ nu = f == 0 ? a.nu1 : (f == 1 ? a.nu3 : a.nu2);
   const int ln = tid / T, t = tid % T;
-  // twiddles: an LDS table behind the 2L lines, or register bases (registers also hold the
-  // carried rows)
-  TwSel<PZ, C, LH_ZASM_TWR> w;
-  tw_setup<LH_ZASM_TWR>(w, lds + 2 * L * LS, t, tw, ln == 0);
+  // twiddles in an LDS table behind the 2L lines (registers hold the carried rows)
+  typename Plan<PZ>::template TwL<C> w;
+  w.init(lds + 2 * L * LS, t, tw, ln == 0);
   // The product rows hold the packed x spectrum P of each row pair (z, z+1): rows z and z+1
   // of column k are (P[k] + conj P[px-k]) / 2 and (P[k] - conj P[px-k]) / 2i, times 1/px.
   // (1/px)(x/2) = (1/(2px)) x exactly: halving is exact
