@@ -168,6 +168,29 @@ int lh_batch_step_cfl(lh_batch* b, double cfl);
 const char* lh_batch_last_error(const lh_batch* b);
 int lh_batch_destroy(lh_batch* b);
 
+/* Learned subfilter closure (no counterpart in the reference: the canopy-layer closure of
+ * the coarse-grid model).  A multilayer perceptron nin -> nhid (tanh) x nlayers -> nout
+ * (linear), trained and evaluated on the GPU by BF16 matrix-core kernels accumulating in
+ * FP32, with FP32 master weights and Adam (beta 0.9, 0.999, eps 1e-8); loss: mean squared
+ * error over the nout outputs.  Arrays are row-major host arrays of floats:
+ *   parameters  per layer W (out x in) then b (out), lh_mlp_param_count values
+ *   x           rows of nin inputs, y rows of nout targets / outputs
+ * lh_mlp_loss_grad evaluates one batch of `batch` rows (loss and the FP32 gradient of every
+ * parameter); lh_mlp_train takes `steps` Adam steps, each on `batch` rows drawn with
+ * replacement from the data set given to lh_mlp_set_data (kept in device memory). */
+typedef struct lh_mlp lh_mlp;
+int lh_mlp_create(int nin, int nhid, int nlayers, int nout, int batch, unsigned long long seed,
+                  lh_mlp** out);
+int lh_mlp_destroy(lh_mlp* m);
+const char* lh_mlp_last_error(lh_mlp* m);
+long long lh_mlp_param_count(lh_mlp* m);
+int lh_mlp_get_params(lh_mlp* m, float* params);
+int lh_mlp_set_params(lh_mlp* m, const float* params);
+int lh_mlp_set_data(lh_mlp* m, const float* x, const float* y, long long n);
+int lh_mlp_loss_grad(lh_mlp* m, const float* x, const float* y, float* loss, float* grad);
+int lh_mlp_train(lh_mlp* m, int steps, float lr, unsigned long long seed, float* loss_last);
+int lh_mlp_predict(lh_mlp* m, const float* x, long long n, float* y);
+
 /* Per-kernel timing (no counterpart in the reference; measurement support).  While
  * enabled, lh_step/lh_step_cfl issue their kernels directly (no graph) with a HIP event
  * pair around every launch on the handle's stream; durations accumulate per kernel

@@ -6,7 +6,7 @@ import subprocess
 import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-SRC = os.path.join(HERE, "csrc", "lh_api.hip")
+SRC = [os.path.join(HERE, "csrc", "lh_api.hip"), os.path.join(HERE, "csrc", "lh_mlp.hip")]
 OUT = os.path.join(HERE, "liblh_amd.so")
 
 HIPCC_FLAGS = [
@@ -20,7 +20,7 @@ HIPCC_FLAGS = [
 def build(verbose: bool = False) -> str:
     hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
     tmp = OUT + ".tmp"
-    cmd = [hipcc, *HIPCC_FLAGS, "-o", tmp, SRC]
+    cmd = [hipcc, *HIPCC_FLAGS, "-o", tmp, *SRC]
     if verbose:
         print(" ".join(cmd), flush=True)
     r = subprocess.run(cmd, capture_output=True, text=True)

@@ -128,6 +128,19 @@ def load_library():
     lib.lh_timing_enable.argtypes = [vp, ctypes.c_int]
     lib.lh_timing_reset.argtypes = [vp]
     lib.lh_timing_read.argtypes = [vp, ctypes.c_int, _dp, ctypes.POINTER(ctypes.c_longlong), _dp]
+    _fp = ctypes.POINTER(ctypes.c_float)
+    lib.lh_mlp_create.argtypes = [ctypes.c_int] * 5 + [ctypes.c_ulonglong, ctypes.POINTER(vp)]
+    lib.lh_mlp_destroy.argtypes = [vp]
+    lib.lh_mlp_last_error.argtypes = [vp]
+    lib.lh_mlp_last_error.restype = ctypes.c_char_p
+    lib.lh_mlp_param_count.argtypes = [vp]
+    lib.lh_mlp_param_count.restype = ctypes.c_longlong
+    lib.lh_mlp_get_params.argtypes = [vp, _fp]
+    lib.lh_mlp_set_params.argtypes = [vp, _fp]
+    lib.lh_mlp_set_data.argtypes = [vp, _fp, _fp, ctypes.c_longlong]
+    lib.lh_mlp_loss_grad.argtypes = [vp, _fp, _fp, _fp, _fp]
+    lib.lh_mlp_train.argtypes = [vp, ctypes.c_int, ctypes.c_float, ctypes.c_ulonglong, _fp]
+    lib.lh_mlp_predict.argtypes = [vp, _fp, ctypes.c_longlong, _fp]
     _lib = lib
     return lib
 
@@ -140,6 +153,9 @@ EXPORTED = [
     "lh_timing_enable", "lh_timing_reset", "lh_timing_read",
     "lh_set_launch", "lh_get_launch", "lh_autotune",
     "lh_batch_create", "lh_batch_step_cfl", "lh_batch_last_error", "lh_batch_destroy",
+    "lh_mlp_create", "lh_mlp_destroy", "lh_mlp_last_error", "lh_mlp_param_count",
+    "lh_mlp_get_params", "lh_mlp_set_params", "lh_mlp_set_data", "lh_mlp_loss_grad",
+    "lh_mlp_train", "lh_mlp_predict",
 ]
 
 KERNEL_NAMES = ["zinv", "xpass_nl", "zfwd", "assemble", "ib_z", "ib_x", "vel", "pressure", "misc",

@@ -1,0 +1,472 @@
+// lh_mlp.hip -- the learned subfilter closure: a small MLP trained and evaluated on gfx950
+// with BF16 matrix cores (v_mfma_f32_16x16x16_bf16) accumulating in FP32.
+//
+// The reference has no counterpart: the closure is the project's canopy-layer subfilter model
+// (inputs: filtered velocity gradients on a 3x3x3 stencil and the solid fraction; outputs:
+// the six subfilter stresses).  Layers: nin -> nhid (tanh) -> ... -> nhid (tanh) -> nout
+// (linear), `nlayers` hidden layers.  FP32 master weights and Adam moments; BF16 copies of
+// the weights and of the activations feed the matrix cores; pre-activations, gradients of
+// the activations and the weight gradients are FP32.
+//
+// Every matrix is stored padded to multiples of 16 (pad entries are zero and stay zero: their
+// gradients are zero).  One wave computes one 16x16 output tile:
+//   lane l holds A[i][k] for i = l % 16, k = 4 (l / 16) + q and B[k][j] for j = l % 16,
+//   k = 4 (l / 16) + q (q = 0..3), and receives D[i][j] for j = l % 16, i = 4 (l / 16) + q.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <string.h>
+
+#include <algorithm>
+#include <cmath>
+#include <string>
+#include <vector>
+
+#include "../../include/littleharsh_amd.h"
+
+namespace {
+
+typedef short bf4 __attribute__((ext_vector_type(4)));
+typedef float f4 __attribute__((ext_vector_type(4)));
+
+// round to nearest even
+__device__ __host__ inline uint16_t f2bf(float f) {
+  union { float f; uint32_t u; } c;
+  c.f = f;
+  const uint32_t u = c.u;
+  if ((u & 0x7f800000u) == 0x7f800000u) return (uint16_t)(u >> 16);   // inf / nan
+  return (uint16_t)((u + 0x7fffu + ((u >> 16) & 1u)) >> 16);
+}
+__device__ __host__ inline float bf2f(uint16_t b) {
+  union { float f; uint32_t u; } c;
+  c.u = (uint32_t)b << 16;
+  return c.f;
+}
+__host__ __device__ inline int pad16(int n) { return (n + 15) & ~15; }
+
+// counter-based hash (SplitMix64 finaliser)
+__device__ __host__ inline uint64_t mix64(uint64_t z) {
+  z += 0x9e3779b97f4a7c15ull;
+  z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+  z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+  return z ^ (z >> 31);
+}
+
+// ---- kernels (one 64-thread block = one wave = one 16x16 tile) -------------------------
+
+// Z = X W^T + b (X: B x K bf16, W: N x K bf16, Z: B x N f32); Xn = tanh(Z) as bf16 when
+// act != 0 (Xn has N columns)
+__global__ void __launch_bounds__(64) k_fwd(const uint16_t* __restrict__ X, const uint16_t* __restrict__ W,
+                                            const float* __restrict__ bias, float* __restrict__ Z,
+                                            uint16_t* __restrict__ Xn, int K, int N, int act) {
+  const int l = threadIdx.x;
+  const int b0 = blockIdx.x * 16, n0 = blockIdx.y * 16;
+  f4 acc = {0.f, 0.f, 0.f, 0.f};
+  for (int k0 = 0; k0 < K; k0 += 16) {
+    bf4 a, b;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int k = k0 + 4 * (l / 16) + q;
+      a[q] = (short)X[(size_t)(b0 + l % 16) * K + k];
+      b[q] = (short)W[(size_t)(n0 + l % 16) * K + k];
+    }
+    acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
+  }
+  const int n = n0 + l % 16;
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const int r = b0 + 4 * (l / 16) + q;
+    const float z = acc[q] + bias[n];
+    Z[(size_t)r * N + n] = z;
+    if (act) Xn[(size_t)r * N + n] = f2bf(tanhf(z));
+  }
+}
+
+// dZ = dH * act'(Z) (dH, Z: B x N f32) -> dZ as f32 and bf16
+__global__ void k_dz(const float* __restrict__ dH, const float* __restrict__ Z, float* __restrict__ dZ,
+                     uint16_t* __restrict__ dZb, int n, int act) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  float g = dH[e];
+  if (act) {
+    const float t = tanhf(Z[e]);
+    g *= 1.f - t * t;
+  }
+  dZ[e] = g;
+  dZb[e] = f2bf(g);
+}
+
+// dW = dZ^T X (dZ: B x N bf16, X: B x K bf16, dW: N x K f32), summed over the B rows
+__global__ void __launch_bounds__(64) k_dw(const uint16_t* __restrict__ dZ, const uint16_t* __restrict__ X,
+                                           float* __restrict__ dW, int B, int K, int N) {
+  const int l = threadIdx.x;
+  const int n0 = blockIdx.x * 16, k0 = blockIdx.y * 16;
+  f4 acc = {0.f, 0.f, 0.f, 0.f};
+  for (int r0 = 0; r0 < B; r0 += 16) {
+    bf4 a, b;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int r = r0 + 4 * (l / 16) + q;
+      a[q] = (short)dZ[(size_t)r * N + n0 + l % 16];   // A[i = neuron][k = row]
+      b[q] = (short)X[(size_t)r * K + k0 + l % 16];    // B[k = row][j = feature]
+    }
+    acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
+  }
+#pragma unroll
+  for (int q = 0; q < 4; ++q) dW[(size_t)(n0 + 4 * (l / 16) + q) * K + k0 + l % 16] = acc[q];
+}
+
+// db = sum over the B rows of dZ (f32), one thread per output
+__global__ void k_db(const float* __restrict__ dZ, float* __restrict__ db, int B, int N) {
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= N) return;
+  float s = 0.f;
+  for (int r = 0; r < B; ++r) s += dZ[(size_t)r * N + n];
+  db[n] = s;
+}
+
+// dX = dZ W (dZ: B x N bf16, W: N x K bf16, dX: B x K f32)
+__global__ void __launch_bounds__(64) k_dx(const uint16_t* __restrict__ dZ, const uint16_t* __restrict__ W,
+                                           float* __restrict__ dX, int K, int N) {
+  const int l = threadIdx.x;
+  const int b0 = blockIdx.x * 16, k0 = blockIdx.y * 16;
+  f4 acc = {0.f, 0.f, 0.f, 0.f};
+  for (int n0 = 0; n0 < N; n0 += 16) {
+    bf4 a, b;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int n = n0 + 4 * (l / 16) + q;
+      a[q] = (short)dZ[(size_t)(b0 + l % 16) * N + n];   // A[i = row][k = neuron]
+      b[q] = (short)W[(size_t)n * K + k0 + l % 16];      // B[k = neuron][j = feature]
+    }
+    acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
+  }
+#pragma unroll
+  for (int q = 0; q < 4; ++q) dX[(size_t)(b0 + 4 * (l / 16) + q) * K + k0 + l % 16] = acc[q];
+}
+
+// mean squared error over the `nout` real outputs of B rows; dY = 2 (Y - T) / (B nout);
+// one block of 256 threads, fixed-order reduction
+__global__ void __launch_bounds__(256) k_mse(const float* __restrict__ Y, const float* __restrict__ T,
+                                             float* __restrict__ dY, float* __restrict__ loss, int B, int NP,
+                                             int nout) {
+  __shared__ float part[256];
+  const float sc = 2.f / (float)(B * nout);
+  float s = 0.f;
+  for (int e = threadIdx.x; e < B * NP; e += 256) {
+    const int r = e / NP, n = e % NP;
+    float d = 0.f;
+    if (n < nout) {
+      d = Y[e] - T[(size_t)r * nout + n];
+      s += d * d;
+    }
+    dY[e] = sc * d;
+  }
+  part[threadIdx.x] = s;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) part[threadIdx.x] += part[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) loss[0] = part[0] / (float)(B * nout);
+}
+
+// batch of B rows drawn with replacement from the data set (x: n x nin f32, y: n x nout f32)
+// -> X (B x K0 bf16, zero pad) and T (B x nout f32)
+__global__ void k_gather(const float* __restrict__ x, const float* __restrict__ y, long long n, int nin, int nout,
+                         int K0, int B, uint64_t seed, uint64_t step, uint16_t* __restrict__ X,
+                         float* __restrict__ T) {
+  const int r = blockIdx.x;
+  if (r >= B) return;
+  const long long src = (long long)(mix64(seed ^ mix64(step * 0x100000001b3ull + (uint64_t)r)) % (uint64_t)n);
+  for (int f = threadIdx.x; f < K0; f += blockDim.x)
+    X[(size_t)r * K0 + f] = f < nin ? f2bf(x[(size_t)src * nin + f]) : (uint16_t)0;
+  for (int f = threadIdx.x; f < nout; f += blockDim.x) T[(size_t)r * nout + f] = y[(size_t)src * nout + f];
+}
+
+// rows of host-given inputs -> X (bf16, zero pad)
+__global__ void k_pack_x(const float* __restrict__ x, int nrow, int nin, int K0, int B, uint16_t* __restrict__ X) {
+  const int r = blockIdx.x;
+  for (int f = threadIdx.x; f < K0; f += blockDim.x)
+    X[(size_t)r * K0 + f] = (r < nrow && f < nin) ? f2bf(x[(size_t)r * nin + f]) : (uint16_t)0;
+}
+
+// Adam on the FP32 master parameters; refreshes the bf16 copy of the weights
+__global__ void k_adam(float* __restrict__ p, const float* __restrict__ g, float* __restrict__ m,
+                       float* __restrict__ v, uint16_t* __restrict__ wb, long long n, float lr, float b1, float b2,
+                       float eps, float c1, float c2) {
+  const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  const float ge = g[e];
+  const float me = b1 * m[e] + (1.f - b1) * ge;
+  const float ve = b2 * v[e] + (1.f - b2) * ge * ge;
+  m[e] = me;
+  v[e] = ve;
+  const float pe = p[e] - lr * (me / c1) / (sqrtf(ve / c2) + eps);
+  p[e] = pe;
+  if (wb) wb[e] = f2bf(pe);
+}
+
+__global__ void k_to_bf(const float* __restrict__ p, uint16_t* __restrict__ wb, long long n) {
+  const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e < n) wb[e] = f2bf(p[e]);
+}
+
+}  // namespace
+
+// ---- handle ------------------------------------------------------------------------------
+struct lh_mlp {
+  int nin = 0, nhid = 0, nlay = 0, nout = 0, B = 0;
+  int L = 0;                                   // weight layers = nlay + 1
+  std::vector<int> Kp, Np, K, N;               // padded / real input and output widths
+  std::vector<long long> offW, offB;           // offsets in the padded parameter vector
+  long long np = 0;                            // padded parameter count
+  float *p = nullptr, *g = nullptr, *m = nullptr, *v = nullptr;   // FP32, padded
+  uint16_t* wb = nullptr;                      // bf16 copy of p (weights and biases)
+  std::vector<uint16_t*> X;                    // layer inputs (bf16), X[0] = batch input
+  std::vector<float*> Z;                       // pre-activations (f32)
+  float *dH = nullptr, *dZ = nullptr, *T = nullptr, *loss = nullptr;
+  uint16_t* dZb = nullptr;
+  float *dx = nullptr, *dy = nullptr;          // data set on the device
+  long long ndata = 0;
+  int t = 0;                                   // Adam step count
+  hipStream_t st = nullptr;
+  std::string err;
+};
+
+static int mfail(lh_mlp* h, int code, const std::string& m) {
+  if (h) h->err = m;
+  return code;
+}
+#define MCHK(h, x)                                                                          \
+  do {                                                                                      \
+    hipError_t e_ = (x);                                                                    \
+    if (e_ != hipSuccess) return mfail(h, LH_ERR_HIP, std::string(#x) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+template <typename T>
+static hipError_t zalloc(T** p, size_t n) {
+  hipError_t e = hipMalloc((void**)p, n * sizeof(T) + 16);
+  if (e == hipSuccess) e = hipMemset(*p, 0, n * sizeof(T) + 16);
+  return e;
+}
+
+// forward pass of the batch in X[0] (B rows); returns the output pre-activations in Z[L-1]
+static void mlp_forward(lh_mlp* h) {
+  for (int l = 0; l < h->L; ++l) {
+    const bool last = l == h->L - 1;
+    hipLaunchKernelGGL(k_fwd, dim3(h->B / 16, h->Np[l] / 16), dim3(64), 0, h->st, h->X[l],
+                       h->wb + h->offW[l], h->p + h->offB[l], h->Z[l], last ? nullptr : h->X[l + 1], h->Kp[l],
+                       h->Np[l], last ? 0 : 1);
+  }
+}
+
+// backward pass from dY (in dH, B x Np[L-1]); fills g
+static void mlp_backward(lh_mlp* h) {
+  for (int l = h->L - 1; l >= 0; --l) {
+    const int n = h->B * h->Np[l];
+    hipLaunchKernelGGL(k_dz, dim3((n + 255) / 256), dim3(256), 0, h->st, h->dH, h->Z[l], h->dZ, h->dZb, n,
+                       l == h->L - 1 ? 0 : 1);
+    hipLaunchKernelGGL(k_dw, dim3(h->Np[l] / 16, h->Kp[l] / 16), dim3(64), 0, h->st, h->dZb, h->X[l],
+                       h->g + h->offW[l], h->B, h->Kp[l], h->Np[l]);
+    hipLaunchKernelGGL(k_db, dim3((h->Np[l] + 63) / 64), dim3(64), 0, h->st, h->dZ, h->g + h->offB[l], h->B,
+                       h->Np[l]);
+    if (l > 0)
+      hipLaunchKernelGGL(k_dx, dim3(h->B / 16, h->Kp[l] / 16), dim3(64), 0, h->st, h->dZb,
+                         h->wb + h->offW[l], h->dH, h->Kp[l], h->Np[l]);
+  }
+}
+
+extern "C" {
+
+int lh_mlp_create(int nin, int nhid, int nlayers, int nout, int batch, unsigned long long seed, lh_mlp** out) {
+  if (!out) return LH_ERR_ARG;
+  *out = nullptr;
+  lh_mlp* h = new lh_mlp();
+  *out = h;
+  if (nin < 1 || nhid < 1 || nlayers < 1 || nout < 1 || batch < 16 || batch % 16)
+    return mfail(h, LH_ERR_ARG, "lh_mlp_create: nin, nhid, nlayers, nout >= 1; batch a multiple of 16");
+  h->nin = nin; h->nhid = nhid; h->nlay = nlayers; h->nout = nout; h->B = batch;
+  h->L = nlayers + 1;
+  long long off = 0;
+  for (int l = 0; l < h->L; ++l) {
+    const int k = l == 0 ? nin : nhid, n = l == h->L - 1 ? nout : nhid;
+    h->K.push_back(k); h->N.push_back(n);
+    h->Kp.push_back(pad16(k)); h->Np.push_back(pad16(n));
+    h->offW.push_back(off); off += (long long)pad16(n) * pad16(k);
+    h->offB.push_back(off); off += pad16(n);
+  }
+  h->np = off;
+  MCHK(h, hipStreamCreate(&h->st));
+  MCHK(h, zalloc(&h->p, h->np)); MCHK(h, zalloc(&h->g, h->np));
+  MCHK(h, zalloc(&h->m, h->np)); MCHK(h, zalloc(&h->v, h->np));
+  MCHK(h, zalloc(&h->wb, h->np));
+  int wmax = 0;
+  for (int l = 0; l < h->L; ++l) {
+    uint16_t* x = nullptr;
+    float* z = nullptr;
+    MCHK(h, zalloc(&x, (size_t)batch * h->Kp[l]));
+    MCHK(h, zalloc(&z, (size_t)batch * h->Np[l]));
+    h->X.push_back(x); h->Z.push_back(z);
+    wmax = std::max(wmax, std::max(h->Kp[l], h->Np[l]));
+  }
+  MCHK(h, zalloc(&h->dH, (size_t)batch * wmax)); MCHK(h, zalloc(&h->dZ, (size_t)batch * wmax));
+  MCHK(h, zalloc(&h->dZb, (size_t)batch * wmax)); MCHK(h, zalloc(&h->T, (size_t)batch * nout));
+  MCHK(h, zalloc(&h->loss, 16));
+  // Glorot-uniform weights from a counter-based generator, zero biases
+  std::vector<float> hp(h->np, 0.f);
+  for (int l = 0; l < h->L; ++l) {
+    const float lim = std::sqrt(6.f / (float)(h->K[l] + h->N[l]));
+    for (int n = 0; n < h->N[l]; ++n)
+      for (int k = 0; k < h->K[l]; ++k) {
+        const uint64_t r = mix64(seed ^ mix64(((uint64_t)l << 40) ^ ((uint64_t)n << 20) ^ (uint64_t)k));
+        const float u = (float)((r >> 11) * (1.0 / 9007199254740992.0));
+        hp[h->offW[l] + (long long)n * h->Kp[l] + k] = lim * (2.f * u - 1.f);
+      }
+  }
+  MCHK(h, hipMemcpy(h->p, hp.data(), h->np * sizeof(float), hipMemcpyHostToDevice));
+  hipLaunchKernelGGL(k_to_bf, dim3((unsigned)((h->np + 255) / 256)), dim3(256), 0, h->st, h->p, h->wb, h->np);
+  MCHK(h, hipStreamSynchronize(h->st));
+  return LH_OK;
+}
+
+int lh_mlp_destroy(lh_mlp* h) {
+  if (!h) return LH_OK;
+  if (h->st) (void)hipStreamSynchronize(h->st);
+  void* ptrs[] = {h->p, h->g, h->m, h->v, h->wb, h->dH, h->dZ, h->dZb, h->T, h->loss, h->dx, h->dy};
+  for (void* q : ptrs)
+    if (q) (void)hipFree(q);
+  for (auto q : h->X) if (q) (void)hipFree(q);
+  for (auto q : h->Z) if (q) (void)hipFree(q);
+  if (h->st) (void)hipStreamDestroy(h->st);
+  delete h;
+  return LH_OK;
+}
+
+const char* lh_mlp_last_error(lh_mlp* h) { return h ? h->err.c_str() : "null handle"; }
+
+long long lh_mlp_param_count(lh_mlp* h) {
+  if (!h) return 0;
+  long long n = 0;
+  for (int l = 0; l < h->L; ++l) n += (long long)h->N[l] * h->K[l] + h->N[l];
+  return n;
+}
+
+// parameters (FP32, unpadded): per layer W (N x K, row-major) then b (N)
+static void pack_params(const lh_mlp* h, const float* padded, float* flat) {
+  long long o = 0;
+  for (int l = 0; l < h->L; ++l) {
+    for (int n = 0; n < h->N[l]; ++n)
+      for (int k = 0; k < h->K[l]; ++k) flat[o++] = padded[h->offW[l] + (long long)n * h->Kp[l] + k];
+    for (int n = 0; n < h->N[l]; ++n) flat[o++] = padded[h->offB[l] + n];
+  }
+}
+
+int lh_mlp_get_params(lh_mlp* h, float* params) {
+  if (!h || !params) return mfail(h, LH_ERR_ARG, "lh_mlp_get_params: null argument");
+  std::vector<float> hp(h->np);
+  MCHK(h, hipStreamSynchronize(h->st));
+  MCHK(h, hipMemcpy(hp.data(), h->p, h->np * sizeof(float), hipMemcpyDeviceToHost));
+  pack_params(h, hp.data(), params);
+  return LH_OK;
+}
+
+int lh_mlp_set_params(lh_mlp* h, const float* params) {
+  if (!h || !params) return mfail(h, LH_ERR_ARG, "lh_mlp_set_params: null argument");
+  std::vector<float> hp(h->np, 0.f);
+  long long o = 0;
+  for (int l = 0; l < h->L; ++l) {
+    for (int n = 0; n < h->N[l]; ++n)
+      for (int k = 0; k < h->K[l]; ++k) hp[h->offW[l] + (long long)n * h->Kp[l] + k] = params[o++];
+    for (int n = 0; n < h->N[l]; ++n) hp[h->offB[l] + n] = params[o++];
+  }
+  MCHK(h, hipStreamSynchronize(h->st));
+  MCHK(h, hipMemcpy(h->p, hp.data(), h->np * sizeof(float), hipMemcpyHostToDevice));
+  MCHK(h, hipMemset(h->m, 0, h->np * sizeof(float)));
+  MCHK(h, hipMemset(h->v, 0, h->np * sizeof(float)));
+  h->t = 0;
+  hipLaunchKernelGGL(k_to_bf, dim3((unsigned)((h->np + 255) / 256)), dim3(256), 0, h->st, h->p, h->wb, h->np);
+  MCHK(h, hipStreamSynchronize(h->st));
+  return LH_OK;
+}
+
+int lh_mlp_set_data(lh_mlp* h, const float* x, const float* y, long long n) {
+  if (!h || !x || !y || n < 1) return mfail(h, LH_ERR_ARG, "lh_mlp_set_data: null argument or n < 1");
+  if (h->dx) (void)hipFree(h->dx);
+  if (h->dy) (void)hipFree(h->dy);
+  h->dx = h->dy = nullptr;
+  MCHK(h, hipMalloc((void**)&h->dx, (size_t)n * h->nin * sizeof(float)));
+  MCHK(h, hipMalloc((void**)&h->dy, (size_t)n * h->nout * sizeof(float)));
+  MCHK(h, hipMemcpy(h->dx, x, (size_t)n * h->nin * sizeof(float), hipMemcpyHostToDevice));
+  MCHK(h, hipMemcpy(h->dy, y, (size_t)n * h->nout * sizeof(float), hipMemcpyHostToDevice));
+  h->ndata = n;
+  return LH_OK;
+}
+
+// loss and FP32 gradients of one batch of `batch` host rows (x: batch x nin, y: batch x nout)
+int lh_mlp_loss_grad(lh_mlp* h, const float* x, const float* y, float* loss, float* grad) {
+  if (!h || !x || !y) return mfail(h, LH_ERR_ARG, "lh_mlp_loss_grad: null argument");
+  float* dx = nullptr;
+  MCHK(h, hipMalloc((void**)&dx, (size_t)h->B * h->nin * sizeof(float)));
+  MCHK(h, hipMemcpy(dx, x, (size_t)h->B * h->nin * sizeof(float), hipMemcpyHostToDevice));
+  MCHK(h, hipMemcpy(h->T, y, (size_t)h->B * h->nout * sizeof(float), hipMemcpyHostToDevice));
+  hipLaunchKernelGGL(k_pack_x, dim3(h->B), dim3(64), 0, h->st, dx, h->B, h->nin, h->Kp[0], h->B, h->X[0]);
+  mlp_forward(h);
+  hipLaunchKernelGGL(k_mse, dim3(1), dim3(256), 0, h->st, h->Z[h->L - 1], h->T, h->dH, h->loss, h->B,
+                     h->Np[h->L - 1], h->nout);
+  mlp_backward(h);
+  MCHK(h, hipStreamSynchronize(h->st));
+  (void)hipFree(dx);
+  if (loss) MCHK(h, hipMemcpy(loss, h->loss, sizeof(float), hipMemcpyDeviceToHost));
+  if (grad) {
+    std::vector<float> hg(h->np);
+    MCHK(h, hipMemcpy(hg.data(), h->g, h->np * sizeof(float), hipMemcpyDeviceToHost));
+    pack_params(h, hg.data(), grad);
+  }
+  return LH_OK;
+}
+
+// `steps` Adam steps on batches drawn from the data set; loss_last = loss of the last batch
+int lh_mlp_train(lh_mlp* h, int steps, float lr, unsigned long long seed, float* loss_last) {
+  if (!h || steps < 0) return mfail(h, LH_ERR_ARG, "lh_mlp_train: bad argument");
+  if (!h->dx) return mfail(h, LH_ERR_ARG, "lh_mlp_train: no data set (lh_mlp_set_data)");
+  const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+  for (int s = 0; s < steps; ++s) {
+    hipLaunchKernelGGL(k_gather, dim3(h->B), dim3(64), 0, h->st, h->dx, h->dy, h->ndata, h->nin, h->nout,
+                       h->Kp[0], h->B, (uint64_t)seed, (uint64_t)(h->t + s), h->X[0], h->T);
+    mlp_forward(h);
+    hipLaunchKernelGGL(k_mse, dim3(1), dim3(256), 0, h->st, h->Z[h->L - 1], h->T, h->dH, h->loss, h->B,
+                       h->Np[h->L - 1], h->nout);
+    mlp_backward(h);
+    const int t = h->t + s + 1;
+    const float c1 = 1.f - std::pow(b1, (float)t), c2 = 1.f - std::pow(b2, (float)t);
+    hipLaunchKernelGGL(k_adam, dim3((unsigned)((h->np + 255) / 256)), dim3(256), 0, h->st, h->p, h->g, h->m,
+                       h->v, h->wb, h->np, lr, b1, b2, eps, c1, c2);
+  }
+  h->t += steps;
+  MCHK(h, hipGetLastError());
+  MCHK(h, hipStreamSynchronize(h->st));
+  if (loss_last) MCHK(h, hipMemcpy(loss_last, h->loss, sizeof(float), hipMemcpyDeviceToHost));
+  return LH_OK;
+}
+
+// outputs of n host rows (x: n x nin) -> y (n x nout), in batches of B
+int lh_mlp_predict(lh_mlp* h, const float* x, long long n, float* y) {
+  if (!h || !x || !y || n < 0) return mfail(h, LH_ERR_ARG, "lh_mlp_predict: bad argument");
+  float* dx = nullptr;
+  MCHK(h, hipMalloc((void**)&dx, (size_t)h->B * h->nin * sizeof(float)));
+  std::vector<float> z((size_t)h->B * h->Np[h->L - 1]);
+  for (long long r0 = 0; r0 < n; r0 += h->B) {
+    const int nr = (int)std::min<long long>(h->B, n - r0);
+    MCHK(h, hipMemcpy(dx, x + (size_t)r0 * h->nin, (size_t)nr * h->nin * sizeof(float), hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(k_pack_x, dim3(h->B), dim3(64), 0, h->st, dx, nr, h->nin, h->Kp[0], h->B, h->X[0]);
+    mlp_forward(h);
+    MCHK(h, hipStreamSynchronize(h->st));
+    MCHK(h, hipMemcpy(z.data(), h->Z[h->L - 1], z.size() * sizeof(float), hipMemcpyDeviceToHost));
+    for (int r = 0; r < nr; ++r)
+      for (int o = 0; o < h->nout; ++o) y[(size_t)(r0 + r) * h->nout + o] = z[(size_t)r * h->Np[h->L - 1] + o];
+  }
+  (void)hipFree(dx);
+  return LH_OK;
+}
+
+}  // extern "C"
