@@ -86,6 +86,15 @@ int lh_get_state(lh_handle* h, double* u1, double* u2, double* u3, double* p);
 int lh_set_nonlinear(lh_handle* h, const double* nu1, const double* nu2, const double* nu3);
 int lh_get_nonlinear(lh_handle* h, double* nu1, double* nu2, double* nu3);
 
+/* Forcing added to the nonlinear term Nu_new of every substep, in the layout of
+ * lh_set_nonlinear (interior planes of u, v, w); the stored Nu (Nu_old of the next substep)
+ * includes it.  A NULL field is zero; all three NULL remove the forcing.  The learned
+ * closure's subfilter-stress divergence d_j tau_ij enters the coarse channel this way.  The
+ * reference has no such term: it is the coupling point of the closure. */
+int lh_set_forcing(lh_handle* h, const double* f1, const double* f2, const double* f3);
+/* The forcing last set (lh_set_forcing or lh_closure_apply), layout of lh_get_nonlinear. */
+int lh_get_forcing(lh_handle* h, double* f1, double* f2, double* f3);
+
 /* One RK3 substep kRK in 1..3 with time step dt: RHS0_u1/u2/u3, nonlinear, solveU/V/U,
  * meanflow_ctU (flag_ctpress == 0), solveP, v_corr (littleharsh.f90:166-188).
  * dt <= 0 uses the device CFL time step of the last lh_cfl_dt / lh_step_cfl.  With
@@ -190,6 +199,39 @@ int lh_mlp_set_data(lh_mlp* m, const float* x, const float* y, long long n);
 int lh_mlp_loss_grad(lh_mlp* m, const float* x, const float* y, float* loss, float* grad);
 int lh_mlp_train(lh_mlp* m, int steps, float lr, unsigned long long seed, float* loss_last);
 int lh_mlp_predict(lh_mlp* m, const float* x, long long n, float* y);
+/* Outputs at every point (j, k, i) of an ny x nz x nx grid, row (j nz + k) nx + i of y, from
+ * inputs gathered on the device: the nch fields g (nch x ny x nz x nx) on the 3x3x3
+ * neighbourhood (feature ((dj+1) 9 + (dk+1) 3 + (di+1)) nch + c, j clamped, k and i
+ * periodic), then the extra field ex (ny x nz x nx) at the point when nin = 27 nch + 1
+ * (ex NULL: nin = 27 nch).  Input f is standardised as (v - mean[f]) / scale[f].  The coarse
+ * channel evaluates the closure this way from its nine velocity gradients and the solid
+ * fraction. */
+int lh_mlp_predict_grid(lh_mlp* h, const float* g, int nch, const float* ex, int ny, int nz, int nx,
+                        const float* mean, const float* scale, float* y);
+
+/* ---- the learned closure inside a coarse channel (no counterpart in the reference) ----
+ * lh_closure_create binds a network (244 inputs: the nine velocity gradients on the 3x3x3
+ * stencil and the solid fraction; 6 outputs: the stresses uu, vv, ww, uv, uw, vw) to a
+ * channel h whose interior u planes are the ny = nyu - 2 coarse centres at heights yc, with
+ * nz x nx coarse points (nx = 2 (mx - 1)) spaced dxc, dzc, shifted by (shift_x, shift_z) to
+ * the box-filter centres the network was trained at.  sf: solid fraction (ny x nz x nx);
+ * xmean / xscale: input standardisation (244); ymean / yscale: output scaling (6).  Creating
+ * it switches the channel's forcing on (zero until the first apply).
+ * lh_closure_apply: from the current state, the forcing d_j tau_ij for the following
+ * substeps, computed on the channel's stream (no host round trip); call it before each
+ * lh_step. */
+typedef struct lh_closure lh_closure;
+int lh_closure_create(lh_handle* h, lh_mlp* net, const double* yc, double dxc, double dzc, double shift_x,
+                      double shift_z, const float* sf, const float* xmean, const float* xscale,
+                      const float* ymean, const float* yscale, lh_closure** out);
+/* The same coupling with the Smagorinsky stresses cs2 (-2 Delta^2 |S| S_ij) in place of the
+ * network, Delta^3 = dxc dzc dy (dy: centred spacing of the centres): the classical closure
+ * the learned one is measured against. */
+int lh_closure_create_smagorinsky(lh_handle* h, const double* yc, double dxc, double dzc, double shift_x,
+                                  double shift_z, double cs2, lh_closure** out);
+int lh_closure_apply(lh_closure* c);
+int lh_closure_destroy(lh_closure* c);
+const char* lh_closure_last_error(const lh_closure* c);
 
 /* Per-kernel timing (no counterpart in the reference; measurement support).  While
  * enabled, lh_step/lh_step_cfl issue their kernels directly (no graph) with a HIP event

@@ -6,7 +6,7 @@ import subprocess
 import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-SRC = [os.path.join(HERE, "csrc", "lh_api.hip"), os.path.join(HERE, "csrc", "lh_mlp.hip")]
+SRC = [os.path.join(HERE, "csrc", n) for n in ("lh_api.hip", "lh_mlp.hip", "lh_closure.hip")]
 OUT = os.path.join(HERE, "liblh_amd.so")
 
 HIPCC_FLAGS = [

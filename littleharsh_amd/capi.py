@@ -104,6 +104,8 @@ def load_library():
     lib.lh_set_state.argtypes = [vp, _dp, _dp, _dp, _dp]
     lib.lh_get_state.argtypes = [vp, _dp, _dp, _dp, _dp]
     lib.lh_set_nonlinear.argtypes = [vp, _dp, _dp, _dp]
+    lib.lh_set_forcing.argtypes = [vp, _dp, _dp, _dp]
+    lib.lh_get_forcing.argtypes = [vp, _dp, _dp, _dp]
     lib.lh_get_nonlinear.argtypes = [vp, _dp, _dp, _dp]
     lib.lh_substep.argtypes = [vp, ctypes.c_int, ctypes.c_double]
     lib.lh_step.argtypes = [vp, ctypes.c_double]
@@ -141,13 +143,23 @@ def load_library():
     lib.lh_mlp_loss_grad.argtypes = [vp, _fp, _fp, _fp, _fp]
     lib.lh_mlp_train.argtypes = [vp, ctypes.c_int, ctypes.c_float, ctypes.c_ulonglong, _fp]
     lib.lh_mlp_predict.argtypes = [vp, _fp, ctypes.c_longlong, _fp]
+    lib.lh_mlp_predict_grid.argtypes = [vp, _fp, ctypes.c_int, _fp, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                        _fp, _fp, _fp]
+    lib.lh_closure_create.argtypes = [vp, vp, _dp, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                      ctypes.c_double, _fp, _fp, _fp, _fp, _fp, ctypes.POINTER(vp)]
+    lib.lh_closure_create_smagorinsky.argtypes = [vp, _dp, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                                  ctypes.c_double, ctypes.c_double, ctypes.POINTER(vp)]
+    lib.lh_closure_apply.argtypes = [vp]
+    lib.lh_closure_destroy.argtypes = [vp]
+    lib.lh_closure_last_error.argtypes = [vp]
+    lib.lh_closure_last_error.restype = ctypes.c_char_p
     _lib = lib
     return lib
 
 
 EXPORTED = [
     "lh_create", "lh_destroy", "lh_last_error", "lh_set_state", "lh_get_state",
-    "lh_set_nonlinear", "lh_get_nonlinear", "lh_substep", "lh_step", "lh_step_cfl",
+    "lh_set_nonlinear", "lh_get_nonlinear", "lh_set_forcing", "lh_get_forcing", "lh_substep", "lh_step", "lh_step_cfl",
     "lh_cfl_dt", "lh_get_scalars", "lh_max_div", "lh_stats_accumulate", "lh_stats_read",
     "lh_cond_stats_size", "lh_cond_stats_read", "lh_spectra_accumulate", "lh_spectra_read",
     "lh_timing_enable", "lh_timing_reset", "lh_timing_read",
@@ -155,7 +167,8 @@ EXPORTED = [
     "lh_batch_create", "lh_batch_step_cfl", "lh_batch_last_error", "lh_batch_destroy",
     "lh_mlp_create", "lh_mlp_destroy", "lh_mlp_last_error", "lh_mlp_param_count",
     "lh_mlp_get_params", "lh_mlp_set_params", "lh_mlp_set_data", "lh_mlp_loss_grad",
-    "lh_mlp_train", "lh_mlp_predict",
+    "lh_mlp_train", "lh_mlp_predict", "lh_mlp_predict_grid", "lh_closure_create", "lh_closure_create_smagorinsky", "lh_closure_apply",
+    "lh_closure_destroy", "lh_closure_last_error",
 ]
 
 KERNEL_NAMES = ["zinv", "xpass_nl", "zfwd", "assemble", "ib_z", "ib_x", "vel", "pressure", "misc",

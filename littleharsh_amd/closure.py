@@ -16,15 +16,21 @@ the box), 244 inputs in all.
 """
 from __future__ import annotations
 
+import os
+from concurrent.futures import ThreadPoolExecutor
+
 import numpy as np
+import scipy.fft as sfft
 
 from .channel import ChannelSetup
+
+_WORKERS = max(1, min(16, os.cpu_count() or 1))
 
 STRESS = ("uu", "vv", "ww", "uv", "uw", "vw")
 NIN = 244
 
 
-def modes_to_phys(s: ChannelSetup, a: np.ndarray) -> np.ndarray:
+def modes_to_phys(s: ChannelSetup, a: np.ndarray, workers: int = _WORKERS) -> np.ndarray:
     """Spectral plane (ncol,) -> physical collocation plane (pz, px), the inverse of
     littleharsh_amd.initial.phys_to_modes (inverse transforms unscaled)."""
     b = a.reshape(s.nz, s.mx)
@@ -32,8 +38,60 @@ def modes_to_phys(s: ChannelSetup, a: np.ndarray) -> np.ndarray:
     full = np.zeros((s.pz, s.px // 2 + 1), dtype=np.complex128)
     full[:h, : s.mx] = b[:h]
     full[s.pz - h:, : s.mx] = b[h:]
-    full = np.fft.ifft(full, axis=0) * s.pz
-    return np.fft.irfft(full, n=s.px, axis=1) * s.px
+    full = sfft.ifft(full, axis=0, workers=workers) * s.pz
+    return sfft.irfft(full, n=s.px, axis=1, workers=workers) * s.px
+
+
+def phys_to_modes96(a: np.ndarray, nz: int, mx: int) -> np.ndarray:
+    """Physical plane(s) (..., nzc, nxc) on the coarse collocation grid -> spectral columns
+    (..., nz * mx) of a channel with nz = nzc z modes and mx = nxc / 2 + 1 x modes; the Nyquist
+    row and column are dropped (set to zero).  Inverse of modes_to_coarse."""
+    nzc, nxc = a.shape[-2:]
+    assert nzc == nz and nxc // 2 + 1 == mx
+    f = np.fft.rfft2(a, axes=(-2, -1)) / (nzc * nxc)
+    f[..., nzc // 2, :] = 0.0
+    f[..., :, nxc // 2] = 0.0
+    return f.reshape(*a.shape[:-2], nz * mx)
+
+
+def modes_to_coarse(b: np.ndarray, nz: int, mx: int) -> np.ndarray:
+    """Spectral columns (..., nz * mx) -> physical plane(s) (..., nz, 2 (mx - 1)) sampled at the
+    coarse points (one per resolved mode, no dealiasing padding)."""
+    nxc = 2 * (mx - 1)
+    f = b.reshape(*b.shape[:-1], nz, mx)
+    return np.fft.irfft2(f, s=(nz, nxc), axes=(-2, -1)) * (nz * nxc)
+
+
+def velocity_gradients(ub, vb, wb, yc, dxc, dzc) -> np.ndarray:
+    """Nine velocity gradients (9, ny, nz, nx) of fields (ny, nz, nx) at the heights yc:
+    d/dx, d/dy, d/dz of u, then of v, then of w; central differences, x and z periodic,
+    one-sided at the end planes in y."""
+    g = []
+    for c in (ub, vb, wb):
+        ddx = (np.roll(c, -1, axis=2) - np.roll(c, 1, axis=2)) / (2 * dxc)
+        ddz = (np.roll(c, -1, axis=1) - np.roll(c, 1, axis=1)) / (2 * dzc)
+        ddy = np.empty_like(c)
+        ddy[1:-1] = (c[2:] - c[:-2]) / (yc[2:] - yc[:-2])[:, None, None]
+        ddy[0] = (c[1] - c[0]) / (yc[1] - yc[0])
+        ddy[-1] = (c[-1] - c[-2]) / (yc[-1] - yc[-2])
+        g += [ddx, ddy, ddz]
+    return np.stack(g)
+
+
+def stencil_features(grad: np.ndarray, sf: np.ndarray, m, kz, ix) -> np.ndarray:
+    """Inputs (n, 244) of the points (m, kz, ix): the nine gradients on the 3x3x3
+    neighbourhood, feature ((dm + 1) 9 + (dz + 1) 3 + (dx + 1)) 9 + c (m clamped, z and x
+    periodic), then the solid fraction of the point."""
+    ny, nz, nx = grad.shape[1:]
+    cols = []
+    for dm in (-1, 0, 1):
+        mm = np.clip(m + dm, 0, ny - 1)
+        for dz in (-1, 0, 1):
+            zz = (kz + dz) % nz
+            for dx in (-1, 0, 1):
+                xx = (ix + dx) % nx
+                cols.append(grad[:, mm, zz, xx].T)
+    return np.concatenate(cols + [sf[m, kz, ix][:, None]], axis=1).astype(np.float32)
 
 
 def _box(a: np.ndarray, fz: int, fx: int) -> np.ndarray:
@@ -72,31 +130,26 @@ class Filter:
         """State arrays (ncol, nplanes) -> dict of coarse fields (nyc, nzc, nxc): ub, vb, wb,
         the six stresses and the solid fraction."""
         s = self.s
-        cache = {}
-
-        def plane(j):
-            if j not in cache:
-                U = modes_to_phys(s, u[:, j])
-                W = modes_to_phys(s, w[:, j])
-                V = 0.5 * (modes_to_phys(s, v[:, j - 1]) + modes_to_phys(s, v[:, j]))
-                bx = lambda a: _box(a, self.fz, self.fx)
-                cache[j] = dict(ub=bx(U), vb=bx(V), wb=bx(W), uu=bx(U * U), vv=bx(V * V), ww=bx(W * W),
-                                uv=bx(U * V), uw=bx(U * W), vw=bx(V * W),
-                                sf=bx(self.mask[j].astype(np.float64)))
-            return cache[j]
-
-        nyc = len(self.centres)
         keys = ("ub", "vb", "wb", "uu", "vv", "ww", "uv", "uw", "vw", "sf")
+
+        def plane(j):                      # one fine u plane -> its box averages
+            U = modes_to_phys(s, u[:, j], 1)
+            W = modes_to_phys(s, w[:, j], 1)
+            V = 0.5 * (modes_to_phys(s, v[:, j - 1], 1) + modes_to_phys(s, v[:, j], 1))
+            bx = lambda a: _box(a, self.fz, self.fx)
+            return dict(ub=bx(U), vb=bx(V), wb=bx(W), uu=bx(U * U), vv=bx(V * V), ww=bx(W * W),
+                        uv=bx(U * V), uw=bx(U * W), vw=bx(V * W), sf=bx(self.mask[j].astype(np.float64)))
+
+        need = sorted({q for j in self.centres for q in self.window(j)[0]})
+        with ThreadPoolExecutor(max_workers=_WORKERS) as ex:
+            planes = dict(zip(need, ex.map(plane, need)))
+        nyc = len(self.centres)
         out = {k: np.zeros((nyc, self.nzc, self.nxc)) for k in keys}
         for m, j in enumerate(self.centres):
             js, wt = self.window(j)
             for q, wq in zip(js, wt):
-                pl = plane(q)
                 for k in keys:
-                    out[k][m] += wq * pl[k]
-            for q in list(cache):
-                if q < j - 1:
-                    del cache[q]
+                    out[k][m] += wq * planes[q][k]
         res = {k: out[k] for k in ("ub", "vb", "wb", "sf")}
         for a, b, name in ((0, 0, "uu"), (1, 1, "vv"), (2, 2, "ww"), (0, 1, "uv"), (0, 2, "uw"), (1, 2, "vw")):
             fa = (out["ub"], out["vb"], out["wb"])[a]
@@ -106,21 +159,12 @@ class Filter:
 
     def gradients(self, f: dict) -> np.ndarray:
         """Nine filtered velocity gradients (9, nyc, nzc, nxc): d/dx, d/dy, d/dz of u, v, w."""
-        y = self.yc
-        g = []
-        for c in (f["ub"], f["vb"], f["wb"]):
-            ddx = (np.roll(c, -1, axis=2) - np.roll(c, 1, axis=2)) / (2 * self.dxc)
-            ddz = (np.roll(c, -1, axis=1) - np.roll(c, 1, axis=1)) / (2 * self.dzc)
-            ddy = np.empty_like(c)
-            ddy[1:-1] = (c[2:] - c[:-2]) / (y[2:] - y[:-2])[:, None, None]
-            ddy[0] = (c[1] - c[0]) / (y[1] - y[0])
-            ddy[-1] = (c[-1] - c[-2]) / (y[-1] - y[-2])
-            g += [ddx, ddy, ddz]
-        return np.stack(g)
+        return velocity_gradients(f["ub"], f["vb"], f["wb"], self.yc, self.dxc, self.dzc)
 
-    def samples(self, f: dict, n: int | None, rng: np.random.Generator):
+    def samples(self, f: dict, n: int | None, rng: np.random.Generator, grad: np.ndarray | None = None):
         """Features (n, 244) and targets (n, 6) at n random coarse points (all when None)."""
-        grad = self.gradients(f)
+        if grad is None:
+            grad = self.gradients(f)
         nyc, nzc, nxc = grad.shape[1:]
         if n is None:
             m, kz, ix = (a.ravel() for a in np.meshgrid(np.arange(nyc), np.arange(nzc), np.arange(nxc),
@@ -129,22 +173,15 @@ class Filter:
             m = rng.integers(0, nyc, n)
             kz = rng.integers(0, nzc, n)
             ix = rng.integers(0, nxc, n)
-        cols = []
-        for dm in (-1, 0, 1):
-            mm = np.clip(m + dm, 0, nyc - 1)
-            for dz in (-1, 0, 1):
-                zz = (kz + dz) % nzc
-                for dx in (-1, 0, 1):
-                    xx = (ix + dx) % nxc
-                    cols.append(grad[:, mm, zz, xx].T)
-        x = np.concatenate(cols + [f["sf"][m, kz, ix][:, None]], axis=1)
+        x = stencil_features(grad, f["sf"], m, kz, ix)
         y = np.stack([f["tau_" + k][m, kz, ix] for k in STRESS], axis=1)
-        return x.astype(np.float32), y.astype(np.float32), (m, kz, ix)
+        return x, y.astype(np.float32), (m, kz, ix)
 
-    def smagorinsky_basis(self, f: dict, idx):
+    def smagorinsky_basis(self, f: dict, idx, grad: np.ndarray | None = None):
         """-2 Delta^2 |S| S_ij at the sample points (n, 6) in the order of STRESS; Delta is the
         cube root of the coarse cell volume.  A fitted constant C_s^2 scales it."""
-        grad = self.gradients(f)
+        if grad is None:
+            grad = self.gradients(f)
         m, kz, ix = idx
         G = grad[:, m, kz, ix].reshape(3, 3, -1)        # G[i, j] = d u_i / d x_j
         S = 0.5 * (G + G.transpose(1, 0, 2))

@@ -1,17 +1,20 @@
 """Learned canopy closure end to end: GPU DNS snapshots -> filtered data set -> MLP trained on
-the GPU (BF16 matrix cores) -> a-priori scores on later, held-out snapshots.
+the GPU (BF16 matrix cores) -> a-priori scores on later, held-out snapshots -> a-posteriori
+runs of the coarse channel with the closure, with a fitted Smagorinsky closure and without
+any closure, judged against the filtered DNS.
 
-    python -m littleharsh_amd.closure_train --spinup 400 --snaps 10 --every 10 --steps 20000
+    python -m littleharsh_amd.closure_train --spinup 1500 --snaps 12 --every 10 --steps 20000
 
 The DNS is the 384x289x384 canopy channel of the benchmark (laminar profile plus
-|k|^-5/3 modes, CFL 0.5).  Snapshots are taken every ``every`` steps after ``spinup``
-steps; the first ``train_frac`` of them train the network, the rest, taken after ``gap``
-further steps, test it.  The network
-(244 -> 64 x 4 -> 6, tanh) predicts the six subfilter stresses from standardised features;
-the baselines are no closure (tau = 0), the mean stress of the training data and the
-Smagorinsky model with its constant fitted by least squares on the training data.
+|k|^-5/3 modes, CFL 0.5).  After ``spinup`` steps, ``snaps`` snapshots ``every`` steps apart
+train the network.  After ``gap`` further steps the judge window starts: its first snapshot
+is the initial state of the coarse runs, and ``judge_snaps`` more follow ``judge_every``
+steps apart.  The judge snapshots are also the a-priori test set.  The network
+(244 -> 64 x 4 -> 6, tanh) predicts the six subfilter stresses from standardised features.
+The a-priori baselines are no closure (tau = 0), the mean stress of the training data and
+the Smagorinsky model with its constant fitted by least squares on the training data.
 
-Prints progress lines and one JSON line.
+Prints progress lines on stderr and one JSON line on stdout.
 """
 from __future__ import annotations
 
@@ -24,6 +27,7 @@ import numpy as np
 
 from .channel import config_production, config_small
 from .closure import NIN, Filter, scores
+from .closure_les import Coarse, DeviceClosure, coarse_setup, compare, plane_stats
 from .initial import laminar_plus_modes
 from .mlp import MLP
 from .solver import Channel
@@ -33,14 +37,78 @@ def log(msg: str) -> None:
     print(msg, file=sys.stderr, flush=True)
 
 
+def dataset(filt, fields, per_snap, rng):
+    xs, ys, bs = [], [], []
+    for f in fields:
+        grad = filt.gradients(f)
+        x, y, idx = filt.samples(f, per_snap, rng, grad)
+        xs.append(x)
+        ys.append(y)
+        bs.append(filt.smagorinsky_basis(f, idx, grad))
+    return np.concatenate(xs), np.concatenate(ys), np.concatenate(bs)
+
+
+def run_coarse(c, co, ic, t0, judge, closure, cfl, tag, device=None):
+    """Advance the coarse channel from ic at time t0 to every judge time, with no closure,
+    a host closure (closure(u, v, w) -> forcing) or a device closure (device(ch) -> an object
+    whose apply() sets the forcing on the GPU); returns the statistics errors per judge time
+    and the cost."""
+    ch = Channel(c)
+    ch.set_state(*ic)
+    dev = device(ch) if device is not None else None
+    t, nstep, t_cl = t0, 0, 0.0
+    rows = []
+    w0 = time.perf_counter()
+    for tj, dns, ref in judge:
+        while t < tj - 1e-12 * tj:
+            if closure is not None:
+                a = time.perf_counter()
+                u, v, w, _ = ch.get_state()
+                ch.set_forcing(*closure(u, v, w))
+                t_cl += time.perf_counter() - a
+            if dev is not None:
+                dev.apply()
+            dt = min(ch.cfl_dt(cfl), tj - t)
+            ch.step(dt)
+            t += dt
+            nstep += 1
+            if nstep % 25 == 0:
+                log(f"# {tag}: step {nstep} t {t:.5f}")
+        sc = ch.scalars()
+        u, v, w, _ = ch.get_state()
+        if not (np.isfinite(sc["qx"]) and np.all(np.isfinite(u))):
+            log(f"# {tag}: state not finite at t {t:.5f}")
+            rows.append({"t": t, "diverged": True})
+            break
+        ub, vb, wb = co.sample(u, v, w)
+        st = plane_stats(ub, vb, wb)
+        err = compare(st, ref, np.abs(co.yc) > 0.85)
+        fl = dns["ub"] - ref["U"][:, None, None]
+        err["u_field"] = float(np.linalg.norm(ub - dns["ub"]) / np.linalg.norm(fl))
+        err["t"] = t
+        err["steps"] = nstep
+        err["flow_rate"] = sc["qx"]
+        rows.append(err)
+        log(f"# {tag}: t {t:.5f} after {nstep} steps: U {err['U']:.3e} urms {err['urms']:.3e} "
+            f"uv {err['uv']:.3e} u-field {err['u_field']:.3e}")
+    wall = time.perf_counter() - w0
+    if dev is not None:
+        dev.close()
+    ch.close()
+    span = t - t0
+    return {"judge": rows, "steps": nstep, "wall_s": wall, "closure_s": t_cl, "sim_time": span,
+            "s_per_unit_time": wall / span if span > 0 else None}
+
+
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--spinup", type=int, default=400)
-    ap.add_argument("--snaps", type=int, default=10)
+    ap.add_argument("--spinup", type=int, default=1500)
+    ap.add_argument("--snaps", type=int, default=12)
     ap.add_argument("--every", type=int, default=10)
-    ap.add_argument("--train-frac", type=float, default=0.8)
-    ap.add_argument("--gap", type=int, default=100, help="extra steps between the training and "
-                    "the test snapshots")
+    ap.add_argument("--gap", type=int, default=100, help="steps between the training snapshots "
+                    "and the judge window")
+    ap.add_argument("--judge-snaps", type=int, default=4)
+    ap.add_argument("--judge-every", type=int, default=250)
     ap.add_argument("--per-snap", type=int, default=150000)
     ap.add_argument("--steps", type=int, default=20000)
     ap.add_argument("--batch", type=int, default=4096)
@@ -52,13 +120,14 @@ def main():
     args = ap.parse_args()
 
     s = config_production() if args.config == "production" else config_small()
+    dyq = 10.0 if args.config == "production" else 6.0
     filt = Filter(s, args.nxc, args.nzc)
     ch = Channel(s)
     ch.set_state(*laminar_plus_modes(s, seed=1234))
     t0 = time.perf_counter()
     tsim = 0.0
 
-    def advance(nsteps: int) -> float:
+    def advance(nsteps: int):
         t = 0.0
         for _ in range(nsteps):
             t += ch.step_cfl(args.cfl)
@@ -72,37 +141,39 @@ def main():
         sc = ch.scalars()
         log(f"# spin-up step {min(k + 100, args.spinup)}: t {tsim:.5f} dt {sc['dt']:.3e} "
             f"flow rate {sc['qx']:.4f} div {ch.max_div():.2e}")
-    ntr = max(1, int(round(args.train_frac * args.snaps)))
-    fields, times = [], []
-    for n in range(args.snaps):
-        if n == ntr:
-            tsim += advance(args.gap)
-        tsim += advance(args.every)
+
+    def snapshot():
         u, v, w, _ = ch.get_state()
-        fields.append(filt.filtered(u, v, w))
-        times.append(tsim)
-        log(f"# snapshot {n + 1}/{args.snaps} at t {tsim:.5f} filtered")
+        return filt.filtered(u, v, w)
+
+    train_f, train_t = [], []
+    for n in range(args.snaps):
+        tsim += advance(args.every)
+        train_f.append(snapshot())
+        train_t.append(tsim)
+        log(f"# training snapshot {n + 1}/{args.snaps} at t {tsim:.5f}")
+    tsim += advance(args.gap)
+    judge_f, judge_t = [snapshot()], [tsim]
+    dns_steps, dns_wall, dns_span = 0, 0.0, 0.0
+    for n in range(args.judge_snaps):
+        a = time.perf_counter()
+        dt = advance(args.judge_every)
+        dns_wall += time.perf_counter() - a
+        dns_steps += args.judge_every
+        dns_span += dt
+        tsim += dt
+        judge_f.append(snapshot())
+        judge_t.append(tsim)
+        log(f"# judge snapshot {n + 1}/{args.judge_snaps} at t {tsim:.5f}")
     t_dns = time.perf_counter() - t0
-    del ch
+    ch.close()
 
+    # ---- data, training
     rng = np.random.default_rng(2024)
-    xs, ys, bs = [], [], []
-    for f in fields[:ntr]:
-        x, y, idx = filt.samples(f, args.per_snap, rng)
-        xs.append(x)
-        ys.append(y)
-        bs.append(filt.smagorinsky_basis(f, idx))
-    xtr, ytr, btr = np.concatenate(xs), np.concatenate(ys), np.concatenate(bs)
-    xs, ys, bs = [], [], []
-    for f in fields[ntr:]:
-        x, y, idx = filt.samples(f, args.per_snap, rng)
-        xs.append(x)
-        ys.append(y)
-        bs.append(filt.smagorinsky_basis(f, idx))
-    xte, yte, bte = np.concatenate(xs), np.concatenate(ys), np.concatenate(bs)
+    xtr, ytr, btr = dataset(filt, train_f, args.per_snap, rng)
+    xte, yte, bte = dataset(filt, judge_f, args.per_snap, rng)
+    del train_f
     log(f"# data: {len(xtr)} training and {len(xte)} test points")
-
-    # standardisation (training statistics)
     xm, xsd = xtr.mean(axis=0), xtr.std(axis=0)
     xsd[xsd < 1e-12] = 1.0
     ym, ysd = ytr.mean(axis=0), ytr.std(axis=0)
@@ -119,33 +190,62 @@ def main():
     t_train = time.perf_counter() - t1
 
     pred = net.predict((xte - xm) / xsd) * ysd + ym
-    # Smagorinsky: tau = mean + c * (-2 Delta^2 |S| S), c = C_s^2 by least squares
     dtr = (ytr - ym).ravel().astype(np.float64)
     btv = btr.ravel()
-    c = float(np.dot(btv, dtr) / max(np.dot(btv, btv), 1e-300))
-    smag = ym + c * bte
+    cs2 = float(np.dot(btv, dtr) / max(np.dot(btv, btv), 1e-300))
+    smag = ym + cs2 * bte
+    nw = sum(a * b for a, b in [(64, NIN), (64, 64), (64, 64), (64, 64), (6, 64)])
+    flops = 6.0 * args.batch * nw * args.steps          # forward 2, backward 4 per weight per row
     res = {
-        "workload": f"{s.nx}x{s.ny}x{s.nz} DNS -> {args.nxc}x{len(filt.centres)}x{args.nzc} "
+        "workload": f"{s.nx}x{s.ny + 2 * s.d}x{s.nz} DNS -> {args.nxc}x{len(filt.centres)}x{args.nzc} "
                     f"box-filtered ({filt.fx}x3x{filt.fz} collocation points)",
-        "snapshots": {"train": ntr, "test": args.snaps - ntr, "every": args.every, "spinup": args.spinup,
-                      "gap": args.gap, "times": times},
+        "snapshots": {"train": args.snaps, "every": args.every, "spinup": args.spinup, "gap": args.gap,
+                      "train_times": train_t, "judge_times": judge_t},
         "points": {"train": int(len(xtr)), "test": int(len(xte))},
         "network": {"inputs": NIN, "hidden": [64] * 4, "outputs": 6, "params": net.nparams,
-                    "batch": args.batch, "steps": args.steps, "lr": args.lr},
+                    "batch": args.batch, "steps": args.steps, "lr": args.lr, "last_batch_loss": loss},
         "train_seconds": t_train,
+        "train_tflops": flops / t_train / 1e12,
+        "train_frac_of_bf16_peak": flops / t_train / 2.5e15,
+        "train_samples_per_s": args.batch * args.steps / t_train,
         "dns_and_filter_seconds": t_dns,
-        "smagorinsky_cs2": c,
-        "scores": {
+        "smagorinsky_cs2": cs2,
+        "apriori": {
             "mlp": scores(pred, yte),
             "smagorinsky": scores(smag, yte),
             "mean": scores(np.broadcast_to(ym, yte.shape), yte),
             "zero": scores(np.zeros_like(yte), yte),
         },
     }
-    nw = sum(a * b for a, b in [(64, NIN), (64, 64), (64, 64), (64, 64), (6, 64)])
-    flops = 6.0 * args.batch * nw * args.steps          # forward 2, backward 4 per weight per row
-    res["train_tflops"] = flops / t_train / 1e12
-    res["train_samples_per_s"] = args.batch * args.steps / t_train
+    del xtr, ytr, btr, xte, yte, bte
+
+    # ---- a posteriori: the coarse channel from the first judge snapshot
+    c = coarse_setup(s, args.nxc, args.nzc, dyq)
+    co = Coarse(c, filt)
+    ic = co.initial_state(judge_f[0])
+    sf = judge_f[0]["sf"].astype(np.float32)
+    # the first entry (no steps) measures how well the coarse initial state represents the DNS
+    judge = [(tj, f, plane_stats(f["ub"], f["vb"], f["wb"])) for tj, f in zip(judge_t, judge_f)]
+
+    def learned(ch):
+        return DeviceClosure(ch, net, co, sf, xm, xsd, ym, ysd)
+
+    def smagorinsky(ch):
+        return DeviceClosure.smagorinsky(ch, co, cs2)
+
+    runs = {}
+    for tag, fn, dev in (("no_closure", None, None), ("smagorinsky", None, smagorinsky),
+                         ("learned", None, learned)):
+        runs[tag] = run_coarse(c, co, ic, judge_t[0], judge, fn, args.cfl, tag, dev)
+    dns_cost = dns_wall / dns_span
+    for r in runs.values():
+        r["speedup_vs_dns"] = dns_cost / r["s_per_unit_time"] if r["s_per_unit_time"] else None
+    res["aposteriori"] = {
+        "coarse": f"{c.nx}x{c.ny + 2 * c.d}x{c.nz} (Ngal {c.px}), posts {c.dstx}x{c.dstz} of "
+                  f"{c.px // c.ntilex}x{c.pz // c.ntilez} points",
+        "dns": {"steps": dns_steps, "wall_s": dns_wall, "sim_time": dns_span, "s_per_unit_time": dns_cost},
+        "runs": runs,
+    }
     print(json.dumps(res), flush=True)
 
 

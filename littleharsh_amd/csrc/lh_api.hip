@@ -26,6 +26,7 @@
 #include "lh_nl.hpp"
 #include "lh_col.hpp"
 #include "lh_stats.hpp"
+#include "lh_internal.hpp"
 
 using namespace lh;
 
@@ -88,6 +89,9 @@ struct lh_handle {
   double2 *nu1 = nullptr, *nu2 = nullptr, *nu3 = nullptr;     // A
   double2 *nb1 = nullptr, *nb2 = nullptr, *nb3 = nullptr;     // B
   int nu_last = 0;                                             // buffer written last (0: A)
+  // optional forcing added to Nu_new in every substep (lh_set_forcing; allocated on first use)
+  double2 *fo1 = nullptr, *fo2 = nullptr, *fo3 = nullptr;
+  bool forcing = false;
   double2 *r1 = nullptr, *r2 = nullptr, *r3 = nullptr, *psi = nullptr, *tipl = nullptr;
   void *zu = nullptr, *zw = nullptr, *zv = nullptr, *zp = nullptr;
   double* dg3 = nullptr;
@@ -336,6 +340,19 @@ static void launch_zfwd(lh_handle* h, const ZJobs& jobs, int njobs, int maxn, bo
 // ---------------------------------------------------------------------------------------
 // substep
 // ---------------------------------------------------------------------------------------
+// Nu_new += forcing, field blockIdx.y (0: u, 1: w, 2: v), whole plane ranges
+__global__ void k_add_forcing(double2* n1, double2* n2, double2* n3, const double2* f1,
+                              const double2* f2, const double2* f3, size_t nu, size_t nv) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int f = blockIdx.y;
+  double2* n = f == 0 ? n1 : (f == 1 ? n3 : n2);
+  const double2* g = f == 0 ? f1 : (f == 1 ? f3 : f2);
+  if (i < (f == 2 ? nv : nu)) {
+    const double2 a = n[i], b = g[i];
+    n[i] = make_double2(a.x + b.x, a.y + b.y);
+  }
+}
+
 static void issue_substep(lh_handle* h, int krk /*0-based*/, bool cfl_mode) {
   const Dims& dm = h->dm;
   const double aR = h->rk[krk], bR = h->rk[3 + krk], gR = h->rk[6 + krk], cR = h->rk[9 + krk],
@@ -405,6 +422,13 @@ static void issue_substep(lh_handle* h, int krk /*0-based*/, bool cfl_mode) {
         LH_DISPATCH_LEN(dm.pz, launch_zasm_t, float, h, a, h->twz)
       }
     });
+  }
+  if (h->forcing) {
+    const size_t nu = (size_t)dm.nyu * dm.ncol, nv = (size_t)dm.nyv * dm.ncol;
+    TIMED(h, LH_K_MISC, 3.0 * nuvw * spl,
+          hipLaunchKernelGGL(k_add_forcing, dim3((unsigned)((nu + 255) / 256), 3), dim3(256), 0,
+                             h->st, nw[0], nw[1], nw[2], (const double2*)h->fo1,
+                             (const double2*)h->fo2, (const double2*)h->fo3, nu, nv));
   }
   RHS ra{};
   ra.u1 = h->u1; ra.u2 = h->u2; ra.u3 = h->u3; ra.p = h->p;
@@ -750,7 +774,7 @@ int lh_destroy(lh_handle* h) {
                   h->k2z, h->scal, h->u1, h->u2, h->u3, h->p, h->nu1, h->nu2, h->nu3, h->nb1, h->nb2, h->nb3, h->r1,
                   h->r2, h->r3, h->psi, h->tipl, h->zu, h->zw, h->zv, h->zp, h->dg3, h->ayu, h->ayv, h->ctwork, h->masks, h->ibu_planes,
                   h->ibv_planes, h->ibflag_u, h->ibflag_v, h->ibent, h->st_psum, h->st_pfold, h->st_acc,
-                  h->cst_acc, h->sp_acc};
+                  h->cst_acc, h->sp_acc, h->fo1, h->fo2, h->fo3};
   for (void* q : ptrs)
     if (q) hipFree(q);
   if (h->twx && h->twx != h->twx64) hipFree(h->twx);
@@ -994,6 +1018,57 @@ int lh_set_launch(lh_handle* h, int xraw_lc, int zasm_lc, int col_bs) {
   h->zasm_lc = zasm_lc;
   h->col_bs = col_bs;
   drop_graphs(h);
+  return LH_OK;
+}
+
+int lh_set_forcing(lh_handle* h, const double* f1, const double* f2, const double* f3) {
+  if (!h) return LH_ERR_ARG;
+  HIPCHK(h, hipStreamSynchronize(h->st));
+  const bool on = f1 || f2 || f3;
+  if (on != h->forcing) drop_graphs(h);   // the captured substeps gain or lose the add
+  h->forcing = on;
+  if (!on) return LH_OK;
+  const size_t nu = (size_t)h->dm.nyu * h->dm.ncol, nv = (size_t)h->dm.nyv * h->dm.ncol;
+  if (!h->fo1) {
+    HIPCHK(h, dalloc(&h->fo1, nu)); HIPCHK(h, dalloc(&h->fo3, nu)); HIPCHK(h, dalloc(&h->fo2, nv));
+  }
+  // a null field is zero; the ghost planes of every field stay zero
+  std::vector<double> zero(2 * nu, 0.0);
+  int r;
+  if ((r = xfer_nl(h, h->fo1, (double*)(f1 ? f1 : zero.data()), h->dm.nyu, true))) return r;
+  if ((r = xfer_nl(h, h->fo2, (double*)(f2 ? f2 : zero.data()), h->dm.nyv, true))) return r;
+  if ((r = xfer_nl(h, h->fo3, (double*)(f3 ? f3 : zero.data()), h->dm.nyu, true))) return r;
+  return LH_OK;
+}
+
+}  // extern "C"
+
+int lh_solver_view(lh_handle* h, LhSolverView* v) {
+  if (!h || !v) return LH_ERR_ARG;
+  HIPCHK(h, hipStreamSynchronize(h->st));
+  const size_t nu = (size_t)h->dm.nyu * h->dm.ncol, nv = (size_t)h->dm.nyv * h->dm.ncol;
+  if (!h->fo1) {
+    HIPCHK(h, dalloc(&h->fo1, nu)); HIPCHK(h, dalloc(&h->fo3, nu)); HIPCHK(h, dalloc(&h->fo2, nv));
+    HIPCHK(h, hipMemset(h->fo1, 0, nu * 16)); HIPCHK(h, hipMemset(h->fo3, 0, nu * 16));
+    HIPCHK(h, hipMemset(h->fo2, 0, nv * 16));
+  }
+  if (!h->forcing) drop_graphs(h);
+  h->forcing = true;
+  *v = LhSolverView{h->u1, h->u2, h->u3, h->fo1, h->fo2, h->fo3, h->k1x, h->k1z,
+                    h->dm.ncol, h->dm.nyu, h->dm.nyv, h->dm.nz, h->dm.mx, h->st};
+  return LH_OK;
+}
+
+extern "C" {
+
+int lh_get_forcing(lh_handle* h, double* f1, double* f2, double* f3) {
+  if (!h) return LH_ERR_ARG;
+  HIPCHK(h, hipStreamSynchronize(h->st));
+  if (!h->fo1) return fail(h, LH_ERR_ARG, "lh_get_forcing: no forcing has been set");
+  int r;
+  if (f1 && (r = xfer_nl(h, h->fo1, f1, h->dm.nyu, false))) return r;
+  if (f2 && (r = xfer_nl(h, h->fo2, f2, h->dm.nyv, false))) return r;
+  if (f3 && (r = xfer_nl(h, h->fo3, f3, h->dm.nyu, false))) return r;
   return LH_OK;
 }
 

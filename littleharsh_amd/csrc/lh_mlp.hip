@@ -22,6 +22,7 @@
 #include <vector>
 
 #include "../../include/littleharsh_amd.h"
+#include "lh_internal.hpp"
 
 namespace {
 
@@ -61,16 +62,11 @@ __global__ void __launch_bounds__(64) k_fwd(const uint16_t* __restrict__ X, cons
   const int l = threadIdx.x;
   const int b0 = blockIdx.x * 16, n0 = blockIdx.y * 16;
   f4 acc = {0.f, 0.f, 0.f, 0.f};
-  for (int k0 = 0; k0 < K; k0 += 16) {
-    bf4 a, b;
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      const int k = k0 + 4 * (l / 16) + q;
-      a[q] = (short)X[(size_t)(b0 + l % 16) * K + k];
-      b[q] = (short)W[(size_t)(n0 + l % 16) * K + k];
-    }
-    acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
-  }
+  // the four k of a lane are contiguous (8 bytes; every width is a multiple of 16)
+  const bf4* xa = reinterpret_cast<const bf4*>(X + (size_t)(b0 + l % 16) * K + 4 * (l / 16));
+  const bf4* wa = reinterpret_cast<const bf4*>(W + (size_t)(n0 + l % 16) * K + 4 * (l / 16));
+  for (int k0 = 0; k0 < K; k0 += 16)
+    acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(xa[k0 / 4], wa[k0 / 4], acc, 0, 0, 0);
   const int n = n0 + l % 16;
 #pragma unroll
   for (int q = 0; q < 4; ++q) {
@@ -95,13 +91,18 @@ __global__ void k_dz(const float* __restrict__ dH, const float* __restrict__ Z, 
   dZb[e] = f2bf(g);
 }
 
-// dW = dZ^T X (dZ: B x N bf16, X: B x K bf16, dW: N x K f32), summed over the B rows
-__global__ void __launch_bounds__(64) k_dw(const uint16_t* __restrict__ dZ, const uint16_t* __restrict__ X,
-                                           float* __restrict__ dW, int B, int K, int N) {
-  const int l = threadIdx.x;
+// dW = dZ^T X (dZ: B x N bf16, X: B x K bf16, dW: N x K f32), summed over the B rows.
+// One block of kDwWaves waves per 16x16 tile: wave w takes the row tiles w, w + kDwWaves, ...;
+// the partial tiles are summed in LDS in wave order (deterministic).
+constexpr int kDwWaves = 16;
+__global__ void __launch_bounds__(64 * kDwWaves) k_dw(const uint16_t* __restrict__ dZ,
+                                                      const uint16_t* __restrict__ X,
+                                                      float* __restrict__ dW, int B, int K, int N) {
+  __shared__ float part[kDwWaves][256];
+  const int l = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int n0 = blockIdx.x * 16, k0 = blockIdx.y * 16;
   f4 acc = {0.f, 0.f, 0.f, 0.f};
-  for (int r0 = 0; r0 < B; r0 += 16) {
+  for (int r0 = 16 * w; r0 < B; r0 += 16 * kDwWaves) {
     bf4 a, b;
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
@@ -112,16 +113,29 @@ __global__ void __launch_bounds__(64) k_dw(const uint16_t* __restrict__ dZ, cons
     acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
   }
 #pragma unroll
-  for (int q = 0; q < 4; ++q) dW[(size_t)(n0 + 4 * (l / 16) + q) * K + k0 + l % 16] = acc[q];
+  for (int q = 0; q < 4; ++q) part[w][(4 * (l / 16) + q) * 16 + l % 16] = acc[q];
+  __syncthreads();
+  if (threadIdx.x < 256) {
+    float s = 0.f;
+    for (int v = 0; v < kDwWaves; ++v) s += part[v][threadIdx.x];
+    dW[(size_t)(n0 + threadIdx.x / 16) * K + k0 + threadIdx.x % 16] = s;
+  }
 }
 
-// db = sum over the B rows of dZ (f32), one thread per output
-__global__ void k_db(const float* __restrict__ dZ, float* __restrict__ db, int B, int N) {
-  const int n = blockIdx.x * blockDim.x + threadIdx.x;
-  if (n >= N) return;
+// db = sum over the B rows of dZ (f32): one 256-thread block per output, fixed-order tree
+__global__ void __launch_bounds__(256) k_db(const float* __restrict__ dZ, float* __restrict__ db, int B,
+                                            int N) {
+  __shared__ float part[256];
+  const int n = blockIdx.x;
   float s = 0.f;
-  for (int r = 0; r < B; ++r) s += dZ[(size_t)r * N + n];
-  db[n] = s;
+  for (int r = threadIdx.x; r < B; r += 256) s += dZ[(size_t)r * N + n];
+  part[threadIdx.x] = s;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) part[threadIdx.x] += part[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) db[n] = part[0];
 }
 
 // dX = dZ W (dZ: B x N bf16, W: N x K bf16, dX: B x K f32)
@@ -130,12 +144,13 @@ __global__ void __launch_bounds__(64) k_dx(const uint16_t* __restrict__ dZ, cons
   const int l = threadIdx.x;
   const int b0 = blockIdx.x * 16, k0 = blockIdx.y * 16;
   f4 acc = {0.f, 0.f, 0.f, 0.f};
+  const bf4* za = reinterpret_cast<const bf4*>(dZ + (size_t)(b0 + l % 16) * N + 4 * (l / 16));
   for (int n0 = 0; n0 < N; n0 += 16) {
-    bf4 a, b;
+    const bf4 a = za[n0 / 4];                            // A[i = row][k = neuron], contiguous
+    bf4 b;
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
       const int n = n0 + 4 * (l / 16) + q;
-      a[q] = (short)dZ[(size_t)(b0 + l % 16) * N + n];   // A[i = row][k = neuron]
       b[q] = (short)W[(size_t)n * K + k0 + l % 16];      // B[k = neuron][j = feature]
     }
     acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
@@ -145,14 +160,14 @@ __global__ void __launch_bounds__(64) k_dx(const uint16_t* __restrict__ dZ, cons
 }
 
 // mean squared error over the `nout` real outputs of B rows; dY = 2 (Y - T) / (B nout);
-// one block of 256 threads, fixed-order reduction
-__global__ void __launch_bounds__(256) k_mse(const float* __restrict__ Y, const float* __restrict__ T,
+// one block of 1024 threads, fixed-order reduction
+__global__ void __launch_bounds__(1024) k_mse(const float* __restrict__ Y, const float* __restrict__ T,
                                              float* __restrict__ dY, float* __restrict__ loss, int B, int NP,
                                              int nout) {
-  __shared__ float part[256];
+  __shared__ float part[1024];
   const float sc = 2.f / (float)(B * nout);
   float s = 0.f;
-  for (int e = threadIdx.x; e < B * NP; e += 256) {
+  for (int e = threadIdx.x; e < B * NP; e += 1024) {
     const int r = e / NP, n = e % NP;
     float d = 0.f;
     if (n < nout) {
@@ -163,7 +178,7 @@ __global__ void __launch_bounds__(256) k_mse(const float* __restrict__ Y, const 
   }
   part[threadIdx.x] = s;
   __syncthreads();
-  for (int w = 128; w > 0; w >>= 1) {
+  for (int w = 512; w > 0; w >>= 1) {
     if ((int)threadIdx.x < w) part[threadIdx.x] += part[threadIdx.x + w];
     __syncthreads();
   }
@@ -173,10 +188,11 @@ __global__ void __launch_bounds__(256) k_mse(const float* __restrict__ Y, const 
 // batch of B rows drawn with replacement from the data set (x: n x nin f32, y: n x nout f32)
 // -> X (B x K0 bf16, zero pad) and T (B x nout f32)
 __global__ void k_gather(const float* __restrict__ x, const float* __restrict__ y, long long n, int nin, int nout,
-                         int K0, int B, uint64_t seed, uint64_t step, uint16_t* __restrict__ X,
+                         int K0, int B, uint64_t seed, const int* __restrict__ tstep, uint16_t* __restrict__ X,
                          float* __restrict__ T) {
   const int r = blockIdx.x;
   if (r >= B) return;
+  const uint64_t step = (uint64_t)tstep[0];
   const long long src = (long long)(mix64(seed ^ mix64(step * 0x100000001b3ull + (uint64_t)r)) % (uint64_t)n);
   for (int f = threadIdx.x; f < K0; f += blockDim.x)
     X[(size_t)r * K0 + f] = f < nin ? f2bf(x[(size_t)src * nin + f]) : (uint16_t)0;
@@ -190,12 +206,56 @@ __global__ void k_pack_x(const float* __restrict__ x, int nrow, int nin, int K0,
     X[(size_t)r * K0 + f] = (r < nrow && f < nin) ? f2bf(x[(size_t)r * nin + f]) : (uint16_t)0;
 }
 
-// Adam on the FP32 master parameters; refreshes the bf16 copy of the weights
+// inputs of grid points r0 .. r0 + nrow - 1 (point q = (j nz + k) nx + i): the nch fields g on
+// the 3x3x3 neighbourhood, feature ((dj + 1) 9 + (dk + 1) 3 + (di + 1)) nch + c, j clamped to
+// 0..ny-1, k and i periodic; then the extra field ex at the point itself when nin = 27 nch + 1.
+// Each input is standardised, (v - mean[f]) / scale[f], and rounded to bf16 (zero pad).
+__global__ void k_pack_grid(const float* __restrict__ g, const float* __restrict__ ex,
+                            const float* __restrict__ mean, const float* __restrict__ scale, int nch, int ny,
+                            int nz, int nx, long long r0, int nrow, int nin, int K0, uint16_t* __restrict__ X) {
+  const int r = blockIdx.x;
+  const long long q = r0 + r;
+  const bool live = r < nrow;
+  int j = 0, k = 0, i = 0;
+  if (live) {
+    i = (int)(q % nx);
+    k = (int)((q / nx) % nz);
+    j = (int)(q / ((long long)nx * nz));
+  }
+  const size_t plane = (size_t)nz * nx, field = plane * ny;
+  for (int f = threadIdx.x; f < K0; f += blockDim.x) {
+    uint16_t out = 0;
+    if (live && f < nin) {
+      float v;
+      if (f < 27 * nch) {
+        const int c = f % nch, s = f / nch;
+        const int dj = s / 9 - 1, dk = (s / 3) % 3 - 1, di = s % 3 - 1;
+        const int jj = min(max(j + dj, 0), ny - 1), kk = (k + dk + nz) % nz, ii = (i + di + nx) % nx;
+        v = g[c * field + (size_t)jj * plane + (size_t)kk * nx + ii];
+      } else {
+        v = ex[(size_t)j * plane + (size_t)k * nx + i];
+      }
+      out = f2bf((v - mean[f]) / scale[f]);
+    }
+    X[(size_t)r * K0 + f] = out;
+  }
+}
+
+// real outputs of nrow rows (Z: B x NP f32) -> y (nrow x nout)
+__global__ void k_unpack_out(const float* __restrict__ Z, int NP, int nout, int nrow, float* __restrict__ y) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e < nrow * nout) y[e] = Z[(size_t)(e / nout) * NP + e % nout];
+}
+
+// Adam on the FP32 master parameters (step t = tstep + 1, bias corrections 1 - beta^t);
+// refreshes the bf16 copy of the weights
 __global__ void k_adam(float* __restrict__ p, const float* __restrict__ g, float* __restrict__ m,
                        float* __restrict__ v, uint16_t* __restrict__ wb, long long n, float lr, float b1, float b2,
-                       float eps, float c1, float c2) {
+                       float eps, const int* __restrict__ tstep) {
   const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= n) return;
+  const float t = (float)(tstep[0] + 1);
+  const float c1 = 1.f - powf(b1, t), c2 = 1.f - powf(b2, t);
   const float ge = g[e];
   const float me = b1 * m[e] + (1.f - b1) * ge;
   const float ve = b2 * v[e] + (1.f - b2) * ge * ge;
@@ -204,6 +264,11 @@ __global__ void k_adam(float* __restrict__ p, const float* __restrict__ g, float
   const float pe = p[e] - lr * (me / c1) / (sqrtf(ve / c2) + eps);
   p[e] = pe;
   if (wb) wb[e] = f2bf(pe);
+}
+
+// the step counter advances once the step's kernels are done (one thread, vector store)
+__global__ void k_tick(int* __restrict__ tstep) {
+  if (threadIdx.x == 0) tstep[0] = tstep[0] + 1;
 }
 
 __global__ void k_to_bf(const float* __restrict__ p, uint16_t* __restrict__ wb, long long n) {
@@ -228,7 +293,15 @@ struct lh_mlp {
   uint16_t* dZb = nullptr;
   float *dx = nullptr, *dy = nullptr;          // data set on the device
   long long ndata = 0;
-  int t = 0;                                   // Adam step count
+  int t = 0;                                   // Adam step count (host copy)
+  int* tdev = nullptr;                         // Adam step count on the device
+  hipGraphExec_t gstep = nullptr;              // one captured training step
+  // inference workspace of Be rows (allocated on first use by the grid evaluation)
+  int Be = 0;
+  std::vector<uint16_t*> Xe;
+  std::vector<float*> Ze;
+  float g_lr = 0.f;
+  unsigned long long g_seed = 0;
   hipStream_t st = nullptr;
   std::string err;
 };
@@ -250,15 +323,18 @@ static hipError_t zalloc(T** p, size_t n) {
   return e;
 }
 
-// forward pass of the batch in X[0] (B rows); returns the output pre-activations in Z[L-1]
-static void mlp_forward(lh_mlp* h) {
+// forward pass of `rows` rows (a multiple of 16) in X[0]; the output pre-activations land in
+// Z[L-1]
+static void forward_rows(lh_mlp* h, int rows, uint16_t* const* X, float* const* Z, hipStream_t st) {
   for (int l = 0; l < h->L; ++l) {
     const bool last = l == h->L - 1;
-    hipLaunchKernelGGL(k_fwd, dim3(h->B / 16, h->Np[l] / 16), dim3(64), 0, h->st, h->X[l],
-                       h->wb + h->offW[l], h->p + h->offB[l], h->Z[l], last ? nullptr : h->X[l + 1], h->Kp[l],
-                       h->Np[l], last ? 0 : 1);
+    hipLaunchKernelGGL(k_fwd, dim3(rows / 16, h->Np[l] / 16), dim3(64), 0, st, X[l], h->wb + h->offW[l],
+                       h->p + h->offB[l], Z[l], last ? nullptr : X[l + 1], h->Kp[l], h->Np[l], last ? 0 : 1);
   }
 }
+
+// forward pass of the batch in X[0] (B rows)
+static void mlp_forward(lh_mlp* h) { forward_rows(h, h->B, h->X.data(), h->Z.data(), h->st); }
 
 // backward pass from dY (in dH, B x Np[L-1]); fills g
 static void mlp_backward(lh_mlp* h) {
@@ -266,10 +342,9 @@ static void mlp_backward(lh_mlp* h) {
     const int n = h->B * h->Np[l];
     hipLaunchKernelGGL(k_dz, dim3((n + 255) / 256), dim3(256), 0, h->st, h->dH, h->Z[l], h->dZ, h->dZb, n,
                        l == h->L - 1 ? 0 : 1);
-    hipLaunchKernelGGL(k_dw, dim3(h->Np[l] / 16, h->Kp[l] / 16), dim3(64), 0, h->st, h->dZb, h->X[l],
-                       h->g + h->offW[l], h->B, h->Kp[l], h->Np[l]);
-    hipLaunchKernelGGL(k_db, dim3((h->Np[l] + 63) / 64), dim3(64), 0, h->st, h->dZ, h->g + h->offB[l], h->B,
-                       h->Np[l]);
+    hipLaunchKernelGGL(k_dw, dim3(h->Np[l] / 16, h->Kp[l] / 16), dim3(64 * kDwWaves), 0, h->st, h->dZb,
+                       h->X[l], h->g + h->offW[l], h->B, h->Kp[l], h->Np[l]);
+    hipLaunchKernelGGL(k_db, dim3(h->Np[l]), dim3(256), 0, h->st, h->dZ, h->g + h->offB[l], h->B, h->Np[l]);
     if (l > 0)
       hipLaunchKernelGGL(k_dx, dim3(h->B / 16, h->Kp[l] / 16), dim3(64), 0, h->st, h->dZb,
                          h->wb + h->offW[l], h->dH, h->Kp[l], h->Np[l]);
@@ -312,6 +387,7 @@ int lh_mlp_create(int nin, int nhid, int nlayers, int nout, int batch, unsigned 
   MCHK(h, zalloc(&h->dH, (size_t)batch * wmax)); MCHK(h, zalloc(&h->dZ, (size_t)batch * wmax));
   MCHK(h, zalloc(&h->dZb, (size_t)batch * wmax)); MCHK(h, zalloc(&h->T, (size_t)batch * nout));
   MCHK(h, zalloc(&h->loss, 16));
+  MCHK(h, zalloc(&h->tdev, 16));
   // Glorot-uniform weights from a counter-based generator, zero biases
   std::vector<float> hp(h->np, 0.f);
   for (int l = 0; l < h->L; ++l) {
@@ -332,11 +408,14 @@ int lh_mlp_create(int nin, int nhid, int nlayers, int nout, int batch, unsigned 
 int lh_mlp_destroy(lh_mlp* h) {
   if (!h) return LH_OK;
   if (h->st) (void)hipStreamSynchronize(h->st);
-  void* ptrs[] = {h->p, h->g, h->m, h->v, h->wb, h->dH, h->dZ, h->dZb, h->T, h->loss, h->dx, h->dy};
+  if (h->gstep) (void)hipGraphExecDestroy(h->gstep);
+  void* ptrs[] = {h->p, h->g, h->m, h->v, h->wb, h->dH, h->dZ, h->dZb, h->T, h->loss, h->dx, h->dy, h->tdev};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
   for (auto q : h->X) if (q) (void)hipFree(q);
   for (auto q : h->Z) if (q) (void)hipFree(q);
+  for (auto q : h->Xe) if (q) (void)hipFree(q);
+  for (auto q : h->Ze) if (q) (void)hipFree(q);
   if (h->st) (void)hipStreamDestroy(h->st);
   delete h;
   return LH_OK;
@@ -383,6 +462,7 @@ int lh_mlp_set_params(lh_mlp* h, const float* params) {
   MCHK(h, hipMemcpy(h->p, hp.data(), h->np * sizeof(float), hipMemcpyHostToDevice));
   MCHK(h, hipMemset(h->m, 0, h->np * sizeof(float)));
   MCHK(h, hipMemset(h->v, 0, h->np * sizeof(float)));
+  MCHK(h, hipMemset(h->tdev, 0, sizeof(int)));
   h->t = 0;
   hipLaunchKernelGGL(k_to_bf, dim3((unsigned)((h->np + 255) / 256)), dim3(256), 0, h->st, h->p, h->wb, h->np);
   MCHK(h, hipStreamSynchronize(h->st));
@@ -394,6 +474,7 @@ int lh_mlp_set_data(lh_mlp* h, const float* x, const float* y, long long n) {
   if (h->dx) (void)hipFree(h->dx);
   if (h->dy) (void)hipFree(h->dy);
   h->dx = h->dy = nullptr;
+  if (h->gstep) { (void)hipGraphExecDestroy(h->gstep); h->gstep = nullptr; }   // it holds dx, dy
   MCHK(h, hipMalloc((void**)&h->dx, (size_t)n * h->nin * sizeof(float)));
   MCHK(h, hipMalloc((void**)&h->dy, (size_t)n * h->nout * sizeof(float)));
   MCHK(h, hipMemcpy(h->dx, x, (size_t)n * h->nin * sizeof(float), hipMemcpyHostToDevice));
@@ -411,7 +492,7 @@ int lh_mlp_loss_grad(lh_mlp* h, const float* x, const float* y, float* loss, flo
   MCHK(h, hipMemcpy(h->T, y, (size_t)h->B * h->nout * sizeof(float), hipMemcpyHostToDevice));
   hipLaunchKernelGGL(k_pack_x, dim3(h->B), dim3(64), 0, h->st, dx, h->B, h->nin, h->Kp[0], h->B, h->X[0]);
   mlp_forward(h);
-  hipLaunchKernelGGL(k_mse, dim3(1), dim3(256), 0, h->st, h->Z[h->L - 1], h->T, h->dH, h->loss, h->B,
+  hipLaunchKernelGGL(k_mse, dim3(1), dim3(1024), 0, h->st, h->Z[h->L - 1], h->T, h->dH, h->loss, h->B,
                      h->Np[h->L - 1], h->nout);
   mlp_backward(h);
   MCHK(h, hipStreamSynchronize(h->st));
@@ -425,23 +506,39 @@ int lh_mlp_loss_grad(lh_mlp* h, const float* x, const float* y, float* loss, flo
   return LH_OK;
 }
 
+// one Adam step on a batch drawn from the data set; the step count lives on the device, so
+// the same launches (captured once as a graph) serve every step
+static void issue_train_step(lh_mlp* h, float lr, unsigned long long seed) {
+  const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+  hipLaunchKernelGGL(k_gather, dim3(h->B), dim3(64), 0, h->st, h->dx, h->dy, h->ndata, h->nin, h->nout,
+                     h->Kp[0], h->B, (uint64_t)seed, (const int*)h->tdev, h->X[0], h->T);
+  mlp_forward(h);
+  hipLaunchKernelGGL(k_mse, dim3(1), dim3(1024), 0, h->st, h->Z[h->L - 1], h->T, h->dH, h->loss, h->B,
+                     h->Np[h->L - 1], h->nout);
+  mlp_backward(h);
+  hipLaunchKernelGGL(k_adam, dim3((unsigned)((h->np + 255) / 256)), dim3(256), 0, h->st, h->p, h->g, h->m,
+                     h->v, h->wb, h->np, lr, b1, b2, eps, (const int*)h->tdev);
+  hipLaunchKernelGGL(k_tick, dim3(1), dim3(64), 0, h->st, h->tdev);
+}
+
 // `steps` Adam steps on batches drawn from the data set; loss_last = loss of the last batch
 int lh_mlp_train(lh_mlp* h, int steps, float lr, unsigned long long seed, float* loss_last) {
   if (!h || steps < 0) return mfail(h, LH_ERR_ARG, "lh_mlp_train: bad argument");
   if (!h->dx) return mfail(h, LH_ERR_ARG, "lh_mlp_train: no data set (lh_mlp_set_data)");
-  const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
-  for (int s = 0; s < steps; ++s) {
-    hipLaunchKernelGGL(k_gather, dim3(h->B), dim3(64), 0, h->st, h->dx, h->dy, h->ndata, h->nin, h->nout,
-                       h->Kp[0], h->B, (uint64_t)seed, (uint64_t)(h->t + s), h->X[0], h->T);
-    mlp_forward(h);
-    hipLaunchKernelGGL(k_mse, dim3(1), dim3(256), 0, h->st, h->Z[h->L - 1], h->T, h->dH, h->loss, h->B,
-                       h->Np[h->L - 1], h->nout);
-    mlp_backward(h);
-    const int t = h->t + s + 1;
-    const float c1 = 1.f - std::pow(b1, (float)t), c2 = 1.f - std::pow(b2, (float)t);
-    hipLaunchKernelGGL(k_adam, dim3((unsigned)((h->np + 255) / 256)), dim3(256), 0, h->st, h->p, h->g, h->m,
-                       h->v, h->wb, h->np, lr, b1, b2, eps, c1, c2);
+  MCHK(h, hipStreamSynchronize(h->st));
+  if (steps > 0 && (!h->gstep || h->g_lr != lr || h->g_seed != seed)) {
+    if (h->gstep) { (void)hipGraphExecDestroy(h->gstep); h->gstep = nullptr; }
+    hipGraph_t g;
+    MCHK(h, hipStreamBeginCapture(h->st, hipStreamCaptureModeThreadLocal));
+    issue_train_step(h, lr, seed);
+    MCHK(h, hipStreamEndCapture(h->st, &g));
+    const hipError_t e = hipGraphInstantiate(&h->gstep, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    MCHK(h, e);
+    h->g_lr = lr;
+    h->g_seed = seed;
   }
+  for (int s = 0; s < steps; ++s) MCHK(h, hipGraphLaunch(h->gstep, h->st));
   h->t += steps;
   MCHK(h, hipGetLastError());
   MCHK(h, hipStreamSynchronize(h->st));
@@ -469,4 +566,83 @@ int lh_mlp_predict(lh_mlp* h, const float* x, long long n, float* y) {
   return LH_OK;
 }
 
+// outputs at every point of an ny x nz x nx grid from stencil inputs gathered on the device
+// (k_pack_grid): g holds nch fields (nch x ny x nz x nx), ex the extra field (ny x nz x nx) or
+// null, mean / scale the nin standardisation constants; y receives ny nz nx rows of nout
+int lh_mlp_predict_grid(lh_mlp* h, const float* g, int nch, const float* ex, int ny, int nz, int nx,
+                        const float* mean, const float* scale, float* y) {
+  if (!h || !g || !mean || !scale || !y || nch < 1 || ny < 1 || nz < 1 || nx < 1)
+    return mfail(h, LH_ERR_ARG, "lh_mlp_predict_grid: bad argument");
+  if (h->nin != 27 * nch + (ex ? 1 : 0))
+    return mfail(h, LH_ERR_ARG, "lh_mlp_predict_grid: nin must be 27 nch (+1 with the extra field)");
+  const long long n = (long long)ny * nz * nx;
+  const size_t field = (size_t)n;
+  float *dg = nullptr, *dex = nullptr, *dms = nullptr, *dy = nullptr;
+  auto cleanup = [&]() {
+    for (float* q : {dg, dex, dms, dy})
+      if (q) (void)hipFree(q);
+  };
+  hipError_t e = hipMalloc((void**)&dg, field * nch * sizeof(float));
+  if (e == hipSuccess && ex) e = hipMalloc((void**)&dex, field * sizeof(float));
+  if (e == hipSuccess) e = hipMalloc((void**)&dms, 2 * (size_t)h->nin * sizeof(float));
+  if (e == hipSuccess) e = hipMalloc((void**)&dy, (size_t)n * h->nout * sizeof(float));
+  if (e == hipSuccess) e = hipMemcpy(dg, g, field * nch * sizeof(float), hipMemcpyHostToDevice);
+  if (e == hipSuccess && ex) e = hipMemcpy(dex, ex, field * sizeof(float), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(dms, mean, h->nin * sizeof(float), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(dms + h->nin, scale, h->nin * sizeof(float), hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    cleanup();
+    return mfail(h, LH_ERR_HIP, std::string("lh_mlp_predict_grid: ") + hipGetErrorString(e));
+  }
+  int r = lh_mlp_eval_grid_dev(h, dg, nch, dex, ny, nz, nx, dms, dms + h->nin, dy, h->st);
+  if (r == LH_OK) {
+    e = hipStreamSynchronize(h->st);
+    if (e == hipSuccess) e = hipMemcpy(y, dy, (size_t)n * h->nout * sizeof(float), hipMemcpyDeviceToHost);
+    if (e != hipSuccess) r = mfail(h, LH_ERR_HIP, std::string("lh_mlp_predict_grid: ") + hipGetErrorString(e));
+  }
+  cleanup();
+  return r;
+}
+
 }  // extern "C"
+
+// ---- views for the closure coupling (lh_internal.hpp) ------------------------------------
+constexpr int kEvalRows = 65536;   // rows per forward pass of the grid evaluation
+
+int lh_mlp_dims(const lh_mlp* h, int* nin, int* nout) {
+  if (!h) return LH_ERR_ARG;
+  if (nin) *nin = h->nin;
+  if (nout) *nout = h->nout;
+  return LH_OK;
+}
+
+int lh_mlp_eval_grid_dev(lh_mlp* h, const float* g, int nch, const float* ex, int ny, int nz, int nx,
+                         const float* mean, const float* scale, float* y, hipStream_t st) {
+  if (!h || !g || !mean || !scale || !y || nch < 1 || ny < 1 || nz < 1 || nx < 1)
+    return mfail(h, LH_ERR_ARG, "lh_mlp_eval_grid: bad argument");
+  if (h->nin != 27 * nch + (ex ? 1 : 0))
+    return mfail(h, LH_ERR_ARG, "lh_mlp_eval_grid: nin must be 27 nch (+1 with the extra field)");
+  if (!h->Be) {
+    for (int l = 0; l < h->L; ++l) {
+      uint16_t* x = nullptr;
+      float* z = nullptr;
+      MCHK(h, zalloc(&x, (size_t)kEvalRows * h->Kp[l]));
+      MCHK(h, zalloc(&z, (size_t)kEvalRows * h->Np[l]));
+      h->Xe.push_back(x);
+      h->Ze.push_back(z);
+    }
+    h->Be = kEvalRows;
+  }
+  const long long n = (long long)ny * nz * nx;
+  for (long long r0 = 0; r0 < n; r0 += h->Be) {
+    const int nr = (int)std::min<long long>(h->Be, n - r0);
+    const int rows = (nr + 15) & ~15;
+    hipLaunchKernelGGL(k_pack_grid, dim3(rows), dim3(64), 0, st, g, ex, mean, scale, nch, ny, nz, nx, r0, nr,
+                       h->nin, h->Kp[0], h->Xe[0]);
+    forward_rows(h, rows, h->Xe.data(), h->Ze.data(), st);
+    hipLaunchKernelGGL(k_unpack_out, dim3((nr * h->nout + 255) / 256), dim3(256), 0, st, h->Ze[h->L - 1],
+                       h->Np[h->L - 1], h->nout, nr, y + (size_t)r0 * h->nout);
+  }
+  MCHK(h, hipGetLastError());
+  return LH_OK;
+}

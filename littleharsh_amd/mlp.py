@@ -87,3 +87,18 @@ class MLP:
         y = np.empty((x.shape[0], self.nout), dtype=np.float32)
         self._check(self.lib.lh_mlp_predict(self.h, _p(x), x.shape[0], _p(y)), "lh_mlp_predict")
         return y
+
+    def predict_grid(self, g, extra, mean, scale) -> np.ndarray:
+        """Outputs (ny, nz, nx, nout) from the nch fields g (nch, ny, nz, nx) gathered on the
+        3x3x3 neighbourhood on the device, plus the field `extra` (ny, nz, nx) or None."""
+        g = _f32(g)
+        nch, ny, nz, nx = g.shape
+        ex = None if extra is None else _f32(extra)
+        if ex is not None:
+            assert ex.shape == (ny, nz, nx)
+        mean, scale = _f32(mean), _f32(scale)
+        assert mean.shape == scale.shape == (self.nin,)
+        y = np.empty((ny, nz, nx, self.nout), dtype=np.float32)
+        self._check(self.lib.lh_mlp_predict_grid(self.h, _p(g), nch, None if ex is None else _p(ex), ny, nz, nx,
+                                                 _p(mean), _p(scale), _p(y)), "lh_mlp_predict_grid")
+        return y

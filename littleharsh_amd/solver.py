@@ -84,6 +84,21 @@ class Channel:
         arrs = [np.ascontiguousarray(a, dtype=np.complex128) for a in (n1, n2, n3)]
         self._chk(self._lib.lh_set_nonlinear(self._h, *[_p(a) for a in arrs]), "lh_set_nonlinear")
 
+    def set_forcing(self, f1=None, f2=None, f3=None):
+        """Forcing added to Nu_new in every substep (layout of set_nonlinear); no argument
+        removes it."""
+        arrs = [None if a is None else np.ascontiguousarray(a, dtype=np.complex128) for a in (f1, f2, f3)]
+        ptrs = [None if a is None else _p(a) for a in arrs]
+        self._chk(self._lib.lh_set_forcing(self._h, *ptrs), "lh_set_forcing")
+
+    def get_forcing(self):
+        s = self.s
+        f1 = np.zeros((s.ncol, s.nyu - 2), complex)
+        f2 = np.zeros((s.ncol, s.nyv - 2), complex)
+        f3 = np.zeros((s.ncol, s.nyu - 2), complex)
+        self._chk(self._lib.lh_get_forcing(self._h, _p(f1), _p(f2), _p(f3)), "lh_get_forcing")
+        return f1, f2, f3
+
     # ---- time stepping
     def substep(self, krk: int, dt: float):
         self._chk(self._lib.lh_substep(self._h, int(krk), float(dt)), "lh_substep")

@@ -155,6 +155,7 @@ typedef struct lho {
   cplx *u1, *u2, *u3, *p, *psi, *div;
   cplx *du1, *du2, *du3, *Lu1, *Lu2, *Lu3;
   cplx *Nu1, *Nu2, *Nu3;
+  cplx *F1, *F2, *F3;          /* optional forcing added to Nu_new (NULL: none) */
   double* DG;  /* DG(3, j, column) as f_dg(3, nyp, ncol) */
   fftplan plx, plz, plnx, plnz;
   double cflmax;
@@ -327,7 +328,7 @@ void lho_destroy(lho* o) {
   free(o->mask_u); free(o->mask_v);
   free(o->u1); free(o->u2); free(o->u3); free(o->p); free(o->psi); free(o->div);
   free(o->du1); free(o->du2); free(o->du3); free(o->Lu1); free(o->Lu2); free(o->Lu3);
-  free(o->Nu1); free(o->Nu2); free(o->Nu3); free(o->DG); free(o->st); free(o->sp); free(o->cst);
+  free(o->Nu1); free(o->Nu2); free(o->Nu3); free(o->F1); free(o->F2); free(o->F3); free(o->DG); free(o->st); free(o->sp); free(o->cst);
   plan_free(&o->plx); plan_free(&o->plz); plan_free(&o->plnx); plan_free(&o->plnz);
   free(o);
 }
@@ -355,6 +356,25 @@ void lho_get_nonlinear(const lho* o, double* n1, double* n2, double* n3) {
     }
     for (int r = 0; r < o->nyv - 2; ++r)
       ((cplx*)n2)[(size_t)c * (o->nyv - 2) + r] = o->Nu2[(size_t)c * o->nyv + r + 1];
+  }
+}
+/* Forcing added to Nu_new in every substep, in the layout of lho_set_nonlinear; all three
+   NULL removes it.  A NULL field among non-NULL ones is zero. */
+void lho_set_forcing(lho* o, const double* f1, const double* f2, const double* f3) {
+  free(o->F1); free(o->F2); free(o->F3);
+  o->F1 = o->F2 = o->F3 = NULL;
+  if (!f1 && !f2 && !f3) return;
+  o->F1 = calloc((size_t)o->ncol * o->nyu, sizeof(cplx));
+  o->F3 = calloc((size_t)o->ncol * o->nyu, sizeof(cplx));
+  o->F2 = calloc((size_t)o->ncol * o->nyv, sizeof(cplx));
+  for (int c = 0; c < o->ncol; ++c) {
+    for (int r = 0; r < o->nyu - 2; ++r) {
+      if (f1) o->F1[(size_t)c * o->nyu + r + 1] = ((const cplx*)f1)[(size_t)c * (o->nyu - 2) + r];
+      if (f3) o->F3[(size_t)c * o->nyu + r + 1] = ((const cplx*)f3)[(size_t)c * (o->nyu - 2) + r];
+    }
+    if (f2)
+      for (int r = 0; r < o->nyv - 2; ++r)
+        o->F2[(size_t)c * o->nyv + r + 1] = ((const cplx*)f2)[(size_t)c * (o->nyv - 2) + r];
   }
 }
 void lho_set_nonlinear(lho* o, const double* n1, const double* n2, const double* n3) {
@@ -509,7 +529,8 @@ static void nonlinear(lho* o, int krk) {
     }
     free(work); free(sp); free(ph);
   }
-  /* assemble Nu_new, du -= gRK*Nu_new, keep Nu_new for the next substep */
+  /* assemble Nu_new (plus the optional forcing), du -= gRK*Nu_new, keep Nu_new for the next
+     substep */
   const double g = o->gRK[krk], dv = o->ddthetavi;
 #pragma omp parallel for schedule(static)
   for (int c = 0; c < ncol; ++c) {
@@ -521,6 +542,7 @@ static void nonlinear(lho* o, int krk) {
                      rmul(m, csub(FV(o, UV, c, j), FV(o, UV, c, j - 1))));
       cplx n3 = cadd(cadd(imul(kx, FU(o, UW, c, j)), imul(kz, FU(o, WW, c, j))),
                      rmul(m, csub(FV(o, VW, c, j), FV(o, VW, c, j - 1))));
+      if (o->F1) { n1 = cadd(n1, FU(o, o->F1, c, j)); n3 = cadd(n3, FU(o, o->F3, c, j)); }
       FU(o, o->Nu1, c, j) = n1;
       FU(o, o->Nu3, c, j) = n3;
       FU(o, o->du1, c, j) = csub(FU(o, o->du1, c, j), rmul(g, n1));
@@ -530,6 +552,7 @@ static void nonlinear(lho* o, int krk) {
       const double m = dv * o->dthdyv[UJ(o, j)];
       cplx n2 = cadd(cadd(imul(kx, FV(o, UV, c, j)), imul(kz, FV(o, VW, c, j))),
                      rmul(m, csub(FU(o, VV, c, j + 1), FU(o, VV, c, j))));
+      if (o->F2) n2 = cadd(n2, FV(o, o->F2, c, j));
       FV(o, o->Nu2, c, j) = n2;
       FV(o, o->du2, c, j) = csub(FV(o, o->du2, c, j), rmul(g, n2));
     }

@@ -36,6 +36,7 @@ def lib():
         L.lho_set_state.argtypes = [vp, _dp, _dp, _dp, _dp]
         L.lho_get_state.argtypes = [vp, _dp, _dp, _dp, _dp]
         L.lho_set_nonlinear.argtypes = [vp, _dp, _dp, _dp]
+        L.lho_set_forcing.argtypes = [vp, _dp, _dp, _dp]
         L.lho_get_nonlinear.argtypes = [vp, _dp, _dp, _dp]
         L.lho_get_scalars.argtypes = [vp, _dp, _dp, _dp]
         L.lho_substep.argtypes = [vp, ctypes.c_int, ctypes.c_double]
@@ -112,6 +113,13 @@ class Oracle:
     def set_nonlinear(self, n1, n2, n3):
         arrs = [np.ascontiguousarray(a, dtype=np.complex128) for a in (n1, n2, n3)]
         lib().lho_set_nonlinear(self._h, *[_p(a) for a in arrs])
+
+    def set_forcing(self, f1=None, f2=None, f3=None):
+        """Forcing added to Nu_new in every substep (layout of set_nonlinear); no argument
+        removes it."""
+        arrs = [None if a is None else np.ascontiguousarray(a, dtype=np.complex128) for a in (f1, f2, f3)]
+        self._forcing = arrs
+        lib().lho_set_forcing(self._h, *[_p(a) for a in arrs])
 
     def scalars(self):
         m, q, c = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()

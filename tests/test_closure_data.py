@@ -54,3 +54,23 @@ def test_normal_stresses_are_non_negative_and_features_are_laid_out():
     assert np.allclose(x[:, 13 * 9:14 * 9], g[:, m, kz, ix].T.astype(np.float32))
     assert np.allclose(x[:, -1], fl["sf"][m, kz, ix])
     assert (x[:, -1] >= 0).all() and (x[:, -1] <= 1).all()
+
+
+def test_coarse_transforms_match_solver_convention():
+    """Coarse-point samples of spectral columns agree with the solver's own (3/2-padded)
+    transform at the coincident points, and phys_to_modes96 inverts modes_to_coarse."""
+    from littleharsh_amd.channel import make_setup
+    from littleharsh_amd.closure import modes_to_coarse, modes_to_phys, phys_to_modes96
+    from littleharsh_amd.initial import laminar_plus_noise
+    c = make_setup(nx=32, nz=32, ny_total=41, dsty=6, re=180.0, alp=1.0, bet=2.0, mpgx=-1.0,
+                   flag_ctpress=1, posth=0.2, dyq=4.0, ntilex=4, ntilez=2, dstx=3, dstz=5)
+    u = laminar_plus_noise(c, seed=3)[0][:, 7]
+    b = u.reshape(c.nz, c.mx).copy()
+    b[c.nz // 2, :] = 0.0
+    b[:, c.mx - 1] = 0.0
+    b = b.ravel()
+    fine = modes_to_phys(c, b)                  # (48, 48) dealiased grid
+    coarse = modes_to_coarse(b, c.nz, c.mx)     # (32, 32) one point per mode
+    assert coarse.shape == (32, 32)
+    assert np.allclose(fine[::3, ::3], coarse[::2, ::2], atol=1e-12 * np.abs(fine).max())
+    assert np.allclose(phys_to_modes96(coarse, c.nz, c.mx), b, atol=1e-12 * np.abs(b).max())

@@ -69,3 +69,71 @@ def test_training_lowers_the_loss(require_gpu):
     l1, _ = net.loss_grad(xe, ye)
     print(f"held-out loss {l0:.4e} -> {l1:.4e}", flush=True)
     assert l1 < 0.3 * l0
+
+
+def _mix64(z):
+    z = z + np.uint64(0x9E3779B97F4A7C15)
+    z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+    z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+    return z ^ (z >> np.uint64(31))
+
+
+def _batch_rows(seed, step, B, n):
+    """Rows the device draws for training step `step` (SplitMix64 finaliser, counter based)."""
+    r = np.arange(B, dtype=np.uint64)
+    inner = _mix64(np.uint64(step) * np.uint64(0x100000001B3) + r)
+    return (_mix64(np.uint64(seed) ^ inner) % np.uint64(n)).astype(np.int64)
+
+
+def test_train_steps_match_oracle_adam(require_gpu):
+    """Adam steps of the captured training graph (batch sampling, step counter, bias
+    correction, re-capture on a new learning rate) against the oracle's gradients and a
+    float64 Adam."""
+    from littleharsh_amd.mlp import MLP
+    rng = np.random.default_rng(15)
+    B, n = 64, 1000
+    net = MLP(NIN, NHID, NLAY, NOUT, batch=B, seed=6)
+    p = net.get_params().astype(np.float64)
+    net.set_params(p.astype(np.float32))              # step counter and moments reset
+    x, y = _data(rng, n)
+    net.set_data(x, y)
+    shp = omlp.shapes(NIN, NHID, NLAY, NOUT)
+    m = np.zeros_like(p)
+    v = np.zeros_like(p)
+    b1, b2, eps = 0.9, 0.999, 1e-8
+    for step, lr in enumerate((1e-3, 1e-3, 5e-4)):
+        rows = _batch_rows(21, step, B, n)
+        _, g = omlp.loss_grad(p, x[rows].astype(np.float64), y[rows].astype(np.float64), shp,
+                              bf16_ops=True)
+        m = b1 * m + (1 - b1) * g
+        v = b2 * v + (1 - b2) * g * g
+        t = step + 1
+        dp = -lr * (m / (1 - b1 ** t)) / (np.sqrt(v / (1 - b2 ** t)) + eps)
+        net.train(1, lr=lr, seed=21)
+        pg = net.get_params().astype(np.float64)
+        e = np.linalg.norm((pg - p) - dp) / np.linalg.norm(dp)
+        print(f"Adam step {t}: update rel err {e:.3e}", flush=True)
+        # sign(g) decides each element's first update: elements whose gradient sits at the
+        # rounding level may flip, so the bound is on the norm
+        assert e < 0.05
+        p = p + dp
+
+
+def test_predict_grid_matches_stencil_rows(require_gpu):
+    """The device-side stencil gather gives the inputs the training rows were built from."""
+    from littleharsh_amd.closure import stencil_features
+    from littleharsh_amd.mlp import MLP
+    rng = np.random.default_rng(16)
+    net = MLP(NIN, NHID, NLAY, NOUT, batch=64, seed=8)
+    ny, nz, nx = 5, 6, 7                       # 210 points, not a multiple of the batch
+    g = rng.normal(size=(9, ny, nz, nx)).astype(np.float32)
+    sf = rng.random((ny, nz, nx)).astype(np.float32)
+    mean = (0.1 * rng.normal(size=NIN)).astype(np.float32)
+    scale = (1.0 + rng.random(NIN)).astype(np.float32)
+    m, kz, ix = (a.ravel() for a in np.meshgrid(np.arange(ny), np.arange(nz), np.arange(nx), indexing="ij"))
+    ref = net.predict((stencil_features(g, sf, m, kz, ix) - mean) / scale)
+    out = net.predict_grid(g, sf, mean, scale).reshape(-1, NOUT)
+    same = np.mean(np.all(out == ref, axis=1))
+    print(f"rows identical: {same:.3f}; max diff {np.abs(out - ref).max():.3e}", flush=True)
+    assert same > 0.95
+    assert np.abs(out - ref).max() <= 2e-2 * np.abs(ref).max()

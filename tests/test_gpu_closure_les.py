@@ -1,0 +1,81 @@
+"""The closure coupled on the GPU (lh_closure_apply) against the host path of the same
+stages (Coarse.sample -> velocity_gradients -> network -> Coarse.forcing, pinned on the CPU
+by tests/test_closure_les.py) on the same coarse state and network."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    return np.linalg.norm(a - b) / np.linalg.norm(b)
+
+
+def test_device_closure_matches_host_path(require_gpu):
+    from littleharsh_amd.channel import config_production
+    from littleharsh_amd.closure import Filter
+    from littleharsh_amd.closure_les import Coarse, DeviceClosure, coarse_setup
+    from littleharsh_amd.initial import laminar_plus_noise
+    from littleharsh_amd.mlp import MLP
+    from littleharsh_amd.solver import Channel
+    s = config_production()
+    co = Coarse(coarse_setup(s), Filter(s))
+    c = co.c
+    u, v, w, p = laminar_plus_noise(c, seed=3)
+    rng = np.random.default_rng(5)
+    grad = co.gradients(*co.sample(u, v, w))
+    gs = grad.reshape(9, -1).std(axis=1)
+    xm = np.zeros(244, np.float32)
+    xs = np.concatenate([np.tile(gs, 27), [1.0]]).astype(np.float32)
+    sf = rng.random((len(co.yc), c.nz, 2 * (c.mx - 1))).astype(np.float32)
+    ym = rng.normal(size=6).astype(np.float32)
+    ys = (0.5 + rng.random(6)).astype(np.float32)
+    net = MLP(244, 64, 4, 6, batch=256, seed=5)
+    tau = net.predict_grid(grad.astype(np.float32), sf, xm, xs) * ys + ym
+    fh = co.forcing(tau)
+    ch = Channel(c)
+    ch.set_state(u, v, w, p)
+    dc = DeviceClosure(ch, net, co, sf, xm, xs, ym, ys)
+    dc.apply()
+    fd = ch.get_forcing()
+    errs = [rel(a, b) for a, b in zip(fd, fh)]
+    print("forcing rel err f1 f2 f3:", " ".join(f"{e:.3e}" for e in errs), flush=True)
+    # FP32 transforms on the device against FP64 on the host: a few network inputs round to
+    # the neighbouring bfloat16; a layout, sign or shift error is O(1)
+    assert max(errs) < 2e-2
+    dc.apply()                                   # deterministic
+    for a, b in zip(ch.get_forcing(), fd):
+        assert np.array_equal(a, b)
+    # the forcing enters the step as the host forcing does
+    ch2 = Channel(c)
+    ch2.set_state(u, v, w, p)
+    ch2.set_forcing(*fd)
+    dt = 2e-5
+    ch.step(dt)
+    ch2.step(dt)
+    for a, b in zip(ch.get_state(), ch2.get_state()):
+        assert rel(a, b) < 1e-12
+    dc.close()
+
+
+def test_device_smagorinsky_matches_host_path(require_gpu):
+    from littleharsh_amd.channel import config_production
+    from littleharsh_amd.closure import Filter
+    from littleharsh_amd.closure_les import Coarse, DeviceClosure, coarse_setup, smagorinsky_tau
+    from littleharsh_amd.initial import laminar_plus_noise
+    from littleharsh_amd.solver import Channel
+    s = config_production()
+    filt = Filter(s)
+    co = Coarse(coarse_setup(s), filt)
+    u, v, w, p = laminar_plus_noise(co.c, seed=4)
+    cs2 = 0.01
+    grad = co.gradients(*co.sample(u, v, w))
+    fh = co.forcing(smagorinsky_tau(grad, co.yc, filt.dxc, filt.dzc, cs2))
+    ch = Channel(co.c)
+    ch.set_state(u, v, w, p)
+    dc = DeviceClosure.smagorinsky(ch, co, cs2)
+    dc.apply()
+    errs = [rel(a, b) for a, b in zip(ch.get_forcing(), fh)]
+    print("Smagorinsky forcing rel err f1 f2 f3:", " ".join(f"{e:.3e}" for e in errs), flush=True)
+    assert max(errs) < 1e-4                      # FP32 transforms against FP64, no bfloat16
+    dc.close()
