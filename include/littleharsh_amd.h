@@ -230,6 +230,9 @@ int lh_closure_create(lh_handle* h, lh_mlp* net, const double* yc, double dxc, d
 int lh_closure_create_smagorinsky(lh_handle* h, const double* yc, double dxc, double dzc, double shift_x,
                                   double shift_z, double cs2, lh_closure** out);
 int lh_closure_apply(lh_closure* c);
+/* on != 0: keep only the plane-mean column (kx = kz = 0) of the closure's forcing, the
+ * stresses' effect on the mean profile (default 0: all columns). */
+int lh_closure_set_mean_only(lh_closure* c, int on);
 int lh_closure_destroy(lh_closure* c);
 const char* lh_closure_last_error(const lh_closure* c);
 

@@ -151,6 +151,7 @@ def load_library():
                                                   ctypes.c_double, ctypes.c_double, ctypes.POINTER(vp)]
     lib.lh_closure_apply.argtypes = [vp]
     lib.lh_closure_destroy.argtypes = [vp]
+    lib.lh_closure_set_mean_only.argtypes = [vp, ctypes.c_int]
     lib.lh_closure_last_error.argtypes = [vp]
     lib.lh_closure_last_error.restype = ctypes.c_char_p
     _lib = lib
@@ -168,7 +169,7 @@ EXPORTED = [
     "lh_mlp_create", "lh_mlp_destroy", "lh_mlp_last_error", "lh_mlp_param_count",
     "lh_mlp_get_params", "lh_mlp_set_params", "lh_mlp_set_data", "lh_mlp_loss_grad",
     "lh_mlp_train", "lh_mlp_predict", "lh_mlp_predict_grid", "lh_closure_create", "lh_closure_create_smagorinsky", "lh_closure_apply",
-    "lh_closure_destroy", "lh_closure_last_error",
+    "lh_closure_set_mean_only", "lh_closure_destroy", "lh_closure_last_error",
 ]
 
 KERNEL_NAMES = ["zinv", "xpass_nl", "zfwd", "assemble", "ib_z", "ib_x", "vel", "pressure", "misc",

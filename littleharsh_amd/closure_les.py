@@ -1,4 +1,4 @@
-"""The coarse channel (LES) in which the learned closure runs, judged against the box-filtered
+"""The coarse channel (LES) in which the learned closure runs, compared with the box-filtered
 DNS.
 
 Coarse set-up (coarse_setup): 96 x 145 x 96 (Ngal 144) for the production channel, with its
@@ -154,6 +154,11 @@ class DeviceClosure:
             msg = self.lib.lh_closure_last_error(self.h).decode() if self.h else ""
             raise RuntimeError(f"{what}: status {r}: {msg}")
 
+    def set_mean_only(self, on: bool = True):
+        """Keep only the plane-mean column of the forcing (the effect on the mean profile)."""
+        self._check(self.lib.lh_closure_set_mean_only(self.h, 1 if on else 0), "lh_closure_set_mean_only")
+        return self
+
     def apply(self):
         self._check(self.lib.lh_closure_apply(self.h), "lh_closure_apply")
 
@@ -181,7 +186,7 @@ def smagorinsky_tau(grad: np.ndarray, yc: np.ndarray, dxc: float, dzc: float, cs
     return np.stack([base * S[comp[k]] for k in STRESS], axis=-1)
 
 
-# ---- what the DNS judges: plane statistics of the filtered fields
+# ---- the comparison with the DNS: plane statistics of the filtered fields
 def plane_stats(ub, vb, wb) -> dict:
     """Per plane: mean U, rms of u', v', w', mean u'v', and the streamwise spectrum of u'
     summed over the planes of each wall half (canopy side first)."""

@@ -1,15 +1,16 @@
 """Learned canopy closure end to end: GPU DNS snapshots -> filtered data set -> MLP trained on
 the GPU (BF16 matrix cores) -> a-priori scores on later, held-out snapshots -> a-posteriori
-runs of the coarse channel with the closure, with a fitted Smagorinsky closure and without
-any closure, judged against the filtered DNS.
+runs of the coarse channel with the closure (all of its forcing, and its plane-mean part
+alone), with a fitted Smagorinsky closure and without any closure, compared with the
+filtered DNS.
 
-    python -m littleharsh_amd.closure_train --spinup 1500 --snaps 12 --every 10 --steps 20000
+    python -m littleharsh_amd.closure_train --spinup 1500 --snaps 30 --every 10 --steps 20000
 
 The DNS is the 384x289x384 canopy channel of the benchmark (laminar profile plus
 |k|^-5/3 modes, CFL 0.5).  After ``spinup`` steps, ``snaps`` snapshots ``every`` steps apart
-train the network.  After ``gap`` further steps the judge window starts: its first snapshot
-is the initial state of the coarse runs, and ``judge_snaps`` more follow ``judge_every``
-steps apart.  The judge snapshots are also the a-priori test set.  The network
+train the network.  After ``gap`` further steps the check window starts: its first snapshot
+is the initial state of the coarse runs, and ``check_snaps`` more follow ``check_every``
+steps apart.  The check snapshots are also the a-priori test set.  The network
 (244 -> 64 x 4 -> 6, tanh) predicts the six subfilter stresses from standardised features.
 The a-priori baselines are no closure (tau = 0), the mean stress of the training data and
 the Smagorinsky model with its constant fitted by least squares on the training data.
@@ -30,7 +31,7 @@ from .closure import NIN, Filter, scores
 from .closure_les import Coarse, DeviceClosure, coarse_setup, compare, plane_stats
 from .initial import laminar_plus_modes
 from .mlp import MLP
-from .solver import Channel
+from .solver import Channel, LHError
 
 
 def log(msg: str) -> None:
@@ -48,18 +49,20 @@ def dataset(filt, fields, per_snap, rng):
     return np.concatenate(xs), np.concatenate(ys), np.concatenate(bs)
 
 
-def run_coarse(c, co, ic, t0, judge, closure, cfl, tag, device=None):
-    """Advance the coarse channel from ic at time t0 to every judge time, with no closure,
+def run_coarse(c, co, ic, t0, checks, closure, cfl, tag, device=None):
+    """Advance the coarse channel from ic at time t0 to every check time, with no closure,
     a host closure (closure(u, v, w) -> forcing) or a device closure (device(ch) -> an object
-    whose apply() sets the forcing on the GPU); returns the statistics errors per judge time
+    whose apply() sets the forcing on the GPU); returns the statistics errors per check time
     and the cost."""
     ch = Channel(c)
     ch.set_state(*ic)
     dev = device(ch) if device is not None else None
     t, nstep, t_cl = t0, 0, 0.0
     rows = []
+    dt_first = None
+    diverged = False
     w0 = time.perf_counter()
-    for tj, dns, ref in judge:
+    for tj, dns, ref, qdns in checks:
         while t < tj - 1e-12 * tj:
             if closure is not None:
                 a = time.perf_counter()
@@ -68,17 +71,25 @@ def run_coarse(c, co, ic, t0, judge, closure, cfl, tag, device=None):
                 t_cl += time.perf_counter() - a
             if dev is not None:
                 dev.apply()
-            dt = min(ch.cfl_dt(cfl), tj - t)
+            try:
+                dcfl = ch.cfl_dt(cfl)
+            except LHError:                      # CFL maximum not finite
+                dcfl = 0.0
+            dt_first = dt_first or dcfl
+            if not dcfl > 1e-3 * dt_first:
+                diverged = True
+                break
+            dt = min(dcfl, tj - t)
             ch.step(dt)
             t += dt
             nstep += 1
             if nstep % 25 == 0:
-                log(f"# {tag}: step {nstep} t {t:.5f}")
+                log(f"# {tag}: step {nstep} t {t:.5f} dt {dt:.3e}")
         sc = ch.scalars()
         u, v, w, _ = ch.get_state()
-        if not (np.isfinite(sc["qx"]) and np.all(np.isfinite(u))):
-            log(f"# {tag}: state not finite at t {t:.5f}")
-            rows.append({"t": t, "diverged": True})
+        if diverged or not (np.isfinite(sc["qx"]) and np.all(np.isfinite(u))):
+            log(f"# {tag}: diverged at t {t:.5f} after {nstep} steps (CFL time step collapsed)")
+            rows.append({"t": t, "steps": nstep, "diverged": True})
             break
         ub, vb, wb = co.sample(u, v, w)
         st = plane_stats(ub, vb, wb)
@@ -87,33 +98,38 @@ def run_coarse(c, co, ic, t0, judge, closure, cfl, tag, device=None):
         err["u_field"] = float(np.linalg.norm(ub - dns["ub"]) / np.linalg.norm(fl))
         err["t"] = t
         err["steps"] = nstep
-        err["flow_rate"] = sc["qx"]
+        # the flow rate is evaluated at the end of a step: none before the first
+        err["flow_rate"] = sc["qx"] if nstep else None
+        err["flow_rate_err"] = abs(sc["qx"] - qdns) / abs(qdns) if nstep else None
         rows.append(err)
+        last_u = st["U"]
         log(f"# {tag}: t {t:.5f} after {nstep} steps: U {err['U']:.3e} urms {err['urms']:.3e} "
-            f"uv {err['uv']:.3e} u-field {err['u_field']:.3e}")
+            f"uv {err['uv']:.3e} u-field {err['u_field']:.3e} flow rate {err['flow_rate_err'] or 0.0:.2e}")
     wall = time.perf_counter() - w0
+    prof = [float(x) for x in last_u] if rows and not rows[-1].get("diverged") else None
     if dev is not None:
         dev.close()
     ch.close()
     span = t - t0
-    return {"judge": rows, "steps": nstep, "wall_s": wall, "closure_s": t_cl, "sim_time": span,
-            "s_per_unit_time": wall / span if span > 0 else None}
+    return {"check": rows, "steps": nstep, "wall_s": wall, "closure_s": t_cl, "sim_time": span,
+            "s_per_unit_time": wall / span if span > 0 else None, "U_last": prof}
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--spinup", type=int, default=1500)
-    ap.add_argument("--snaps", type=int, default=12)
+    ap.add_argument("--snaps", type=int, default=30)
     ap.add_argument("--every", type=int, default=10)
-    ap.add_argument("--gap", type=int, default=100, help="steps between the training snapshots "
-                    "and the judge window")
-    ap.add_argument("--judge-snaps", type=int, default=4)
-    ap.add_argument("--judge-every", type=int, default=250)
-    ap.add_argument("--per-snap", type=int, default=150000)
+    ap.add_argument("--gap", type=int, default=50, help="steps between the training snapshots "
+                    "and the check window")
+    ap.add_argument("--check-snaps", type=int, default=5)
+    ap.add_argument("--check-every", type=int, default=250)
+    ap.add_argument("--per-snap", type=int, default=100000)
     ap.add_argument("--steps", type=int, default=20000)
     ap.add_argument("--batch", type=int, default=4096)
     ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--cfl", type=float, default=0.5)
+    ap.add_argument("--les-cfl", type=float, default=0.5, help="CFL number of the coarse runs")
     ap.add_argument("--config", choices=("production", "small"), default="production")
     ap.add_argument("--nxc", type=int, default=96)
     ap.add_argument("--nzc", type=int, default=96)
@@ -153,25 +169,26 @@ def main():
         train_t.append(tsim)
         log(f"# training snapshot {n + 1}/{args.snaps} at t {tsim:.5f}")
     tsim += advance(args.gap)
-    judge_f, judge_t = [snapshot()], [tsim]
+    check_f, check_t, check_q = [snapshot()], [tsim], [ch.scalars()["qx"]]
     dns_steps, dns_wall, dns_span = 0, 0.0, 0.0
-    for n in range(args.judge_snaps):
+    for n in range(args.check_snaps):
         a = time.perf_counter()
-        dt = advance(args.judge_every)
+        dt = advance(args.check_every)
         dns_wall += time.perf_counter() - a
-        dns_steps += args.judge_every
+        dns_steps += args.check_every
         dns_span += dt
         tsim += dt
-        judge_f.append(snapshot())
-        judge_t.append(tsim)
-        log(f"# judge snapshot {n + 1}/{args.judge_snaps} at t {tsim:.5f}")
+        check_f.append(snapshot())
+        check_t.append(tsim)
+        check_q.append(ch.scalars()["qx"])
+        log(f"# check snapshot {n + 1}/{args.check_snaps} at t {tsim:.5f}")
     t_dns = time.perf_counter() - t0
     ch.close()
 
     # ---- data, training
     rng = np.random.default_rng(2024)
     xtr, ytr, btr = dataset(filt, train_f, args.per_snap, rng)
-    xte, yte, bte = dataset(filt, judge_f, args.per_snap, rng)
+    xte, yte, bte = dataset(filt, check_f, args.per_snap, rng)
     del train_f
     log(f"# data: {len(xtr)} training and {len(xte)} test points")
     xm, xsd = xtr.mean(axis=0), xtr.std(axis=0)
@@ -200,7 +217,7 @@ def main():
         "workload": f"{s.nx}x{s.ny + 2 * s.d}x{s.nz} DNS -> {args.nxc}x{len(filt.centres)}x{args.nzc} "
                     f"box-filtered ({filt.fx}x3x{filt.fz} collocation points)",
         "snapshots": {"train": args.snaps, "every": args.every, "spinup": args.spinup, "gap": args.gap,
-                      "train_times": train_t, "judge_times": judge_t},
+                      "train_times": train_t, "check_times": check_t},
         "points": {"train": int(len(xtr)), "test": int(len(xte))},
         "network": {"inputs": NIN, "hidden": [64] * 4, "outputs": 6, "params": net.nparams,
                     "batch": args.batch, "steps": args.steps, "lr": args.lr, "last_batch_loss": loss},
@@ -219,31 +236,36 @@ def main():
     }
     del xtr, ytr, btr, xte, yte, bte
 
-    # ---- a posteriori: the coarse channel from the first judge snapshot
+    # ---- a posteriori: the coarse channel from the first check snapshot
     c = coarse_setup(s, args.nxc, args.nzc, dyq)
     co = Coarse(c, filt)
-    ic = co.initial_state(judge_f[0])
-    sf = judge_f[0]["sf"].astype(np.float32)
+    ic = co.initial_state(check_f[0])
+    sf = check_f[0]["sf"].astype(np.float32)
     # the first entry (no steps) measures how well the coarse initial state represents the DNS
-    judge = [(tj, f, plane_stats(f["ub"], f["vb"], f["wb"])) for tj, f in zip(judge_t, judge_f)]
+    checks = [(tj, f, plane_stats(f["ub"], f["vb"], f["wb"]), q) for tj, f, q in zip(check_t, check_f, check_q)]
 
     def learned(ch):
         return DeviceClosure(ch, net, co, sf, xm, xsd, ym, ysd)
+
+    def learned_mean(ch):
+        return DeviceClosure(ch, net, co, sf, xm, xsd, ym, ysd).set_mean_only()
 
     def smagorinsky(ch):
         return DeviceClosure.smagorinsky(ch, co, cs2)
 
     runs = {}
-    for tag, fn, dev in (("no_closure", None, None), ("smagorinsky", None, smagorinsky),
-                         ("learned", None, learned)):
-        runs[tag] = run_coarse(c, co, ic, judge_t[0], judge, fn, args.cfl, tag, dev)
+    for tag, dev in (("no_closure", None), ("smagorinsky", smagorinsky), ("learned", learned),
+                     ("learned_mean", learned_mean)):
+        runs[tag] = run_coarse(c, co, ic, check_t[0], checks, None, args.les_cfl, tag, dev)
     dns_cost = dns_wall / dns_span
     for r in runs.values():
         r["speedup_vs_dns"] = dns_cost / r["s_per_unit_time"] if r["s_per_unit_time"] else None
     res["aposteriori"] = {
         "coarse": f"{c.nx}x{c.ny + 2 * c.d}x{c.nz} (Ngal {c.px}), posts {c.dstx}x{c.dstz} of "
                   f"{c.px // c.ntilex}x{c.pz // c.ntilez} points",
-        "dns": {"steps": dns_steps, "wall_s": dns_wall, "sim_time": dns_span, "s_per_unit_time": dns_cost},
+        "dns": {"steps": dns_steps, "wall_s": dns_wall, "sim_time": dns_span, "s_per_unit_time": dns_cost,
+                "flow_rate": check_q, "U_last": [float(x) for x in checks[-1][2]["U"]],
+                "yc": [float(x) for x in co.yc]},
         "runs": runs,
     }
     print(json.dumps(res), flush=True)

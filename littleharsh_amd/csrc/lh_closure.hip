@@ -18,7 +18,8 @@
 //   k_cl_div     f1 = i kx T_uu + i kz T_uw + dy T_uv and f3 = i kx T_uw + i kz T_ww + dy T_vw
 //                at the centres; f2 = the x and z derivatives of T_uv, T_vw averaged to the
 //                face plus (T_vv(m+1) - T_vv(m)) / (y(m+1) - y(m)) at the interior faces;
-//                written into the channel's forcing buffers (FP64).
+//                written into the channel's forcing buffers (FP64); with mean_only, only the
+//                plane-mean column (the stresses' effect on the mean profile) is kept.
 #include <hip/hip_runtime.h>
 #include <math.h>
 
@@ -36,6 +37,7 @@ constexpr int kNGrad = 9;     // d/dx, d/dy, d/dz of u, v, w
 
 struct CL {
   int ny, nz, nx, mx, ncol;   // centre planes, coarse points (nx = 2 (mx - 1)), columns
+  int mean_only;              // 1: only the plane-mean column (kx = kz = 0) of the forcing
   const float2* shift;        // exp(i (kx dx + kz dz)) per column
   const float2* twz;          // exp(2 pi i n / nz)
   const float2* twx;          // exp(2 pi i n / nx)
@@ -212,6 +214,12 @@ __global__ void k_cl_div(LhSolverView sv, CL c, const double* __restrict__ yc, c
   const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= (long long)c.ncol * c.ny) return;
   const int e = (int)(g % c.ncol), m = (int)(g / c.ncol);
+  if (c.mean_only && e != 0) {
+    sv.f1[(size_t)(m + 1) * c.ncol + e] = make_double2(0.0, 0.0);
+    sv.f3[(size_t)(m + 1) * c.ncol + e] = make_double2(0.0, 0.0);
+    if (m + 1 < c.ny) sv.f2[(size_t)(m + 1) * c.ncol + e] = make_double2(0.0, 0.0);
+    return;
+  }
   const double kx = sv.k1x[e % c.mx], kz = sv.k1z[e / c.mx];
   auto at = [&](int q, int mm) {
     const float2 v = T[((size_t)q * c.ny + mm) * c.ncol + e];
@@ -401,6 +409,12 @@ int lh_closure_apply(lh_closure* c) {
   hipLaunchKernelGGL(k_cl_div, dim3((unsigned)(((long long)cl.ncol * cl.ny + 255) / 256)), dim3(256), 0, st,
                      c->sv, cl, c->yc, c->T);
   CCHK(c, hipGetLastError());
+  return LH_OK;
+}
+
+int lh_closure_set_mean_only(lh_closure* c, int on) {
+  if (!c) return LH_ERR_ARG;
+  c->c.mean_only = on ? 1 : 0;
   return LH_OK;
 }
 

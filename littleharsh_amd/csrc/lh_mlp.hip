@@ -241,6 +241,153 @@ __global__ void k_pack_grid(const float* __restrict__ g, const float* __restrict
   }
 }
 
+// The whole network on 16-point tiles of a grid, one wave per tile: the weights (bf16) sit
+// in LDS for the block; the first layer's operands are gathered and standardised straight
+// into the matrix-core registers (the inputs k_pack_grid would write); each hidden layer's
+// activations go through a 16 x 64 bf16 tile of the wave's own LDS.  Same operands, k order
+// and instructions as k_pack_grid + k_fwd, so the outputs are the same bits.
+constexpr int kFuseMaxL = 6;
+constexpr int kFuseWaves = 8;     // waves per block sharing the staged weights
+constexpr int kFuseMaxK0 = 256;   // first-layer width held in registers
+struct FusedNet {
+  int L, np, nin, nout;
+  int Kp[kFuseMaxL], Np[kFuseMaxL];
+  long long offW[kFuseMaxL], offB[kFuseMaxL];
+};
+__global__ void __launch_bounds__(64 * kFuseWaves) k_mlp_grid(FusedNet fn, const uint16_t* __restrict__ wb,
+                                                              const float* __restrict__ p, const float* __restrict__ g,
+                                                              const float* __restrict__ ex,
+                                                              const float* __restrict__ mean,
+                                                              const float* __restrict__ scale, int nch, int ny, int nz,
+                                                              int nx, long long n, float* __restrict__ y) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  uint16_t* Wl = reinterpret_cast<uint16_t*>(smem);
+  const int wbytes = (2 * fn.np + 15) & ~15;
+  const int K0 = fn.Kp[0];
+  float* ms = reinterpret_cast<float*>(smem + wbytes);
+  int* fdec = reinterpret_cast<int*>(ms + 2 * K0);   // per input: channel, stencil offsets (or ex / pad)
+  const int wave = threadIdx.x >> 6, l = threadIdx.x & 63;
+  uint16_t* act = reinterpret_cast<uint16_t*>(fdec + K0) + wave * 16 * 64;
+  for (int e = threadIdx.x; e < fn.np; e += blockDim.x) Wl[e] = wb[e];
+  for (int e = threadIdx.x; e < K0; e += blockDim.x) {
+    ms[e] = e < fn.nin ? mean[e] : 0.f;
+    ms[K0 + e] = e < fn.nin ? scale[e] : 1.f;
+    int d = -1;                                    // padding
+    if (e < 27 * nch) {
+      const int c = e % nch, s = e / nch;
+      d = c | ((s / 9) << 8) | (((s / 3) % 3) << 10) | ((s % 3) << 12);
+    } else if (e < fn.nin) {
+      d = -2;                                      // the extra field at the point
+    }
+    fdec[e] = d;
+  }
+  __syncthreads();
+  const size_t plane = (size_t)nz * nx, field = plane * ny;
+  const long long ntile = (n + 15) / 16;
+  for (long long tile = (long long)blockIdx.x * kFuseWaves + wave; tile < ntile;
+       tile += (long long)gridDim.x * kFuseWaves) {
+    const long long r0 = tile * 16;
+    const long long q = r0 + l % 16;               // this lane's point (rows of A)
+    const bool live = q < n;
+    int i = 0, k = 0, j = 0;
+    if (live) {
+      i = (int)(q % nx);
+      k = (int)((q / nx) % nz);
+      j = (int)(q / ((long long)nx * nz));
+    }
+    // every first-layer operand of the lane first (the loads overlap), then the products
+    float v[kFuseMaxK0 / 16][4];
+#pragma unroll
+    for (int c = 0; c < kFuseMaxK0 / 16; ++c)
+#pragma unroll
+      for (int qq = 0; qq < 4; ++qq) {
+        const int f = c * 16 + 4 * (l / 16) + qq;
+        float x = 0.f;
+        if (c * 16 < K0 && live) {
+          const int d = fdec[f];
+          if (d >= 0) {
+            int jj = j + ((d >> 8) & 3) - 1, kk = k + ((d >> 10) & 3) - 1, ii = i + ((d >> 12) & 3) - 1;
+            jj = jj < 0 ? 0 : (jj >= ny ? ny - 1 : jj);
+            kk = kk < 0 ? kk + nz : (kk >= nz ? kk - nz : kk);
+            ii = ii < 0 ? ii + nx : (ii >= nx ? ii - nx : ii);
+            x = g[(size_t)(d & 255) * field + (size_t)jj * plane + (size_t)kk * nx + ii];
+          } else if (d == -2) {
+            x = ex[(size_t)j * plane + (size_t)k * nx + i];
+          }
+        }
+        v[c][qq] = x;
+      }
+    f4 acc[4];
+    {
+      const int nnt = fn.Np[0] / 16;
+#pragma unroll
+      for (int t = 0; t < 4; ++t) acc[t] = f4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int c = 0; c < kFuseMaxK0 / 16; ++c)
+        if (c * 16 < K0) {
+          bf4 a;
+#pragma unroll
+          for (int qq = 0; qq < 4; ++qq) {
+            const int f = c * 16 + 4 * (l / 16) + qq;
+            a[qq] = (short)((live && f < fn.nin) ? f2bf((v[c][qq] - ms[f]) / ms[K0 + f]) : (uint16_t)0);
+          }
+#pragma unroll
+          for (int t = 0; t < 4; ++t)
+            if (t < nnt) {
+              const bf4 b = *reinterpret_cast<const bf4*>(Wl + fn.offW[0] + (size_t)(t * 16 + l % 16) * K0 +
+                                                          c * 16 + 4 * (l / 16));
+              acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[t], 0, 0, 0);
+            }
+        }
+    }
+    for (int layer = 0; layer < fn.L; ++layer) {
+      const int nnt = fn.Np[layer] / 16;
+      const float* bias = p + fn.offB[layer];
+      if (layer == fn.L - 1) {                     // output rows (pre-activations)
+#pragma unroll
+        for (int t = 0; t < 4; ++t)
+          if (t < nnt) {
+            const int nn = t * 16 + l % 16;
+#pragma unroll
+            for (int qq = 0; qq < 4; ++qq) {
+              const long long r = r0 + 4 * (l / 16) + qq;
+              if (r < n && nn < fn.nout) y[(size_t)r * fn.nout + nn] = acc[t][qq] + bias[nn];
+            }
+          }
+        break;
+      }
+      // activations of this layer -> the wave's tile (D layout in, row-major out)
+#pragma unroll
+      for (int t = 0; t < 4; ++t)
+        if (t < nnt) {
+          const int nn = t * 16 + l % 16;
+#pragma unroll
+          for (int qq = 0; qq < 4; ++qq)
+            act[(4 * (l / 16) + qq) * 64 + nn] = f2bf(tanhf(acc[t][qq] + bias[nn]));
+        }
+      // next layer: A from the tile (row l % 16, four contiguous k per 16-wide chunk)
+      const int K = fn.Kp[layer + 1], nnx = fn.Np[layer + 1] / 16;
+      bf4 a[4];
+#pragma unroll
+      for (int c = 0; c < 4; ++c)
+        if (c * 16 < K) a[c] = *reinterpret_cast<const bf4*>(act + (l % 16) * 64 + c * 16 + 4 * (l / 16));
+#pragma unroll
+      for (int t = 0; t < 4; ++t) acc[t] = f4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int c = 0; c < 4; ++c)
+        if (c * 16 < K) {
+#pragma unroll
+          for (int t = 0; t < 4; ++t)
+            if (t < nnx) {
+              const bf4 b = *reinterpret_cast<const bf4*>(Wl + fn.offW[layer + 1] + (size_t)(t * 16 + l % 16) * K +
+                                                          c * 16 + 4 * (l / 16));
+              acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[c], b, acc[t], 0, 0, 0);
+            }
+        }
+    }
+  }
+}
+
 // real outputs of nrow rows (Z: B x NP f32) -> y (nrow x nout)
 __global__ void k_unpack_out(const float* __restrict__ Z, int NP, int nout, int nrow, float* __restrict__ y) {
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
@@ -298,6 +445,7 @@ struct lh_mlp {
   hipGraphExec_t gstep = nullptr;              // one captured training step
   // inference workspace of Be rows (allocated on first use by the grid evaluation)
   int Be = 0;
+  int ncu = 0;                                 // compute units (fused grid evaluation)
   std::vector<uint16_t*> Xe;
   std::vector<float*> Ze;
   float g_lr = 0.f;
@@ -622,6 +770,39 @@ int lh_mlp_eval_grid_dev(lh_mlp* h, const float* g, int nch, const float* ex, in
     return mfail(h, LH_ERR_ARG, "lh_mlp_eval_grid: bad argument");
   if (h->nin != 27 * nch + (ex ? 1 : 0))
     return mfail(h, LH_ERR_ARG, "lh_mlp_eval_grid: nin must be 27 nch (+1 with the extra field)");
+  // fused path: every width <= 64, weights and tiles within the LDS of a block
+  bool fuse = h->L <= kFuseMaxL && h->Kp[0] <= kFuseMaxK0 && nch < 256;
+  for (int l = 0; l < h->L; ++l) fuse = fuse && h->Np[l] <= 64 && (l == 0 || h->Kp[l] <= 64);
+  const size_t lds = (size_t)((2 * h->np + 15) & ~15LL) + 3 * (size_t)h->Kp[0] * sizeof(float) +
+                     kFuseWaves * 16 * 64 * 2;
+  fuse = fuse && lds <= 160 * 1024;
+  if (fuse) {
+    FusedNet fn{};
+    fn.L = h->L;
+    fn.np = (int)h->np;
+    fn.nin = h->nin;
+    fn.nout = h->nout;
+    for (int l = 0; l < h->L; ++l) {
+      fn.Kp[l] = h->Kp[l];
+      fn.Np[l] = h->Np[l];
+      fn.offW[l] = h->offW[l];
+      fn.offB[l] = h->offB[l];
+    }
+    if (!h->ncu) {
+      int dev = 0;
+      MCHK(h, hipGetDevice(&dev));
+      MCHK(h, hipDeviceGetAttribute(&h->ncu, hipDeviceAttributeMultiprocessorCount, dev));
+      MCHK(h, hipFuncSetAttribute((const void*)k_mlp_grid, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  160 * 1024));
+    }
+    const long long n = (long long)ny * nz * nx;
+    const long long tiles = (n + 15) / 16;
+    const int blocks = (int)std::min<long long>((tiles + kFuseWaves - 1) / kFuseWaves, 2LL * h->ncu);
+    hipLaunchKernelGGL(k_mlp_grid, dim3(blocks), dim3(64 * kFuseWaves), lds, st, fn, h->wb, h->p, g, ex, mean, scale, nch,
+                       ny, nz, nx, n, y);
+    MCHK(h, hipGetLastError());
+    return LH_OK;
+  }
   if (!h->Be) {
     for (int l = 0; l < h->L; ++l) {
       uint16_t* x = nullptr;

@@ -60,6 +60,20 @@ module lh_amd
       import :: c_int, c_ptr
       type(c_ptr), value :: h, n1, n2, n3
     end function
+    ! forcing added to Nu_new every substep (a coarse channel's closure term)
+    integer(c_int) function lh_set_forcing(h, f1, f2, f3) bind(C, name="lh_set_forcing")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: h, f1, f2, f3
+    end function
+    integer(c_int) function lh_get_forcing(h, f1, f2, f3) bind(C, name="lh_get_forcing")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: h, f1, f2, f3
+    end function
+    ! the closure bound to a coarse channel: its forcing from the current state, on the GPU
+    integer(c_int) function lh_closure_apply(c) bind(C, name="lh_closure_apply")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: c
+    end function
     integer(c_int) function lh_substep(h, krk, dt) bind(C, name="lh_substep")
       import :: c_int, c_ptr, c_double
       type(c_ptr), value :: h

@@ -137,3 +137,22 @@ def test_predict_grid_matches_stencil_rows(require_gpu):
     print(f"rows identical: {same:.3f}; max diff {np.abs(out - ref).max():.3e}", flush=True)
     assert same > 0.95
     assert np.abs(out - ref).max() <= 2e-2 * np.abs(ref).max()
+
+
+def test_predict_grid_unfused_shape(require_gpu):
+    """A network wider than the fused kernel takes (80 hidden) runs the pack + layer-by-layer
+    path and gives the same rows."""
+    from littleharsh_amd.closure import stencil_features
+    from littleharsh_amd.mlp import MLP
+    rng = np.random.default_rng(17)
+    net = MLP(NIN, 80, 2, NOUT, batch=64, seed=9)
+    ny, nz, nx = 4, 5, 6
+    g = rng.normal(size=(9, ny, nz, nx)).astype(np.float32)
+    sf = rng.random((ny, nz, nx)).astype(np.float32)
+    mean = np.zeros(NIN, np.float32)
+    scale = np.ones(NIN, np.float32)
+    m, kz, ix = (a.ravel() for a in np.meshgrid(np.arange(ny), np.arange(nz), np.arange(nx), indexing="ij"))
+    ref = net.predict(stencil_features(g, sf, m, kz, ix))
+    out = net.predict_grid(g, sf, mean, scale).reshape(-1, NOUT)
+    assert np.mean(np.all(out == ref, axis=1)) > 0.95
+    assert np.abs(out - ref).max() <= 2e-2 * np.abs(ref).max()

@@ -46,6 +46,12 @@ def test_device_closure_matches_host_path(require_gpu):
     dc.apply()                                   # deterministic
     for a, b in zip(ch.get_forcing(), fd):
         assert np.array_equal(a, b)
+    dc.set_mean_only(True)                       # only the plane-mean column survives
+    dc.apply()
+    for a, b in zip(ch.get_forcing(), fd):
+        assert np.array_equal(a[0], b[0]) and not np.any(a[1:])
+    dc.set_mean_only(False)
+    dc.apply()
     # the forcing enters the step as the host forcing does
     ch2 = Channel(c)
     ch2.set_state(u, v, w, p)
