@@ -49,7 +49,7 @@ def dataset(filt, fields, per_snap, rng):
     return np.concatenate(xs), np.concatenate(ys), np.concatenate(bs)
 
 
-def run_coarse(c, co, ic, t0, checks, closure, cfl, tag, device=None):
+def run_coarse(c, co, ic, t0, checks, closure, cfl, tag, device=None, per_substep=False):
     """Advance the coarse channel from ic at time t0 to every check time, with no closure,
     a host closure (closure(u, v, w) -> forcing) or a device closure (device(ch) -> an object
     whose apply() sets the forcing on the GPU); returns the statistics errors per check time
@@ -69,7 +69,7 @@ def run_coarse(c, co, ic, t0, checks, closure, cfl, tag, device=None):
                 u, v, w, _ = ch.get_state()
                 ch.set_forcing(*closure(u, v, w))
                 t_cl += time.perf_counter() - a
-            if dev is not None:
+            if dev is not None and not per_substep:
                 dev.apply()
             try:
                 dcfl = ch.cfl_dt(cfl)
@@ -80,7 +80,12 @@ def run_coarse(c, co, ic, t0, checks, closure, cfl, tag, device=None):
                 diverged = True
                 break
             dt = min(dcfl, tj - t)
-            ch.step(dt)
+            if dev is not None and per_substep:      # the closure re-evaluated before each substep
+                for krk in (1, 2, 3):
+                    dev.apply()
+                    ch.substep(krk, dt)
+            else:
+                ch.step(dt)
             t += dt
             nstep += 1
             if nstep % 25 == 0:
@@ -130,6 +135,8 @@ def main():
     ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--cfl", type=float, default=0.5)
     ap.add_argument("--les-cfl", type=float, default=0.5, help="CFL number of the coarse runs")
+    ap.add_argument("--per-substep", action="store_true", help="evaluate the closures before every RK3 "
+                    "substep instead of once per step")
     ap.add_argument("--config", choices=("production", "small"), default="production")
     ap.add_argument("--nxc", type=int, default=96)
     ap.add_argument("--nzc", type=int, default=96)
@@ -256,7 +263,7 @@ def main():
     runs = {}
     for tag, dev in (("no_closure", None), ("smagorinsky", smagorinsky), ("learned", learned),
                      ("learned_mean", learned_mean)):
-        runs[tag] = run_coarse(c, co, ic, check_t[0], checks, None, args.les_cfl, tag, dev)
+        runs[tag] = run_coarse(c, co, ic, check_t[0], checks, None, args.les_cfl, tag, dev, args.per_substep)
     dns_cost = dns_wall / dns_span
     for r in runs.values():
         r["speedup_vs_dns"] = dns_cost / r["s_per_unit_time"] if r["s_per_unit_time"] else None
