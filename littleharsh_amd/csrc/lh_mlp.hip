@@ -77,30 +77,35 @@ __global__ void __launch_bounds__(64) k_fwd(const uint16_t* __restrict__ X, cons
   }
 }
 
-// dZ = dH * act'(Z) (dH, Z: B x N f32) -> dZ as f32 and bf16
-__global__ void k_dz(const float* __restrict__ dH, const float* __restrict__ Z, float* __restrict__ dZ,
-                     uint16_t* __restrict__ dZb, int n, int act) {
-  const int e = blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= n) return;
-  float g = dH[e];
-  if (act) {
-    const float t = tanhf(Z[e]);
-    g *= 1.f - t * t;
-  }
-  dZ[e] = g;
-  dZb[e] = f2bf(g);
-}
-
-// dW = dZ^T X (dZ: B x N bf16, X: B x K bf16, dW: N x K f32), summed over the B rows.
+// dW = dZ^T X (dZ: B x N bf16, X: B x K bf16, dW: N x K f32), summed over the B rows; the
+// blocks of the extra last row (blockIdx.y = K / 16) form db = the column sums of the FP32 dZf
+// instead, beside the products.
 // One block of kDwWaves waves per 16x16 tile: wave w takes the row tiles w, w + kDwWaves, ...;
 // the partial tiles are summed in LDS in wave order (deterministic).
 constexpr int kDwWaves = 16;
 __global__ void __launch_bounds__(64 * kDwWaves) k_dw(const uint16_t* __restrict__ dZ,
                                                       const uint16_t* __restrict__ X,
-                                                      float* __restrict__ dW, int B, int K, int N) {
+                                                      float* __restrict__ dW, int B, int K, int N,
+                                                      const float* __restrict__ dZf, float* __restrict__ db) {
   __shared__ float part[kDwWaves][256];
+  __shared__ float dbp[kDwWaves][64];
   const int l = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int n0 = blockIdx.x * 16, k0 = blockIdx.y * 16;
+  if (k0 >= K) {                                    // bias row: column sums of dZf
+    float sb = 0.f;
+    for (int r0 = 16 * w; r0 < B; r0 += 16 * kDwWaves)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) sb += dZf[(size_t)(r0 + 4 * (l / 16) + q) * N + n0 + l % 16];
+    dbp[w][l] = sb;
+    __syncthreads();
+    if (threadIdx.x < 16) {
+      float s = 0.f;
+      for (int v = 0; v < kDwWaves; ++v)
+        for (int gq = 0; gq < 4; ++gq) s += dbp[v][gq * 16 + threadIdx.x];
+      db[n0 + threadIdx.x] = s;
+    }
+    return;
+  }
   f4 acc = {0.f, 0.f, 0.f, 0.f};
   for (int r0 = 16 * w; r0 < B; r0 += 16 * kDwWaves) {
     bf4 a, b;
@@ -122,25 +127,11 @@ __global__ void __launch_bounds__(64 * kDwWaves) k_dw(const uint16_t* __restrict
   }
 }
 
-// db = sum over the B rows of dZ (f32): one 256-thread block per output, fixed-order tree
-__global__ void __launch_bounds__(256) k_db(const float* __restrict__ dZ, float* __restrict__ db, int B,
-                                            int N) {
-  __shared__ float part[256];
-  const int n = blockIdx.x;
-  float s = 0.f;
-  for (int r = threadIdx.x; r < B; r += 256) s += dZ[(size_t)r * N + n];
-  part[threadIdx.x] = s;
-  __syncthreads();
-  for (int w = 128; w > 0; w >>= 1) {
-    if ((int)threadIdx.x < w) part[threadIdx.x] += part[threadIdx.x + w];
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) db[n] = part[0];
-}
-
-// dX = dZ W (dZ: B x N bf16, W: N x K bf16, dX: B x K f32)
+// the previous layer's dZ = (dZ W) * (1 - tanh(Zp)^2) (dZ: B x N bf16, W: N x K bf16, Zp: its
+// pre-activations, B x K f32) -> dZo as f32 and bf16
 __global__ void __launch_bounds__(64) k_dx(const uint16_t* __restrict__ dZ, const uint16_t* __restrict__ W,
-                                           float* __restrict__ dX, int K, int N) {
+                                           const float* __restrict__ Zp, float* __restrict__ dZo,
+                                           uint16_t* __restrict__ dZbo, int K, int N) {
   const int l = threadIdx.x;
   const int b0 = blockIdx.x * 16, k0 = blockIdx.y * 16;
   f4 acc = {0.f, 0.f, 0.f, 0.f};
@@ -156,33 +147,57 @@ __global__ void __launch_bounds__(64) k_dx(const uint16_t* __restrict__ dZ, cons
     acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
   }
 #pragma unroll
-  for (int q = 0; q < 4; ++q) dX[(size_t)(b0 + 4 * (l / 16) + q) * K + k0 + l % 16] = acc[q];
+  for (int q = 0; q < 4; ++q) {
+    const size_t o = (size_t)(b0 + 4 * (l / 16) + q) * K + k0 + l % 16;
+    const float th = tanhf(Zp[o]);
+    const float gq = acc[q] * (1.f - th * th);
+    dZo[o] = gq;
+    dZbo[o] = f2bf(gq);
+  }
 }
 
-// mean squared error over the `nout` real outputs of B rows; dY = 2 (Y - T) / (B nout);
-// one block of 1024 threads, fixed-order reduction
-__global__ void __launch_bounds__(1024) k_mse(const float* __restrict__ Y, const float* __restrict__ T,
-                                             float* __restrict__ dY, float* __restrict__ loss, int B, int NP,
+// squared error of the `nout` real outputs of B rows: dZ of the output layer,
+// dY = 2 (Y - T) / (B nout), as f32 and bf16, and one partial sum of (Y - T)^2 per 256-element
+// block (fixed-order tree); with tstep, the first thread advances the step counter (every
+// block of the step's gather has finished)
+__global__ void __launch_bounds__(256) k_mse(const float* __restrict__ Y, const float* __restrict__ T,
+                                             float* __restrict__ dZ, uint16_t* __restrict__ dZb,
+                                             float* __restrict__ part, int* __restrict__ tstep, int B, int NP,
                                              int nout) {
-  __shared__ float part[1024];
+  __shared__ float red[256];
+  const int e = blockIdx.x * 256 + threadIdx.x;
   const float sc = 2.f / (float)(B * nout);
   float s = 0.f;
-  for (int e = threadIdx.x; e < B * NP; e += 1024) {
+  if (e < B * NP) {
     const int r = e / NP, n = e % NP;
     float d = 0.f;
     if (n < nout) {
       d = Y[e] - T[(size_t)r * nout + n];
-      s += d * d;
+      s = d * d;
     }
-    dY[e] = sc * d;
+    dZ[e] = sc * d;
+    dZb[e] = f2bf(sc * d);
   }
-  part[threadIdx.x] = s;
+  red[threadIdx.x] = s;
   __syncthreads();
-  for (int w = 512; w > 0; w >>= 1) {
-    if ((int)threadIdx.x < w) part[threadIdx.x] += part[threadIdx.x + w];
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
     __syncthreads();
   }
-  if (threadIdx.x == 0) loss[0] = part[0] / (float)(B * nout);
+  if (threadIdx.x == 0) {
+    part[blockIdx.x] = red[0];
+    if (tstep && blockIdx.x == 0) tstep[0] = tstep[0] + 1;
+  }
+}
+
+// the loss from the partial sums, in a fixed order (one thread)
+__device__ inline void sum_loss(const float* __restrict__ part, int nparts, float den, float* __restrict__ loss) {
+  float s = 0.f;
+  for (int i = 0; i < nparts; ++i) s += part[i];
+  loss[0] = s / den;
+}
+__global__ void k_loss(const float* __restrict__ part, int nparts, float den, float* __restrict__ loss) {
+  if (threadIdx.x == 0) sum_loss(part, nparts, den, loss);
 }
 
 // batch of B rows drawn with replacement from the data set (x: n x nin f32, y: n x nout f32)
@@ -394,14 +409,28 @@ __global__ void k_unpack_out(const float* __restrict__ Z, int NP, int nout, int 
   if (e < nrow * nout) y[e] = Z[(size_t)(e / nout) * NP + e % nout];
 }
 
-// Adam on the FP32 master parameters (step t = tstep + 1, bias corrections 1 - beta^t);
-// refreshes the bf16 copy of the weights
+// Adam on the FP32 master parameters (step t = tstep, already advanced by k_mse; bias
+// corrections 1 - beta^t); refreshes the bf16 copy of the weights; the first thread also sums
+// the step's loss
 __global__ void k_adam(float* __restrict__ p, const float* __restrict__ g, float* __restrict__ m,
                        float* __restrict__ v, uint16_t* __restrict__ wb, long long n, float lr, float b1, float b2,
-                       float eps, const int* __restrict__ tstep) {
+                       float eps, const int* __restrict__ tstep, const float* __restrict__ part, int nparts,
+                       float den, float* __restrict__ loss) {
   const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (blockIdx.x == 0) {                            // the step's loss: fixed-order tree
+    __shared__ float red[256];
+    float s = 0.f;
+    for (int i = threadIdx.x; i < nparts; i += 256) s += part[i];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+      if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) loss[0] = red[0] / den;
+  }
   if (e >= n) return;
-  const float t = (float)(tstep[0] + 1);
+  const float t = (float)tstep[0];
   const float c1 = 1.f - powf(b1, t), c2 = 1.f - powf(b2, t);
   const float ge = g[e];
   const float me = b1 * m[e] + (1.f - b1) * ge;
@@ -411,11 +440,6 @@ __global__ void k_adam(float* __restrict__ p, const float* __restrict__ g, float
   const float pe = p[e] - lr * (me / c1) / (sqrtf(ve / c2) + eps);
   p[e] = pe;
   if (wb) wb[e] = f2bf(pe);
-}
-
-// the step counter advances once the step's kernels are done (one thread, vector store)
-__global__ void k_tick(int* __restrict__ tstep) {
-  if (threadIdx.x == 0) tstep[0] = tstep[0] + 1;
 }
 
 __global__ void k_to_bf(const float* __restrict__ p, uint16_t* __restrict__ wb, long long n) {
@@ -436,13 +460,15 @@ struct lh_mlp {
   uint16_t* wb = nullptr;                      // bf16 copy of p (weights and biases)
   std::vector<uint16_t*> X;                    // layer inputs (bf16), X[0] = batch input
   std::vector<float*> Z;                       // pre-activations (f32)
-  float *dH = nullptr, *dZ = nullptr, *T = nullptr, *loss = nullptr;
-  uint16_t* dZb = nullptr;
+  float *dZ = nullptr, *dZ2 = nullptr, *T = nullptr, *loss = nullptr, *part = nullptr;
+  uint16_t *dZb = nullptr, *dZb2 = nullptr;     // dZ of layers of even / odd index (f32, bf16)
+  int nparts = 0;                               // loss partial sums per batch
   float *dx = nullptr, *dy = nullptr;          // data set on the device
   long long ndata = 0;
   int t = 0;                                   // Adam step count (host copy)
   int* tdev = nullptr;                         // Adam step count on the device
   hipGraphExec_t gstep = nullptr;              // one captured training step
+  hipGraphExec_t gmulti = nullptr;             // kTrainSteps steps in one graph
   // inference workspace of Be rows (allocated on first use by the grid evaluation)
   int Be = 0;
   int ncu = 0;                                 // compute units (fused grid evaluation)
@@ -453,6 +479,15 @@ struct lh_mlp {
   hipStream_t st = nullptr;
   std::string err;
 };
+
+// training steps per captured graph
+constexpr int kTrainSteps = 16;
+
+static void drop_train_graphs(lh_mlp* h) {
+  if (h->gstep) (void)hipGraphExecDestroy(h->gstep);
+  if (h->gmulti) (void)hipGraphExecDestroy(h->gmulti);
+  h->gstep = h->gmulti = nullptr;
+}
 
 static int mfail(lh_mlp* h, int code, const std::string& m) {
   if (h) h->err = m;
@@ -484,18 +519,25 @@ static void forward_rows(lh_mlp* h, int rows, uint16_t* const* X, float* const* 
 // forward pass of the batch in X[0] (B rows)
 static void mlp_forward(lh_mlp* h) { forward_rows(h, h->B, h->X.data(), h->Z.data(), h->st); }
 
-// backward pass from dY (in dH, B x Np[L-1]); fills g
+static float* dz_f(lh_mlp* h, int l) { return l % 2 ? h->dZ2 : h->dZ; }
+static uint16_t* dz_b(lh_mlp* h, int l) { return l % 2 ? h->dZb2 : h->dZb; }
+
+// loss terms of the batch (output rows Z[L-1] against T): dZ of the output layer and the loss
+// partial sums; with tick, the step counter advances
+static void mlp_loss(lh_mlp* h, bool tick) {
+  const int l = h->L - 1;
+  hipLaunchKernelGGL(k_mse, dim3(h->nparts), dim3(256), 0, h->st, h->Z[l], h->T, dz_f(h, l), dz_b(h, l), h->part,
+                     tick ? h->tdev : nullptr, h->B, h->Np[l], h->nout);
+}
+
+// backward pass from the output layer's dZ; fills g (weights and biases)
 static void mlp_backward(lh_mlp* h) {
   for (int l = h->L - 1; l >= 0; --l) {
-    const int n = h->B * h->Np[l];
-    hipLaunchKernelGGL(k_dz, dim3((n + 255) / 256), dim3(256), 0, h->st, h->dH, h->Z[l], h->dZ, h->dZb, n,
-                       l == h->L - 1 ? 0 : 1);
-    hipLaunchKernelGGL(k_dw, dim3(h->Np[l] / 16, h->Kp[l] / 16), dim3(64 * kDwWaves), 0, h->st, h->dZb,
-                       h->X[l], h->g + h->offW[l], h->B, h->Kp[l], h->Np[l]);
-    hipLaunchKernelGGL(k_db, dim3(h->Np[l]), dim3(256), 0, h->st, h->dZ, h->g + h->offB[l], h->B, h->Np[l]);
+    hipLaunchKernelGGL(k_dw, dim3(h->Np[l] / 16, h->Kp[l] / 16 + 1), dim3(64 * kDwWaves), 0, h->st, dz_b(h, l),
+                       h->X[l], h->g + h->offW[l], h->B, h->Kp[l], h->Np[l], dz_f(h, l), h->g + h->offB[l]);
     if (l > 0)
-      hipLaunchKernelGGL(k_dx, dim3(h->B / 16, h->Kp[l] / 16), dim3(64), 0, h->st, h->dZb,
-                         h->wb + h->offW[l], h->dH, h->Kp[l], h->Np[l]);
+      hipLaunchKernelGGL(k_dx, dim3(h->B / 16, h->Kp[l] / 16), dim3(64), 0, h->st, dz_b(h, l), h->wb + h->offW[l],
+                         h->Z[l - 1], dz_f(h, l - 1), dz_b(h, l - 1), h->Kp[l], h->Np[l]);
   }
 }
 
@@ -532,8 +574,11 @@ int lh_mlp_create(int nin, int nhid, int nlayers, int nout, int batch, unsigned 
     h->X.push_back(x); h->Z.push_back(z);
     wmax = std::max(wmax, std::max(h->Kp[l], h->Np[l]));
   }
-  MCHK(h, zalloc(&h->dH, (size_t)batch * wmax)); MCHK(h, zalloc(&h->dZ, (size_t)batch * wmax));
-  MCHK(h, zalloc(&h->dZb, (size_t)batch * wmax)); MCHK(h, zalloc(&h->T, (size_t)batch * nout));
+  MCHK(h, zalloc(&h->dZ, (size_t)batch * wmax)); MCHK(h, zalloc(&h->dZ2, (size_t)batch * wmax));
+  MCHK(h, zalloc(&h->dZb, (size_t)batch * wmax)); MCHK(h, zalloc(&h->dZb2, (size_t)batch * wmax));
+  MCHK(h, zalloc(&h->T, (size_t)batch * nout));
+  h->nparts = (batch * h->Np[h->L - 1] + 255) / 256;
+  MCHK(h, zalloc(&h->part, (size_t)h->nparts));
   MCHK(h, zalloc(&h->loss, 16));
   MCHK(h, zalloc(&h->tdev, 16));
   // Glorot-uniform weights from a counter-based generator, zero biases
@@ -557,7 +602,9 @@ int lh_mlp_destroy(lh_mlp* h) {
   if (!h) return LH_OK;
   if (h->st) (void)hipStreamSynchronize(h->st);
   if (h->gstep) (void)hipGraphExecDestroy(h->gstep);
-  void* ptrs[] = {h->p, h->g, h->m, h->v, h->wb, h->dH, h->dZ, h->dZb, h->T, h->loss, h->dx, h->dy, h->tdev};
+  if (h->gmulti) (void)hipGraphExecDestroy(h->gmulti);
+  void* ptrs[] = {h->p,  h->g,    h->m,    h->v,  h->wb,   h->dZ,  h->dZ2, h->dZb,
+                  h->dZb2, h->T, h->loss, h->part, h->dx, h->dy, h->tdev};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
   for (auto q : h->X) if (q) (void)hipFree(q);
@@ -622,7 +669,7 @@ int lh_mlp_set_data(lh_mlp* h, const float* x, const float* y, long long n) {
   if (h->dx) (void)hipFree(h->dx);
   if (h->dy) (void)hipFree(h->dy);
   h->dx = h->dy = nullptr;
-  if (h->gstep) { (void)hipGraphExecDestroy(h->gstep); h->gstep = nullptr; }   // it holds dx, dy
+  drop_train_graphs(h);                        // they hold dx, dy
   MCHK(h, hipMalloc((void**)&h->dx, (size_t)n * h->nin * sizeof(float)));
   MCHK(h, hipMalloc((void**)&h->dy, (size_t)n * h->nout * sizeof(float)));
   MCHK(h, hipMemcpy(h->dx, x, (size_t)n * h->nin * sizeof(float), hipMemcpyHostToDevice));
@@ -640,9 +687,9 @@ int lh_mlp_loss_grad(lh_mlp* h, const float* x, const float* y, float* loss, flo
   MCHK(h, hipMemcpy(h->T, y, (size_t)h->B * h->nout * sizeof(float), hipMemcpyHostToDevice));
   hipLaunchKernelGGL(k_pack_x, dim3(h->B), dim3(64), 0, h->st, dx, h->B, h->nin, h->Kp[0], h->B, h->X[0]);
   mlp_forward(h);
-  hipLaunchKernelGGL(k_mse, dim3(1), dim3(1024), 0, h->st, h->Z[h->L - 1], h->T, h->dH, h->loss, h->B,
-                     h->Np[h->L - 1], h->nout);
+  mlp_loss(h, false);
   mlp_backward(h);
+  hipLaunchKernelGGL(k_loss, dim3(1), dim3(64), 0, h->st, h->part, h->nparts, (float)(h->B * h->nout), h->loss);
   MCHK(h, hipStreamSynchronize(h->st));
   (void)hipFree(dx);
   if (loss) MCHK(h, hipMemcpy(loss, h->loss, sizeof(float), hipMemcpyDeviceToHost));
@@ -661,32 +708,40 @@ static void issue_train_step(lh_mlp* h, float lr, unsigned long long seed) {
   hipLaunchKernelGGL(k_gather, dim3(h->B), dim3(64), 0, h->st, h->dx, h->dy, h->ndata, h->nin, h->nout,
                      h->Kp[0], h->B, (uint64_t)seed, (const int*)h->tdev, h->X[0], h->T);
   mlp_forward(h);
-  hipLaunchKernelGGL(k_mse, dim3(1), dim3(1024), 0, h->st, h->Z[h->L - 1], h->T, h->dH, h->loss, h->B,
-                     h->Np[h->L - 1], h->nout);
+  mlp_loss(h, true);
   mlp_backward(h);
   hipLaunchKernelGGL(k_adam, dim3((unsigned)((h->np + 255) / 256)), dim3(256), 0, h->st, h->p, h->g, h->m,
-                     h->v, h->wb, h->np, lr, b1, b2, eps, (const int*)h->tdev);
-  hipLaunchKernelGGL(k_tick, dim3(1), dim3(64), 0, h->st, h->tdev);
+                     h->v, h->wb, h->np, lr, b1, b2, eps, (const int*)h->tdev, h->part, h->nparts,
+                     (float)(h->B * h->nout), h->loss);
 }
 
-// `steps` Adam steps on batches drawn from the data set; loss_last = loss of the last batch
+// captures `n` consecutive training steps as one graph
+static int capture_steps(lh_mlp* h, int n, float lr, unsigned long long seed, hipGraphExec_t* out) {
+  hipGraph_t g;
+  MCHK(h, hipStreamBeginCapture(h->st, hipStreamCaptureModeThreadLocal));
+  for (int s = 0; s < n; ++s) issue_train_step(h, lr, seed);
+  MCHK(h, hipStreamEndCapture(h->st, &g));
+  const hipError_t e = hipGraphInstantiate(out, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  MCHK(h, e);
+  return LH_OK;
+}
+
+// `steps` Adam steps on batches drawn from the data set; loss_last = loss of the last batch.
+// The steps replay a graph of kTrainSteps steps, the remainder a graph of one step: each graph
+// launch has a fixed cost that one step alone does not cover.
 int lh_mlp_train(lh_mlp* h, int steps, float lr, unsigned long long seed, float* loss_last) {
   if (!h || steps < 0) return mfail(h, LH_ERR_ARG, "lh_mlp_train: bad argument");
   if (!h->dx) return mfail(h, LH_ERR_ARG, "lh_mlp_train: no data set (lh_mlp_set_data)");
   MCHK(h, hipStreamSynchronize(h->st));
-  if (steps > 0 && (!h->gstep || h->g_lr != lr || h->g_seed != seed)) {
-    if (h->gstep) { (void)hipGraphExecDestroy(h->gstep); h->gstep = nullptr; }
-    hipGraph_t g;
-    MCHK(h, hipStreamBeginCapture(h->st, hipStreamCaptureModeThreadLocal));
-    issue_train_step(h, lr, seed);
-    MCHK(h, hipStreamEndCapture(h->st, &g));
-    const hipError_t e = hipGraphInstantiate(&h->gstep, g, nullptr, nullptr, 0);
-    (void)hipGraphDestroy(g);
-    MCHK(h, e);
-    h->g_lr = lr;
-    h->g_seed = seed;
-  }
-  for (int s = 0; s < steps; ++s) MCHK(h, hipGraphLaunch(h->gstep, h->st));
+  if ((h->gstep || h->gmulti) && (h->g_lr != lr || h->g_seed != seed)) drop_train_graphs(h);
+  h->g_lr = lr;
+  h->g_seed = seed;
+  int r;
+  if (steps >= kTrainSteps && !h->gmulti && (r = capture_steps(h, kTrainSteps, lr, seed, &h->gmulti))) return r;
+  if (steps % kTrainSteps && !h->gstep && (r = capture_steps(h, 1, lr, seed, &h->gstep))) return r;
+  for (int s = 0; s + kTrainSteps <= steps; s += kTrainSteps) MCHK(h, hipGraphLaunch(h->gmulti, h->st));
+  for (int s = 0; s < steps % kTrainSteps; ++s) MCHK(h, hipGraphLaunch(h->gstep, h->st));
   h->t += steps;
   MCHK(h, hipGetLastError());
   MCHK(h, hipStreamSynchronize(h->st));
@@ -704,7 +759,7 @@ int lh_mlp_predict(lh_mlp* h, const float* x, long long n, float* y) {
     const int nr = (int)std::min<long long>(h->B, n - r0);
     MCHK(h, hipMemcpy(dx, x + (size_t)r0 * h->nin, (size_t)nr * h->nin * sizeof(float), hipMemcpyHostToDevice));
     hipLaunchKernelGGL(k_pack_x, dim3(h->B), dim3(64), 0, h->st, dx, nr, h->nin, h->Kp[0], h->B, h->X[0]);
-    mlp_forward(h);
+    forward_rows(h, h->B, h->X.data(), h->Z.data(), h->st);
     MCHK(h, hipStreamSynchronize(h->st));
     MCHK(h, hipMemcpy(z.data(), h->Z[h->L - 1], z.size() * sizeof(float), hipMemcpyDeviceToHost));
     for (int r = 0; r < nr; ++r)
