@@ -403,6 +403,154 @@ __global__ void __launch_bounds__(64 * kFuseWaves) k_mlp_grid(FusedNet fn, const
   }
 }
 
+// k_mlp_grid with the closure network's widths known at compile time (first layer K0 wide,
+// NHH + 1 hidden layers NH wide, 16 padded outputs): every LDS operand address is a per-lane
+// base plus a constant; the gather offsets come from an LDS table four at a time, and tiles
+// whose 16 points (one x row, nx a multiple of 16) need no clamp or wrap skip that logic.
+// Same operands, k order and instructions as k_mlp_grid.
+constexpr int kOffEx = -2147483647;      // input offset sentinels: the extra field, padding
+constexpr int kOffPad = -2147483647 - 1;
+template <int K0, int NH, int NHH>
+__global__ void __launch_bounds__(64 * kFuseWaves) k_mlp_grid_s(FusedNet fn, const uint16_t* __restrict__ wb,
+                                                                const float* __restrict__ p,
+                                                                const float* __restrict__ g,
+                                                                const float* __restrict__ ex,
+                                                                const float* __restrict__ mean,
+                                                                const float* __restrict__ scale, int nch, int ny,
+                                                                int nz, int nx, long long n, float* __restrict__ y) {
+  constexpr int NO = 16;
+  constexpr int WH = NH * K0, WO = WH + NHH * NH * NH, WN = WO + NO * NH;   // LDS weight offsets
+  extern __shared__ __align__(16) unsigned char smem[];
+  uint16_t* Wl = reinterpret_cast<uint16_t*>(smem);
+  float* ms = reinterpret_cast<float*>(smem + ((2 * WN + 15) & ~15));   // mean[K0], scale[K0]
+  int* offs = reinterpret_cast<int*>(ms + 2 * K0);                      // interior offset per input
+  int* fdec = offs + K0;                                                // channel / stencil code
+  const int wave = threadIdx.x >> 6, l = threadIdx.x & 63, li = l & 15, lg = l >> 4;
+  uint16_t* act = reinterpret_cast<uint16_t*>(fdec + K0) + wave * 16 * NH;
+  const int plane = nz * nx, field = plane * ny;
+  for (int e = threadIdx.x; e < WN; e += blockDim.x) {
+    long long src;
+    if (e < WH) src = fn.offW[0] + e;
+    else if (e < WO) src = fn.offW[1 + (e - WH) / (NH * NH)] + (e - WH) % (NH * NH);
+    else src = fn.offW[1 + NHH] + (e - WO);
+    Wl[e] = wb[src];
+  }
+  for (int e = threadIdx.x; e < K0; e += blockDim.x) {
+    ms[e] = e < fn.nin ? mean[e] : 0.f;
+    ms[K0 + e] = e < fn.nin ? scale[e] : 1.f;
+    int d = -1, o = kOffPad;
+    if (e < 27 * nch) {
+      const int c = e % nch, s = e / nch;
+      d = c | ((s / 9) << 8) | (((s / 3) % 3) << 10) | ((s % 3) << 12);
+      o = c * field + (s / 9 - 1) * plane + ((s / 3) % 3 - 1) * nx + (s % 3 - 1);
+    } else if (e < fn.nin) {
+      d = -2;
+      o = kOffEx;
+    }
+    fdec[e] = d;
+    offs[e] = o;
+  }
+  __syncthreads();
+  const uint16_t* w0 = Wl + li * K0 + 4 * lg;        // B operands: lane base + constants
+  const uint16_t* wh = Wl + WH + li * NH + 4 * lg;
+  const uint16_t* wo = Wl + WO + li * NH + 4 * lg;
+  const long long ntile = (n + 15) / 16;
+  for (long long tile = (long long)blockIdx.x * kFuseWaves + wave; tile < ntile;
+       tile += (long long)gridDim.x * kFuseWaves) {
+    const long long r0 = tile * 16;
+    const int i0 = (int)(r0 % nx), k = (int)((r0 / nx) % nz), j = (int)(r0 / ((long long)nx * nz));
+    const int i = i0 + li;
+    const bool live = r0 + li < n;
+    const bool inner = r0 + 16 <= n && i0 > 0 && i0 + 16 < nx && k > 0 && k + 1 < nz && j > 0 && j + 1 < ny;
+    const int pidx = j * plane + k * nx + i;
+    float v[K0 / 16][4];
+    if (inner) {
+#pragma unroll
+      for (int c = 0; c < K0 / 16; ++c) {
+        const int4 o4 = *reinterpret_cast<const int4*>(offs + c * 16 + 4 * lg);
+        const int oo[4] = {o4.x, o4.y, o4.z, o4.w};
+#pragma unroll
+        for (int qq = 0; qq < 4; ++qq)
+          v[c][qq] = oo[qq] > kOffEx ? g[pidx + oo[qq]] : (oo[qq] == kOffEx ? ex[pidx] : 0.f);
+      }
+    } else {
+#pragma unroll
+      for (int c = 0; c < K0 / 16; ++c)
+#pragma unroll
+        for (int qq = 0; qq < 4; ++qq) {
+          const int d = fdec[c * 16 + 4 * lg + qq];
+          float x = 0.f;
+          if (live && d >= 0) {
+            int jj = j + ((d >> 8) & 3) - 1, kk = k + ((d >> 10) & 3) - 1, ii = i + ((d >> 12) & 3) - 1;
+            jj = jj < 0 ? 0 : (jj >= ny ? ny - 1 : jj);
+            kk = kk < 0 ? kk + nz : (kk >= nz ? kk - nz : kk);
+            ii = ii < 0 ? ii + nx : (ii >= nx ? ii - nx : ii);
+            x = g[(size_t)(d & 255) * field + (size_t)jj * plane + (size_t)kk * nx + ii];
+          } else if (live && d == -2) {
+            x = ex[pidx];
+          }
+          v[c][qq] = x;
+        }
+    }
+    f4 acc[NH / 16];
+#pragma unroll
+    for (int t = 0; t < NH / 16; ++t) acc[t] = f4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int c = 0; c < K0 / 16; ++c) {
+      const float4 mu = *reinterpret_cast<const float4*>(ms + c * 16 + 4 * lg);
+      const float4 sg = *reinterpret_cast<const float4*>(ms + K0 + c * 16 + 4 * lg);
+      const float mv[4] = {mu.x, mu.y, mu.z, mu.w}, sv[4] = {sg.x, sg.y, sg.z, sg.w};
+      bf4 a;
+#pragma unroll
+      for (int qq = 0; qq < 4; ++qq) {
+        const int f = c * 16 + 4 * lg + qq;
+        a[qq] = (short)((live && f < fn.nin) ? f2bf((v[c][qq] - mv[qq]) / sv[qq]) : (uint16_t)0);
+      }
+#pragma unroll
+      for (int t = 0; t < NH / 16; ++t)
+        acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(
+            a, *reinterpret_cast<const bf4*>(w0 + t * 16 * K0 + c * 16), acc[t], 0, 0, 0);
+    }
+    // hidden layers: activations through the wave's tile, then the next product
+#pragma unroll
+    for (int h = 0; h <= NHH; ++h) {
+      const float* bias = p + fn.offB[h];
+#pragma unroll
+      for (int t = 0; t < NH / 16; ++t) {
+        const float bb = bias[t * 16 + li];
+#pragma unroll
+        for (int qq = 0; qq < 4; ++qq)
+          act[(4 * lg + qq) * NH + t * 16 + li] = f2bf(tanhf(acc[t][qq] + bb));
+      }
+      bf4 a[NH / 16];
+#pragma unroll
+      for (int c = 0; c < NH / 16; ++c) a[c] = *reinterpret_cast<const bf4*>(act + li * NH + c * 16 + 4 * lg);
+      if (h < NHH) {
+#pragma unroll
+        for (int t = 0; t < NH / 16; ++t) {
+          acc[t] = f4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+          for (int c = 0; c < NH / 16; ++c)
+            acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(
+                a[c], *reinterpret_cast<const bf4*>(wh + h * NH * NH + t * 16 * NH + c * 16), acc[t], 0, 0, 0);
+        }
+      } else {
+        f4 ao = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int c = 0; c < NH / 16; ++c)
+          ao = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[c], *reinterpret_cast<const bf4*>(wo + c * 16), ao, 0,
+                                                         0, 0);
+        const float bo = p[fn.offB[NHH + 1] + li];
+#pragma unroll
+        for (int qq = 0; qq < 4; ++qq) {
+          const long long r = r0 + 4 * lg + qq;
+          if (r < n && li < fn.nout) y[(size_t)r * fn.nout + li] = ao[qq] + bo;
+        }
+      }
+    }
+  }
+}
+
 // real outputs of nrow rows (Z: B x NP f32) -> y (nrow x nout)
 __global__ void k_unpack_out(const float* __restrict__ Z, int NP, int nout, int nrow, float* __restrict__ y) {
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
@@ -472,6 +620,7 @@ struct lh_mlp {
   // inference workspace of Be rows (allocated on first use by the grid evaluation)
   int Be = 0;
   int ncu = 0;                                 // compute units (fused grid evaluation)
+  int ncu_s = 0;                               // the same, for the compile-time-shape kernel
   std::vector<uint16_t*> Xe;
   std::vector<float*> Ze;
   float g_lr = 0.f;
@@ -825,6 +974,39 @@ int lh_mlp_eval_grid_dev(lh_mlp* h, const float* g, int nch, const float* ex, in
     return mfail(h, LH_ERR_ARG, "lh_mlp_eval_grid: bad argument");
   if (h->nin != 27 * nch + (ex ? 1 : 0))
     return mfail(h, LH_ERR_ARG, "lh_mlp_eval_grid: nin must be 27 nch (+1 with the extra field)");
+  // the closure network's shape: widths known at compile time
+  bool spec = h->L == 5 && h->Kp[0] == 256 && h->Np[h->L - 1] == 16 && nx % 16 == 0 && nch < 256 &&
+              (long long)ny * nz * nx * nch < (1LL << 31);
+  for (int l = 0; l + 1 < h->L; ++l) spec = spec && h->Np[l] == 64;
+  if (spec) {
+    FusedNet fn{};
+    fn.L = h->L;
+    fn.np = (int)h->np;
+    fn.nin = h->nin;
+    fn.nout = h->nout;
+    for (int l = 0; l < h->L; ++l) {
+      fn.Kp[l] = h->Kp[l];
+      fn.Np[l] = h->Np[l];
+      fn.offW[l] = h->offW[l];
+      fn.offB[l] = h->offB[l];
+    }
+    constexpr int WN = 64 * 256 + 3 * 64 * 64 + 16 * 64;
+    const size_t lds_s = (size_t)((2 * WN + 15) & ~15) + 4 * 256 * sizeof(int) + kFuseWaves * 16 * 64 * 2;
+    if (!h->ncu_s) {
+      int dev = 0;
+      MCHK(h, hipGetDevice(&dev));
+      MCHK(h, hipDeviceGetAttribute(&h->ncu_s, hipDeviceAttributeMultiprocessorCount, dev));
+      MCHK(h, hipFuncSetAttribute((const void*)k_mlp_grid_s<256, 64, 3>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  160 * 1024));
+    }
+    const long long n = (long long)ny * nz * nx;
+    const long long tiles = (n + 15) / 16;
+    const int blocks = (int)std::min<long long>((tiles + kFuseWaves - 1) / kFuseWaves, 2LL * h->ncu_s);
+    hipLaunchKernelGGL((k_mlp_grid_s<256, 64, 3>), dim3(blocks), dim3(64 * kFuseWaves), lds_s, st, fn, h->wb, h->p,
+                       g, ex, mean, scale, nch, ny, nz, nx, n, y);
+    MCHK(h, hipGetLastError());
+    return LH_OK;
+  }
   // fused path: every width <= 64, weights and tiles within the LDS of a block
   bool fuse = h->L <= kFuseMaxL && h->Kp[0] <= kFuseMaxK0 && nch < 256;
   for (int l = 0; l < h->L; ++l) fuse = fuse && h->Np[l] <= 64 && (l == 0 || h->Kp[l] <= 64);
