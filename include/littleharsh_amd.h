@@ -233,6 +233,8 @@ int lh_closure_apply(lh_closure* c);
 /* on != 0: keep only the plane-mean column (kx = kz = 0) of the closure's forcing, the
  * stresses' effect on the mean profile (default 0: all columns). */
 int lh_closure_set_mean_only(lh_closure* c, int on);
+/* Destroy the closure before its channel and its network: it holds their device buffers and
+ * the channel's stream.  The channel's forcing stays as last applied. */
 int lh_closure_destroy(lh_closure* c);
 const char* lh_closure_last_error(const lh_closure* c);
 
