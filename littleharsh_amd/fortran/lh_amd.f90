@@ -69,8 +69,68 @@ module lh_amd
       import :: c_int, c_ptr
       type(c_ptr), value :: h, f1, f2, f3
     end function
-    ! the closure bound to a coarse channel: its forcing from the current state, on the GPU
+    ! the learned closure's network (trained and evaluated on the GPU)
+    integer(c_int) function lh_mlp_create(nin, nhid, nlayers, nout, batch, seed, m) bind(C, name="lh_mlp_create")
+      import :: c_int, c_ptr, c_long_long
+      integer(c_int), value :: nin, nhid, nlayers, nout, batch
+      integer(c_long_long), value :: seed
+      type(c_ptr), intent(out) :: m
+    end function
+    integer(c_int) function lh_mlp_destroy(m) bind(C, name="lh_mlp_destroy")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: m
+    end function
+    integer(c_long_long) function lh_mlp_param_count(m) bind(C, name="lh_mlp_param_count")
+      import :: c_long_long, c_ptr
+      type(c_ptr), value :: m
+    end function
+    integer(c_int) function lh_mlp_set_params(m, params) bind(C, name="lh_mlp_set_params")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: m, params
+    end function
+    integer(c_int) function lh_mlp_get_params(m, params) bind(C, name="lh_mlp_get_params")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: m, params
+    end function
+    integer(c_int) function lh_mlp_set_data(m, x, y, n) bind(C, name="lh_mlp_set_data")
+      import :: c_int, c_ptr, c_long_long
+      type(c_ptr), value :: m, x, y
+      integer(c_long_long), value :: n
+    end function
+    integer(c_int) function lh_mlp_train(m, steps, lr, seed, loss_last) bind(C, name="lh_mlp_train")
+      import :: c_int, c_ptr, c_float, c_long_long
+      type(c_ptr), value :: m
+      integer(c_int), value :: steps
+      real(c_float), value :: lr
+      integer(c_long_long), value :: seed
+      real(c_float), intent(out) :: loss_last
+    end function
+    ! the closure bound to a coarse channel (arrays: c_loc of the host's own storage)
+    integer(c_int) function lh_closure_create(h, m, yc, dxc, dzc, shift_x, shift_z, sf, xmean, xscale, &
+                                              ymean, yscale, c) bind(C, name="lh_closure_create")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: h, m, yc, sf, xmean, xscale, ymean, yscale
+      real(c_double), value :: dxc, dzc, shift_x, shift_z
+      type(c_ptr), intent(out) :: c
+    end function
+    integer(c_int) function lh_closure_create_smagorinsky(h, yc, dxc, dzc, shift_x, shift_z, cs2, c) &
+        bind(C, name="lh_closure_create_smagorinsky")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: h, yc
+      real(c_double), value :: dxc, dzc, shift_x, shift_z, cs2
+      type(c_ptr), intent(out) :: c
+    end function
+    ! its forcing from the current state, on the GPU (call before each lh_step)
     integer(c_int) function lh_closure_apply(c) bind(C, name="lh_closure_apply")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: c
+    end function
+    integer(c_int) function lh_closure_set_mean_only(c, on) bind(C, name="lh_closure_set_mean_only")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: c
+      integer(c_int), value :: on
+    end function
+    integer(c_int) function lh_closure_destroy(c) bind(C, name="lh_closure_destroy")
       import :: c_int, c_ptr
       type(c_ptr), value :: c
     end function
