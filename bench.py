@@ -5,7 +5,7 @@ Workload: Re_tau = 550 channel 2pi x 2 x pi, Nx = Nz = 384 (Ngal 576), 289 v pla
 CFL 0.5 time step chosen on the device.  Inputs are resident in HBM before the timed
 region; the timed region is K full RK3 steps (3 substeps each) after W warm-up steps.
 
-Prints one JSON line (rank 0).  With --gpus N > 1 it exits: this build runs on one GPU.
+Prints one JSON line (rank 0).  With --gpus N > 1 it exits: the solver runs on one GPU.
 """
 from __future__ import annotations
 
@@ -84,7 +84,7 @@ def main():
         calibrate_copy()
         return
     if args.gpus != 1:
-        print(f"bench.py: this build runs on exactly one GPU; --gpus {args.gpus} is not supported",
+        print(f"bench.py: the solver runs on exactly one GPU; --gpus {args.gpus} is not supported",
               file=sys.stderr)
         sys.exit(2)
 

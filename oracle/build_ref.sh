@@ -1,7 +1,7 @@
 #!/bin/bash
 # Compile the reference's own FFT and solver routines from their sources in the reference
-# tree, plus the project test driver oracle/refdrv.f90, into oracle/_ref/ (checker only).
-# The missing FOU3D.f90 symbols and the MPI entry points stay unresolved: the driver never
+# tree, plus the project test program oracle/refdrv.f90, into oracle/_ref/ (checker only).
+# The missing FOU3D.f90 symbols and the MPI entry points stay unresolved: that program never
 # reaches them, and the library is loaded lazily by tests/golden/make_golden.py.
 set -euo pipefail
 HERE="$(cd "$(dirname "${BASH_SOURCE[0]}")" && pwd)"

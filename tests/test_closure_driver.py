@@ -18,7 +18,7 @@ def fine_setup():
 
 
 class OracleChannel:
-    """The oracle behind the solver's Channel methods the driver uses."""
+    """The oracle behind the Channel methods run_coarse uses."""
 
     collapse_after = None                       # steps after which cfl_dt collapses
     raise_after = None                          # steps after which cfl_dt raises
