@@ -1,5 +1,5 @@
 // lh_mlp.hip -- the learned subfilter closure: a small MLP trained and evaluated on gfx950
-// with BF16 matrix cores (v_mfma_f32_16x16x16_bf16) accumulating in FP32.
+// with BF16 matrix cores (v_mfma_f32_16x16x32_bf16 and v_mfma_f32_16x16x16_bf16) accumulating in FP32.
 //
 // The reference has no counterpart: the closure is the project's canopy-layer subfilter model
 // (inputs: filtered velocity gradients on a 3x3x3 stencil and the solid fraction; outputs:
@@ -12,6 +12,8 @@
 // gradients are zero).  One wave computes one 16x16 output tile:
 //   lane l holds A[i][k] for i = l % 16, k = 4 (l / 16) + q and B[k][j] for j = l % 16,
 //   k = 4 (l / 16) + q (q = 0..3), and receives D[i][j] for j = l % 16, i = 4 (l / 16) + q.
+// The forward products step through k 32 at a time with v_mfma_f32_16x16x32_bf16 (gfx950),
+// whose lane l holds the eight k = 8 (l / 16) + 0..7 of its row (A) or column (B).
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 #include <string.h>
@@ -28,6 +30,14 @@ namespace {
 
 typedef short bf4 __attribute__((ext_vector_type(4)));
 typedef float f4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
+typedef short s8 __attribute__((ext_vector_type(8)));
+
+// eight contiguous bf16 (16 bytes) as a v_mfma_f32_16x16x32_bf16 operand
+__device__ inline bf8 ld8(const uint16_t* p) { return __builtin_bit_cast(bf8, *reinterpret_cast<const s8*>(p)); }
+// 32-wide step: lane l holds A[i = l % 16][k = 8 (l / 16) + 0..7] and B[k = 8 (l / 16) + 0..7]
+// [j = l % 16]; D as in the 16-wide instruction
+__device__ inline f4 mfma32(bf8 a, bf8 b, f4 c) { return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0); }
 
 // round to nearest even
 __device__ __host__ inline uint16_t f2bf(float f) {
@@ -62,11 +72,16 @@ __global__ void __launch_bounds__(64) k_fwd(const uint16_t* __restrict__ X, cons
   const int l = threadIdx.x;
   const int b0 = blockIdx.x * 16, n0 = blockIdx.y * 16;
   f4 acc = {0.f, 0.f, 0.f, 0.f};
-  // the four k of a lane are contiguous (8 bytes; every width is a multiple of 16)
-  const bf4* xa = reinterpret_cast<const bf4*>(X + (size_t)(b0 + l % 16) * K + 4 * (l / 16));
-  const bf4* wa = reinterpret_cast<const bf4*>(W + (size_t)(n0 + l % 16) * K + 4 * (l / 16));
-  for (int k0 = 0; k0 < K; k0 += 16)
-    acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(xa[k0 / 4], wa[k0 / 4], acc, 0, 0, 0);
+  // a lane's k are contiguous (every width is a multiple of 16): 32 at a time, then a last
+  // 16-wide step when K is an odd multiple of 16
+  const uint16_t* xr = X + (size_t)(b0 + l % 16) * K;
+  const uint16_t* wr = W + (size_t)(n0 + l % 16) * K;
+  int k0 = 0;
+  for (; k0 + 32 <= K; k0 += 32) acc = mfma32(ld8(xr + k0 + 8 * (l / 16)), ld8(wr + k0 + 8 * (l / 16)), acc);
+  if (k0 < K)
+    acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(*reinterpret_cast<const bf4*>(xr + k0 + 4 * (l / 16)),
+                                                    *reinterpret_cast<const bf4*>(wr + k0 + 4 * (l / 16)), acc,
+                                                    0, 0, 0);
   const int n = n0 + l % 16;
 #pragma unroll
   for (int q = 0; q < 4; ++q) {
@@ -260,7 +275,8 @@ __global__ void k_pack_grid(const float* __restrict__ g, const float* __restrict
 // in LDS for the block; the first layer's operands are gathered and standardised straight
 // into the matrix-core registers (the inputs k_pack_grid would write); each hidden layer's
 // activations go through a 16 x 64 bf16 tile of the wave's own LDS.  Same operands, k order
-// and instructions as k_pack_grid + k_fwd, so the outputs are the same bits.
+// and instructions as k_pack_grid + k_fwd, so the outputs are the same bits (every input
+// width a multiple of 32, so that k_fwd takes 32-wide steps only).
 constexpr int kFuseMaxL = 6;
 constexpr int kFuseWaves = 8;     // waves per block sharing the staged weights
 constexpr int kFuseMaxK0 = 256;   // first-layer width held in registers
@@ -311,14 +327,14 @@ __global__ void __launch_bounds__(64 * kFuseWaves) k_mlp_grid(FusedNet fn, const
       j = (int)(q / ((long long)nx * nz));
     }
     // every first-layer operand of the lane first (the loads overlap), then the products
-    float v[kFuseMaxK0 / 16][4];
+    float v[kFuseMaxK0 / 32][8];
 #pragma unroll
-    for (int c = 0; c < kFuseMaxK0 / 16; ++c)
+    for (int c = 0; c < kFuseMaxK0 / 32; ++c)
 #pragma unroll
-      for (int qq = 0; qq < 4; ++qq) {
-        const int f = c * 16 + 4 * (l / 16) + qq;
+      for (int qq = 0; qq < 8; ++qq) {
+        const int f = c * 32 + 8 * (l / 16) + qq;
         float x = 0.f;
-        if (c * 16 < K0 && live) {
+        if (c * 32 < K0 && live) {
           const int d = fdec[f];
           if (d >= 0) {
             int jj = j + ((d >> 8) & 3) - 1, kk = k + ((d >> 10) & 3) - 1, ii = i + ((d >> 12) & 3) - 1;
@@ -338,21 +354,19 @@ __global__ void __launch_bounds__(64 * kFuseWaves) k_mlp_grid(FusedNet fn, const
 #pragma unroll
       for (int t = 0; t < 4; ++t) acc[t] = f4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-      for (int c = 0; c < kFuseMaxK0 / 16; ++c)
-        if (c * 16 < K0) {
-          bf4 a;
+      for (int c = 0; c < kFuseMaxK0 / 32; ++c)
+        if (c * 32 < K0) {
+          s8 a;
 #pragma unroll
-          for (int qq = 0; qq < 4; ++qq) {
-            const int f = c * 16 + 4 * (l / 16) + qq;
+          for (int qq = 0; qq < 8; ++qq) {
+            const int f = c * 32 + 8 * (l / 16) + qq;
             a[qq] = (short)((live && f < fn.nin) ? f2bf((v[c][qq] - ms[f]) / ms[K0 + f]) : (uint16_t)0);
           }
 #pragma unroll
           for (int t = 0; t < 4; ++t)
-            if (t < nnt) {
-              const bf4 b = *reinterpret_cast<const bf4*>(Wl + fn.offW[0] + (size_t)(t * 16 + l % 16) * K0 +
-                                                          c * 16 + 4 * (l / 16));
-              acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[t], 0, 0, 0);
-            }
+            if (t < nnt)
+              acc[t] = mfma32(__builtin_bit_cast(bf8, a),
+                              ld8(Wl + fn.offW[0] + (size_t)(t * 16 + l % 16) * K0 + c * 32 + 8 * (l / 16)), acc[t]);
         }
     }
     for (int layer = 0; layer < fn.L; ++layer) {
@@ -380,24 +394,22 @@ __global__ void __launch_bounds__(64 * kFuseWaves) k_mlp_grid(FusedNet fn, const
           for (int qq = 0; qq < 4; ++qq)
             act[(4 * (l / 16) + qq) * 64 + nn] = f2bf(tanhf(acc[t][qq] + bias[nn]));
         }
-      // next layer: A from the tile (row l % 16, four contiguous k per 16-wide chunk)
+      // next layer: A from the tile (row l % 16, eight contiguous k per 32-wide chunk)
       const int K = fn.Kp[layer + 1], nnx = fn.Np[layer + 1] / 16;
-      bf4 a[4];
+      bf8 a[2];
 #pragma unroll
-      for (int c = 0; c < 4; ++c)
-        if (c * 16 < K) a[c] = *reinterpret_cast<const bf4*>(act + (l % 16) * 64 + c * 16 + 4 * (l / 16));
+      for (int c = 0; c < 2; ++c)
+        if (c * 32 < K) a[c] = ld8(act + (l % 16) * 64 + c * 32 + 8 * (l / 16));
 #pragma unroll
       for (int t = 0; t < 4; ++t) acc[t] = f4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-      for (int c = 0; c < 4; ++c)
-        if (c * 16 < K) {
+      for (int c = 0; c < 2; ++c)
+        if (c * 32 < K) {
 #pragma unroll
           for (int t = 0; t < 4; ++t)
-            if (t < nnx) {
-              const bf4 b = *reinterpret_cast<const bf4*>(Wl + fn.offW[layer + 1] + (size_t)(t * 16 + l % 16) * K +
-                                                          c * 16 + 4 * (l / 16));
-              acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[c], b, acc[t], 0, 0, 0);
-            }
+            if (t < nnx)
+              acc[t] = mfma32(a[c], ld8(Wl + fn.offW[layer + 1] + (size_t)(t * 16 + l % 16) * K + c * 32 + 8 * (l / 16)),
+                              acc[t]);
         }
     }
   }
@@ -418,6 +430,7 @@ __global__ void __launch_bounds__(64 * kFuseWaves) k_mlp_grid_s(FusedNet fn, con
                                                                 const float* __restrict__ mean,
                                                                 const float* __restrict__ scale, int nch, int ny,
                                                                 int nz, int nx, long long n, float* __restrict__ y) {
+  static_assert(K0 % 32 == 0 && NH % 32 == 0, "32-wide matrix-core steps");
   constexpr int NO = 16;
   constexpr int WH = NH * K0, WO = WH + NHH * NH * NH, WN = WO + NO * NH;   // LDS weight offsets
   extern __shared__ __align__(16) unsigned char smem[];
@@ -451,9 +464,9 @@ __global__ void __launch_bounds__(64 * kFuseWaves) k_mlp_grid_s(FusedNet fn, con
     offs[e] = o;
   }
   __syncthreads();
-  const uint16_t* w0 = Wl + li * K0 + 4 * lg;        // B operands: lane base + constants
-  const uint16_t* wh = Wl + WH + li * NH + 4 * lg;
-  const uint16_t* wo = Wl + WO + li * NH + 4 * lg;
+  const uint16_t* w0 = Wl + li * K0 + 8 * lg;        // B operands: lane base + constants
+  const uint16_t* wh = Wl + WH + li * NH + 8 * lg;
+  const uint16_t* wo = Wl + WO + li * NH + 8 * lg;
   const long long ntile = (n + 15) / 16;
   for (long long tile = (long long)blockIdx.x * kFuseWaves + wave; tile < ntile;
        tile += (long long)gridDim.x * kFuseWaves) {
@@ -463,22 +476,23 @@ __global__ void __launch_bounds__(64 * kFuseWaves) k_mlp_grid_s(FusedNet fn, con
     const bool live = r0 + li < n;
     const bool inner = r0 + 16 <= n && i0 > 0 && i0 + 16 < nx && k > 0 && k + 1 < nz && j > 0 && j + 1 < ny;
     const int pidx = j * plane + k * nx + i;
-    float v[K0 / 16][4];
+    float v[K0 / 32][8];
     if (inner) {
 #pragma unroll
-      for (int c = 0; c < K0 / 16; ++c) {
-        const int4 o4 = *reinterpret_cast<const int4*>(offs + c * 16 + 4 * lg);
-        const int oo[4] = {o4.x, o4.y, o4.z, o4.w};
+      for (int c = 0; c < K0 / 32; ++c) {
+        const int4 oa = *reinterpret_cast<const int4*>(offs + c * 32 + 8 * lg);
+        const int4 ob = *reinterpret_cast<const int4*>(offs + c * 32 + 8 * lg + 4);
+        const int oo[8] = {oa.x, oa.y, oa.z, oa.w, ob.x, ob.y, ob.z, ob.w};
 #pragma unroll
-        for (int qq = 0; qq < 4; ++qq)
+        for (int qq = 0; qq < 8; ++qq)
           v[c][qq] = oo[qq] > kOffEx ? g[pidx + oo[qq]] : (oo[qq] == kOffEx ? ex[pidx] : 0.f);
       }
     } else {
 #pragma unroll
-      for (int c = 0; c < K0 / 16; ++c)
+      for (int c = 0; c < K0 / 32; ++c)
 #pragma unroll
-        for (int qq = 0; qq < 4; ++qq) {
-          const int d = fdec[c * 16 + 4 * lg + qq];
+        for (int qq = 0; qq < 8; ++qq) {
+          const int d = fdec[c * 32 + 8 * lg + qq];
           float x = 0.f;
           if (live && d >= 0) {
             int jj = j + ((d >> 8) & 3) - 1, kk = k + ((d >> 10) & 3) - 1, ii = i + ((d >> 12) & 3) - 1;
@@ -496,20 +510,21 @@ __global__ void __launch_bounds__(64 * kFuseWaves) k_mlp_grid_s(FusedNet fn, con
 #pragma unroll
     for (int t = 0; t < NH / 16; ++t) acc[t] = f4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-    for (int c = 0; c < K0 / 16; ++c) {
-      const float4 mu = *reinterpret_cast<const float4*>(ms + c * 16 + 4 * lg);
-      const float4 sg = *reinterpret_cast<const float4*>(ms + K0 + c * 16 + 4 * lg);
-      const float mv[4] = {mu.x, mu.y, mu.z, mu.w}, sv[4] = {sg.x, sg.y, sg.z, sg.w};
-      bf4 a;
+    for (int c = 0; c < K0 / 32; ++c) {
+      const float4 m0 = *reinterpret_cast<const float4*>(ms + c * 32 + 8 * lg);
+      const float4 m1 = *reinterpret_cast<const float4*>(ms + c * 32 + 8 * lg + 4);
+      const float4 s0 = *reinterpret_cast<const float4*>(ms + K0 + c * 32 + 8 * lg);
+      const float4 s1 = *reinterpret_cast<const float4*>(ms + K0 + c * 32 + 8 * lg + 4);
+      const float mv[8] = {m0.x, m0.y, m0.z, m0.w, m1.x, m1.y, m1.z, m1.w};
+      const float sv[8] = {s0.x, s0.y, s0.z, s0.w, s1.x, s1.y, s1.z, s1.w};
+      s8 a;
 #pragma unroll
-      for (int qq = 0; qq < 4; ++qq) {
-        const int f = c * 16 + 4 * lg + qq;
+      for (int qq = 0; qq < 8; ++qq) {
+        const int f = c * 32 + 8 * lg + qq;
         a[qq] = (short)((live && f < fn.nin) ? f2bf((v[c][qq] - mv[qq]) / sv[qq]) : (uint16_t)0);
       }
 #pragma unroll
-      for (int t = 0; t < NH / 16; ++t)
-        acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(
-            a, *reinterpret_cast<const bf4*>(w0 + t * 16 * K0 + c * 16), acc[t], 0, 0, 0);
+      for (int t = 0; t < NH / 16; ++t) acc[t] = mfma32(__builtin_bit_cast(bf8, a), ld8(w0 + t * 16 * K0 + c * 32), acc[t]);
     }
     // hidden layers: activations through the wave's tile, then the next product
 #pragma unroll
@@ -522,24 +537,20 @@ __global__ void __launch_bounds__(64 * kFuseWaves) k_mlp_grid_s(FusedNet fn, con
         for (int qq = 0; qq < 4; ++qq)
           act[(4 * lg + qq) * NH + t * 16 + li] = f2bf(tanhf(acc[t][qq] + bb));
       }
-      bf4 a[NH / 16];
+      bf8 a[NH / 32];
 #pragma unroll
-      for (int c = 0; c < NH / 16; ++c) a[c] = *reinterpret_cast<const bf4*>(act + li * NH + c * 16 + 4 * lg);
+      for (int c = 0; c < NH / 32; ++c) a[c] = ld8(act + li * NH + c * 32 + 8 * lg);
       if (h < NHH) {
 #pragma unroll
         for (int t = 0; t < NH / 16; ++t) {
           acc[t] = f4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-          for (int c = 0; c < NH / 16; ++c)
-            acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(
-                a[c], *reinterpret_cast<const bf4*>(wh + h * NH * NH + t * 16 * NH + c * 16), acc[t], 0, 0, 0);
+          for (int c = 0; c < NH / 32; ++c) acc[t] = mfma32(a[c], ld8(wh + h * NH * NH + t * 16 * NH + c * 32), acc[t]);
         }
       } else {
         f4 ao = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-        for (int c = 0; c < NH / 16; ++c)
-          ao = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[c], *reinterpret_cast<const bf4*>(wo + c * 16), ao, 0,
-                                                         0, 0);
+        for (int c = 0; c < NH / 32; ++c) ao = mfma32(a[c], ld8(wo + c * 32), ao);
         const float bo = p[fn.offB[NHH + 1] + li];
 #pragma unroll
         for (int qq = 0; qq < 4; ++qq) {
@@ -1009,7 +1020,7 @@ int lh_mlp_eval_grid_dev(lh_mlp* h, const float* g, int nch, const float* ex, in
   }
   // fused path: every width <= 64, weights and tiles within the LDS of a block
   bool fuse = h->L <= kFuseMaxL && h->Kp[0] <= kFuseMaxK0 && nch < 256;
-  for (int l = 0; l < h->L; ++l) fuse = fuse && h->Np[l] <= 64 && (l == 0 || h->Kp[l] <= 64);
+  for (int l = 0; l < h->L; ++l) fuse = fuse && h->Np[l] <= 64 && (l == 0 || h->Kp[l] <= 64) && h->Kp[l] % 32 == 0;
   const size_t lds = (size_t)((2 * h->np + 15) & ~15LL) + 3 * (size_t)h->Kp[0] * sizeof(float) +
                      kFuseWaves * 16 * 64 * 2;
   fuse = fuse && lds <= 160 * 1024;
