@@ -15,6 +15,8 @@
 //                k_cl_smag from the gradients at the point;
 //   k_cl_tau     de-standardised stresses -> spectral columns: a forward DFT in x then in z
 //                scaled by 1/(nz nx), Nyquist row and column dropped, the shift undone;
+//   (k_cl_sample_fft and k_cl_tau_fft replace the two DFT kernels when nx = nz is a length
+//   with a Stockham plan in lh_fft.hpp: 32, 48, 64 or 96)
 //   k_cl_div     f1 = i kx T_uu + i kz T_uw + dy T_uv and f3 = i kx T_uw + i kz T_ww + dy T_vw
 //                at the centres; f2 = the x and z derivatives of T_uv, T_vw averaged to the
 //                face plus (T_vv(m+1) - T_vv(m)) / (y(m+1) - y(m)) at the interior faces;
@@ -28,6 +30,7 @@
 #include <vector>
 
 #include "../../include/littleharsh_amd.h"
+#include "lh_fft.hpp"
 #include "lh_internal.hpp"
 
 namespace {
@@ -209,6 +212,118 @@ __global__ void __launch_bounds__(256) k_cl_tau(CL c, const float* __restrict__ 
   }
 }
 
+// FFT forms of k_cl_sample and k_cl_tau for nx = nz = N (O(N log N) per line instead of
+// O(N^2)).  Two sets of MX = N / 2 + 1 LDS lines of stride N + 1:
+//   Z: line kx holds one spectral column along z (the z transforms);
+//   X: line p holds the rows z = 2p and 2p + 1 as one complex line a + i b (the x
+//      transforms of two real rows at once); its last line is zero and only keeps the
+//      MX-th thread group in step with the others at the transforms' barriers.
+// Block: MX groups of Plan<N>::T threads, one line per group.  tw: exp(-2 pi i m / N).
+template <int N> constexpr int fft_threads() { return (N / 2 + 1) * lh::Plan<N>::T; }
+
+template <int N>
+__global__ void __launch_bounds__(fft_threads<N>()) k_cl_sample_fft(LhSolverView sv, CL c,
+                                                                    const float2* __restrict__ tw,
+                                                                    float* __restrict__ phys) {
+  constexpr int MX = N / 2 + 1, LS = N + 1, T = lh::Plan<N>::T;
+  extern __shared__ float2 sm[];
+  float2* Z = sm;
+  float2* X = Z + MX * LS;
+  const int m = blockIdx.x, q = blockIdx.y;
+  const int ln = threadIdx.x / T, t = threadIdx.x % T;
+  typename lh::Plan<N>::template Tw<float2> w;
+  w.load(t, tw);
+  for (int e = threadIdx.x; e < N * MX; e += blockDim.x) {   // e = kz row * MX + kx
+    double2 a;
+    if (q == 0) {
+      a = sv.u[(size_t)(m + 1) * c.ncol + e];
+    } else if (q == 2) {
+      a = sv.w[(size_t)(m + 1) * c.ncol + e];
+    } else {
+      const double2 p = sv.v[(size_t)m * c.ncol + e], r = sv.v[(size_t)(m + 1) * c.ncol + e];
+      a = make_double2(0.5 * (p.x + r.x), 0.5 * (p.y + r.y));
+    }
+    Z[(e % MX) * LS + e / MX] = cmul(make_float2((float)a.x, (float)a.y), c.shift[e]);
+  }
+  __syncthreads();
+  lh::Plan<N>::template run<true>(Z + ln * LS, t, w);        // inverse z transforms
+  // Hermitian x lines of rows 2p (real part) and 2p + 1 (imaginary part); the zero and
+  // Nyquist modes contribute their real parts, as in the one-sided sum of k_cl_sample
+  for (int e = threadIdx.x; e < MX * N; e += blockDim.x) {
+    const int p = e / N, n = e % N;
+    float2 v = make_float2(0.f, 0.f);
+    if (p < N / 2) {
+      if (n == 0 || 2 * n == N) {
+        v = make_float2(Z[n * LS + 2 * p].x, Z[n * LS + 2 * p + 1].x);
+      } else if (2 * n < N) {
+        const float2 a = Z[n * LS + 2 * p], b = Z[n * LS + 2 * p + 1];
+        v = make_float2(a.x - b.y, a.y + b.x);
+      } else {
+        const float2 a = Z[(N - n) * LS + 2 * p], b = Z[(N - n) * LS + 2 * p + 1];
+        v = make_float2(a.x + b.y, b.x - a.y);
+      }
+    }
+    X[p * LS + n] = v;
+  }
+  __syncthreads();
+  lh::Plan<N>::template run<true>(X + ln * LS, t, w);        // inverse x transforms
+  float* out = phys + ((size_t)q * c.ny + m) * (N * N);
+  for (int e = threadIdx.x; e < N * N; e += blockDim.x) {
+    const float2 v = X[(e / N / 2) * LS + e % N];
+    out[e] = ((e / N) & 1) ? v.y : v.x;
+  }
+}
+
+template <int N>
+__global__ void __launch_bounds__(fft_threads<N>()) k_cl_tau_fft(CL c, const float* __restrict__ y,
+                                                                 const float* __restrict__ ym,
+                                                                 const float* __restrict__ ysd,
+                                                                 const float2* __restrict__ tw,
+                                                                 float2* __restrict__ T) {
+  constexpr int MX = N / 2 + 1, LS = N + 1, TL = lh::Plan<N>::T;
+  extern __shared__ float2 sm[];
+  float2* X = sm;
+  float2* Z = X + MX * LS;
+  const int m = blockIdx.x, q = blockIdx.y;
+  const int ln = threadIdx.x / TL, t = threadIdx.x % TL;
+  typename lh::Plan<N>::template Tw<float2> w;
+  w.load(t, tw);
+  const float a = ysd[q], b = ym[q];
+  const float* src = y + (size_t)m * (N * N) * kNStress + q;
+  for (int e = threadIdx.x; e < MX * N; e += blockDim.x) {
+    const int p = e / N, n = e % N;
+    float2 v = make_float2(0.f, 0.f);
+    if (p < N / 2)
+      v = make_float2(src[(size_t)(2 * p * N + n) * kNStress] * a + b,
+                      src[(size_t)((2 * p + 1) * N + n) * kNStress] * a + b);
+    X[p * LS + n] = v;
+  }
+  __syncthreads();
+  lh::Plan<N>::template run<false>(X + ln * LS, t, w);       // forward x transforms
+  // the half spectra of the two rows of each line, into the z lines
+  for (int e = threadIdx.x; e < (N / 2) * MX; e += blockDim.x) {
+    const int p = e / MX, k = e % MX;
+    const float2 zk = X[p * LS + k], zn = X[p * LS + (k == 0 ? 0 : N - k)];
+    Z[k * LS + 2 * p] = make_float2(0.5f * (zk.x + zn.x), 0.5f * (zk.y - zn.y));
+    Z[k * LS + 2 * p + 1] = make_float2(0.5f * (zk.y + zn.y), -0.5f * (zk.x - zn.x));
+  }
+  __syncthreads();
+  lh::Plan<N>::template run<false>(Z + ln * LS, t, w);       // forward z transforms
+  const float sc = 1.f / (float)(N * N);
+  float2* out = T + ((size_t)q * c.ny + m) * c.ncol;
+  for (int e = threadIdx.x; e < N * MX; e += blockDim.x) {   // e = kz row * MX + kx
+    const int r = e / MX, cx = e % MX;
+    float2 s = make_float2(0.f, 0.f);
+    if (2 * r != N && 2 * cx != N) {
+      const float2 v = Z[cx * LS + r];
+      s = cmulc(make_float2(v.x * sc, v.y * sc), c.shift[e]);
+    }
+    out[e] = s;
+  }
+}
+
+template <int N> constexpr size_t fft_lds() { return 2 * (size_t)(N / 2 + 1) * (N + 1) * sizeof(float2); }
+
 // one thread per (column, centre plane): the divergence into the forcing buffers
 __global__ void k_cl_div(LhSolverView sv, CL c, const double* __restrict__ yc, const float2* __restrict__ T) {
   const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -262,6 +377,8 @@ struct lh_closure {
   float *sf = nullptr, *stdz = nullptr, *phys = nullptr, *grad = nullptr, *y = nullptr;
   float2* T = nullptr;
   size_t lds_sample = 0, lds_tau = 0;
+  int fft_n = 0;                // N of the FFT kernels (nx = nz = N), 0: the DFT kernels
+  float2* twf = nullptr;        // exp(-2 pi i m / N) for the FFT kernels
   std::string err;
 };
 
@@ -278,6 +395,14 @@ static int cfail(lh_closure* c, int code, const std::string& m) {
 extern "C" {
 
 }  // extern "C"
+
+template <int N> static hipError_t fft_attr() {
+  hipError_t e = hipFuncSetAttribute((const void*)k_cl_sample_fft<N>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                     (int)fft_lds<N>());
+  if (e != hipSuccess) return e;
+  return hipFuncSetAttribute((const void*)k_cl_tau_fft<N>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                             (int)fft_lds<N>());
+}
 
 // shared set-up: channel view, shapes, shifts, twiddles, work buffers and the output scaling
 // (ymean, yscale: 6 each); the network inputs' standardisation (nin each) comes first in stdz
@@ -345,7 +470,34 @@ static int closure_setup(lh_closure* c, lh_handle* h, const double* yc, double d
                               (int)c->lds_sample));
   CCHK(c, hipFuncSetAttribute((const void*)k_cl_tau, hipFuncAttributeMaxDynamicSharedMemorySize,
                               (int)c->lds_tau));
+  if (cl.nx == cl.nz && (cl.nx == 32 || cl.nx == 48 || cl.nx == 64 || cl.nx == 96)) {
+    const int n = cl.nx;
+    std::vector<float2> hf(n);
+    for (int k = 0; k < n; ++k) {
+      const double ph = 2.0 * M_PI * k / n;
+      hf[k] = make_float2((float)std::cos(ph), (float)-std::sin(ph));
+    }
+    CCHK(c, hipMalloc((void**)&c->twf, n * sizeof(float2)));
+    CCHK(c, hipMemcpy(c->twf, hf.data(), n * sizeof(float2), hipMemcpyHostToDevice));
+    switch (n) {
+      case 32: CCHK(c, fft_attr<32>()); break;
+      case 48: CCHK(c, fft_attr<48>()); break;
+      case 64: CCHK(c, fft_attr<64>()); break;
+      default: CCHK(c, fft_attr<96>()); break;
+    }
+    c->fft_n = n;
+  }
   return LH_OK;
+}
+
+// the closure's two spectral stages with the FFT kernels of length N
+template <int N> static void launch_fft_sample(lh_closure* c, hipStream_t st) {
+  hipLaunchKernelGGL(k_cl_sample_fft<N>, dim3(c->c.ny, 3), dim3(fft_threads<N>()), fft_lds<N>(), st, c->sv, c->c,
+                     c->twf, c->phys);
+}
+template <int N> static void launch_fft_tau(lh_closure* c, hipStream_t st, const float* ym, const float* ysd) {
+  hipLaunchKernelGGL(k_cl_tau_fft<N>, dim3(c->c.ny, kNStress), dim3(fft_threads<N>()), fft_lds<N>(), st, c->c, c->y,
+                     ym, ysd, c->twf, c->T);
 }
 
 extern "C" {
@@ -392,7 +544,13 @@ int lh_closure_apply(lh_closure* c) {
   const hipStream_t st = c->sv.st;
   const long long npt = (long long)cl.ny * cl.nz * cl.nx;
   const int nin = c->nin;
-  hipLaunchKernelGGL(k_cl_sample, dim3(cl.ny, 3), dim3(256), c->lds_sample, st, c->sv, cl, c->phys);
+  switch (c->fft_n) {
+    case 32: launch_fft_sample<32>(c, st); break;
+    case 48: launch_fft_sample<48>(c, st); break;
+    case 64: launch_fft_sample<64>(c, st); break;
+    case 96: launch_fft_sample<96>(c, st); break;
+    default: hipLaunchKernelGGL(k_cl_sample, dim3(cl.ny, 3), dim3(256), c->lds_sample, st, c->sv, cl, c->phys);
+  }
   hipLaunchKernelGGL(k_cl_grad, dim3((unsigned)((npt + 255) / 256)), dim3(256), 0, st, cl, c->yc,
                      (float)(0.5 / c->dxc), (float)(0.5 / c->dzc), c->phys, c->grad);
   CCHK(c, hipGetLastError());
@@ -404,8 +562,14 @@ int lh_closure_apply(lh_closure* c) {
     hipLaunchKernelGGL(k_cl_smag, dim3((unsigned)((npt + 255) / 256)), dim3(256), 0, st, cl, c->yc, c->dxc, c->dzc,
                        c->cs2, c->grad, c->y);
   }
-  hipLaunchKernelGGL(k_cl_tau, dim3(cl.ny, kNStress), dim3(256), c->lds_tau, st, cl, c->y, c->stdz + 2 * nin,
-                     c->stdz + 2 * nin + kNStress, c->T);
+  const float *ym = c->stdz + 2 * nin, *ysd = c->stdz + 2 * nin + kNStress;
+  switch (c->fft_n) {
+    case 32: launch_fft_tau<32>(c, st, ym, ysd); break;
+    case 48: launch_fft_tau<48>(c, st, ym, ysd); break;
+    case 64: launch_fft_tau<64>(c, st, ym, ysd); break;
+    case 96: launch_fft_tau<96>(c, st, ym, ysd); break;
+    default: hipLaunchKernelGGL(k_cl_tau, dim3(cl.ny, kNStress), dim3(256), c->lds_tau, st, cl, c->y, ym, ysd, c->T);
+  }
   hipLaunchKernelGGL(k_cl_div, dim3((unsigned)(((long long)cl.ncol * cl.ny + 255) / 256)), dim3(256), 0, st,
                      c->sv, cl, c->yc, c->T);
   CCHK(c, hipGetLastError());
@@ -421,7 +585,7 @@ int lh_closure_set_mean_only(lh_closure* c, int on) {
 int lh_closure_destroy(lh_closure* c) {
   if (!c) return LH_OK;
   if (c->sv.st) (void)hipStreamSynchronize(c->sv.st);
-  void* ptrs[] = {c->yc, c->shift, c->tw, c->sf, c->stdz, c->phys, c->grad, c->y, c->T};
+  void* ptrs[] = {c->yc, c->shift, c->tw, c->twf, c->sf, c->stdz, c->phys, c->grad, c->y, c->T};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
   delete c;

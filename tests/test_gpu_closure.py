@@ -20,14 +20,17 @@ def _data(rng, n):
     return x, y
 
 
-def test_gradient_matches_oracle(require_gpu):
+# (hidden width, batch): the products step through 32 rows or columns at a time; width 48
+# and batch 272 (odd multiples of 16) also take the last 16-wide step of every product
+@pytest.mark.parametrize("nhid,batch", [(NHID, 256), (48, 272)])
+def test_gradient_matches_oracle(require_gpu, nhid, batch):
     from littleharsh_amd.mlp import MLP
     rng = np.random.default_rng(11)
-    net = MLP(NIN, NHID, NLAY, NOUT, batch=256, seed=3)
-    x, y = _data(rng, 256)
+    net = MLP(NIN, nhid, NLAY, NOUT, batch=batch, seed=3)
+    x, y = _data(rng, batch)
     loss, g = net.loss_grad(x, y)
     p = net.get_params().astype(np.float64)
-    shp = omlp.shapes(NIN, NHID, NLAY, NOUT)
+    shp = omlp.shapes(NIN, nhid, NLAY, NOUT)
     lo, go = omlp.loss_grad(p, x.astype(np.float64), y.astype(np.float64), shp, bf16_ops=True)
     print(f"loss gpu {loss:.6e} oracle {lo:.6e}; grad rel err "
           f"{np.linalg.norm(g - go) / np.linalg.norm(go):.3e}", flush=True)
