@@ -203,7 +203,8 @@ int lh_mlp_predict(lh_mlp* m, const float* x, long long n, float* y);
  * inputs gathered on the device: the nch fields g (nch x ny x nz x nx) on the 3x3x3
  * neighbourhood (feature ((dj+1) 9 + (dk+1) 3 + (di+1)) nch + c, j clamped, k and i
  * periodic), then the extra field ex (ny x nz x nx) at the point when nin = 27 nch + 1
- * (ex NULL: nin = 27 nch).  Input f is standardised as (v - mean[f]) / scale[f].  The coarse
+ * (ex NULL: nin = 27 nch).  Input f is standardised as (v - mean[f]) * (1 / scale[f]), the
+ * reciprocal rounded to float (as numpy's float32 (v - mean) * (1 / scale)).  The coarse
  * channel evaluates the closure this way from its nine velocity gradients and the solid
  * fraction. */
 int lh_mlp_predict_grid(lh_mlp* h, const float* g, int nch, const float* ex, int ny, int nz, int nx,

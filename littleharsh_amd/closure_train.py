@@ -30,7 +30,7 @@ from .channel import config_production, config_small
 from .closure import NIN, Filter, scores
 from .closure_les import Coarse, DeviceClosure, coarse_setup, compare, plane_stats
 from .initial import laminar_plus_modes
-from .mlp import MLP
+from .mlp import MLP, standardise
 from .solver import Channel, LHError
 
 
@@ -203,7 +203,7 @@ def main():
     ym, ysd = ytr.mean(axis=0), ytr.std(axis=0)
     ysd[ysd < 1e-30] = 1.0
     net = MLP(NIN, 64, 4, 6, batch=args.batch, seed=7)
-    net.set_data((xtr - xm) / xsd, (ytr - ym) / ysd)
+    net.set_data(standardise(xtr, xm, xsd), (ytr - ym) / ysd)   # as the coupling does
     t1 = time.perf_counter()
     done, chunk = 0, 1000
     while done < args.steps:
@@ -213,7 +213,7 @@ def main():
         log(f"# training step {done}: batch loss {loss:.4e}")
     t_train = time.perf_counter() - t1
 
-    pred = net.predict((xte - xm) / xsd) * ysd + ym
+    pred = net.predict(standardise(xte, xm, xsd)) * ysd + ym
     dtr = (ytr - ym).ravel().astype(np.float64)
     btv = btr.ravel()
     cs2 = float(np.dot(btv, dtr) / max(np.dot(btv, btv), 1e-300))

@@ -454,7 +454,9 @@ static int closure_setup(lh_closure* c, lh_handle* h, const double* yc, double d
   CCHK(c, hipMemcpy(c->tw, ht.data(), ht.size() * sizeof(float2), hipMemcpyHostToDevice));
   if (nin > 0) {
     CCHK(c, hipMemcpy(c->stdz, xmean, nin * sizeof(float), hipMemcpyHostToDevice));
-    CCHK(c, hipMemcpy(c->stdz + nin, xscale, nin * sizeof(float), hipMemcpyHostToDevice));
+    std::vector<float> rs(nin);                     // the network kernels take 1 / scale
+    for (int f = 0; f < nin; ++f) rs[f] = 1.0f / xscale[f];
+    CCHK(c, hipMemcpy(c->stdz + nin, rs.data(), nin * sizeof(float), hipMemcpyHostToDevice));
   }
   CCHK(c, hipMemcpy(c->stdz + 2 * nin, ymean, kNStress * sizeof(float), hipMemcpyHostToDevice));
   CCHK(c, hipMemcpy(c->stdz + 2 * nin + kNStress, yscale, kNStress * sizeof(float), hipMemcpyHostToDevice));

@@ -21,8 +21,9 @@ struct LhSolverView {
 int lh_solver_view(lh_handle* h, LhSolverView* v);
 
 // Outputs (n x nout, device) of the network at every point of an ny x nz x nx grid from the
-// nch device fields g and the device extra field ex (or null), standardised with the device
-// arrays mean / scale; launched on stream st.  nin must be 27 nch (+1 with ex).
+// nch device fields g and the device extra field ex (or null), standardised as
+// (v - mean[f]) * rscale[f] with the device arrays mean and rscale (the reciprocals of the
+// scales, rounded to float); launched on stream st.  nin must be 27 nch (+1 with ex).
 int lh_mlp_eval_grid_dev(lh_mlp* h, const float* g, int nch, const float* ex, int ny, int nz, int nx,
-                         const float* mean, const float* scale, float* y, hipStream_t st);
+                         const float* mean, const float* rscale, float* y, hipStream_t st);
 int lh_mlp_dims(const lh_mlp* h, int* nin, int* nout);

@@ -39,20 +39,28 @@ __device__ inline bf8 ld8(const uint16_t* p) { return __builtin_bit_cast(bf8, *r
 // [j = l % 16]; D as in the 16-wide instruction
 __device__ inline f4 mfma32(bf8 a, bf8 b, f4 c) { return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0); }
 
-// round to nearest even
-__device__ __host__ inline uint16_t f2bf(float f) {
-  union { float f; uint32_t u; } c;
-  c.f = f;
-  const uint32_t u = c.u;
-  if ((u & 0x7f800000u) == 0x7f800000u) return (uint16_t)(u >> 16);   // inf / nan
-  return (uint16_t)((u + 0x7fffu + ((u >> 16) & 1u)) >> 16);
-}
+// round to nearest even (gfx950's v_cvt_pk_bf16_f32)
+__device__ inline uint16_t f2bf(float f) { return __builtin_bit_cast(uint16_t, (__bf16)f); }
 __device__ __host__ inline float bf2f(uint16_t b) {
   union { float f; uint32_t u; } c;
   c.u = (uint32_t)b << 16;
   return c.f;
 }
 __host__ __device__ inline int pad16(int n) { return (n + 15) & ~15; }
+
+// The activation, tanh, from the exponential: tanh|x| = (1 - t) / (1 + t) with t = exp(-2|x|)
+// (hardware exp2 and reciprocal), and below |x| = 1/16, where 1 - t cancels, the Taylor series
+// x - x^3/3 + 2 x^5/15 - 17 x^7/315 (truncation below 1e-12).  Relative error about 1e-6, far
+// below the bf16 rounding the activations take next; the forward pass, its derivative and
+// inference all use this one function.
+__device__ inline float tanh_act(float x) {
+  const float ax = fabsf(x);
+  const float t = __expf(-2.f * ax);
+  const float big = copysignf((1.f - t) * __builtin_amdgcn_rcpf(1.f + t), x);
+  const float x2 = x * x;
+  const float small = x + x * x2 * (-1.f / 3.f + x2 * (2.f / 15.f - x2 * (17.f / 315.f)));
+  return ax < 0.0625f ? small : big;
+}
 
 // counter-based hash (SplitMix64 finaliser)
 __device__ __host__ inline uint64_t mix64(uint64_t z) {
@@ -88,7 +96,7 @@ __global__ void __launch_bounds__(64) k_fwd(const uint16_t* __restrict__ X, cons
     const int r = b0 + 4 * (l / 16) + q;
     const float z = acc[q] + bias[n];
     Z[(size_t)r * N + n] = z;
-    if (act) Xn[(size_t)r * N + n] = f2bf(tanhf(z));
+    if (act) Xn[(size_t)r * N + n] = f2bf(tanh_act(z));
   }
 }
 
@@ -164,7 +172,7 @@ __global__ void __launch_bounds__(64) k_dx(const uint16_t* __restrict__ dZ, cons
 #pragma unroll
   for (int q = 0; q < 4; ++q) {
     const size_t o = (size_t)(b0 + 4 * (l / 16) + q) * K + k0 + l % 16;
-    const float th = tanhf(Zp[o]);
+    const float th = tanh_act(Zp[o]);
     const float gq = acc[q] * (1.f - th * th);
     dZo[o] = gq;
     dZbo[o] = f2bf(gq);
@@ -239,9 +247,10 @@ __global__ void k_pack_x(const float* __restrict__ x, int nrow, int nin, int K0,
 // inputs of grid points r0 .. r0 + nrow - 1 (point q = (j nz + k) nx + i): the nch fields g on
 // the 3x3x3 neighbourhood, feature ((dj + 1) 9 + (dk + 1) 3 + (di + 1)) nch + c, j clamped to
 // 0..ny-1, k and i periodic; then the extra field ex at the point itself when nin = 27 nch + 1.
-// Each input is standardised, (v - mean[f]) / scale[f], and rounded to bf16 (zero pad).
+// Each input is standardised, (v - mean[f]) * rscale[f] with rscale = 1 / scale rounded to
+// float, and rounded to bf16 (zero pad).
 __global__ void k_pack_grid(const float* __restrict__ g, const float* __restrict__ ex,
-                            const float* __restrict__ mean, const float* __restrict__ scale, int nch, int ny,
+                            const float* __restrict__ mean, const float* __restrict__ rscale, int nch, int ny,
                             int nz, int nx, long long r0, int nrow, int nin, int K0, uint16_t* __restrict__ X) {
   const int r = blockIdx.x;
   const long long q = r0 + r;
@@ -265,7 +274,7 @@ __global__ void k_pack_grid(const float* __restrict__ g, const float* __restrict
       } else {
         v = ex[(size_t)j * plane + (size_t)k * nx + i];
       }
-      out = f2bf((v - mean[f]) / scale[f]);
+      out = f2bf((v - mean[f]) * rscale[f]);
     }
     X[(size_t)r * K0 + f] = out;
   }
@@ -289,7 +298,7 @@ __global__ void __launch_bounds__(64 * kFuseWaves) k_mlp_grid(FusedNet fn, const
                                                               const float* __restrict__ p, const float* __restrict__ g,
                                                               const float* __restrict__ ex,
                                                               const float* __restrict__ mean,
-                                                              const float* __restrict__ scale, int nch, int ny, int nz,
+                                                              const float* __restrict__ rscale, int nch, int ny, int nz,
                                                               int nx, long long n, float* __restrict__ y) {
   extern __shared__ __align__(16) unsigned char smem[];
   uint16_t* Wl = reinterpret_cast<uint16_t*>(smem);
@@ -302,7 +311,7 @@ __global__ void __launch_bounds__(64 * kFuseWaves) k_mlp_grid(FusedNet fn, const
   for (int e = threadIdx.x; e < fn.np; e += blockDim.x) Wl[e] = wb[e];
   for (int e = threadIdx.x; e < K0; e += blockDim.x) {
     ms[e] = e < fn.nin ? mean[e] : 0.f;
-    ms[K0 + e] = e < fn.nin ? scale[e] : 1.f;
+    ms[K0 + e] = e < fn.nin ? rscale[e] : 1.f;
     int d = -1;                                    // padding
     if (e < 27 * nch) {
       const int c = e % nch, s = e / nch;
@@ -360,7 +369,7 @@ __global__ void __launch_bounds__(64 * kFuseWaves) k_mlp_grid(FusedNet fn, const
 #pragma unroll
           for (int qq = 0; qq < 8; ++qq) {
             const int f = c * 32 + 8 * (l / 16) + qq;
-            a[qq] = (short)((live && f < fn.nin) ? f2bf((v[c][qq] - ms[f]) / ms[K0 + f]) : (uint16_t)0);
+            a[qq] = (short)((live && f < fn.nin) ? f2bf((v[c][qq] - ms[f]) * ms[K0 + f]) : (uint16_t)0);
           }
 #pragma unroll
           for (int t = 0; t < 4; ++t)
@@ -392,7 +401,7 @@ __global__ void __launch_bounds__(64 * kFuseWaves) k_mlp_grid(FusedNet fn, const
           const int nn = t * 16 + l % 16;
 #pragma unroll
           for (int qq = 0; qq < 4; ++qq)
-            act[(4 * (l / 16) + qq) * 64 + nn] = f2bf(tanhf(acc[t][qq] + bias[nn]));
+            act[(4 * (l / 16) + qq) * 64 + nn] = f2bf(tanh_act(acc[t][qq] + bias[nn]));
         }
       // next layer: A from the tile (row l % 16, eight contiguous k per 32-wide chunk)
       const int K = fn.Kp[layer + 1], nnx = fn.Np[layer + 1] / 16;
@@ -428,14 +437,14 @@ __global__ void __launch_bounds__(64 * kFuseWaves) k_mlp_grid_s(FusedNet fn, con
                                                                 const float* __restrict__ g,
                                                                 const float* __restrict__ ex,
                                                                 const float* __restrict__ mean,
-                                                                const float* __restrict__ scale, int nch, int ny,
+                                                                const float* __restrict__ rscale, int nch, int ny,
                                                                 int nz, int nx, long long n, float* __restrict__ y) {
   static_assert(K0 % 32 == 0 && NH % 32 == 0, "32-wide matrix-core steps");
   constexpr int NO = 16;
   constexpr int WH = NH * K0, WO = WH + NHH * NH * NH, WN = WO + NO * NH;   // LDS weight offsets
   extern __shared__ __align__(16) unsigned char smem[];
   uint16_t* Wl = reinterpret_cast<uint16_t*>(smem);
-  float* ms = reinterpret_cast<float*>(smem + ((2 * WN + 15) & ~15));   // mean[K0], scale[K0]
+  float* ms = reinterpret_cast<float*>(smem + ((2 * WN + 15) & ~15));   // mean[K0], rscale[K0]
   int* offs = reinterpret_cast<int*>(ms + 2 * K0);                      // interior offset per input
   int* fdec = offs + K0;                                                // channel / stencil code
   const int wave = threadIdx.x >> 6, l = threadIdx.x & 63, li = l & 15, lg = l >> 4;
@@ -450,7 +459,7 @@ __global__ void __launch_bounds__(64 * kFuseWaves) k_mlp_grid_s(FusedNet fn, con
   }
   for (int e = threadIdx.x; e < K0; e += blockDim.x) {
     ms[e] = e < fn.nin ? mean[e] : 0.f;
-    ms[K0 + e] = e < fn.nin ? scale[e] : 1.f;
+    ms[K0 + e] = e < fn.nin ? rscale[e] : 1.f;
     int d = -1, o = kOffPad;
     if (e < 27 * nch) {
       const int c = e % nch, s = e / nch;
@@ -478,14 +487,18 @@ __global__ void __launch_bounds__(64 * kFuseWaves) k_mlp_grid_s(FusedNet fn, con
     const int pidx = j * plane + k * nx + i;
     float v[K0 / 32][8];
     if (inner) {
+      // no branches: every input loads (a sentinel loads the point itself) and selects
+      const float exv = ex ? ex[pidx] : 0.f;
 #pragma unroll
       for (int c = 0; c < K0 / 32; ++c) {
         const int4 oa = *reinterpret_cast<const int4*>(offs + c * 32 + 8 * lg);
         const int4 ob = *reinterpret_cast<const int4*>(offs + c * 32 + 8 * lg + 4);
         const int oo[8] = {oa.x, oa.y, oa.z, oa.w, ob.x, ob.y, ob.z, ob.w};
 #pragma unroll
-        for (int qq = 0; qq < 8; ++qq)
-          v[c][qq] = oo[qq] > kOffEx ? g[pidx + oo[qq]] : (oo[qq] == kOffEx ? ex[pidx] : 0.f);
+        for (int qq = 0; qq < 8; ++qq) {
+          const float gv = g[pidx + (oo[qq] > kOffEx ? oo[qq] : 0)];
+          v[c][qq] = oo[qq] > kOffEx ? gv : (oo[qq] == kOffEx ? exv : 0.f);
+        }
       }
     } else {
 #pragma unroll
@@ -521,7 +534,7 @@ __global__ void __launch_bounds__(64 * kFuseWaves) k_mlp_grid_s(FusedNet fn, con
 #pragma unroll
       for (int qq = 0; qq < 8; ++qq) {
         const int f = c * 32 + 8 * lg + qq;
-        a[qq] = (short)((live && f < fn.nin) ? f2bf((v[c][qq] - mv[qq]) / sv[qq]) : (uint16_t)0);
+        a[qq] = (short)((live && f < fn.nin) ? f2bf((v[c][qq] - mv[qq]) * sv[qq]) : (uint16_t)0);
       }
 #pragma unroll
       for (int t = 0; t < NH / 16; ++t) acc[t] = mfma32(__builtin_bit_cast(bf8, a), ld8(w0 + t * 16 * K0 + c * 32), acc[t]);
@@ -535,7 +548,7 @@ __global__ void __launch_bounds__(64 * kFuseWaves) k_mlp_grid_s(FusedNet fn, con
         const float bb = bias[t * 16 + li];
 #pragma unroll
         for (int qq = 0; qq < 4; ++qq)
-          act[(4 * lg + qq) * NH + t * 16 + li] = f2bf(tanhf(acc[t][qq] + bb));
+          act[(4 * lg + qq) * NH + t * 16 + li] = f2bf(tanh_act(acc[t][qq] + bb));
       }
       bf8 a[NH / 32];
 #pragma unroll
@@ -931,7 +944,8 @@ int lh_mlp_predict(lh_mlp* h, const float* x, long long n, float* y) {
 
 // outputs at every point of an ny x nz x nx grid from stencil inputs gathered on the device
 // (k_pack_grid): g holds nch fields (nch x ny x nz x nx), ex the extra field (ny x nz x nx) or
-// null, mean / scale the nin standardisation constants; y receives ny nz nx rows of nout
+// null, mean / scale the nin standardisation constants (input f becomes (v - mean[f]) *
+// (1 / scale[f]), the reciprocal rounded to float); y receives ny nz nx rows of nout
 int lh_mlp_predict_grid(lh_mlp* h, const float* g, int nch, const float* ex, int ny, int nz, int nx,
                         const float* mean, const float* scale, float* y) {
   if (!h || !g || !mean || !scale || !y || nch < 1 || ny < 1 || nz < 1 || nx < 1)
@@ -952,7 +966,9 @@ int lh_mlp_predict_grid(lh_mlp* h, const float* g, int nch, const float* ex, int
   if (e == hipSuccess) e = hipMemcpy(dg, g, field * nch * sizeof(float), hipMemcpyHostToDevice);
   if (e == hipSuccess && ex) e = hipMemcpy(dex, ex, field * sizeof(float), hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipMemcpy(dms, mean, h->nin * sizeof(float), hipMemcpyHostToDevice);
-  if (e == hipSuccess) e = hipMemcpy(dms + h->nin, scale, h->nin * sizeof(float), hipMemcpyHostToDevice);
+  std::vector<float> rs(h->nin);
+  for (int f = 0; f < h->nin; ++f) rs[f] = 1.0f / scale[f];
+  if (e == hipSuccess) e = hipMemcpy(dms + h->nin, rs.data(), h->nin * sizeof(float), hipMemcpyHostToDevice);
   if (e != hipSuccess) {
     cleanup();
     return mfail(h, LH_ERR_HIP, std::string("lh_mlp_predict_grid: ") + hipGetErrorString(e));
@@ -980,8 +996,8 @@ int lh_mlp_dims(const lh_mlp* h, int* nin, int* nout) {
 }
 
 int lh_mlp_eval_grid_dev(lh_mlp* h, const float* g, int nch, const float* ex, int ny, int nz, int nx,
-                         const float* mean, const float* scale, float* y, hipStream_t st) {
-  if (!h || !g || !mean || !scale || !y || nch < 1 || ny < 1 || nz < 1 || nx < 1)
+                         const float* mean, const float* rscale, float* y, hipStream_t st) {
+  if (!h || !g || !mean || !rscale || !y || nch < 1 || ny < 1 || nz < 1 || nx < 1)
     return mfail(h, LH_ERR_ARG, "lh_mlp_eval_grid: bad argument");
   if (h->nin != 27 * nch + (ex ? 1 : 0))
     return mfail(h, LH_ERR_ARG, "lh_mlp_eval_grid: nin must be 27 nch (+1 with the extra field)");
@@ -1014,7 +1030,7 @@ int lh_mlp_eval_grid_dev(lh_mlp* h, const float* g, int nch, const float* ex, in
     const long long tiles = (n + 15) / 16;
     const int blocks = (int)std::min<long long>((tiles + kFuseWaves - 1) / kFuseWaves, 2LL * h->ncu_s);
     hipLaunchKernelGGL((k_mlp_grid_s<256, 64, 3>), dim3(blocks), dim3(64 * kFuseWaves), lds_s, st, fn, h->wb, h->p,
-                       g, ex, mean, scale, nch, ny, nz, nx, n, y);
+                       g, ex, mean, rscale, nch, ny, nz, nx, n, y);
     MCHK(h, hipGetLastError());
     return LH_OK;
   }
@@ -1046,7 +1062,7 @@ int lh_mlp_eval_grid_dev(lh_mlp* h, const float* g, int nch, const float* ex, in
     const long long n = (long long)ny * nz * nx;
     const long long tiles = (n + 15) / 16;
     const int blocks = (int)std::min<long long>((tiles + kFuseWaves - 1) / kFuseWaves, 2LL * h->ncu);
-    hipLaunchKernelGGL(k_mlp_grid, dim3(blocks), dim3(64 * kFuseWaves), lds, st, fn, h->wb, h->p, g, ex, mean, scale, nch,
+    hipLaunchKernelGGL(k_mlp_grid, dim3(blocks), dim3(64 * kFuseWaves), lds, st, fn, h->wb, h->p, g, ex, mean, rscale, nch,
                        ny, nz, nx, n, y);
     MCHK(h, hipGetLastError());
     return LH_OK;
@@ -1066,7 +1082,7 @@ int lh_mlp_eval_grid_dev(lh_mlp* h, const float* g, int nch, const float* ex, in
   for (long long r0 = 0; r0 < n; r0 += h->Be) {
     const int nr = (int)std::min<long long>(h->Be, n - r0);
     const int rows = (nr + 15) & ~15;
-    hipLaunchKernelGGL(k_pack_grid, dim3(rows), dim3(64), 0, st, g, ex, mean, scale, nch, ny, nz, nx, r0, nr,
+    hipLaunchKernelGGL(k_pack_grid, dim3(rows), dim3(64), 0, st, g, ex, mean, rscale, nch, ny, nz, nx, r0, nr,
                        h->nin, h->Kp[0], h->Xe[0]);
     forward_rows(h, rows, h->Xe.data(), h->Ze.data(), st);
     hipLaunchKernelGGL(k_unpack_out, dim3((nr * h->nout + 255) / 256), dim3(256), 0, st, h->Ze[h->L - 1],

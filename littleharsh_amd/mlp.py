@@ -22,6 +22,14 @@ def _p(a: np.ndarray):
     return a.ctypes.data_as(_fp)
 
 
+def standardise(x, mean, scale) -> np.ndarray:
+    """The network's input standardisation, in float32: (x - mean) * (1 / scale), the
+    reciprocal rounded to float32.  The device-side gathers (predict_grid, the closure
+    coupling) compute the same operations, so rows standardised here match theirs exactly."""
+    r = np.float32(1.0) / _f32(scale)
+    return (_f32(x) - _f32(mean)) * r
+
+
 class MLP:
     """nin -> nhid (tanh) x nlayers -> nout (linear), batch rows per step."""
 
@@ -90,7 +98,8 @@ class MLP:
 
     def predict_grid(self, g, extra, mean, scale) -> np.ndarray:
         """Outputs (ny, nz, nx, nout) from the nch fields g (nch, ny, nz, nx) gathered on the
-        3x3x3 neighbourhood on the device, plus the field `extra` (ny, nz, nx) or None."""
+        3x3x3 neighbourhood on the device, plus the field `extra` (ny, nz, nx) or None, each
+        input standardised as standardise(v, mean, scale)."""
         g = _f32(g)
         nch, ny, nz, nx = g.shape
         ex = None if extra is None else _f32(extra)

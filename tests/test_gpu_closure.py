@@ -128,7 +128,7 @@ def test_predict_grid_matches_stencil_rows(require_gpu, ny, nz, nx):
     nx = 7 runs the general fused kernel; nx = 48 (a multiple of 16) the one with the closure
     network's widths compiled in, whose tiles have an interior and a boundary path."""
     from littleharsh_amd.closure import stencil_features
-    from littleharsh_amd.mlp import MLP
+    from littleharsh_amd.mlp import MLP, standardise
     rng = np.random.default_rng(16)
     net = MLP(NIN, NHID, NLAY, NOUT, batch=64, seed=8)
     g = rng.normal(size=(9, ny, nz, nx)).astype(np.float32)
@@ -136,7 +136,7 @@ def test_predict_grid_matches_stencil_rows(require_gpu, ny, nz, nx):
     mean = (0.1 * rng.normal(size=NIN)).astype(np.float32)
     scale = (1.0 + rng.random(NIN)).astype(np.float32)
     m, kz, ix = (a.ravel() for a in np.meshgrid(np.arange(ny), np.arange(nz), np.arange(nx), indexing="ij"))
-    ref = net.predict((stencil_features(g, sf, m, kz, ix) - mean) / scale)
+    ref = net.predict(standardise(stencil_features(g, sf, m, kz, ix), mean, scale))
     out = net.predict_grid(g, sf, mean, scale).reshape(-1, NOUT)
     same = np.mean(np.all(out == ref, axis=1))
     print(f"rows identical: {same:.3f}; max diff {np.abs(out - ref).max():.3e}", flush=True)
