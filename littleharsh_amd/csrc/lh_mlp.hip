@@ -445,8 +445,10 @@ __global__ void __launch_bounds__(64 * kFuseWaves) k_mlp_grid_s(FusedNet fn, con
   extern __shared__ __align__(16) unsigned char smem[];
   uint16_t* Wl = reinterpret_cast<uint16_t*>(smem);
   float* ms = reinterpret_cast<float*>(smem + ((2 * WN + 15) & ~15));   // mean[K0], rscale[K0]
-  int* offs = reinterpret_cast<int*>(ms + 2 * K0);                      // interior offset per input
-  int* fdec = offs + K0;                                                // channel / stencil code
+  // interior offset per input, three tables: as is, and with the x neighbour wrapped for the
+  // first (i = 0: dx = -1 reads x = nx - 1) and the last point of a row (dx = +1 reads x = 0)
+  int* offs = reinterpret_cast<int*>(ms + 2 * K0);
+  int* fdec = offs + 3 * K0;                                            // channel / stencil code
   const int wave = threadIdx.x >> 6, l = threadIdx.x & 63, li = l & 15, lg = l >> 4;
   uint16_t* act = reinterpret_cast<uint16_t*>(fdec + K0) + wave * 16 * NH;
   const int plane = nz * nx, field = plane * ny;
@@ -460,17 +462,21 @@ __global__ void __launch_bounds__(64 * kFuseWaves) k_mlp_grid_s(FusedNet fn, con
   for (int e = threadIdx.x; e < K0; e += blockDim.x) {
     ms[e] = e < fn.nin ? mean[e] : 0.f;
     ms[K0 + e] = e < fn.nin ? rscale[e] : 1.f;
-    int d = -1, o = kOffPad;
+    int d = -1, o = kOffPad, ol = kOffPad, orr = kOffPad;
     if (e < 27 * nch) {
-      const int c = e % nch, s = e / nch;
+      const int c = e % nch, s = e / nch, dx = s % 3 - 1;
       d = c | ((s / 9) << 8) | (((s / 3) % 3) << 10) | ((s % 3) << 12);
-      o = c * field + (s / 9 - 1) * plane + ((s / 3) % 3 - 1) * nx + (s % 3 - 1);
+      o = c * field + (s / 9 - 1) * plane + ((s / 3) % 3 - 1) * nx + dx;
+      ol = o + (dx == -1 ? nx : 0);
+      orr = o - (dx == 1 ? nx : 0);
     } else if (e < fn.nin) {
       d = -2;
-      o = kOffEx;
+      o = ol = orr = kOffEx;
     }
     fdec[e] = d;
     offs[e] = o;
+    offs[K0 + e] = ol;
+    offs[2 * K0 + e] = orr;
   }
   __syncthreads();
   const uint16_t* w0 = Wl + li * K0 + 8 * lg;        // B operands: lane base + constants
@@ -483,16 +489,19 @@ __global__ void __launch_bounds__(64 * kFuseWaves) k_mlp_grid_s(FusedNet fn, con
     const int i0 = (int)(r0 % nx), k = (int)((r0 / nx) % nz), j = (int)(r0 / ((long long)nx * nz));
     const int i = i0 + li;
     const bool live = r0 + li < n;
-    const bool inner = r0 + 16 <= n && i0 > 0 && i0 + 16 < nx && k > 0 && k + 1 < nz && j > 0 && j + 1 < ny;
+    // full tiles off the y end planes and the z end rows (a row of nx % 16 == 0 points holds
+    // whole tiles; its end points take the wrapped tables)
+    const bool inner = r0 + 16 <= n && k > 0 && k + 1 < nz && j > 0 && j + 1 < ny;
     const int pidx = j * plane + k * nx + i;
     float v[K0 / 32][8];
     if (inner) {
       // no branches: every input loads (a sentinel loads the point itself) and selects
       const float exv = ex ? ex[pidx] : 0.f;
+      const int* ot = offs + (i == 0 ? K0 : (i == nx - 1 ? 2 * K0 : 0));
 #pragma unroll
       for (int c = 0; c < K0 / 32; ++c) {
-        const int4 oa = *reinterpret_cast<const int4*>(offs + c * 32 + 8 * lg);
-        const int4 ob = *reinterpret_cast<const int4*>(offs + c * 32 + 8 * lg + 4);
+        const int4 oa = *reinterpret_cast<const int4*>(ot + c * 32 + 8 * lg);
+        const int4 ob = *reinterpret_cast<const int4*>(ot + c * 32 + 8 * lg + 4);
         const int oo[8] = {oa.x, oa.y, oa.z, oa.w, ob.x, ob.y, ob.z, ob.w};
 #pragma unroll
         for (int qq = 0; qq < 8; ++qq) {
@@ -1018,7 +1027,7 @@ int lh_mlp_eval_grid_dev(lh_mlp* h, const float* g, int nch, const float* ex, in
       fn.offB[l] = h->offB[l];
     }
     constexpr int WN = 64 * 256 + 3 * 64 * 64 + 16 * 64;
-    const size_t lds_s = (size_t)((2 * WN + 15) & ~15) + 4 * 256 * sizeof(int) + kFuseWaves * 16 * 64 * 2;
+    const size_t lds_s = (size_t)((2 * WN + 15) & ~15) + 6 * 256 * sizeof(int) + kFuseWaves * 16 * 64 * 2;
     if (!h->ncu_s) {
       int dev = 0;
       MCHK(h, hipGetDevice(&dev));
