@@ -378,6 +378,7 @@ struct lh_closure {
   float2* T = nullptr;
   size_t lds_sample = 0, lds_tau = 0;
   int fft_n = 0;                // N of the FFT kernels (nx = nz = N), 0: the DFT kernels
+  bool ready = false;           // set when creation completed; apply refuses a handle without it
   float2* twf = nullptr;        // exp(-2 pi i m / N) for the FFT kernels
   std::string err;
 };
@@ -523,6 +524,7 @@ int lh_closure_create(lh_handle* h, lh_mlp* net, const double* yc, double dxc, d
   const size_t npt = (size_t)c->c.ny * c->c.nz * c->c.nx;
   CCHK(c, hipMalloc((void**)&c->sf, npt * sizeof(float)));
   CCHK(c, hipMemcpy(c->sf, sf, npt * sizeof(float), hipMemcpyHostToDevice));
+  c->ready = true;
   return LH_OK;
 }
 
@@ -536,11 +538,14 @@ int lh_closure_create_smagorinsky(lh_handle* h, const double* yc, double dxc, do
     return cfail(c, LH_ERR_ARG, "lh_closure_create_smagorinsky: null argument or non-positive spacing");
   c->cs2 = cs2;
   const float zero[kNStress] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f}, one[kNStress] = {1.f, 1.f, 1.f, 1.f, 1.f, 1.f};
-  return closure_setup(c, h, yc, dxc, dzc, shift_x, shift_z, 0, nullptr, nullptr, zero, one);
+  const int r = closure_setup(c, h, yc, dxc, dzc, shift_x, shift_z, 0, nullptr, nullptr, zero, one);
+  c->ready = r == LH_OK;
+  return r;
 }
 
 int lh_closure_apply(lh_closure* c) {
-  if (!c || !c->h) return LH_ERR_ARG;
+  if (!c) return LH_ERR_ARG;
+  if (!c->ready || !c->h) return cfail(c, LH_ERR_ARG, "lh_closure_apply: the closure was not created successfully");
   if (lh_solver_view(c->h, &c->sv) != LH_OK) return cfail(c, LH_ERR_HIP, "lh_closure_apply: channel view failed");
   const CL& cl = c->c;
   const hipStream_t st = c->sv.st;

@@ -64,16 +64,18 @@ def test_device_closure_matches_host_path(require_gpu):
     dc.close()
 
 
-# coarse nx x nz: 96 x 96 and 64 x 64 take the FFT kernels of the spectral stages, 64 x 48
-# (no common length) the direct DFT kernels
-@pytest.mark.parametrize("nxc,nzc", [(96, 96), (64, 64), (64, 48)])
-def test_device_smagorinsky_matches_host_path(require_gpu, nxc, nzc):
-    from littleharsh_amd.channel import config_production
+# fine channel, coarse nx x nz: 96, 64 and 32 (production) and 48 (the small channel, whose
+# post tiling allows it) take the FFT kernels of the spectral stages, one for each length;
+# 64 x 48 (no common length) the direct DFT kernels
+@pytest.mark.parametrize("fine,nxc,nzc", [("production", 96, 96), ("production", 64, 64), ("production", 32, 32),
+                                          ("small", 48, 48), ("production", 64, 48)])
+def test_device_smagorinsky_matches_host_path(require_gpu, fine, nxc, nzc):
+    from littleharsh_amd.channel import config_production, config_small
     from littleharsh_amd.closure import Filter
     from littleharsh_amd.closure_les import Coarse, DeviceClosure, coarse_setup, smagorinsky_tau
     from littleharsh_amd.initial import laminar_plus_noise
     from littleharsh_amd.solver import Channel
-    s = config_production()
+    s = config_production() if fine == "production" else config_small()
     filt = Filter(s, nxc, nzc)
     co = Coarse(coarse_setup(s, nxc, nzc), filt)
     u, v, w, p = laminar_plus_noise(co.c, seed=4)
