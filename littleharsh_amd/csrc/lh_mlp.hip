@@ -106,14 +106,13 @@ __global__ void __launch_bounds__(64) k_fwd(const uint16_t* __restrict__ X, cons
 // One block of kDwWaves waves per 16x16 tile: wave w takes the row tiles w, w + kDwWaves, ...;
 // the partial tiles are summed in LDS in wave order (deterministic).
 constexpr int kDwWaves = 16;
-__global__ void __launch_bounds__(64 * kDwWaves) k_dw(const uint16_t* __restrict__ dZ,
-                                                      const uint16_t* __restrict__ X,
-                                                      float* __restrict__ dW, int B, int K, int N,
-                                                      const float* __restrict__ dZf, float* __restrict__ db) {
+__device__ __forceinline__ void dw_tile(const uint16_t* __restrict__ dZ, const uint16_t* __restrict__ X,
+                                        float* __restrict__ dW, int B, int K, int N, const float* __restrict__ dZf,
+                                        float* __restrict__ db, int bx, int by) {
   __shared__ float part[kDwWaves][256];
   __shared__ float dbp[kDwWaves][64];
   const int l = threadIdx.x & 63, w = threadIdx.x >> 6;
-  const int n0 = blockIdx.x * 16, k0 = blockIdx.y * 16;
+  const int n0 = bx * 16, k0 = by * 16;
   if (k0 >= K) {                                    // bias row: column sums of dZf
     float sb = 0.f;
     for (int r0 = 16 * w; r0 < B; r0 += 16 * kDwWaves)
@@ -148,6 +147,34 @@ __global__ void __launch_bounds__(64 * kDwWaves) k_dw(const uint16_t* __restrict
     for (int v = 0; v < kDwWaves; ++v) s += part[v][threadIdx.x];
     dW[(size_t)(n0 + threadIdx.x / 16) * K + k0 + threadIdx.x % 16] = s;
   }
+}
+__global__ void __launch_bounds__(64 * kDwWaves) k_dw(const uint16_t* __restrict__ dZ,
+                                                      const uint16_t* __restrict__ X,
+                                                      float* __restrict__ dW, int B, int K, int N,
+                                                      const float* __restrict__ dZf, float* __restrict__ db) {
+  dw_tile(dZ, X, dW, B, K, N, dZf, db, blockIdx.x, blockIdx.y);
+}
+
+// The weight gradients of every layer in one launch: block b of layer l is the block
+// (b % (N / 16), b / (N / 16)) of that layer's k_dw grid.  With tstep, block 0 advances the
+// step counter (the step's gather has finished and no block here reads it).
+constexpr int kMaxL = 8;
+struct DwAll {
+  const uint16_t* dZ[kMaxL];
+  const uint16_t* X[kMaxL];
+  float* dW[kMaxL];
+  const float* dZf[kMaxL];
+  float* db[kMaxL];
+  int K[kMaxL], N[kMaxL], first[kMaxL + 1];   // first block of each layer
+  int L, B;
+  int* tstep;
+};
+__global__ void __launch_bounds__(64 * kDwWaves) k_dw_all(DwAll a) {
+  int l = 0;
+  while (l + 1 < a.L && (int)blockIdx.x >= a.first[l + 1]) ++l;
+  const int b = blockIdx.x - a.first[l], nbx = a.N[l] / 16;
+  dw_tile(a.dZ[l], a.X[l], a.dW[l], a.B, a.K[l], a.N[l], a.dZf[l], a.db[l], b % nbx, b / nbx);
+  if (a.tstep && blockIdx.x == 0 && threadIdx.x == 0) a.tstep[0] = a.tstep[0] + 1;
 }
 
 // the previous layer's dZ = (dZ W) * (1 - tanh(Zp)^2) (dZ: B x N bf16, W: N x K bf16, Zp: its
@@ -210,6 +237,146 @@ __global__ void __launch_bounds__(256) k_mse(const float* __restrict__ Y, const 
   if (threadIdx.x == 0) {
     part[blockIdx.x] = red[0];
     if (tstep && blockIdx.x == 0) tstep[0] = tstep[0] + 1;
+  }
+}
+
+// One training pass of 16 batch rows per wave (one wave per block) for the closure network's
+// shape: K0 inputs, NHH + 1 hidden layers of NH (tanh), 16 padded outputs.  It runs the
+// forward pass, the loss terms and the input gradients of every layer with the operands,
+// instructions and order of k_fwd, k_mse and k_dx, so its results are bit-identical to theirs:
+//   forward: acc over k in 32-wide steps per 16x16 output tile, z = acc + b, x = bf16(tanh z);
+//   loss: dZ = 2 (Y - T) / (B nout) and the tile's sum of (Y - T)^2 by k_mse's 256-element
+//   tree (one part per 16-row tile: 256 = 16 rows x 16 outputs);
+//   input gradients: acc over n in 16-wide steps, dZ = acc (1 - tanh(z)^2).
+// A lane keeps the pre-activations it produced (rows 4 (l / 16) + q, column l % 16 of each
+// tile) in registers: the input-gradient epilogue reads them in the same lanes.  Layer inputs
+// and dZ pass to the next layer through the wave's LDS tiles (LDS operations of one wave
+// complete in order) and go to global memory for the weight gradients (k_dw_all).
+struct TrainNet {
+  const uint16_t* w[kMaxL];   // bf16 weights of each layer (N x K)
+  const float* b[kMaxL];      // FP32 biases
+  uint16_t* X[kMaxL];         // layer inputs (bf16, B x K): X[0] read, the others written
+  float* dZf[kMaxL];          // dZ of each layer (B x N), FP32 and bf16
+  uint16_t* dZb[kMaxL];
+  const float* T;             // targets (B x nout)
+  float* part;                // loss partial sums, one per 16-row tile
+  int B, nout;
+};
+template <int K0, int NH, int NHH>
+__global__ void __launch_bounds__(64) k_train_s(TrainNet tn) {
+  static_assert(K0 % 32 == 0 && NH % 32 == 0, "32-wide forward steps without a tail");
+  constexpr int L = NHH + 2, NO = 16, NT = NH / 16;
+  __shared__ __align__(16) uint16_t xt[16 * NH];    // the next layer's input tile
+  __shared__ __align__(16) uint16_t dzt[16 * NH];   // the current dZ tile (row stride N)
+  __shared__ float red[256];
+  const int l = threadIdx.x, li = l & 15, lg = l >> 4;
+  const int b0 = blockIdx.x * 16;
+  f4 zr[NHH + 1][NT];                               // hidden pre-activations
+  // ---- forward: the input layer (A from the batch rows in X[0])
+  {
+    const uint16_t* xr = tn.X[0] + (size_t)(b0 + li) * K0;
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+      const uint16_t* wr = tn.w[0] + (size_t)(t * 16 + li) * K0;
+      f4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int k0 = 0; k0 < K0; k0 += 32) acc = mfma32(ld8(xr + k0 + 8 * lg), ld8(wr + k0 + 8 * lg), acc);
+      zr[0][t] = acc;
+    }
+  }
+  // ---- forward: hidden epilogue h, then layer h + 1 on the tile it wrote
+#pragma unroll
+  for (int h = 0; h <= NHH; ++h) {
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+      const float bb = tn.b[h][t * 16 + li];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const float z = zr[h][t][q] + bb;
+        zr[h][t][q] = z;
+        const uint16_t x = f2bf(tanh_act(z));
+        xt[(4 * lg + q) * NH + t * 16 + li] = x;
+        tn.X[h + 1][(size_t)(b0 + 4 * lg + q) * NH + t * 16 + li] = x;
+      }
+    }
+    bf8 a[NH / 32];
+#pragma unroll
+    for (int c = 0; c < NH / 32; ++c) a[c] = ld8(xt + li * NH + c * 32 + 8 * lg);
+    if (h < NHH) {
+#pragma unroll
+      for (int t = 0; t < NT; ++t) {
+        const uint16_t* wr = tn.w[h + 1] + (size_t)(t * 16 + li) * NH;
+        f4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int c = 0; c < NH / 32; ++c) acc = mfma32(a[c], ld8(wr + c * 32 + 8 * lg), acc);
+        zr[h + 1][t] = acc;
+      }
+    } else {
+      // ---- output layer and the loss terms
+      const uint16_t* wr = tn.w[L - 1] + (size_t)li * NH;
+      f4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int c = 0; c < NH / 32; ++c) acc = mfma32(a[c], ld8(wr + c * 32 + 8 * lg), acc);
+      const float bo = tn.b[L - 1][li];
+      const float sc = 2.f / (float)(tn.B * tn.nout);
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int r = b0 + 4 * lg + q;
+        const float y = acc[q] + bo;
+        float d = 0.f, s = 0.f;
+        if (li < tn.nout) {
+          d = y - tn.T[(size_t)r * tn.nout + li];
+          s = d * d;
+        }
+        const size_t o = (size_t)r * NO + li;
+        tn.dZf[L - 1][o] = sc * d;
+        const uint16_t db = f2bf(sc * d);
+        tn.dZb[L - 1][o] = db;
+        dzt[(4 * lg + q) * NO + li] = db;
+        red[(4 * lg + q) * 16 + li] = s;
+      }
+    }
+  }
+  // the tile's loss partial: k_mse's tree (red[i] += red[i + w], w = 128 .. 1)
+  red[l] += red[l + 128];
+  red[l + 64] += red[l + 192];
+#pragma unroll
+  for (int w = 64; w > 0; w >>= 1)
+    if (l < w) red[l] += red[l + w];
+  if (l == 0) tn.part[blockIdx.x] = red[0];
+  // ---- input gradients, layer L - 1 down to 1: dZ(l - 1) = (dZ(l) W(l)) (1 - tanh(z)^2)
+#pragma unroll
+  for (int ly = L - 1; ly >= 1; --ly) {
+    constexpr int K = NH;
+    const int N = ly == L - 1 ? NO : NH;
+    bf4 a[NT];
+#pragma unroll
+    for (int n0 = 0; n0 < NH; n0 += 16)
+      if (n0 < N) a[n0 / 16] = *reinterpret_cast<const bf4*>(dzt + li * N + n0 + 4 * lg);
+    const uint16_t* W = tn.w[ly];
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+      f4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int n0 = 0; n0 < NH; n0 += 16) {
+        if (n0 < N) {
+          bf4 b;
+#pragma unroll
+          for (int q = 0; q < 4; ++q) b[q] = (short)W[(size_t)(n0 + 4 * lg + q) * K + t * 16 + li];
+          acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[n0 / 16], b, acc, 0, 0, 0);
+        }
+      }
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const float th = tanh_act(zr[ly - 1][t][q]);
+        const float gq = acc[q] * (1.f - th * th);
+        const size_t o = (size_t)(b0 + 4 * lg + q) * K + t * 16 + li;
+        tn.dZf[ly - 1][o] = gq;
+        const uint16_t gb = f2bf(gq);
+        tn.dZb[ly - 1][o] = gb;
+        dzt[(4 * lg + q) * K + t * 16 + li] = gb;
+      }
+    }
   }
 }
 
@@ -644,6 +811,11 @@ struct lh_mlp {
   float *dZ = nullptr, *dZ2 = nullptr, *T = nullptr, *loss = nullptr, *part = nullptr;
   uint16_t *dZb = nullptr, *dZb2 = nullptr;     // dZ of layers of even / odd index (f32, bf16)
   int nparts = 0;                               // loss partial sums per batch
+  // the closure network's shape (256 inputs, four hidden layers of 64, 16 padded outputs)
+  // trains through k_train_s and k_dw_all, with one dZ buffer per layer
+  bool spec = false;
+  std::vector<float*> dZl;
+  std::vector<uint16_t*> dZbl;
   float *dx = nullptr, *dy = nullptr;          // data set on the device
   long long ndata = 0;
   int t = 0;                                   // Adam step count (host copy)
@@ -723,6 +895,41 @@ static void mlp_backward(lh_mlp* h) {
   }
 }
 
+// the closure network's shape: forward, loss terms and input gradients of the batch in X[0]
+// (k_train_s), then every layer's weight gradients (k_dw_all); with tick, the step counter
+// advances.  The same results as mlp_forward + mlp_loss + mlp_backward, in two launches.
+static void spec_pass(lh_mlp* h, bool tick) {
+  TrainNet tn{};
+  DwAll a{};
+  int nb = 0;
+  for (int l = 0; l < h->L; ++l) {
+    tn.w[l] = h->wb + h->offW[l];
+    tn.b[l] = h->p + h->offB[l];
+    tn.X[l] = h->X[l];
+    tn.dZf[l] = h->dZl[l];
+    tn.dZb[l] = h->dZbl[l];
+    a.dZ[l] = h->dZbl[l];
+    a.X[l] = h->X[l];
+    a.dW[l] = h->g + h->offW[l];
+    a.dZf[l] = h->dZl[l];
+    a.db[l] = h->g + h->offB[l];
+    a.K[l] = h->Kp[l];
+    a.N[l] = h->Np[l];
+    a.first[l] = nb;
+    nb += (h->Np[l] / 16) * (h->Kp[l] / 16 + 1);
+  }
+  a.first[h->L] = nb;
+  a.L = h->L;
+  a.B = h->B;
+  a.tstep = tick ? h->tdev : nullptr;
+  tn.T = h->T;
+  tn.part = h->part;
+  tn.B = h->B;
+  tn.nout = h->nout;
+  hipLaunchKernelGGL((k_train_s<256, 64, 3>), dim3(h->B / 16), dim3(64), 0, h->st, tn);
+  hipLaunchKernelGGL(k_dw_all, dim3(nb), dim3(64 * kDwWaves), 0, h->st, a);
+}
+
 extern "C" {
 
 int lh_mlp_create(int nin, int nhid, int nlayers, int nout, int batch, unsigned long long seed, lh_mlp** out) {
@@ -759,6 +966,18 @@ int lh_mlp_create(int nin, int nhid, int nlayers, int nout, int batch, unsigned 
   MCHK(h, zalloc(&h->dZ, (size_t)batch * wmax)); MCHK(h, zalloc(&h->dZ2, (size_t)batch * wmax));
   MCHK(h, zalloc(&h->dZb, (size_t)batch * wmax)); MCHK(h, zalloc(&h->dZb2, (size_t)batch * wmax));
   MCHK(h, zalloc(&h->T, (size_t)batch * nout));
+  h->spec = h->L == 5 && h->Kp[0] == 256 && h->Np[h->L - 1] == 16;
+  for (int l = 0; l + 1 < h->L; ++l) h->spec = h->spec && h->Np[l] == 64;
+  if (h->spec) {
+    for (int l = 0; l < h->L; ++l) {
+      float* f = nullptr;
+      uint16_t* b = nullptr;
+      MCHK(h, zalloc(&f, (size_t)batch * h->Np[l]));
+      h->dZl.push_back(f);
+      MCHK(h, zalloc(&b, (size_t)batch * h->Np[l]));
+      h->dZbl.push_back(b);
+    }
+  }
   h->nparts = (batch * h->Np[h->L - 1] + 255) / 256;
   MCHK(h, zalloc(&h->part, (size_t)h->nparts));
   MCHK(h, zalloc(&h->loss, 16));
@@ -793,6 +1012,8 @@ int lh_mlp_destroy(lh_mlp* h) {
   for (auto q : h->Z) if (q) (void)hipFree(q);
   for (auto q : h->Xe) if (q) (void)hipFree(q);
   for (auto q : h->Ze) if (q) (void)hipFree(q);
+  for (auto q : h->dZl) if (q) (void)hipFree(q);
+  for (auto q : h->dZbl) if (q) (void)hipFree(q);
   if (h->st) (void)hipStreamDestroy(h->st);
   delete h;
   return LH_OK;
@@ -868,9 +1089,13 @@ int lh_mlp_loss_grad(lh_mlp* h, const float* x, const float* y, float* loss, flo
   MCHK(h, hipMemcpy(dx, x, (size_t)h->B * h->nin * sizeof(float), hipMemcpyHostToDevice));
   MCHK(h, hipMemcpy(h->T, y, (size_t)h->B * h->nout * sizeof(float), hipMemcpyHostToDevice));
   hipLaunchKernelGGL(k_pack_x, dim3(h->B), dim3(64), 0, h->st, dx, h->B, h->nin, h->Kp[0], h->B, h->X[0]);
-  mlp_forward(h);
-  mlp_loss(h, false);
-  mlp_backward(h);
+  if (h->spec) {
+    spec_pass(h, false);
+  } else {
+    mlp_forward(h);
+    mlp_loss(h, false);
+    mlp_backward(h);
+  }
   hipLaunchKernelGGL(k_loss, dim3(1), dim3(64), 0, h->st, h->part, h->nparts, (float)(h->B * h->nout), h->loss);
   MCHK(h, hipStreamSynchronize(h->st));
   (void)hipFree(dx);
@@ -889,9 +1114,13 @@ static void issue_train_step(lh_mlp* h, float lr, unsigned long long seed) {
   const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
   hipLaunchKernelGGL(k_gather, dim3(h->B), dim3(64), 0, h->st, h->dx, h->dy, h->ndata, h->nin, h->nout,
                      h->Kp[0], h->B, (uint64_t)seed, (const int*)h->tdev, h->X[0], h->T);
-  mlp_forward(h);
-  mlp_loss(h, true);
-  mlp_backward(h);
+  if (h->spec) {
+    spec_pass(h, true);
+  } else {
+    mlp_forward(h);
+    mlp_loss(h, true);
+    mlp_backward(h);
+  }
   hipLaunchKernelGGL(k_adam, dim3((unsigned)((h->np + 255) / 256)), dim3(256), 0, h->st, h->p, h->g, h->m,
                      h->v, h->wb, h->np, lr, b1, b2, eps, (const int*)h->tdev, h->part, h->nparts,
                      (float)(h->B * h->nout), h->loss);
