@@ -198,6 +198,12 @@ int lh_mlp_set_params(lh_mlp* m, const float* params);
 int lh_mlp_set_data(lh_mlp* m, const float* x, const float* y, long long n);
 int lh_mlp_loss_grad(lh_mlp* m, const float* x, const float* y, float* loss, float* grad);
 int lh_mlp_train(lh_mlp* m, int steps, float lr, unsigned long long seed, float* loss_last);
+/* The closure's shape (244 or up to 256 inputs, four hidden layers of 64, up to 16 outputs)
+ * trains by one fused pass per 16 batch rows plus one launch for every layer's weight
+ * gradients; lh_mlp_set_fused(m, 0) selects the per-layer kernels instead (1 restores the
+ * default).  Both give bit-identical gradients and parameters; the switch is for tests and
+ * timing.  Other shapes always use the per-layer kernels. */
+int lh_mlp_set_fused(lh_mlp* m, int on);
 int lh_mlp_predict(lh_mlp* m, const float* x, long long n, float* y);
 /* Outputs at every point (j, k, i) of an ny x nz x nx grid, row (j nz + k) nx + i of y, from
  * inputs gathered on the device: the nch fields g (nch x ny x nz x nx) on the 3x3x3

@@ -142,6 +142,7 @@ def load_library():
     lib.lh_mlp_set_data.argtypes = [vp, _fp, _fp, ctypes.c_longlong]
     lib.lh_mlp_loss_grad.argtypes = [vp, _fp, _fp, _fp, _fp]
     lib.lh_mlp_train.argtypes = [vp, ctypes.c_int, ctypes.c_float, ctypes.c_ulonglong, _fp]
+    lib.lh_mlp_set_fused.argtypes = [vp, ctypes.c_int]
     lib.lh_mlp_predict.argtypes = [vp, _fp, ctypes.c_longlong, _fp]
     lib.lh_mlp_predict_grid.argtypes = [vp, _fp, ctypes.c_int, _fp, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                         _fp, _fp, _fp]
@@ -168,7 +169,7 @@ EXPORTED = [
     "lh_batch_create", "lh_batch_step_cfl", "lh_batch_last_error", "lh_batch_destroy",
     "lh_mlp_create", "lh_mlp_destroy", "lh_mlp_last_error", "lh_mlp_param_count",
     "lh_mlp_get_params", "lh_mlp_set_params", "lh_mlp_set_data", "lh_mlp_loss_grad",
-    "lh_mlp_train", "lh_mlp_predict", "lh_mlp_predict_grid", "lh_closure_create", "lh_closure_create_smagorinsky", "lh_closure_apply",
+    "lh_mlp_train", "lh_mlp_set_fused", "lh_mlp_predict", "lh_mlp_predict_grid", "lh_closure_create", "lh_closure_create_smagorinsky", "lh_closure_apply",
     "lh_closure_set_mean_only", "lh_closure_destroy", "lh_closure_last_error",
 ]
 

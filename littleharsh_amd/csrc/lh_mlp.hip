@@ -814,6 +814,7 @@ struct lh_mlp {
   // the closure network's shape (256 inputs, four hidden layers of 64, 16 padded outputs)
   // trains through k_train_s and k_dw_all, with one dZ buffer per layer
   bool spec = false;
+  bool fused = true;                           // lh_mlp_set_fused
   std::vector<float*> dZl;
   std::vector<uint16_t*> dZbl;
   float *dx = nullptr, *dy = nullptr;          // data set on the device
@@ -1089,7 +1090,7 @@ int lh_mlp_loss_grad(lh_mlp* h, const float* x, const float* y, float* loss, flo
   MCHK(h, hipMemcpy(dx, x, (size_t)h->B * h->nin * sizeof(float), hipMemcpyHostToDevice));
   MCHK(h, hipMemcpy(h->T, y, (size_t)h->B * h->nout * sizeof(float), hipMemcpyHostToDevice));
   hipLaunchKernelGGL(k_pack_x, dim3(h->B), dim3(64), 0, h->st, dx, h->B, h->nin, h->Kp[0], h->B, h->X[0]);
-  if (h->spec) {
+  if (h->spec && h->fused) {
     spec_pass(h, false);
   } else {
     mlp_forward(h);
@@ -1114,7 +1115,7 @@ static void issue_train_step(lh_mlp* h, float lr, unsigned long long seed) {
   const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
   hipLaunchKernelGGL(k_gather, dim3(h->B), dim3(64), 0, h->st, h->dx, h->dy, h->ndata, h->nin, h->nout,
                      h->Kp[0], h->B, (uint64_t)seed, (const int*)h->tdev, h->X[0], h->T);
-  if (h->spec) {
+  if (h->spec && h->fused) {
     spec_pass(h, true);
   } else {
     mlp_forward(h);
@@ -1157,6 +1158,14 @@ int lh_mlp_train(lh_mlp* h, int steps, float lr, unsigned long long seed, float*
   MCHK(h, hipGetLastError());
   MCHK(h, hipStreamSynchronize(h->st));
   if (loss_last) MCHK(h, hipMemcpy(loss_last, h->loss, sizeof(float), hipMemcpyDeviceToHost));
+  return LH_OK;
+}
+
+int lh_mlp_set_fused(lh_mlp* h, int on) {
+  if (!h) return LH_ERR_ARG;
+  MCHK(h, hipStreamSynchronize(h->st));
+  if (h->fused != (on != 0)) drop_train_graphs(h);   // the captured steps hold the other path
+  h->fused = on != 0;
   return LH_OK;
 }
 

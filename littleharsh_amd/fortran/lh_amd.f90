@@ -105,6 +105,11 @@ module lh_amd
       integer(c_long_long), value :: seed
       real(c_float), intent(out) :: loss_last
     end function
+    integer(c_int) function lh_mlp_set_fused(m, on) bind(C, name="lh_mlp_set_fused")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: m
+      integer(c_int), value :: on
+    end function
     ! the closure bound to a coarse channel (arrays: c_loc of the host's own storage)
     integer(c_int) function lh_closure_create(h, m, yc, dxc, dzc, shift_x, shift_z, sf, xmean, xscale, &
                                               ymean, yscale, c) bind(C, name="lh_closure_create")

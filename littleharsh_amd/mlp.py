@@ -90,6 +90,11 @@ class MLP:
         self._check(self.lib.lh_mlp_train(self.h, steps, lr, seed, _p(loss)), "lh_mlp_train")
         return float(loss[0])
 
+    def set_fused(self, on: bool):
+        """The closure's shape trains by the fused pass (default) or, with False, the per-layer
+        kernels; both give bit-identical results."""
+        self._check(self.lib.lh_mlp_set_fused(self.h, 1 if on else 0), "lh_mlp_set_fused")
+
     def predict(self, x) -> np.ndarray:
         x = _f32(x)
         y = np.empty((x.shape[0], self.nout), dtype=np.float32)

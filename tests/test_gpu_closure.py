@@ -122,6 +122,29 @@ def test_train_steps_match_oracle_adam(require_gpu):
         p = p + dp
 
 
+def test_fused_training_is_bit_identical_to_the_per_layer_kernels(require_gpu):
+    """The closure's shape trains by a fused pass (forward, loss terms and input gradients per
+    16 batch rows; every layer's weight gradients in one launch).  The per-layer kernels give
+    the same loss, gradients and parameters bit for bit: batch 272 (17 row tiles), 40 steps
+    (both captured graphs: 2 x 16 steps and 8 single steps)."""
+    from littleharsh_amd.mlp import MLP
+    rng = np.random.default_rng(18)
+    x, y = _data(rng, 3000)
+    out = []
+    for fused in (True, False):
+        net = MLP(NIN, NHID, NLAY, NOUT, batch=272, seed=10)
+        net.set_fused(fused)
+        loss, g = net.loss_grad(x[:272], y[:272])
+        net.set_data(x, y)
+        last = net.train(40, lr=1e-3, seed=4)
+        out.append((loss, g, last, net.get_params()))
+        net.close()
+    (la, ga, ta, pa), (lb, gb, tb, pb) = out
+    print(f"fused / per-layer: loss {la:.6e} {lb:.6e}, last batch loss {ta:.6e} {tb:.6e}", flush=True)
+    assert la == lb and np.array_equal(ga, gb)
+    assert ta == tb and np.array_equal(pa, pb)
+
+
 @pytest.mark.parametrize("ny,nz,nx", [(5, 6, 7), (4, 5, 48)])
 def test_predict_grid_matches_stencil_rows(require_gpu, ny, nz, nx):
     """The device-side stencil gather gives the inputs the training rows were built from.
