@@ -240,18 +240,19 @@ __global__ void __launch_bounds__(256) k_mse(const float* __restrict__ Y, const 
   }
 }
 
-// One training pass of 16 batch rows per wave (one wave per block) for the closure network's
-// shape: K0 inputs, NHH + 1 hidden layers of NH (tanh), 16 padded outputs.  It runs the
-// forward pass, the loss terms and the input gradients of every layer with the operands,
-// instructions and order of k_fwd, k_mse and k_dx, so its results are bit-identical to theirs:
-//   forward: acc over k in 32-wide steps per 16x16 output tile, z = acc + b, x = bf16(tanh z);
-//   loss: dZ = 2 (Y - T) / (B nout) and the tile's sum of (Y - T)^2 by k_mse's 256-element
-//   tree (one part per 16-row tile: 256 = 16 rows x 16 outputs);
+// One training pass of 16 batch rows per block for the closure network's shape: K0 inputs,
+// NHH + 1 hidden layers of NH (tanh), 16 padded outputs; NH / 16 waves, wave w computing the
+// output columns 16 w .. 16 w + 15 of every hidden layer.  It runs the forward pass, the loss
+// terms and the input gradients with the operands, instructions and order of k_fwd, k_mse
+// and k_dx for every 16x16 tile, so its results are bit-identical to theirs:
+//   forward: acc over k in 32-wide steps, z = acc + b, x = bf16(tanh z);
+//   loss (wave 0): dZ = 2 (Y - T) / (B nout) and the tile's sum of (Y - T)^2 by k_mse's
+//   256-element tree (one part per 16-row tile: 256 = 16 rows x 16 outputs);
 //   input gradients: acc over n in 16-wide steps, dZ = acc (1 - tanh(z)^2).
-// A lane keeps the pre-activations it produced (rows 4 (l / 16) + q, column l % 16 of each
-// tile) in registers: the input-gradient epilogue reads them in the same lanes.  Layer inputs
-// and dZ pass to the next layer through the wave's LDS tiles (LDS operations of one wave
-// complete in order) and go to global memory for the weight gradients (k_dw_all).
+// A lane keeps the pre-activations it produced (rows 4 (l / 16) + q, column l % 16 of its
+// wave's tile) in registers: the input-gradient epilogue of the same tile reads them in the
+// same lanes.  Layer inputs and dZ pass between layers through ping-pong LDS tiles (one
+// barrier per layer) and go to global memory for the weight gradients (k_dw_all).
 struct TrainNet {
   const uint16_t* w[kMaxL];   // bf16 weights of each layer (N x K)
   const float* b[kMaxL];      // FP32 biases
@@ -263,56 +264,47 @@ struct TrainNet {
   int B, nout;
 };
 template <int K0, int NH, int NHH>
-__global__ void __launch_bounds__(64) k_train_s(TrainNet tn) {
+__global__ void __launch_bounds__(4 * NH) k_train_s(TrainNet tn) {
   static_assert(K0 % 32 == 0 && NH % 32 == 0, "32-wide forward steps without a tail");
-  constexpr int L = NHH + 2, NO = 16, NT = NH / 16;
-  __shared__ __align__(16) uint16_t xt[16 * NH];    // the next layer's input tile
-  __shared__ __align__(16) uint16_t dzt[16 * NH];   // the current dZ tile (row stride N)
+  constexpr int L = NHH + 2, NO = 16;
+  __shared__ __align__(16) uint16_t xt[2][16 * NH];    // layer inputs (ping-pong)
+  __shared__ __align__(16) uint16_t dzt[2][16 * NH];   // dZ tiles (ping-pong, row stride N)
   __shared__ float red[256];
-  const int l = threadIdx.x, li = l & 15, lg = l >> 4;
-  const int b0 = blockIdx.x * 16;
-  f4 zr[NHH + 1][NT];                               // hidden pre-activations
-  // ---- forward: the input layer (A from the batch rows in X[0])
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63, li = l & 15, lg = l >> 4;
+  const int b0 = blockIdx.x * 16, col = w * 16 + li;
+  f4 zr[NHH + 1];                                      // this wave's hidden pre-activations
   {
     const uint16_t* xr = tn.X[0] + (size_t)(b0 + li) * K0;
+    const uint16_t* wr = tn.w[0] + (size_t)col * K0;
+    f4 acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-    for (int t = 0; t < NT; ++t) {
-      const uint16_t* wr = tn.w[0] + (size_t)(t * 16 + li) * K0;
-      f4 acc = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-      for (int k0 = 0; k0 < K0; k0 += 32) acc = mfma32(ld8(xr + k0 + 8 * lg), ld8(wr + k0 + 8 * lg), acc);
-      zr[0][t] = acc;
-    }
+    for (int k0 = 0; k0 < K0; k0 += 32) acc = mfma32(ld8(xr + k0 + 8 * lg), ld8(wr + k0 + 8 * lg), acc);
+    zr[0] = acc;
   }
-  // ---- forward: hidden epilogue h, then layer h + 1 on the tile it wrote
 #pragma unroll
   for (int h = 0; h <= NHH; ++h) {
+    uint16_t* xo = xt[(h + 1) & 1];
+    const float bb = tn.b[h][col];
 #pragma unroll
-    for (int t = 0; t < NT; ++t) {
-      const float bb = tn.b[h][t * 16 + li];
-#pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        const float z = zr[h][t][q] + bb;
-        zr[h][t][q] = z;
-        const uint16_t x = f2bf(tanh_act(z));
-        xt[(4 * lg + q) * NH + t * 16 + li] = x;
-        tn.X[h + 1][(size_t)(b0 + 4 * lg + q) * NH + t * 16 + li] = x;
-      }
+    for (int q = 0; q < 4; ++q) {
+      const float z = zr[h][q] + bb;
+      zr[h][q] = z;
+      const uint16_t x = f2bf(tanh_act(z));
+      xo[(4 * lg + q) * NH + col] = x;
+      tn.X[h + 1][(size_t)(b0 + 4 * lg + q) * NH + col] = x;
     }
+    __syncthreads();
     bf8 a[NH / 32];
 #pragma unroll
-    for (int c = 0; c < NH / 32; ++c) a[c] = ld8(xt + li * NH + c * 32 + 8 * lg);
+    for (int c = 0; c < NH / 32; ++c) a[c] = ld8(xo + li * NH + c * 32 + 8 * lg);
     if (h < NHH) {
+      const uint16_t* wr = tn.w[h + 1] + (size_t)col * NH;
+      f4 acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-      for (int t = 0; t < NT; ++t) {
-        const uint16_t* wr = tn.w[h + 1] + (size_t)(t * 16 + li) * NH;
-        f4 acc = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-        for (int c = 0; c < NH / 32; ++c) acc = mfma32(a[c], ld8(wr + c * 32 + 8 * lg), acc);
-        zr[h + 1][t] = acc;
-      }
-    } else {
-      // ---- output layer and the loss terms
+      for (int c = 0; c < NH / 32; ++c) acc = mfma32(a[c], ld8(wr + c * 32 + 8 * lg), acc);
+      zr[h + 1] = acc;
+    } else if (w == 0) {
+      // output layer and the loss terms
       const uint16_t* wr = tn.w[L - 1] + (size_t)li * NH;
       f4 acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
@@ -332,51 +324,54 @@ __global__ void __launch_bounds__(64) k_train_s(TrainNet tn) {
         tn.dZf[L - 1][o] = sc * d;
         const uint16_t db = f2bf(sc * d);
         tn.dZb[L - 1][o] = db;
-        dzt[(4 * lg + q) * NO + li] = db;
+        dzt[0][(4 * lg + q) * NO + li] = db;
         red[(4 * lg + q) * 16 + li] = s;
       }
     }
   }
-  // the tile's loss partial: k_mse's tree (red[i] += red[i + w], w = 128 .. 1)
-  red[l] += red[l + 128];
-  red[l + 64] += red[l + 192];
+  __syncthreads();
+  if (w == 0) {
+    // the tile's loss partial: k_mse's tree (red[i] += red[i + w], w = 128 .. 1)
+    red[l] += red[l + 128];
+    red[l + 64] += red[l + 192];
 #pragma unroll
-  for (int w = 64; w > 0; w >>= 1)
-    if (l < w) red[l] += red[l + w];
-  if (l == 0) tn.part[blockIdx.x] = red[0];
-  // ---- input gradients, layer L - 1 down to 1: dZ(l - 1) = (dZ(l) W(l)) (1 - tanh(z)^2)
+    for (int s = 64; s > 0; s >>= 1)
+      if (l < s) red[l] += red[l + s];
+    if (l == 0) tn.part[blockIdx.x] = red[0];
+  }
+  // input gradients, layer L - 1 down to 1: dZ(l - 1) = (dZ(l) W(l)) (1 - tanh(z)^2)
 #pragma unroll
   for (int ly = L - 1; ly >= 1; --ly) {
     constexpr int K = NH;
     const int N = ly == L - 1 ? NO : NH;
-    bf4 a[NT];
+    const uint16_t* src = dzt[(L - 1 - ly) & 1];
+    uint16_t* dst = dzt[(L - ly) & 1];
+    bf4 a[NH / 16];
 #pragma unroll
     for (int n0 = 0; n0 < NH; n0 += 16)
-      if (n0 < N) a[n0 / 16] = *reinterpret_cast<const bf4*>(dzt + li * N + n0 + 4 * lg);
+      if (n0 < N) a[n0 / 16] = *reinterpret_cast<const bf4*>(src + li * N + n0 + 4 * lg);
     const uint16_t* W = tn.w[ly];
+    f4 acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-    for (int t = 0; t < NT; ++t) {
-      f4 acc = {0.f, 0.f, 0.f, 0.f};
+    for (int n0 = 0; n0 < NH; n0 += 16) {
+      if (n0 < N) {
+        bf4 b;
 #pragma unroll
-      for (int n0 = 0; n0 < NH; n0 += 16) {
-        if (n0 < N) {
-          bf4 b;
-#pragma unroll
-          for (int q = 0; q < 4; ++q) b[q] = (short)W[(size_t)(n0 + 4 * lg + q) * K + t * 16 + li];
-          acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[n0 / 16], b, acc, 0, 0, 0);
-        }
-      }
-#pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        const float th = tanh_act(zr[ly - 1][t][q]);
-        const float gq = acc[q] * (1.f - th * th);
-        const size_t o = (size_t)(b0 + 4 * lg + q) * K + t * 16 + li;
-        tn.dZf[ly - 1][o] = gq;
-        const uint16_t gb = f2bf(gq);
-        tn.dZb[ly - 1][o] = gb;
-        dzt[(4 * lg + q) * K + t * 16 + li] = gb;
+        for (int q = 0; q < 4; ++q) b[q] = (short)W[(size_t)(n0 + 4 * lg + q) * K + col];
+        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[n0 / 16], b, acc, 0, 0, 0);
       }
     }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const float th = tanh_act(zr[ly - 1][q]);
+      const float gq = acc[q] * (1.f - th * th);
+      const size_t o = (size_t)(b0 + 4 * lg + q) * K + col;
+      tn.dZf[ly - 1][o] = gq;
+      const uint16_t gb = f2bf(gq);
+      tn.dZb[ly - 1][o] = gb;
+      dst[(4 * lg + q) * K + col] = gb;
+    }
+    if (ly > 1) __syncthreads();
   }
 }
 
@@ -927,7 +922,7 @@ static void spec_pass(lh_mlp* h, bool tick) {
   tn.part = h->part;
   tn.B = h->B;
   tn.nout = h->nout;
-  hipLaunchKernelGGL((k_train_s<256, 64, 3>), dim3(h->B / 16), dim3(64), 0, h->st, tn);
+  hipLaunchKernelGGL((k_train_s<256, 64, 3>), dim3(h->B / 16), dim3(4 * 64), 0, h->st, tn);
   hipLaunchKernelGGL(k_dw_all, dim3(nb), dim3(64 * kDwWaves), 0, h->st, a);
 }
 
