@@ -121,6 +121,9 @@ struct lh_handle {
   int xraw_lc = LH_XRAW_LC;
   int zasm_lc = LH_ZASM_LC;
   int col_bs = LH_COL_BS;
+  // rows per step of the velocity sweep: 1 on large grids (fewer registers, more resident
+  // waves), kVelUSmall when few columns leave the sweeps latency-bound (set at creation)
+  int vel_u = 1;
   // per-kernel timing
   bool timing = false;
   std::vector<hipEvent_t> evpool;
@@ -209,6 +212,12 @@ static bool supported_len(int n) {
 #ifndef LH_VEL_U
 #define LH_VEL_U 1
 #endif
+// Grids of at most kVelSmallCols spectral columns run the velocity sweep kVelUSmall rows per
+// step: with one thread per column too few waves are resident to hide each row's loads
+// (96x145x96: 1.37 -> 1.24 ms per step; 64x97x64: 0.89 -> 0.82 ms; the pressure sweep's
+// unroll and the block size change nothing there).
+constexpr int kVelSmallCols = 16384;
+constexpr int kVelUSmall = 4;
 template <int N> constexpr int zlines() {
   return N >= 384 ? LH_ZLINES_LONG : (Plan<N>::T >= 8 ? 16 : 32);
 }
@@ -497,9 +506,15 @@ static void issue_substep(lh_handle* h, int krk /*0-based*/, bool cfl_mode) {
     const double by = spl * (2.0 * nuvw + 2.0 * dm.nyp) + 3.0 * 8.0 * dm.ncol * (double)dm.nyp;
     // in: u, p, Nu_old, Nu_new (rhs on canopy planes); eliminated rows out and back; out: u
     TIMED(h, LH_K_VEL, spl * (7.0 * nuvw),
-          hipLaunchKernelGGL(k_vel_rhs<LH_VEL_U>, dim3((dm.ncol + h->col_bs - 1) / h->col_bs, 3),
-                             dim3(h->col_bs), 0, h->st, ra, dm, (const double*)h->ayu,
-                             (const double*)h->ayv));
+          if (h->vel_u == kVelUSmall) {
+            hipLaunchKernelGGL(k_vel_rhs<kVelUSmall>, dim3((dm.ncol + h->col_bs - 1) / h->col_bs, 3),
+                               dim3(h->col_bs), 0, h->st, ra, dm, (const double*)h->ayu,
+                               (const double*)h->ayv);
+          } else {
+            hipLaunchKernelGGL(k_vel_rhs<LH_VEL_U>, dim3((dm.ncol + h->col_bs - 1) / h->col_bs, 3),
+                               dim3(h->col_bs), 0, h->st, ra, dm, (const double*)h->ayu,
+                               (const double*)h->ayv);
+          });
     TIMED(h, LH_K_PRES, by - 2.0 * nuvw * spl + 2.0 * nuvw * spl,
           hipLaunchKernelGGL(k_pres<LH_COL_U>, dim3((dm.ncol + h->col_bs - 1) / h->col_bs),
                              dim3(h->col_bs), 0, h->st, a, dm));
@@ -539,6 +554,7 @@ int lh_create(const lh_config* c, lh_handle** out) {
   Dims& dm = h->dm;
   dm.nx = c->nx; dm.nz = c->nz; dm.px = c->px; dm.pz = c->pz;
   dm.mx = c->nx / 2 + 1; dm.ncol = dm.mx * c->nz;
+  h->vel_u = dm.ncol <= kVelSmallCols ? kVelUSmall : LH_VEL_U;
   dm.ny = c->ny; dm.d = c->dsty;
   dm.nyu = c->ny + 2 * c->dsty + 1; dm.nyv = c->ny + 2 * c->dsty; dm.nyp = c->ny + 2 * c->dsty - 1;
   dm.dxi = c->px * c->alp / (2.0 * M_PI);
