@@ -126,6 +126,10 @@ struct lh_handle {
   int vel_u = 1;
   // per-kernel timing
   bool timing = false;
+  // set while the handle belongs to a batch: k_prep stays on the handle's stream, since the
+  // batch's graph already runs the instances side by side and a side branch per substep and
+  // instance (48 x 3 cross-stream joins per step) serialised them
+  bool in_batch = false;
   std::vector<hipEvent_t> evpool;
   size_t evnext = 0;
   struct Pend { int k; hipEvent_t a, b; };
@@ -374,10 +378,10 @@ static void issue_substep(lh_handle* h, int krk /*0-based*/, bool cfl_mode) {
   const double spl = 16.0 * dm.ncol;                         // bytes per spectral plane
   const double nuvw = 2.0 * dm.nyu + dm.nyv;                 // u, w, v planes
   // k_prep needs only dt: in graph and direct issue it runs on the side stream beside the
-  // nonlinear stage (timing mode keeps it in line); at kRK = 1 of a CFL step dt comes from
-  // the first x pass, so it forks after k_cfl_dt
+  // nonlinear stage (timing mode and batch members keep it in line); at kRK = 1 of a CFL
+  // step dt comes from the first x pass, so it forks after k_cfl_dt
   const PREP prep{h->dyu2i, h->dyv2i, h->scal, h->ayu, h->ayv, h->gw[0], h->gw[1], bre, dm.nyu, dm.nyv};
-  const bool side = !h->timing;
+  const bool side = !h->timing && !h->in_batch;
   auto fork_prep = [&]() {
     (void)hipEventRecord(h->ev_fork, h->st);
     (void)hipStreamWaitEvent(h->side, h->ev_fork, 0);
@@ -1187,6 +1191,10 @@ int lh_batch_create(int n, lh_handle* const* hs, lh_batch** out) {
     if (!hs[i]) { delete b; return LH_ERR_ARG; }
     b->hs.push_back(hs[i]);
   }
+  for (auto* h : b->hs) {
+    h->in_batch = true;
+    drop_graphs(h);                                 // captured with the side branch
+  }
   if (hipStreamCreate(&b->master) != hipSuccess) { delete b; return LH_ERR_HIP; }
   b->ev.assign(n + 1, nullptr);
   for (auto& e : b->ev)
@@ -1241,6 +1249,10 @@ int lh_batch_step_cfl(lh_batch* b, double cfl) {
 int lh_batch_destroy(lh_batch* b) {
   if (!b) return LH_OK;
   if (b->master) (void)hipStreamSynchronize(b->master);
+  for (auto* h : b->hs) {
+    h->in_batch = false;
+    drop_graphs(h);                                 // captured with k_prep in line
+  }
   if (b->g) (void)hipGraphExecDestroy(b->g);
   for (auto e : b->ev)
     if (e) (void)hipEventDestroy(e);
