@@ -593,8 +593,12 @@ __global__ void __launch_bounds__(64 * kFuseWaves) k_mlp_grid(FusedNet fn, const
 // Same operands, k order and instructions as k_mlp_grid.
 constexpr int kOffEx = -2147483647;      // input offset sentinels: the extra field, padding
 constexpr int kOffPad = -2147483647 - 1;
+// waves per block of k_mlp_grid_s: one block per CU (the weights take 59 KB of LDS per block),
+// four waves per SIMD (the compiler fits 128 VGPRs with 36 bytes of spill; 1.05x faster per
+// coupling than three waves at 138 VGPRs, 1.15x than two)
+constexpr int kSpecWaves = 16;
 template <int K0, int NH, int NHH>
-__global__ void __launch_bounds__(64 * kFuseWaves) k_mlp_grid_s(FusedNet fn, const uint16_t* __restrict__ wb,
+__global__ void __launch_bounds__(64 * kSpecWaves) k_mlp_grid_s(FusedNet fn, const uint16_t* __restrict__ wb,
                                                                 const float* __restrict__ p,
                                                                 const float* __restrict__ g,
                                                                 const float* __restrict__ ex,
@@ -645,8 +649,8 @@ __global__ void __launch_bounds__(64 * kFuseWaves) k_mlp_grid_s(FusedNet fn, con
   const uint16_t* wh = Wl + WH + li * NH + 8 * lg;
   const uint16_t* wo = Wl + WO + li * NH + 8 * lg;
   const long long ntile = (n + 15) / 16;
-  for (long long tile = (long long)blockIdx.x * kFuseWaves + wave; tile < ntile;
-       tile += (long long)gridDim.x * kFuseWaves) {
+  for (long long tile = (long long)blockIdx.x * kSpecWaves + wave; tile < ntile;
+       tile += (long long)gridDim.x * kSpecWaves) {
     const long long r0 = tile * 16;
     const int i0 = (int)(r0 % nx), k = (int)((r0 / nx) % nz), j = (int)(r0 / ((long long)nx * nz));
     const int i = i0 + li;
@@ -1260,7 +1264,7 @@ int lh_mlp_eval_grid_dev(lh_mlp* h, const float* g, int nch, const float* ex, in
       fn.offB[l] = h->offB[l];
     }
     constexpr int WN = 64 * 256 + 3 * 64 * 64 + 16 * 64;
-    const size_t lds_s = (size_t)((2 * WN + 15) & ~15) + 6 * 256 * sizeof(int) + kFuseWaves * 16 * 64 * 2;
+    const size_t lds_s = (size_t)((2 * WN + 15) & ~15) + 6 * 256 * sizeof(int) + kSpecWaves * 16 * 64 * 2;
     if (!h->ncu_s) {
       int dev = 0;
       MCHK(h, hipGetDevice(&dev));
@@ -1270,8 +1274,8 @@ int lh_mlp_eval_grid_dev(lh_mlp* h, const float* g, int nch, const float* ex, in
     }
     const long long n = (long long)ny * nz * nx;
     const long long tiles = (n + 15) / 16;
-    const int blocks = (int)std::min<long long>((tiles + kFuseWaves - 1) / kFuseWaves, 2LL * h->ncu_s);
-    hipLaunchKernelGGL((k_mlp_grid_s<256, 64, 3>), dim3(blocks), dim3(64 * kFuseWaves), lds_s, st, fn, h->wb, h->p,
+    const int blocks = (int)std::min<long long>((tiles + kSpecWaves - 1) / kSpecWaves, (long long)h->ncu_s);
+    hipLaunchKernelGGL((k_mlp_grid_s<256, 64, 3>), dim3(blocks), dim3(64 * kSpecWaves), lds_s, st, fn, h->wb, h->p,
                        g, ex, mean, rscale, nch, ny, nz, nx, n, y);
     MCHK(h, hipGetLastError());
     return LH_OK;
