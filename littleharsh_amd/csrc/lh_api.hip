@@ -126,10 +126,10 @@ struct lh_handle {
   int vel_u = 1;
   // per-kernel timing
   bool timing = false;
-  // set while the handle belongs to a batch: k_prep stays on the handle's stream, since the
+  // batches the handle belongs to: while any, k_prep stays on the handle's stream, since the
   // batch's graph already runs the instances side by side and a side branch per substep and
   // instance (48 x 3 cross-stream joins per step) serialised them
-  bool in_batch = false;
+  int in_batch = 0;
   std::vector<hipEvent_t> evpool;
   size_t evnext = 0;
   struct Pend { int k; hipEvent_t a, b; };
@@ -381,7 +381,7 @@ static void issue_substep(lh_handle* h, int krk /*0-based*/, bool cfl_mode) {
   // nonlinear stage (timing mode and batch members keep it in line); at kRK = 1 of a CFL
   // step dt comes from the first x pass, so it forks after k_cfl_dt
   const PREP prep{h->dyu2i, h->dyv2i, h->scal, h->ayu, h->ayv, h->gw[0], h->gw[1], bre, dm.nyu, dm.nyv};
-  const bool side = !h->timing && !h->in_batch;
+  const bool side = !h->timing && h->in_batch == 0;
   auto fork_prep = [&]() {
     (void)hipEventRecord(h->ev_fork, h->st);
     (void)hipStreamWaitEvent(h->side, h->ev_fork, 0);
@@ -1180,6 +1180,7 @@ struct lh_batch {
   hipGraphExec_t g = nullptr;
   double cfl = -1.0;
   std::vector<hipEvent_t> ev;
+  bool joined = false;          // the handles count this batch (in_batch)
   std::string err;
 };
 
@@ -1191,10 +1192,6 @@ int lh_batch_create(int n, lh_handle* const* hs, lh_batch** out) {
     if (!hs[i]) { delete b; return LH_ERR_ARG; }
     b->hs.push_back(hs[i]);
   }
-  for (auto* h : b->hs) {
-    h->in_batch = true;
-    drop_graphs(h);                                 // captured with the side branch
-  }
   if (hipStreamCreate(&b->master) != hipSuccess) { delete b; return LH_ERR_HIP; }
   b->ev.assign(n + 1, nullptr);
   for (auto& e : b->ev)
@@ -1202,6 +1199,11 @@ int lh_batch_create(int n, lh_handle* const* hs, lh_batch** out) {
       lh_batch_destroy(b);
       return LH_ERR_HIP;
     }
+  for (auto* h : b->hs) {
+    h->in_batch += 1;
+    drop_graphs(h);                                 // captured with the side branch
+  }
+  b->joined = true;
   *out = b;
   return LH_OK;
 }
@@ -1249,10 +1251,11 @@ int lh_batch_step_cfl(lh_batch* b, double cfl) {
 int lh_batch_destroy(lh_batch* b) {
   if (!b) return LH_OK;
   if (b->master) (void)hipStreamSynchronize(b->master);
-  for (auto* h : b->hs) {
-    h->in_batch = false;
-    drop_graphs(h);                                 // captured with k_prep in line
-  }
+  if (b->joined)
+    for (auto* h : b->hs) {
+      h->in_batch -= 1;
+      drop_graphs(h);                               // captured with k_prep in line
+    }
   if (b->g) (void)hipGraphExecDestroy(b->g);
   for (auto e : b->ev)
     if (e) (void)hipEventDestroy(e);
