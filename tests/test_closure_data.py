@@ -74,3 +74,22 @@ def test_coarse_transforms_match_solver_convention():
     assert coarse.shape == (32, 32)
     assert np.allclose(fine[::3, ::3], coarse[::2, ::2], atol=1e-12 * np.abs(fine).max())
     assert np.allclose(phys_to_modes96(coarse, c.nz, c.mx), b, atol=1e-12 * np.abs(b).max())
+
+
+def test_standardise_is_the_float32_reciprocal_form():
+    """The network's input standardisation (the form the device gathers compute): float32
+    (x - mean) * (1 / scale), the reciprocal rounded to float32.  It agrees with the division
+    to within one rounding and differs from it in some elements, so a switch back to
+    dividing (on the host while the device multiplies) would be caught."""
+    from littleharsh_amd.mlp import standardise
+    rng = np.random.default_rng(21)
+    x = rng.normal(size=(4000, 8)).astype(np.float32)
+    mean = rng.normal(size=8).astype(np.float32)
+    scale = (0.3 + rng.random(8)).astype(np.float32)
+    out = standardise(x, mean, scale)
+    assert out.dtype == np.float32
+    r = (np.float32(1.0) / scale).astype(np.float32)
+    assert np.array_equal(out, (x - mean) * r)
+    div = (x - mean) / scale
+    assert np.any(out != div)
+    assert np.max(np.abs(out - div) / np.maximum(np.abs(div), 1e-30)) < 3e-7
