@@ -1134,6 +1134,12 @@ void lho_get_nonlinear_full(const lho* o, double* n1, double* n2, double* n3) {
   memcpy(n2, o->Nu2, sizeof(cplx) * (size_t)o->ncol * o->nyv);
   memcpy(n3, o->Nu3, sizeof(cplx) * (size_t)o->ncol * o->nyu);
 }
+/* solveU / solveV alone (littleharsh.f90:733-824 with LUsolU / LUsolV): u (in/out) and du
+   (in, overwritten by the solve's work) on the u grid (grid_u) or the v grid, with the
+   y-Laplacian lu for the canopy tip planes; reference layout, full plane ranges */
+void lho_hook_solve_vel(lho* o, int grid_u, int krk1, double dt, double* u, double* du, double* lu) {
+  solve_vel(o, (cplx*)u, (cplx*)du, (const cplx*)lu, grid_u, krk1 - 1, dt);
+}
 /* immersed-boundary forcing alone on x (reference layout, grid_u: u grid else v grid)
    with the y-Laplacian Lu and beta */
 void lho_hook_ib(lho* o, int grid_u, double beta, double* x, const double* lu) {

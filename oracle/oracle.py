@@ -66,6 +66,7 @@ def lib():
         L.lho_hook_nonlinear.argtypes = [vp, ctypes.c_int]
         L.lho_get_nonlinear_full.argtypes = [vp, _dp, _dp, _dp]
         L.lho_hook_ib.argtypes = [vp, ctypes.c_int, ctypes.c_double, _dp, _dp]
+        L.lho_hook_solve_vel.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_double, _dp, _dp, _dp]
         _lib = L
     return _lib
 
@@ -190,6 +191,15 @@ class Oracle:
         lu = np.ascontiguousarray(lu, dtype=np.complex128)
         lib().lho_hook_ib(self._h, 1 if grid_u else 0, beta, _p(x), _p(lu))
         return x
+
+    def solve_vel(self, grid_u, krk, dt, u, du, lu):
+        """solveU / solveV (littleharsh.f90:733-824): the new velocity from u, du and the
+        y-Laplacian lu (used on canopy tip planes); inputs are not modified."""
+        u = np.array(u, dtype=np.complex128, order="C", copy=True)
+        du = np.array(du, dtype=np.complex128, order="C", copy=True)
+        lu = np.ascontiguousarray(lu, dtype=np.complex128)
+        lib().lho_hook_solve_vel(self._h, 1 if grid_u else 0, int(krk), float(dt), _p(u), _p(du), _p(lu))
+        return u
 
     def spectra(self):
         s = self.s

@@ -211,6 +211,27 @@ contains
     deallocate(ay, axz)
   end subroutine
 
+  ! LU_build alone (tridLU_3D_IB.f90:427-493): the unfactored rows of the velocity matrix
+  ! (lower, diagonal, upper per plane) and axz per column
+  subroutine ref_lu_build(krk_in, dt_in, grid, ay_o, axz_o) bind(C, name="ref_lu_build")
+    integer(c_int), value :: krk_in, grid
+    real(c_double), value :: dt_in
+    real(c_double), intent(out) :: ay_o(*), axz_o(*)
+    real(8), allocatable :: ay(:,:,:), axz(:,:)
+    integer :: nj
+    kRK = krk_in
+    dt = dt_in
+    allocate(ay(3,jlim(1,grid,2):jlim(2,grid,2),nband))
+    allocate(axz(maxval(columns_num),nband))
+    ay = 0d0
+    axz = 0d0
+    call LU_build(jlim(1,grid,midband), jlim(2,grid,midband), grid, 0, midband, axz, ay)
+    nj = jlim(2,grid,2) - jlim(1,grid,2) + 1
+    ay_o(1:3*nj) = reshape(ay(:,:,midband), (/3*nj/))
+    axz_o(1:columns_num(midband,0)) = axz(1:columns_num(midband,0),midband)
+    deallocate(ay, axz)
+  end subroutine
+
   ! DG factors after LU_buildP/LU_decP (tridLU_3D_IB.f90:497-571, 643-684)
   subroutine ref_dg(dg_o) bind(C, name="ref_dg")
     real(c_double), intent(out) :: dg_o(*)

@@ -7,6 +7,7 @@ inputs and the reference routines' outputs for one small channel:
 * y_grid_canopy metrics, def_k wavenumbers, gridweighting        (y_grid_canopy.f90, start.f90)
 * boundary_canopies point lists                                    (geom_sym_canopy.f90)
 * LU_build + LU_dec factors of the u and v matrices, axz           (tridLU_3D_IB.f90:427-640)
+* LU_build alone: the unfactored rows of those matrices            (tridLU_3D_IB.f90:427-493)
 * LU_buildP + LU_decP factors DG                                   (tridLU_3D_IB.f90:497-684)
 * RHS0_u1/u2/u3 outputs du and Lu on a seeded state               (littleharsh.f90:614-731)
 * solveP + v_corr on a seeded state                                (littleharsh.f90:557-937)
@@ -120,6 +121,11 @@ def main():
             ref.ref_lu_vel(ctypes.c_int(2), ctypes.c_double(3e-4), ctypes.c_int(grid), _dp(ay), _dp(axz))
             out[f"ay_{name}"] = ay.reshape(nj, 3)
             out[f"axz_{name}"] = axz
+            # the same matrix before LU_dec: the system the velocity solve inverts
+            ayb, axzb = np.zeros(3 * nj), np.zeros(ncol)
+            ref.ref_lu_build(ctypes.c_int(2), ctypes.c_double(3e-4), ctypes.c_int(grid), _dp(ayb), _dp(axzb))
+            out[f"ayb_{name}"] = ayb.reshape(nj, 3)
+            out[f"axzb_{name}"] = axzb
         dg = np.zeros(3 * nyp * ncol)
         ref.ref_dg(_dp(dg))
         out["dg"] = dg.reshape(ncol, nyp, 3)

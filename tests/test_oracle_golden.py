@@ -83,6 +83,55 @@ def test_velocity_lu_factors(g, setup):
         assert rel(axz, g[f"axz_{name}"]) < 1e-15
 
 
+def _setup_without_canopy():
+    c = CFG
+    s = make_setup(nx=c["nx"], nz=c["nz"], ny_total=c["ny_total"], dsty=c["dsty"], re=c["re"],
+                   alp=c["alp"], bet=c["bet"], mpgx=c["mpgx"], posth=c["posth"], dyq=c["dyq"],
+                   ntilex=c["ntilex"], ntilez=c["ntilez"], dstx=c["dstx"], dstz=c["dstz"])
+    s.list_u = s.list_u[:, :0]
+    s.list_v = s.list_v[:, :0]
+    return s
+
+
+@pytest.mark.parametrize("canopy", [False, True])
+@pytest.mark.parametrize("grid", ["u", "v"])
+def test_velocity_solve_matches_dense_solve(g, setup, grid, canopy):
+    """solveU / solveV with LUsolU / LUsolV (littleharsh.f90:733-824, tridLU_3D_IB.f90:50-194):
+    the boundary rows of the rhs are 0, rows 1..nj-2 are (u + dt du) / axz, the canopy
+    planes get the immersed-boundary forcing (pinned by its own brute-force check), and the
+    new velocity solves the system whose rows the reference's LU_build wrote (golden, before
+    LU_dec).  Checked by a dense solve per column against the oracle's Thomas sweeps."""
+    s = setup if canopy else _setup_without_canopy()
+    o = Oracle(s)
+    krk, dt = 2, 3e-4            # the golden LU_build was called with these
+    beta = (1.0 / 15.0) * dt / s.re
+    nj = s.nyu if grid == "u" else s.nyv
+    rng = np.random.default_rng(101 if grid == "u" else 102)
+    cr = lambda: rng.standard_normal((s.ncol, nj)) + 1j * rng.standard_normal((s.ncol, nj))
+    u, du, lu = cr(), cr(), cr()
+    ours = o.solve_vel(grid == "u", krk, dt, u, du, lu)
+    rows, axz = g[f"ayb_{grid}"], g[f"axzb_{grid}"]
+    assert rel(axz, g[f"axz_{grid}"]) == 0.0
+    M = np.diag(rows[:, 1]) + np.diag(rows[1:, 0], -1) + np.diag(rows[:-1, 2], 1)
+    b = np.zeros((s.ncol, nj), complex)
+    b[:, 1:-1] = (u[:, 1:-1] + dt * du[:, 1:-1]) / axz[:, None]
+    if canopy:
+        b = o.ib(grid == "u", beta, b, lu)
+    ref = np.linalg.solve(M, b.T).T
+    assert rel(ours, ref) < 1e-13
+    # the boundary rows hold the wall conditions: u rows 0 and nj-1 are tied to their
+    # neighbours by gridweighting, v is zero at the walls
+    if grid == "u":
+        assert np.abs(ours[:, 0] + rows[0, 2] * ours[:, 1]).max() < 1e-14 * np.abs(ours).max()
+    else:
+        assert np.abs(ours[:, 0]).max() == 0.0 and np.abs(ours[:, -1]).max() == 0.0
+    if canopy:
+        # a planted mistake must show: leaving out the division by axz
+        wrong = b.copy()
+        wrong[:, 1:-1] *= axz[:, None]
+        assert rel(ours, np.linalg.solve(M, wrong.T).T) > 1e-9
+
+
 def test_pressure_lu_factors(g, setup):
     o = Oracle(setup)
     dg = o.dg()

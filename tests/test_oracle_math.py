@@ -151,31 +151,46 @@ def test_nonlinear_vanishes_for_a_mean_profile():
         assert np.abs(n).max() < 1e-14
 
 
-def test_immersed_boundary_masking_matches_numpy():
-    """tridLU_3D_IB.f90:224-245: on canopy planes rhs -> physical, 0 at listed points,
-    -beta*Lu at listed points of tip planes, back to Fourier."""
+@pytest.mark.parametrize("grid", ["u", "v"])
+def test_immersed_boundary_masking_matches_numpy(grid):
+    """immersed_boundaries_U / _V (tridLU_3D_IB.f90:224-245, 304-325): on canopy planes
+    rhs -> physical, 0 at listed points, -beta*Lu at listed points of the tip planes, back to
+    Fourier.  The tip planes follow the reference's tests literally: j == 0 or
+    j == Ngal(4,3)-dsty+1 on the u grid (line 237), j == -1 or j == Ngal(3,3)-dsty+2 on the v
+    grid (line 317), with Ngal(4,3) = ny-1+dsty and Ngal(3,3) = ny-2+dsty (y_grid_canopy.f90:
+    46-52)."""
     s = tiny()
-    u, v, w, p = random_state(s, seed=9)
-    lu = random_state(s, seed=10)[0]
+    nj = s.nyu if grid == "u" else s.nyv
+    lst = s.list_u if grid == "u" else s.list_v
+    rng = np.random.default_rng(9 if grid == "u" else 19)
+    cr = lambda: rng.standard_normal((s.ncol, nj)) + 1j * rng.standard_normal((s.ncol, nj))
+    x, lu = cr(), cr()
     beta = 1.7e-3
     o = Oracle(s)
-    out = o.ib(True, beta, u, lu)
+    out = o.ib(grid == "u", beta, x, lu)
+    if grid == "u":
+        tips = (0, (s.ny - 1 + s.d) - s.d + 1)
+    else:
+        tips = (-1, (s.ny - 2 + s.d) - s.d + 2)
     P = (s.pz, s.px)
-    ref = u.copy()
-    for r in range(s.nyu):
+    ref = x.copy()
+    ntip = 0
+    for r in range(nj):
         j = r - s.d
-        pts = s.list_u[:, s.list_u[2] == j]
+        pts = lst[:, lst[2] == j]
         if pts.shape[1] == 0:
             continue
-        ph = to_phys(s, u[:, r].reshape(s.nz, s.mx), P)
+        ph = to_phys(s, x[:, r].reshape(s.nz, s.mx), P)
         lph = to_phys(s, lu[:, r].reshape(s.nz, s.mx), P)
-        tip = j in s.tips["u"]
+        tip = j in tips
+        ntip += tip
         for i, k in zip(pts[0] - 1, pts[1] - 1):
             ph[k, i] = -beta * lph[k, i] if tip else 0.0
         ref[:, r] = to_spec(s, ph, P).ravel()
+    assert ntip == 2                      # both tip planes hold canopy points
     assert rel(out, ref) < 1e-13
-    untouched = [r for r in range(s.nyu) if (r - s.d) not in set(s.list_u[2].tolist())]
-    assert np.array_equal(out[:, untouched], u[:, untouched])
+    untouched = [r for r in range(nj) if (r - s.d) not in set(lst[2].tolist())]
+    assert np.array_equal(out[:, untouched], x[:, untouched])
 
 
 def test_velocity_lu_reconstructs_matrix():
