@@ -75,6 +75,7 @@ typedef struct lh_handle lh_handle;
  * (start.f90:392-652: FFT initialisation, RK tables, LU_buildP).  All pointers are copied
  * and may be freed on return. */
 int lh_create(const lh_config* cfg, lh_handle** out);
+/* Fails with LH_ERR_ARG while the handle belongs to a batch: destroy the batch first. */
 int lh_destroy(lh_handle* h);
 const char* lh_last_error(const lh_handle* h);
 
@@ -99,7 +100,12 @@ int lh_get_forcing(lh_handle* h, double* f1, double* f2, double* f3);
  * meanflow_ctU (flag_ctpress == 0), solveP, v_corr (littleharsh.f90:166-188).
  * dt <= 0 uses the device CFL time step of the last lh_cfl_dt / lh_step_cfl.  With
  * kRK = 3 and flag_ctpress == 1 it also evaluates the flow rate (meanflow_ctP,
- * littleharsh.f90:193-197), so three lh_substep calls do what one lh_step does. */
+ * littleharsh.f90:193-197), so three lh_substep calls do what one lh_step does.
+ * Non-finite values: the first substep of a step (kRK = 1), lh_step, lh_step_cfl and
+ * lh_batch_step_cfl take the CFL maximum of the state at the start of the step with a
+ * reduction that keeps NaN, and return LH_ERR_NUMERIC when it is not finite (or, for the
+ * CFL-controlled steps, when the time step is not finite and positive).  The kernels run to
+ * completion either way; the state is then not usable. */
 int lh_substep(lh_handle* h, int krk, double dt);
 
 /* One full RK3 step (three substeps) followed by meanflow_ctP when flag_ctpress == 1
@@ -120,7 +126,8 @@ int lh_cfl_dt(lh_handle* h, double cfl, double* dt);
  * the last dt and the last CFL maximum. */
 int lh_get_scalars(lh_handle* h, double* mpgx, double* qx, double* dt, double* cflmax);
 
-/* max |div u| over the p grid (error, littleharsh.f90:323-352). */
+/* max |div u| over the p grid (error, littleharsh.f90:323-352); NaN when the velocity holds
+ * a NaN (the reduction keeps it). */
 int lh_max_div(lh_handle* h, double* err);
 
 /* Statistics (stats.f90:1-108) of the current state: per-plane sums over the physical
@@ -171,6 +178,9 @@ int lh_autotune(lh_handle* h, int steps);
  * CFL step of every instance per call, issued as one captured HIP graph in which the
  * instances are parallel branches, each on its handle's stream (no counterpart in the
  * reference; the instances share nothing). */
+/* A batch re-captures its graph when the CFL number changes or when any member's own graphs
+ * were dropped (lh_set_launch, lh_autotune, lh_set_forcing switching the forcing on or off,
+ * lh_closure_create); each handle may appear once.  Destroy the batch before its members. */
 typedef struct lh_batch lh_batch;
 int lh_batch_create(int n, lh_handle* const* handles, lh_batch** out);
 int lh_batch_step_cfl(lh_batch* b, double cfl);
@@ -245,12 +255,14 @@ int lh_closure_set_mean_only(lh_closure* c, int on);
 int lh_closure_destroy(lh_closure* c);
 const char* lh_closure_last_error(const lh_closure* c);
 
-/* Per-kernel timing (no counterpart in the reference; measurement support).  While
- * enabled, lh_step/lh_step_cfl issue their kernels directly (no graph) with a HIP event
- * pair around every launch on the handle's stream; durations accumulate per kernel
- * class.  lh_timing_read returns the accumulated milliseconds, the launch count and the
- * algorithmic bytes one launch of that class moves (each input read once, each output
- * written once). */
+/* Timing (no counterpart in the reference; measurement support).  Mode 1: lh_step and
+ * lh_step_cfl issue their kernels directly (no graph) with a HIP event pair around every
+ * launch on the handle's stream; durations accumulate per kernel class.  Mode 2: the
+ * captured step graph is replayed as usual, with one event pair around each replay on the
+ * handle's stream (class LH_K_STEP, one launch per RK3 step).  Mode 0: off.
+ * lh_timing_read returns the accumulated milliseconds, the launch count and the algorithmic
+ * bytes one launch of that class moves (each input read once, each output written once;
+ * 0 for LH_K_STEP). */
 #define LH_K_ZINV 0   /* inverse z transforms of u, w, v                  */
 #define LH_K_XNL 1    /* x transforms + products                          */
 #define LH_K_ZFWD 2   /* forward z transforms of the products             */
@@ -261,7 +273,8 @@ const char* lh_closure_last_error(const lh_closure* c);
 #define LH_K_PRES 7   /* pressure solve + v_corr                          */
 #define LH_K_MISC 8   /* small kernels: LU factors, mean flow, scalars     */
 #define LH_K_RHS 9    /* RHS0 + nonlinear + rhs of the solves, per column  */
-#define LH_K_COUNT 10
+#define LH_K_STEP 10  /* one replay of the captured RK3 step graph (mode 2) */
+#define LH_K_COUNT 11
 int lh_timing_enable(lh_handle* h, int enable);
 int lh_timing_read(lh_handle* h, int kernel, double* total_ms, long long* launches,
                    double* bytes_per_launch);

@@ -14,6 +14,9 @@ import numpy as np
 from .channel import ChannelSetup
 
 LH_OK = 0
+LH_ERR_ARG = 1
+LH_ERR_HIP = 2
+LH_ERR_NUMERIC = 3
 LH_PREC_FP64 = 0
 LH_PREC_MIXED = 1
 
@@ -174,4 +177,4 @@ EXPORTED = [
 ]
 
 KERNEL_NAMES = ["zinv", "xpass_nl", "zfwd", "assemble", "ib_z", "ib_x", "vel", "pressure", "misc",
-                "rhs"]
+                "rhs", "step"]

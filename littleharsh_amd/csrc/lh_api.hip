@@ -24,6 +24,7 @@
 #include "../../include/littleharsh_amd.h"
 #include "lh_kernels.hpp"
 #include "lh_nl.hpp"
+#include "lh_xw.hpp"
 #include "lh_col.hpp"
 #include "lh_stats.hpp"
 #include "lh_internal.hpp"
@@ -98,6 +99,8 @@ struct lh_handle {
   double *ayu = nullptr, *ayv = nullptr, *ctwork = nullptr;
   void *twx = nullptr, *twz = nullptr;         // twiddles in the NL precision
   void *twx64 = nullptr, *twz64 = nullptr;     // FP64 twiddles (IB, statistics)
+  // the one-wave-per-line x pass (FP64, 576 collocation points in x): its twiddle table
+  double2* twxw = nullptr;
   // canopy
   unsigned char* masks = nullptr;
   int nmask = 0;
@@ -124,12 +127,17 @@ struct lh_handle {
   // rows per step of the velocity sweep: 1 on large grids (fewer registers, more resident
   // waves), kVelUSmall when few columns leave the sweeps latency-bound (set at creation)
   int vel_u = 1;
-  // per-kernel timing
+  // per-kernel timing (direct issue, an event pair per launch) or, with timing_steps, one
+  // event pair around each replay of the captured step graph
   bool timing = false;
+  bool timing_steps = false;
   // batches the handle belongs to: while any, k_prep stays on the handle's stream, since the
   // batch's graph already runs the instances side by side and a side branch per substep and
   // instance (48 x 3 cross-stream joins per step) serialised them
   int in_batch = 0;
+  // bumped whenever the captured graphs are dropped (launch parameters, forcing switched on
+  // or off, batch membership): a batch re-captures when a member's generation changed
+  unsigned gen = 0;
   std::vector<hipEvent_t> evpool;
   size_t evnext = 0;
   struct Pend { int k; hipEvent_t a, b; };
@@ -140,8 +148,8 @@ struct lh_handle {
 };
 
 // event pair around one launch (timing mode only; never inside graph capture)
-static int tbeg(lh_handle* h, int k, double bytes) {
-  if (!h->timing) return -1;
+static int tbeg(lh_handle* h, int k, double bytes, bool steps = false) {
+  if (!(steps ? h->timing_steps : h->timing)) return -1;
   if (h->evnext + 2 > h->evpool.size()) {
     for (int q = 0; q < 64; ++q) {
       hipEvent_t e;
@@ -188,6 +196,26 @@ static int fail(lh_handle* h, int code, const std::string& msg) {
     if (e_ != hipSuccess)                                                                    \
       return fail(h, LH_ERR_HIP, std::string(#x) + ": " + hipGetErrorString(e_));            \
   } while (0)
+
+// Non-finite detection after a step (the reference's only monitor is `error`,
+// littleharsh.f90:323-352).  The first x pass of every step takes the CFL maximum of the
+// state at the start of the step with a reduction that keeps NaN, so a non-finite value
+// anywhere in u, v or w shows there.  With need_dt the device CFL time step must also be
+// finite and positive.  Reads back the device scalars into s.
+static int numeric_check(lh_handle* h, bool need_dt, double* s) {
+  HIPCHK(h, hipMemcpy(s, h->scal, S_NSCAL * sizeof(double), hipMemcpyDeviceToHost));
+  char msg[160];
+  if (!std::isfinite(s[S_CFLMAX])) {
+    snprintf(msg, sizeof msg, "non-finite velocity at the start of the step (CFL maximum %g)", s[S_CFLMAX]);
+    return fail(h, LH_ERR_NUMERIC, msg);
+  }
+  if (need_dt && !(std::isfinite(s[S_DT]) && s[S_DT] > 0.0)) {
+    snprintf(msg, sizeof msg, "CFL time step %g is not finite and positive (CFL maximum %g)", s[S_DT],
+             s[S_CFLMAX]);
+    return fail(h, LH_ERR_NUMERIC, msg);
+  }
+  return LH_OK;
+}
 
 template <typename T>
 static hipError_t dalloc(T** p, size_t n) {
@@ -277,6 +305,19 @@ static void launch_xraw_t(lh_handle* h, const XRAW& a, const void* tw) {
   dim3 grid(h->dm.pz / 2, (nlev + lc - 1) / lc);
   hipLaunchKernelGGL((k_xraw2<PX, TR>), grid, dim3(6 * Plan<PX>::T), sh, h->st, a, h->dm,
                      (const cx<TR>*)tw, lc);
+}
+// the one-wave-per-line x pass: WPB waves per block, one block per CU (LDS), two waves per
+// SIMD (registers); each wave takes one row pair and xraw_lc levels
+constexpr int kXwWaves = 8;
+constexpr bool kXwOn = false;
+static bool use_xw(const lh_handle* h) { return kXwOn && h->dm.px == 576 && h->prec == LH_PREC_FP64; }
+static void launch_xw(lh_handle* h, int want_cfl) {
+  XW a{(const double2*)h->zu, (const double2*)h->zw, (const double2*)h->zv, (double2*)h->zp, h->twxw,
+       h->dthdyu, h->scal, want_cfl, h->xraw_lc};
+  const int nlev = h->dm.nyu - 2;
+  const long waves = (long)(h->dm.px / 2) * ((nlev + h->xraw_lc - 1) / h->xraw_lc);
+  hipLaunchKernelGGL(k_xw576<kXwWaves>, dim3((unsigned)((waves + kXwWaves - 1) / kXwWaves)), dim3(64 * kXwWaves),
+                     xw_lds_bytes<kXwWaves>(), h->st, a, h->dm);
 }
 constexpr int kZasmCols = LH_ZASM_COLS;
 template <int PZ, typename TR>
@@ -404,7 +445,9 @@ static void issue_substep(lh_handle* h, int krk /*0-based*/, bool cfl_mode) {
   {
     XRAW a{h->zu, h->zw, h->zv, h->zp, h->dthdyu, h->scal, krk == 0 ? 1 : 0};
     TIMED(h, LH_K_XNL, nuvw * zsl + 6.0 * nlev * zsl, {
-      if (f64) {
+      if (use_xw(h)) {
+        launch_xw(h, krk == 0 ? 1 : 0);
+      } else if (f64) {
         LH_DISPATCH_LEN(dm.px, launch_xraw_t, double, h, a, h->twx)
       } else {
         LH_DISPATCH_LEN(dm.px, launch_xraw_t, float, h, a, h->twx)
@@ -604,6 +647,14 @@ int lh_create(const lh_config* c, lh_handle** out) {
     }
     if (bx < 1) bx = 1;
     if (bz < 1) bz = 1;
+    if (use_xw(h)) {
+      // one block of kXwWaves waves per CU; a chunk's start-up is three inverse transforms
+      // against nine transforms per level
+      HIPCHK(h, hipFuncSetAttribute((const void*)k_xw576<kXwWaves>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    (int)xw_lds_bytes<kXwWaves>()));
+      if (h->xraw_lc <= 0)
+        h->xraw_lc = chunk_levels(dm.nyu - 2, (long)dm.px / 2, ncu * kXwWaves, 4, 96, 1.0 / 3.0);
+    }
     if (h->xraw_lc <= 0) h->xraw_lc = chunk_levels(dm.nyu - 2, dm.pz / 2, ncu * bx, 4, 64, 1.0);
     if (h->zasm_lc <= 0)
       h->zasm_lc = chunk_levels(dm.nyu - 2, 3L * ((dm.mx + LH_ZASM_COLS - 1) / LH_ZASM_COLS), ncu * bz, 2, 12, 0.25);
@@ -681,6 +732,21 @@ int lh_create(const lh_config* c, lh_handle** out) {
   };
   if (mk_tw(dm.px, &h->twx64, &h->twx) || mk_tw(dm.pz, &h->twz64, &h->twz))
     return fail(h, LH_ERR_HIP, "twiddle upload failed");
+  if (use_xw(h)) {
+    // w576^{+n1 t} (n1 = 0..8, t = 0..63) and w64^{+j} (j = 0..63)
+    std::vector<double2> t(xw::TWN);
+    for (int n1 = 0; n1 < 9; ++n1)
+      for (int l = 0; l < 64; ++l) {
+        const double a = 2.0 * M_PI * (double)(n1 * l) / 576.0;
+        t[xw::TW576 + n1 * 64 + l] = make_double2(cos(a), sin(a));
+      }
+    for (int j = 0; j < 64; ++j) {
+      const double a = 2.0 * M_PI * (double)j / 64.0;
+      t[xw::TW64 + j] = make_double2(cos(a), sin(a));
+    }
+    HIPCHK(h, dalloc(&h->twxw, t.size()));
+    HIPCHK(h, hipMemcpy(h->twxw, t.data(), t.size() * sizeof(double2), hipMemcpyHostToDevice));
+  }
   // canopy masks: one per distinct footprint; planes -> mask index
   {
     const size_t pp = (size_t)dm.px * dm.pz;
@@ -785,6 +851,8 @@ int lh_create(const lh_config* c, lh_handle** out) {
 
 int lh_destroy(lh_handle* h) {
   if (!h) return LH_OK;
+  // a batch keeps pointers to its members: it must be destroyed first
+  if (h->in_batch > 0) return fail(h, LH_ERR_ARG, "lh_destroy: the handle belongs to a batch; destroy the batch first");
   if (h->st) hipStreamSynchronize(h->st);
   if (h->side) hipStreamSynchronize(h->side);
   for (auto& g : h->gexec)
@@ -794,7 +862,7 @@ int lh_destroy(lh_handle* h) {
                   h->k2z, h->scal, h->u1, h->u2, h->u3, h->p, h->nu1, h->nu2, h->nu3, h->nb1, h->nb2, h->nb3, h->r1,
                   h->r2, h->r3, h->psi, h->tipl, h->zu, h->zw, h->zv, h->zp, h->dg3, h->ayu, h->ayv, h->ctwork, h->masks, h->ibu_planes,
                   h->ibv_planes, h->ibflag_u, h->ibflag_v, h->ibent, h->st_psum, h->st_pfold, h->st_acc,
-                  h->cst_acc, h->sp_acc, h->fo1, h->fo2, h->fo3};
+                  h->cst_acc, h->sp_acc, h->fo1, h->fo2, h->fo3, h->twxw};
   for (void* q : ptrs)
     if (q) hipFree(q);
   if (h->twx && h->twx != h->twx64) hipFree(h->twx);
@@ -902,6 +970,10 @@ int lh_substep(lh_handle* h, int krk, double dt) {
   HIPCHK(h, hipGetLastError());
   HIPCHK(h, hipStreamSynchronize(h->st));
   tflush(h);
+  if (krk == 1) {   // the first substep takes the CFL maximum of the state
+    double s[S_NSCAL];
+    return numeric_check(h, false, s);
+  }
   return LH_OK;
 }
 
@@ -918,7 +990,9 @@ static int run_graph(lh_handle* h, int mode) {
     HIPCHK(h, hipGraphInstantiate(&h->gexec[mode], g, nullptr, nullptr, 0));
     HIPCHK(h, hipGraphDestroy(g));
   }
+  const int ti = tbeg(h, LH_K_STEP, 0.0, true);
   HIPCHK(h, hipGraphLaunch(h->gexec[mode], h->st));
+  tend(h, ti);
   return LH_OK;
 }
 
@@ -931,7 +1005,8 @@ int lh_step(lh_handle* h, double dt) {
   HIPCHK(h, hipGetLastError());
   HIPCHK(h, hipStreamSynchronize(h->st));
   tflush(h);
-  return LH_OK;
+  double s[S_NSCAL];
+  return numeric_check(h, false, s);
 }
 
 int lh_step_cfl(lh_handle* h, double cfl, double* dt_used) {
@@ -943,8 +1018,10 @@ int lh_step_cfl(lh_handle* h, double cfl, double* dt_used) {
   HIPCHK(h, hipGetLastError());
   HIPCHK(h, hipStreamSynchronize(h->st));
   tflush(h);
-  if (dt_used) HIPCHK(h, hipMemcpy(dt_used, h->scal + S_DT, 8, hipMemcpyDeviceToHost));
-  return LH_OK;
+  double s[S_NSCAL];
+  r = numeric_check(h, true, s);
+  if (dt_used) *dt_used = s[S_DT];
+  return r;
 }
 
 int lh_get_scalars(lh_handle* h, double* mpgx, double* qx, double* dt, double* cflmax) {
@@ -982,7 +1059,9 @@ int lh_cfl_dt(lh_handle* h, double cfl, double* dt) {
   jobs.j[2] = ZJob{h->u2, h->zv, nullptr, 0, dm.nyv, 0};
   launch_zinv(h, jobs, 3, dm.nyu, false);
   XRAW a{h->zu, h->zw, h->zv, h->zp, h->dthdyu, h->scal, 1};
-  if (h->prec == LH_PREC_FP64) {
+  if (use_xw(h)) {
+    launch_xw(h, 1);
+  } else if (h->prec == LH_PREC_FP64) {
     LH_DISPATCH_LEN(dm.px, launch_xraw_t, double, h, a, h->twx)
   } else {
     LH_DISPATCH_LEN(dm.px, launch_xraw_t, float, h, a, h->twx)
@@ -1000,9 +1079,11 @@ int lh_cfl_dt(lh_handle* h, double cfl, double* dt) {
 
 int lh_timing_enable(lh_handle* h, int enable) {
   if (!h) return LH_ERR_ARG;
+  if (enable < 0 || enable > 2) return fail(h, LH_ERR_ARG, "lh_timing_enable: mode 0, 1 or 2");
   HIPCHK(h, hipStreamSynchronize(h->st));
   tflush(h);
-  h->timing = enable != 0;
+  h->timing = enable == 1;
+  h->timing_steps = enable == 2;
   return LH_OK;
 }
 
@@ -1027,6 +1108,7 @@ int lh_timing_read(lh_handle* h, int kernel, double* total_ms, long long* launch
 static void drop_graphs(lh_handle* h) {
   for (auto& g : h->gexec)
     if (g) { (void)hipGraphExecDestroy(g); g = nullptr; }
+  h->gen += 1;
 }
 
 int lh_set_launch(lh_handle* h, int xraw_lc, int zasm_lc, int col_bs) {
@@ -1176,6 +1258,7 @@ int lh_autotune(lh_handle* h, int steps) {
 // ---- many independent channels advanced by one graph launch per step
 struct lh_batch {
   std::vector<lh_handle*> hs;
+  std::vector<unsigned> gens;   // member generations the graph was captured at
   hipStream_t master = nullptr;
   hipGraphExec_t g = nullptr;
   double cfl = -1.0;
@@ -1190,6 +1273,8 @@ int lh_batch_create(int n, lh_handle* const* hs, lh_batch** out) {
   lh_batch* b = new lh_batch();
   for (int i = 0; i < n; ++i) {
     if (!hs[i]) { delete b; return LH_ERR_ARG; }
+    // a handle listed twice would be stepped twice per call on its one stream
+    if (std::find(b->hs.begin(), b->hs.end(), hs[i]) != b->hs.end()) { delete b; return LH_ERR_ARG; }
     b->hs.push_back(hs[i]);
   }
   if (hipStreamCreate(&b->master) != hipSuccess) { delete b; return LH_ERR_HIP; }
@@ -1215,7 +1300,9 @@ int lh_batch_step_cfl(lh_batch* b, double cfl) {
   if (!(cfl > 0.0)) { b->err = "cfl must be positive"; return LH_ERR_ARG; }
   for (auto* h : b->hs)
     if (h->timing) { b->err = "disable timing on every handle of a batch"; return LH_ERR_ARG; }
-  if (!b->g || b->cfl != cfl) {
+  bool stale = b->gens.size() != b->hs.size();
+  for (size_t i = 0; i < b->hs.size() && !stale; ++i) stale = b->gens[i] != b->hs[i]->gen;
+  if (!b->g || b->cfl != cfl || stale) {
     if (b->g) { (void)hipGraphExecDestroy(b->g); b->g = nullptr; }
     hipGraph_t g;
     auto chk = [&](hipError_t e, const char* what) {
@@ -1240,10 +1327,20 @@ int lh_batch_step_cfl(lh_batch* b, double cfl) {
     }
     (void)hipGraphDestroy(g);
     b->cfl = cfl;
+    b->gens.clear();
+    for (auto* h : b->hs) b->gens.push_back(h->gen);
   }
   if (hipGraphLaunch(b->g, b->master) != hipSuccess || hipStreamSynchronize(b->master) != hipSuccess) {
     b->err = "batch graph launch failed";
     return LH_ERR_HIP;
+  }
+  for (size_t i = 0; i < b->hs.size(); ++i) {
+    double s[S_NSCAL];
+    const int r = numeric_check(b->hs[i], true, s);
+    if (r != LH_OK) {
+      b->err = "instance " + std::to_string(i) + ": " + b->hs[i]->err;
+      return r;
+    }
   }
   return LH_OK;
 }

@@ -275,9 +275,14 @@ __device__ __forceinline__ void unpack_rows(const C* line, int k, C& a, C& b) {
 
 
 __device__ __forceinline__ void atomic_max_pos(double* addr, double v) {
-  // non-negative doubles order like their bit patterns
+  // non-negative doubles order like their bit patterns; a NaN (exponent all ones, non-zero
+  // mantissa) compares above +inf, so a NaN operand always wins the maximum
   atomicMax(reinterpret_cast<unsigned long long*>(addr), (unsigned long long)__double_as_longlong(v));
 }
+
+// Maximum that keeps a NaN: fmax would drop it and report a finite maximum of a field that
+// holds non-finite values.
+__device__ __forceinline__ double nanmax(double e, double v) { return (v > e || v != v) ? v : e; }
 
 // Immersed-boundary x pass: rhs rows -> physical, rhs = 0 on canopy points, -beta*Lu_y on
 // canopy points of tip planes, -> Fourier (tridLU_3D_IB.f90:224-245, 306-325)
@@ -518,10 +523,11 @@ __global__ void __launch_bounds__(256) k_maxdiv(const double2* u1, const double2
       const size_t ou = (size_t)(r + 1) * dm.ncol + c;
       const double2 dvg = dadd(dadd(dim(kx, u1[ou]), dim(kz, u3[ou])),
                                drm(dm.ddthetavi * dthdyu[r + 1], dsub(u2[ou], u2[ou - dm.ncol])));
-      e = fmax(e, hypot(dvg.x, dvg.y));
+      // |div| without hypot, which returns +inf for (inf, NaN): sqrt of the sum keeps NaN
+      e = nanmax(e, sqrt(dvg.x * dvg.x + dvg.y * dvg.y));
     }
   }
-  for (int off = 32; off > 0; off >>= 1) e = fmax(e, __shfl_down(e, off, 64));
+  for (int off = 32; off > 0; off >>= 1) e = nanmax(e, __shfl_down(e, off, 64));
   if ((threadIdx.x & 63) == 0) atomic_max_pos(scal + S_MAXDIV, e);
 }
 

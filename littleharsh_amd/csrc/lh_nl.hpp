@@ -170,7 +170,7 @@ __global__ void __launch_bounds__(6 * Plan<PX>::T)
         if (a.want_cfl) {
           const double s0 = fabs((double)U.x) * dm.dxi + fabs((double)vc0) * dyi + fabs((double)W.x) * dm.dzi;
           const double s1 = fabs((double)U.y) * dm.dxi + fabs((double)vc1) * dyi + fabs((double)W.y) * dm.dzi;
-          cmax = fmax(cmax, fmax(s0, s1));
+          cmax = nanmax(cmax, nanmax(s0, s1));
         }
       }
     }
@@ -237,13 +237,13 @@ __global__ void __launch_bounds__(6 * Plan<PX>::T)
     const int act = min(64, (int)blockDim.x - wv * 64);
     for (int off = 32; off > 0; off >>= 1) {
       const double o = __shfl_down(cmax, off, 64);
-      if (lane + off < act) cmax = fmax(cmax, o);
+      if (lane + off < act) cmax = nanmax(cmax, o);
     }
     if (lane == 0) red[wv] = cmax;
     __syncthreads();
     if (tid == 0) {
       double m = 0.0;
-      for (int q = 0; q < (int)((blockDim.x + 63) / 64); ++q) m = fmax(m, red[q]);
+      for (int q = 0; q < (int)((blockDim.x + 63) / 64); ++q) m = nanmax(m, red[q]);
       atomic_max_pos(a.scal + S_CFLMAX, m);
     }
   }

@@ -23,7 +23,12 @@ def _p(a):
 
 
 class LHError(RuntimeError):
-    pass
+    """A non-zero status of the C-ABI; ``status`` holds it (LH_ERR_ARG, LH_ERR_HIP,
+    LH_ERR_NUMERIC)."""
+
+    def __init__(self, msg, status=None):
+        super().__init__(msg)
+        self.status = status
 
 
 class Channel:
@@ -43,16 +48,18 @@ class Channel:
         del keep
 
     def close(self):
+        """Destroy the handle (a batch holding it must be closed first: lh_destroy refuses,
+        and the handle then stays alive)."""
         if getattr(self, "_h", None):
-            self._lib.lh_destroy(self._h)
-            self._h = None
+            if self._lib.lh_destroy(self._h) == capi.LH_OK:
+                self._h = None
 
     def __del__(self):
         self.close()
 
     def _chk(self, r, what):
         if r != capi.LH_OK:
-            raise LHError(f"{what}: {self._lib.lh_last_error(self._h).decode()} (status {r})")
+            raise LHError(f"{what}: {self._lib.lh_last_error(self._h).decode()} (status {r})", r)
 
     # ---- state
     def set_state(self, u, v, w, p):
@@ -172,8 +179,10 @@ class Channel:
         return self.get_launch()
 
     # ---- per-kernel timing (HIP events on the handle's stream)
-    def timing(self, enable: bool):
-        self._chk(self._lib.lh_timing_enable(self._h, 1 if enable else 0), "lh_timing_enable")
+    def timing(self, enable):
+        """False/0 off, True/1 per-kernel events (direct issue), 2 per-step events around
+        the graph replays."""
+        self._chk(self._lib.lh_timing_enable(self._h, int(enable)), "lh_timing_enable")
 
     def timing_reset(self):
         self._chk(self._lib.lh_timing_reset(self._h), "lh_timing_reset")
@@ -199,14 +208,14 @@ class Batch:
         b = ctypes.c_void_p()
         r = self._lib.lh_batch_create(len(self.channels), arr, ctypes.byref(b))
         if r != capi.LH_OK:
-            raise LHError(f"lh_batch_create failed (status {r})")
+            raise LHError(f"lh_batch_create failed (status {r})", r)
         self._b = b
 
     def step_cfl(self, cfl: float):
         r = self._lib.lh_batch_step_cfl(self._b, float(cfl))
         if r != capi.LH_OK:
             raise LHError(f"lh_batch_step_cfl: {self._lib.lh_batch_last_error(self._b).decode()} "
-                          f"(status {r})")
+                          f"(status {r})", r)
 
     def close(self):
         if getattr(self, "_b", None):

@@ -61,3 +61,59 @@ def test_batch_matches_single_instances(require_gpu):
         for x, y in zip(c.get_state(), r.get_state()):
             assert np.array_equal(x, y)
     b.close()
+
+
+def test_batch_recaptures_when_a_member_changes(require_gpu):
+    """Switching a member's forcing on (and off) after the batch graph was captured must be
+    seen by the next batch step: the batch re-captures when a member's graphs were dropped."""
+    from littleharsh_amd.solver import Batch, Channel
+    s = tiny()
+    st = laminar_plus_noise(s, seed=44)
+    a, b, ra, rb = Channel(s), Channel(s), Channel(s), Channel(s)
+    for c in (a, b, ra, rb):
+        c.set_state(*st)
+    rng = np.random.default_rng(45)
+    f = [rng.standard_normal((s.ncol, n)) * 1e-2 + 0j for n in (s.nyu - 2, s.nyv - 2, s.nyu - 2)]
+    bt = Batch([a, b])
+    bt.step_cfl(0.5)
+    ra.step_cfl(0.5)
+    rb.step_cfl(0.5)
+    b.set_forcing(*f)
+    rb.set_forcing(*f)
+    bt.step_cfl(0.5)
+    ra.step_cfl(0.5)
+    rb.step_cfl(0.5)
+    b.set_forcing()
+    rb.set_forcing()
+    bt.step_cfl(0.5)
+    ra.step_cfl(0.5)
+    rb.step_cfl(0.5)
+    for c, r in ((a, ra), (b, rb)):
+        for x, y in zip(c.get_state(), r.get_state()):
+            assert np.array_equal(x, y)
+    # the forcing made a difference (the check above is not vacuous)
+    assert not np.array_equal(a.get_state()[0], b.get_state()[0])
+    bt.close()
+
+
+def test_destroy_order_is_enforced(require_gpu):
+    from littleharsh_amd import capi
+    from littleharsh_amd.solver import Batch, Channel
+    s = tiny()
+    c = Channel(s)
+    c.set_state(*laminar_plus_noise(s, seed=46))
+    bt = Batch([c])
+    lib = capi.load_library()
+    assert lib.lh_destroy(c._h) == capi.LH_ERR_ARG     # still a member
+    bt.step_cfl(0.5)                                    # the handle is intact
+    bt.close()
+    assert lib.lh_destroy(c._h) == capi.LH_OK
+    c._h = None
+
+
+def test_batch_rejects_a_repeated_handle(require_gpu):
+    from littleharsh_amd.solver import Batch, Channel, LHError
+    s = tiny()
+    c = Channel(s)
+    with pytest.raises(LHError):
+        Batch([c, c])

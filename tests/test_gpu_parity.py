@@ -96,13 +96,18 @@ def test_constant_flow_rate_tiny(require_gpu):
     errs = report("ctU 3 steps", g.get_state(), o.get_state())
     assert max(errs.values()) < TOL
     gm, om = g.scalars()["mpgx"], o.scalars()["mpgx"]
-    print("mpgx", gm, om, flush=True)
-    # g = (QxT - Qx0)/Qx1 (littleharsh.f90:478): Qx0 ~ QxT ~ 120 while Qx1 ~ 2*dRK*dt is
-    # O(1e-4), so a round-off of eps*Qx in Qx0 moves mpgx by ~eps*Qx/Qx1 ~ 1e-10 per
-    # substep.  The tolerance is 1e3 times that bound over the nine substeps.
-    qx1 = 2.0 * (8.0 / 15.0) * dt
-    bound = 9 * 2.2e-16 * 120.0 / qx1
-    assert abs(gm - om) < 1e3 * bound
+    # Each substep adds g = (QxT - Qx0)/Qx1 to mpgx (flowrate_corr, littleharsh.f90:453-485).
+    # Qx0 is a sum of n = ny-1 terms dy*u between the tips (flowrateRe) of total ~QxT; the
+    # GPU and the oracle add them in different orders, so their g differ by at most
+    # n*eps*QxT/|Qx1|.  The unit response u11 is -dRK*dt between the tips up to O(beta), so
+    # |Qx1| = dRK*dt*(yv(ny-1) - yv(0)).  The bound is that worst case summed over the nine
+    # substeps (about 7e-8 here; a correction error of 1e-7 would fail).
+    eps = np.finfo(float).eps
+    span = s.grid.yv[s.ny - 1 + s.d] - s.grid.yv[s.d]
+    dRK = s.rk.ravel()[12:15]
+    bound = 3 * sum((s.ny - 1) * eps * s.qxt / (d * dt * span) for d in dRK)
+    print(f"mpgx gpu {gm:.17e} oracle {om:.17e} |diff| {abs(gm - om):.2e} bound {bound:.2e}", flush=True)
+    assert abs(gm - om) < bound
 
 
 def test_one_substep_production(require_gpu):
