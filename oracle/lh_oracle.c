@@ -186,11 +186,12 @@ static double* dupd(const double* a, size_t n) {
 /* phys_to_four: forward rft along x (x 1/px, rft_buff.f:63-73), forward cft along z    */
 /* (x 1/pz, cft_buff.f:37-46), truncation to the nx x nz modes.                          */
 /* ------------------------------------------------------------------------------------ */
-static void four_to_phys(const lho* o, const cplx* sp, double* ph, cplx* work /* >= pz*mx + 2*px */) {
+static void four_to_phys(const lho* o, const cplx* sp, double* ph, cplx* work /* >= pz*mx + 2*max(px,pz) */) {
   const int mx = o->mx, nz = o->nz, pz = o->pz, px = o->px;
   cplx* hz = work;                 /* [z][i], i < mx */
   cplx* line = work + (size_t)pz * mx;
-  cplx* lout = line + px;
+  /* line holds a z line (pz) or an x line (px): the output starts after the longer one */
+  cplx* lout = line + (px > pz ? px : pz);
   cplx* col = lout;                /* reuse */
   for (int i = 0; i < mx; ++i) {
     for (int z = 0; z < pz; ++z) line[z] = (cplx){0.0, 0.0};

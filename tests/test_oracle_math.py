@@ -83,9 +83,30 @@ def random_state(s, seed=3):
     return u, v, w, p
 
 
-def test_nonlinear_matches_numpy_same_padding():
+# (nx, nz): wider in x, wider in z (the z line is then longer than the x line) and square
+SHAPES = [(16, 8), (8, 16), (16, 16)]
+
+
+@pytest.mark.parametrize("nx,nz", SHAPES)
+def test_plane_transforms_match_numpy(nx, nz):
+    """Both plane transforms of the oracle against numpy's, for every aspect of the plane."""
+    s = tiny(nx=nx, nz=nz)
+    o = Oracle(s)
+    rng = np.random.default_rng(nx + 7 * nz)
+    sp = rng.standard_normal((s.nz, s.mx)) + 1j * rng.standard_normal((s.nz, s.mx))
+    sp[:, 0] = sp[:, 0].real
+    sp[s.nz // 2:, 0] = np.conj(sp[1:s.nz // 2 + 1, 0][::-1])    # kx = 0 column Hermitian in z
+    sp[s.nz // 2, 0] = sp[s.nz // 2, 0].real
+    ph = o.four_to_phys(sp.ravel())
+    assert rel(ph, to_phys(s, sp, (s.pz, s.px))) < 1e-13
+    ph = rng.standard_normal((s.pz, s.px))
+    assert rel(o.phys_to_four(ph).reshape(s.nz, s.mx), to_spec(s, ph, (s.pz, s.px))) < 1e-13
+
+
+@pytest.mark.parametrize("nx,nz", SHAPES)
+def test_nonlinear_matches_numpy_same_padding(nx, nz):
     """Same 3/2 padding, independent implementation (numpy FFTs, dict-based planes)."""
-    s = tiny()
+    s = tiny(nx=nx, nz=nz)
     u, v, w, p = random_state(s)
     o = Oracle(s)
     o.set_state(u, v, w, p)
@@ -222,11 +243,12 @@ def test_step_is_divergence_free():
     assert o.max_div() < 1e-10
 
 
-def test_cfl_maximum_matches_numpy():
+@pytest.mark.parametrize("nx,nz", SHAPES)
+def test_cfl_maximum_matches_numpy(nx, nz):
     """max over centres and collocation points of |u|/dx + |v_c| dthdyu + |w|/dz, v_c the
     average of the faces below and above (the CFL restatement; dtc/dti are declared at
     declaration.f90:51 but set by the absent nonlinear routine)."""
-    s = tiny()
+    s = tiny(nx=nx, nz=nz)
     u, v, w, p = random_state(s, seed=12)
     o = Oracle(s)
     o.set_state(u, v, w, p)
