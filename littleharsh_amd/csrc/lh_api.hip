@@ -309,7 +309,7 @@ static void launch_xraw_t(lh_handle* h, const XRAW& a, const void* tw) {
 // the one-wave-per-line x pass: WPB waves per block, one block per CU (LDS), two waves per
 // SIMD (registers); each wave takes one row pair and xraw_lc levels
 constexpr int kXwWaves = 8;
-constexpr bool kXwOn = false;
+constexpr bool kXwOn = true;
 static bool use_xw(const lh_handle* h) { return kXwOn && h->dm.px == 576 && h->prec == LH_PREC_FP64; }
 static void launch_xw(lh_handle* h, int want_cfl) {
   XW a{(const double2*)h->zu, (const double2*)h->zw, (const double2*)h->zv, (double2*)h->zp, h->twxw,

@@ -264,27 +264,33 @@ __global__ void __launch_bounds__(256) k_pres(COL a, Dims dm) {
   const double k2x = a.k2x[i], k2z = a.k2z[k];
   double a3p = 0.0;
   int r = 0;
+  // rows in chunks of U; every load of a chunk is issued (rows past the last one read the
+  // last one again), so that a chunk waits for memory once
+  const double dypin = a.yu[2 + dm.d] - a.yu[1 + dm.d];
   for (; r < np; r += U) {
     double2 a1[U], a3[U], a2[U];
+    double hu[U], hv[U + 1];              // dthdyu(rr+1), dthdyv(rr) of the chunk's rows
 #pragma unroll
     for (int q = 0; q < U; ++q) {
-      if (r + q < np) {
-        const size_t ou = (size_t)(r + q + 1) * ncol + c;
-        a1[q] = u1[ou];
-        a3[q] = u3[ou];
-        a2[q] = u2[ou];
-      }
+      const int rq = min(r + q, np - 1);
+      const size_t ou = (size_t)(rq + 1) * ncol + c;
+      a1[q] = u1[ou];
+      a3[q] = u3[ou];
+      a2[q] = u2[ou];
+      hu[q] = a.dthdyu[rq + 1];
     }
+#pragma unroll
+    for (int q = 0; q <= U; ++q) hv[q] = a.dthdyv[min(r + q, np)];
 #pragma unroll
     for (int q = 0; q < U; ++q) {
       const int rr = r + q;
       if (rr < np) {
-        const PoisRow w = poisson_row(rr, c == 0, k2x, k2z, np, dv, a.dthdyu, a.dthdyv, a.yu, dm.d);
+        const PoisRow w = poisson_row_v(rr, c == 0, k2x, k2z, np, dv, hu[q], hv[q], hv[q + 1], dypin);
         const double g1 = w.lo;
         const double g2 = 1.0 / (w.dg - w.lo * a3p);
         a3p = w.up * g2;
         const double2 dvg = dadd(dadd(dim(kx, a1[q]), dim(kz, a3[q])),
-                                 drm(dv * a.dthdyu[rr + 1], dsub(a2[q], vlow)));
+                                 drm(dv * hu[q], dsub(a2[q], vlow)));
         vlow = a2[q];
         double2 ps = drm(dtirk, dvg);
         if (c == 0 && rr == 0) ps = d2(0.0, 0.0);
@@ -308,19 +314,18 @@ __global__ void __launch_bounds__(256) k_pres(COL a, Dims dm) {
   r = np - 2;
   for (; r >= 0; r -= U) {
     double2 sv[U], pv[U], a1[U], a3[U], a2[U];
-    double g3[U];
+    double g3[U], hv[U];
 #pragma unroll
     for (int q = 0; q < U; ++q) {
-      const int rr = r - q;
-      if (rr >= 0) {
-        const size_t o = (size_t)rr * ncol + c, ou = (size_t)(rr + 1) * ncol + c;
-        sv[q] = psi[o];
-        g3[q] = a.dg3[o];
-        pv[q] = pp[o];
-        a1[q] = u1[ou];
-        a3[q] = u3[ou];
-        a2[q] = u2[ou];
-      }
+      const int rr = max(r - q, 0);
+      const size_t o = (size_t)rr * ncol + c, ou = (size_t)(rr + 1) * ncol + c;
+      sv[q] = psi[o];
+      g3[q] = a.dg3[o];
+      pv[q] = pp[o];
+      a1[q] = u1[ou];
+      a3[q] = u3[ou];
+      a2[q] = u2[ou];
+      hv[q] = a.dthdyv[rr + 1];
     }
 #pragma unroll
     for (int q = 0; q < U; ++q) {
@@ -331,7 +336,7 @@ __global__ void __launch_bounds__(256) k_pres(COL a, Dims dm) {
         pp[o] = dadd(pv[q], ps);
         u1[ou] = dsub(a1[q], dim(kxr, ps));
         u3[ou] = dsub(a3[q], dim(kzr, ps));
-        u2[ou] = dsub(a2[q], drm(dtrk_u2 * a.dthdyv[rr + 1], dsub(pnext, ps)));
+        u2[ou] = dsub(a2[q], drm(dtrk_u2 * hv[q], dsub(pnext, ps)));
         pnext = ps;
       }
     }

@@ -436,38 +436,41 @@ struct CTU {
 struct PoisRow {
   double lo, dg, up;
 };
-__device__ __forceinline__ PoisRow poisson_row(int r, bool pin, double k2x, double k2z, int np, double dv,
-                                               const double* __restrict__ dthdyu,
-                                               const double* __restrict__ dthdyv,
-                                               const double* __restrict__ yu, int d) {
+// Row r of the Poisson operator from the metrics it touches: hu = dthdyu(r+1), hv0 = dthdyv(r),
+// hv1 = dthdyv(r+1); dypin = yu(2+d) - yu(1+d), the row pinned in column 0.
+__device__ __forceinline__ PoisRow poisson_row_v(int r, bool pin, double k2x, double k2z, int np, double dv,
+                                                 double hu, double hv0, double hv1, double dypin) {
   // D_vec_b(j) = -dv*dthdyu(j+1), G_vec_b(j) = -dv*dthdyv(j), D_vec_t(j) = dv*dthdyu(j),
-  // G_vec_t(j) = dv*dthdyv(j-1)
-  auto DB = [&](int q) { return -dv * dthdyu[q + 2]; };
-  auto GB = [&](int q) { return -dv * dthdyv[q + 1]; };
-  auto DT = [&](int q) { return dv * dthdyu[q + 1]; };
-  auto GT = [&](int q) { return dv * dthdyv[q]; };
+  // G_vec_t(j) = dv*dthdyv(j-1); in row r: DB(r-1) = -dv hu, GB(r-1) = -dv hv0, DT(r) = dv hu,
+  // GT(r) = dv hv0, GB(r) = -dv hv1, GT(r+1) = dv hv1
   PoisRow w;
   if (r == 0) {
     w.lo = 0.0;
     if (pin) {
-      const double dy = yu[2 + d] - yu[1 + d];
-      w.dg = 1.0 / (dy * dy);
+      w.dg = 1.0 / (dypin * dypin);
       w.up = 0.0;
     } else {
-      w.dg = (DT(0) * GB(0)) + k2x + k2z;
-      w.up = (DT(0) * GT(1));
+      w.dg = ((dv * hu) * (-dv * hv1)) + k2x + k2z;
+      w.up = ((dv * hu) * (dv * hv1));
     }
   } else {
-    w.lo = (DB(r - 1) * GB(r - 1));
+    w.lo = ((-dv * hu) * (-dv * hv0));
     if (r < np - 1) {
-      w.dg = (DB(r - 1) * GT(r)) + (DT(r) * GB(r)) + k2x + k2z;
-      w.up = (DT(r) * GT(r + 1));
+      w.dg = ((-dv * hu) * (dv * hv0)) + ((dv * hu) * (-dv * hv1)) + k2x + k2z;
+      w.up = ((dv * hu) * (dv * hv1));
     } else {
-      w.dg = (DB(r - 1) * GT(r)) + k2x + k2z;
+      w.dg = ((-dv * hu) * (dv * hv0)) + k2x + k2z;
       w.up = 0.0;
     }
   }
   return w;
+}
+__device__ __forceinline__ PoisRow poisson_row(int r, bool pin, double k2x, double k2z, int np, double dv,
+                                               const double* __restrict__ dthdyu,
+                                               const double* __restrict__ dthdyv,
+                                               const double* __restrict__ yu, int d) {
+  return poisson_row_v(r, pin, k2x, k2z, np, dv, dthdyu[r + 1], dthdyv[r], dthdyv[r + 1],
+                       yu[2 + d] - yu[1 + d]);
 }
 
 // LU_decP once at start (tridLU_3D_IB.f90:643-684): the modified uppers a3 = up / pivot,

@@ -262,6 +262,7 @@ struct ZASM {
   int lc;                                    // levels per block
 };
 
+// items per load batch of the z assembly (more loads in flight against more registers)
 #ifndef LH_ZASM_ZB
 #define LH_ZASM_ZB 1
 #endif
@@ -275,6 +276,7 @@ __global__ void __launch_bounds__(2 * L * Plan<PZ>::T)
   constexpr int NPR = PZ / 2;                          // row pairs
   constexpr int ITZ = (L * NPR + NT - 1) / NT;         // (column, row pair) items per thread
   constexpr int NZ = 2 * PZ / 3;
+  constexpr int ZB = LH_ZASM_ZB < ITZ ? LH_ZASM_ZB : ITZ;   // items per load batch
   extern __shared__ char smem_raw[];
   C* lds = reinterpret_cast<C*>(smem_raw);
   const int tid = threadIdx.x;
@@ -320,64 +322,91 @@ __global__ void __launch_bounds__(2 * L * Plan<PZ>::T)
   // of column k are (P[k] + conj P[px-k]) / 2 and (P[k] - conj P[px-k]) / 2i, times 1/px.
   // (1/px)(x/2) = (1/(2px)) x exactly: halving is exact
   const TR sxh = (TR)0.5 * ((TR)1 / (TR)dm.px);
-  auto row_pair = [&](const C* row, int k, C& ra, C& rb) {
-    const C zk = row[k], zn = k == 0 ? zk : row[mx + k];
+  // Item q of this thread: column i0 + e % L and row pair e / L, e = tid + q NT.  Items
+  // outside the plane read a valid element (clamped) and are masked afterwards, so that every
+  // load of a level is issued unconditionally and the level waits for memory once.  Column 0
+  // reads its packed value twice (slot mx + 0 is never written); the other columns read
+  // P[k] and P[px-k].
+  auto item_col = [&](int q) { return min(i0 + (tid + q * NT) % L, mx - 1); };
+  auto item_ok = [&](int q) {
+    const int e = tid + q * NT;
+    return (ITZ * NT == L * NPR || e < L * NPR) && i0 + e % L < mx;
+  };
+  auto item_ro = [&](int q) { return min((tid + q * NT) / L, NPR - 1) * 2 * mx + item_col(q); };
+  auto item_rn = [&](int q) { return item_col(q) == 0 ? item_ro(q) : item_ro(q) + mx; };
+  auto split = [&](C zk, C zn, bool k0, C& ra, C& rb) {
     ra = mk<C>(sxh * (zk.x + zn.x), sxh * (zk.y - zn.y));
     rb = mk<C>(sxh * (zk.y + zn.y), -(sxh * (zk.x - zn.x)));
-    if (k == 0) { ra.y = 0; rb.y = 0; }
+    if (k0) { ra.y = 0; rb.y = 0; }
   };
   // lower operand of the y difference at the first level (rows 2p, 2p+1 of item q)
   C rl[ITZ][2];
   {
     const int l_lo = f < 2 ? l0 - 1 : l0;
+    const C* row = zo + qd * ps + (size_t)max(l_lo, 0) * slot;
+    C zk[ITZ], zn[ITZ];
 #pragma unroll
     for (int q = 0; q < ITZ; ++q) {
-      const int e = tid + q * NT;
-      const int l2 = e % L, pr = e / L, i = i0 + l2;
-      rl[q][0] = mk<C>(0, 0);
-      rl[q][1] = mk<C>(0, 0);
-      if (e < L * NPR && i < mx && l_lo >= 0)
-        row_pair(zo + qd * ps + (size_t)l_lo * slot + (size_t)pr * 2 * mx, i, rl[q][0], rl[q][1]);
+      zk[q] = row[item_ro(q)];
+      zn[q] = row[item_rn(q)];
+    }
+#pragma unroll
+    for (int q = 0; q < ITZ; ++q) {
+      split(zk[q], zn[q], item_col(q) == 0, rl[q][0], rl[q][1]);
+      if (!item_ok(q) || l_lo < 0) rl[q][0] = rl[q][1] = mk<C>(0, 0);
     }
   }
+  // Nu stores of this thread: element e = tid + n NT of the L x NZ retained block
+  constexpr int NW = (L * NZ + NT - 1) / NT;
   const double sz = 1.0 / (double)PZ;
   for (int l = l0; l < l1; ++l) {
     const int l_hi = f < 2 ? l : l + 1;
     const TR mlev = (TR)(dm.ddthetavi * (f < 2 ? a.dthdyu[l + 1] : a.dthdyv[l + 1]));
-    // loads batched LH_ZASM_ZB items at a time: the carried rows and the twiddles hold registers
+    const C* row0 = zo + q0 * ps + (size_t)l * slot;
+    const C* rowb = zo + qb * ps + (size_t)l * slot;
+    const C* rowh = zo + qd * ps + (size_t)l_hi * slot;
+    // items in batches of ZB: the loads of a batch are issued together, then combined
 #pragma unroll
-    for (int q1 = 0; q1 < ITZ; q1 += LH_ZASM_ZB) {
-      C r0[LH_ZASM_ZB][2], rh[LH_ZASM_ZB][2], rb[LH_ZASM_ZB][2];
+    for (int q1 = 0; q1 < ITZ; q1 += ZB) {
+      C z0[ZB][2], zb[ZB][2], zh[ZB][2];
+      TR kxq[ZB];
 #pragma unroll
-      for (int u = 0; u < LH_ZASM_ZB; ++u) {
-        const int q = q1 + u;
-        const int e = tid + q * NT;
-        const int l2 = e % L, pr = e / L, i = i0 + l2;
-        if (q < ITZ && e < L * NPR && i < mx) {
-          const size_t o = (size_t)pr * 2 * mx;
-          row_pair(zo + q0 * ps + (size_t)l * slot + o, i, r0[u][0], r0[u][1]);
-          row_pair(zo + qb * ps + (size_t)l * slot + o, i, rb[u][0], rb[u][1]);
-          row_pair(zo + qd * ps + (size_t)l_hi * slot + o, i, rh[u][0], rh[u][1]);
-        }
+      for (int u = 0; u < ZB; ++u) {
+        const int q = min(q1 + u, ITZ - 1);
+        const int o = item_ro(q), on = item_rn(q);
+        z0[u][0] = row0[o];
+        z0[u][1] = row0[on];
+        zb[u][0] = rowb[o];
+        zb[u][1] = rowb[on];
+        zh[u][0] = rowh[o];
+        zh[u][1] = rowh[on];
+        kxq[u] = (TR)a.k1x[item_col(q)];
       }
 #pragma unroll
-      for (int u = 0; u < LH_ZASM_ZB; ++u) {
+      for (int u = 0; u < ZB; ++u) {
         const int q = q1 + u;
         const int e = tid + q * NT;
-        const int l2 = e % L, pr = e / L, i = i0 + l2;
-        if (q < ITZ && e < L * NPR) {
+        const int l2 = e % L, pr = e / L;
+        if (q < ITZ && (ITZ * NT == L * NPR || e < L * NPR)) {
+          const bool k0 = item_col(q) == 0;
+          C r0[2], rb[2], rh[2];
+          split(z0[u][0], z0[u][1], k0, r0[0], r0[1]);
+          split(zb[u][0], zb[u][1], k0, rb[0], rb[1]);
+          split(zh[u][0], zh[u][1], k0, rh[0], rh[1]);
 #pragma unroll
           for (int h2 = 0; h2 < 2; ++h2) {
             C A = mk<C>(0, 0), B = mk<C>(0, 0);
-            if (i < mx) {
-              const C x0 = r0[u][h2], xh = rh[u][h2], xl = rl[q][h2];
-              const TR kx = (TR)a.k1x[i];
+            if (item_ok(q)) {
+              const C x0 = r0[h2], xh = rh[h2], xl = rl[q][h2];
+              const TR kx = kxq[u];
               A = mk<C>(-kx * x0.y + mlev * (xh.x - xl.x), kx * x0.x + mlev * (xh.y - xl.y));
-              B = rb[u][h2];
+              B = rb[h2];
               rl[q][h2] = xh;
             }
-            lds[(2 * l2) * LS + 2 * pr + h2] = A;
-            lds[(2 * l2 + 1) * LS + 2 * pr + h2] = B;
+            // A of column l2 in line l2, B in line L + l2: the eight columns of a write then
+            // start LS * 4 dwords = 4 banks apart, eight distinct bank groups
+            lds[l2 * LS + 2 * pr + h2] = A;
+            lds[(L + l2) * LS + 2 * pr + h2] = B;
           }
         }
       }
@@ -386,14 +415,20 @@ __global__ void __launch_bounds__(2 * L * Plan<PZ>::T)
     Plan<PZ>::template run<false>(lds + ln * LS, t, w);
     // Nu = (F(A) + ikz F(B)) / pz on the retained modes
     const int s = l + 1;                      // u slot of centre j / v slot of face j
-    for (int e = tid; e < L * NZ; e += NT) {
+    double kzn[NW];
+#pragma unroll
+    for (int n = 0; n < NW; ++n) kzn[n] = a.k1z[min((tid + n * NT) / L, NZ - 1)];
+#pragma unroll
+    for (int n = 0; n < NW; ++n) {
+      const int e = tid + n * NT;
       const int l2 = e % L, k = e / L, i = i0 + l2;
-      if (i >= mx) continue;
-      const int kk = k < h ? k : k + (PZ - nz);
-      const C A = lds[(2 * l2) * LS + kk], B = lds[(2 * l2 + 1) * LS + kk];
-      const double kz = a.k1z[k];
-      nu[(size_t)s * ncol + k * mx + i] =
-          d2(sz * ((double)A.x - kz * (double)B.y), sz * ((double)A.y + kz * (double)B.x));
+      if ((NW * NT == L * NZ || e < L * NZ) && i < mx) {
+        const int kk = k < h ? k : k + (PZ - nz);
+        const C A = lds[l2 * LS + kk], B = lds[(L + l2) * LS + kk];
+        const double kz = kzn[n];
+        nu[(size_t)s * ncol + k * mx + i] =
+            d2(sz * ((double)A.x - kz * (double)B.y), sz * ((double)A.y + kz * (double)B.x));
+      }
     }
     __syncthreads();
   }

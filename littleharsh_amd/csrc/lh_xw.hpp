@@ -114,44 +114,73 @@ template <bool INV> __device__ __forceinline__ void dft8_t(C (&v)[8]) {
 }
 __device__ __host__ constexpr int brev3(int k) { return ((k & 1) << 2) | (k & 2) | ((k >> 2) & 1); }
 
-// slot 8: the 64-point DFT across the lanes, decimation in frequency (inverse sign);
-// lane t ends with output bitrev6(t).  Stage of span h: the lower lane of a pair keeps
-// v + p, the upper one (p - v) w with w = e^{+2 pi i (t mod h) / 2h}; written without
-// branches (the lower lane multiplies by table entry 0, which is 1).
-__device__ __forceinline__ C dif64_inv(C v, int t, const C* tw) {
-#pragma unroll
-  for (int h = 32; h >= 1; h >>= 1) {
-    C p;
-    p.x = __shfl_xor(v.x, h, 64);
-    p.y = __shfl_xor(v.y, h, 64);
-    const bool up = (t & h) != 0;
-    const double sg = up ? -1.0 : 1.0;
-    const C d = mk<C>(fma(sg, v.x, p.x), fma(sg, v.y, p.y));
-    if (h > 1) {
-      const int e = up ? (t % h) * (32 / h) : 0;
-      v = cm(d, tw[TW64 + e]);
-    } else {
-      v = d;
-    }
+// Value of lane t ^ H, for H = 1, 2, 8 by DPP (quad_perm [1,0,3,2], [2,3,0,1]; row_ror:8),
+// 4 and 16 by ds_swizzle in bit mode (xor within 32 lanes, no memory access) and 32 by
+// v_permlane32_swap, which exchanges the two halves of the wave.
+template <int H> __device__ __forceinline__ double lane_xor(double v) {
+  int lo = __double2loint(v), hi = __double2hiint(v);
+  if constexpr (H == 1 || H == 2 || H == 8) {
+    constexpr int ctl = H == 1 ? 0xB1 : (H == 2 ? 0x4E : 0x128);
+    lo = __builtin_amdgcn_mov_dpp(lo, ctl, 0xF, 0xF, false);
+    hi = __builtin_amdgcn_mov_dpp(hi, ctl, 0xF, 0xF, false);
+  } else if constexpr (H == 4 || H == 16) {
+    constexpr int off = 0x1F | (H << 10);
+    lo = __builtin_amdgcn_ds_swizzle(lo, off);
+    hi = __builtin_amdgcn_ds_swizzle(hi, off);
+  } else {
+    static_assert(H == 32, "lane_xor: H in 1, 2, 4, 8, 16, 32");
+    const bool upper = (__lane_id() & 32) != 0;
+    const auto a = __builtin_amdgcn_permlane32_swap(lo, lo, false, false);
+    const auto b = __builtin_amdgcn_permlane32_swap(hi, hi, false, false);
+    lo = upper ? a[0] : a[1];
+    hi = upper ? b[0] : b[1];
   }
-  return v;
+  return __hiloint2double(hi, lo);
+}
+template <int H> __device__ __forceinline__ C lane_xor(C v) { return mk<C>(lane_xor<H>(v.x), lane_xor<H>(v.y)); }
+
+// slot 8: the 64-point DFT across the lanes, decimation in frequency (inverse sign);
+// lane t ends with output bitrev6(t).  Stage of span H: the lower lane of a pair keeps
+// v + p, the upper one (p - v) w with w = e^{+2 pi i (t mod H) / 2H}; written without
+// branches (the lower lane multiplies by table entry 0, which is 1).
+template <int H> __device__ __forceinline__ C dif_stage(C v, int t, const C* tw) {
+  const C p = lane_xor<H>(v);
+  const bool up = (t & H) != 0;
+  const double sg = up ? -1.0 : 1.0;
+  const C d = mk<C>(fma(sg, v.x, p.x), fma(sg, v.y, p.y));
+  if constexpr (H > 1) {
+    const int e = up ? (t % H) * (32 / H) : 0;
+    return cm(d, tw[TW64 + e]);
+  } else {
+    return d;
+  }
+}
+__device__ __forceinline__ C dif64_inv(C v, int t, const C* tw) {
+  v = dif_stage<32>(v, t, tw);
+  v = dif_stage<16>(v, t, tw);
+  v = dif_stage<8>(v, t, tw);
+  v = dif_stage<4>(v, t, tw);
+  v = dif_stage<2>(v, t, tw);
+  return dif_stage<1>(v, t, tw);
 }
 // its transpose with the forward sign: input in bit-reversed lane order, output natural.
 // Lower lane: v + conj(w) p; upper lane: p - conj(w) v.
+template <int H> __device__ __forceinline__ C dit_stage(C v, int t, const C* tw) {
+  const C p = lane_xor<H>(v);
+  const bool up = (t & H) != 0;
+  const C a = up ? p : v;
+  C b = up ? v : p;
+  if constexpr (H > 1) b = cmc(b, tw[TW64 + (t % H) * (32 / H)]);
+  const double sg = up ? -1.0 : 1.0;
+  return mk<C>(fma(sg, b.x, a.x), fma(sg, b.y, a.y));
+}
 __device__ __forceinline__ C dit64_fwd(C v, int t, const C* tw) {
-#pragma unroll
-  for (int h = 1; h <= 32; h <<= 1) {
-    C p;
-    p.x = __shfl_xor(v.x, h, 64);
-    p.y = __shfl_xor(v.y, h, 64);
-    const bool up = (t & h) != 0;
-    const C a = up ? p : v;
-    C b = up ? v : p;
-    if (h > 1) b = cmc(b, tw[TW64 + (t % h) * (32 / h)]);
-    const double sg = up ? -1.0 : 1.0;
-    v = mk<C>(fma(sg, b.x, a.x), fma(sg, b.y, a.y));
-  }
-  return v;
+  v = dit_stage<1>(v, t, tw);
+  v = dit_stage<2>(v, t, tw);
+  v = dit_stage<4>(v, t, tw);
+  v = dit_stage<8>(v, t, tw);
+  v = dit_stage<16>(v, t, tw);
+  return dit_stage<32>(v, t, tw);
 }
 
 // inverse 576-point transform of one wave: v[r] = X[t + 64 r] in; v[s] = x at the lane
@@ -226,19 +255,23 @@ __device__ __forceinline__ void fwd576(C (&v)[9], int t, C* buf, const C* tw) {
 
 // packed rows (z0, z0+1) of one Z slot -> the natural-order line of lane t:
 // line[n] = A[n] + i B[n] for n < MX, A[576-n] conj... (rows' Hermitian halves), 0 between
+// Every lane issues every load (no load is left to one lane), so that the line waits for
+// memory once: lane 0's mode 0 drops the imaginary parts, and mode 192 (lane 0 of slot 3;
+// the other lanes of that slot are zero padding), by factors 0 and 1.
 __device__ __forceinline__ void load_line(C (&v)[9], const C* __restrict__ A, int t) {
   const C* B = A + MX;
+  const double lane0 = t == 0 ? 1.0 : 0.0, other = 1.0 - lane0;
+  C a[3], b[3];
 #pragma unroll
   for (int r = 0; r < 3; ++r) {
-    const int n = t + 64 * r;
-    const C a = A[n], b = B[n];
-    v[r] = n == 0 ? mk<C>(a.x, b.x) : mk<C>(a.x - b.y, a.y + b.x);
+    a[r] = A[t + 64 * r];
+    b[r] = B[t + 64 * r];
   }
-  {
-    // mode 192 sits in lane 0 of slot 3; the other lanes of that slot are zero padding
-    const C a = A[192], b = B[192];
-    v[3] = t == 0 ? mk<C>(a.x - b.y, a.y + b.x) : mk<C>(0.0, 0.0);
-  }
+  const C a3 = A[192], b3 = B[192];
+  v[0] = mk<C>(fma(-other, b[0].y, a[0].x), fma(other, a[0].y, b[0].x));
+#pragma unroll
+  for (int r = 1; r < 3; ++r) v[r] = mk<C>(a[r].x - b[r].y, a[r].y + b[r].x);
+  v[3] = mk<C>(lane0 * (a3.x - b3.y), lane0 * (a3.y + b3.x));
   v[4] = mk<C>(0.0, 0.0);
   v[5] = mk<C>(0.0, 0.0);
 #pragma unroll
@@ -392,5 +425,6 @@ __global__ void __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(2
     if (t == 0) atomic_max_pos(a.scal + S_CFLMAX, cmax);
   }
 }
+
 
 }  // namespace lh
