@@ -130,6 +130,15 @@ int lh_get_scalars(lh_handle* h, double* mpgx, double* qx, double* dt, double* c
  * a NaN (the reduction keeps it). */
 int lh_max_div(lh_handle* h, double* err);
 
+/* lh_max_div without any device allocation (no counterpart in the reference): for callers
+ * that cannot allocate device memory, such as code under HIP graph capture, tight serving
+ * loops and harnesses that forbid hipMalloc.  Writes max |div u| (NaN kept) to the device
+ * double err_dev, in order on the handle's stream, and returns without waiting for it.  Its
+ * partial maxima live in a static device array of 4096 doubles per GPU, so the grid may hold
+ * at most 1,048,576 spectral columns (LH_ERR_ARG beyond).  Calls that overlap on one GPU
+ * (two streams or two host threads) share that array and race. */
+int lh_max_div_noalloc(lh_handle* h, double* err_dev);
+
 /* Statistics (stats.f90:1-108) of the current state: per-plane sums over the physical
  * collocation points of 0 U, 1 U2, 2 V, 3 V2, 4 W, 5 W2, 6 P, 7 P2, 8 U*V_itp, 9 U*W,
  * 10 V*W_itp, 11 wx, 12 wx^2 (the order of write_stats, stats.f90:337-349).  Rows 0-1, 4-5,

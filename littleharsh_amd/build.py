@@ -6,7 +6,7 @@ import subprocess
 import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-SRC = [os.path.join(HERE, "csrc", n) for n in ("lh_api.hip", "lh_mlp.hip", "lh_closure.hip")]
+SRC = [os.path.join(HERE, "csrc", n) for n in ("lh_api.hip", "lh_mlp.hip", "lh_closure.hip", "lh_noalloc.hip")]
 OUT = os.path.join(HERE, "liblh_amd.so")
 
 HIPCC_FLAGS = [

@@ -116,6 +116,7 @@ def load_library():
     lib.lh_cfl_dt.argtypes = [vp, ctypes.c_double, _dp]
     lib.lh_get_scalars.argtypes = [vp, _dp, _dp, _dp, _dp]
     lib.lh_max_div.argtypes = [vp, _dp]
+    lib.lh_max_div_noalloc.argtypes = [vp, vp]
     lib.lh_stats_accumulate.argtypes = [vp]
     lib.lh_stats_read.argtypes = [vp, _dp, _ip, ctypes.c_int]
     lib.lh_cond_stats_size.argtypes = [vp, ctypes.POINTER(ctypes.c_longlong)]
@@ -165,7 +166,7 @@ def load_library():
 EXPORTED = [
     "lh_create", "lh_destroy", "lh_last_error", "lh_set_state", "lh_get_state",
     "lh_set_nonlinear", "lh_get_nonlinear", "lh_set_forcing", "lh_get_forcing", "lh_substep", "lh_step", "lh_step_cfl",
-    "lh_cfl_dt", "lh_get_scalars", "lh_max_div", "lh_stats_accumulate", "lh_stats_read",
+    "lh_cfl_dt", "lh_get_scalars", "lh_max_div", "lh_max_div_noalloc", "lh_stats_accumulate", "lh_stats_read",
     "lh_cond_stats_size", "lh_cond_stats_read", "lh_spectra_accumulate", "lh_spectra_read",
     "lh_timing_enable", "lh_timing_reset", "lh_timing_read",
     "lh_set_launch", "lh_get_launch", "lh_autotune",

@@ -1145,6 +1145,14 @@ int lh_set_forcing(lh_handle* h, const double* f1, const double* f2, const doubl
 
 }  // extern "C"
 
+int lh_div_view(lh_handle* h, LhDivView* v) {
+  if (!h || !v) return LH_ERR_ARG;
+  *v = LhDivView{h->u1, h->u2, h->u3, h->dthdyu, h->k1x, h->k1z, h->dm.ddthetavi,
+                 h->dm.ncol, h->dm.mx, h->dm.nyp, h->dev, h->st};
+  return LH_OK;
+}
+int lh_fail(lh_handle* h, int code, const char* msg) { return fail(h, code, msg); }
+
 int lh_solver_view(lh_handle* h, LhSolverView* v) {
   if (!h || !v) return LH_ERR_ARG;
   HIPCHK(h, hipStreamSynchronize(h->st));

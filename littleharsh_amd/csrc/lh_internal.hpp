@@ -27,3 +27,17 @@ int lh_solver_view(lh_handle* h, LhSolverView* v);
 int lh_mlp_eval_grid_dev(lh_mlp* h, const float* g, int nch, const float* ex, int ny, int nz, int nx,
                          const float* mean, const float* rscale, float* y, hipStream_t st);
 int lh_mlp_dims(const lh_mlp* h, int* nin, int* nout);
+
+// What the divergence monitor reads (lh_noalloc.hip): the handle's state and metrics on the
+// device, its device and its stream.
+struct LhDivView {
+  const double2 *u1, *u2, *u3;     // state, slot-major: [slot][col]
+  const double *dthdyu, *k1x, *k1z;
+  double ddthetavi;
+  int ncol, mx, nyp;
+  int dev;
+  hipStream_t st;
+};
+int lh_div_view(lh_handle* h, LhDivView* v);
+// records msg as the handle's last error and returns code
+int lh_fail(lh_handle* h, int code, const char* msg);

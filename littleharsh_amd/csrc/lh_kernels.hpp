@@ -11,6 +11,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#include "lh_div.hpp"
 #include "lh_fft.hpp"
 
 namespace lh {
@@ -282,7 +283,6 @@ __device__ __forceinline__ void atomic_max_pos(double* addr, double v) {
 
 // Maximum that keeps a NaN: fmax would drop it and report a finite maximum of a field that
 // holds non-finite values.
-__device__ __forceinline__ double nanmax(double e, double v) { return (v > e || v != v) ? v : e; }
 
 // Immersed-boundary x pass: rhs rows -> physical, rhs = 0 on canopy points, -beta*Lu_y on
 // canopy points of tip planes, -> Fourier (tridLU_3D_IB.f90:224-245, 306-325)
@@ -519,17 +519,7 @@ __global__ void __launch_bounds__(256) k_maxdiv(const double2* u1, const double2
                                                 const double* k1z, double* scal, Dims dm) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   double e = 0.0;
-  if (c < dm.ncol) {
-    const int i = c % dm.mx, k = c / dm.mx;
-    const double kx = k1x[i], kz = k1z[k];
-    for (int r = 0; r < dm.nyp; ++r) {
-      const size_t ou = (size_t)(r + 1) * dm.ncol + c;
-      const double2 dvg = dadd(dadd(dim(kx, u1[ou]), dim(kz, u3[ou])),
-                               drm(dm.ddthetavi * dthdyu[r + 1], dsub(u2[ou], u2[ou - dm.ncol])));
-      // |div| without hypot, which returns +inf for (inf, NaN): sqrt of the sum keeps NaN
-      e = nanmax(e, sqrt(dvg.x * dvg.x + dvg.y * dvg.y));
-    }
-  }
+  if (c < dm.ncol) e = col_maxdiv(c, u1, u2, u3, dthdyu, k1x, k1z, dm.ddthetavi, dm.ncol, dm.mx, dm.nyp);
   for (int off = 32; off > 0; off >>= 1) e = nanmax(e, __shfl_down(e, off, 64));
   if ((threadIdx.x & 63) == 0) atomic_max_pos(scal + S_MAXDIV, e);
 }

@@ -172,6 +172,11 @@ module lh_amd
       type(c_ptr), value :: h
       real(c_double), intent(out) :: err
     end function
+    ! err_dev: a device address (for example from hipMalloc at set-up); nothing is allocated
+    integer(c_int) function lh_max_div_noalloc(h, err_dev) bind(C, name="lh_max_div_noalloc")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: h, err_dev
+    end function
     integer(c_int) function lh_stats_accumulate(h) bind(C, name="lh_stats_accumulate")
       import :: c_int, c_ptr
       type(c_ptr), value :: h
