@@ -316,13 +316,16 @@ static void launch_xw(lh_handle* h, int want_cfl) {
        h->dthdyu, h->scal, want_cfl, h->xraw_lc};
   const int nlev = h->dm.nyu - 2;
   const long waves = (long)(h->dm.px / 2) * ((nlev + h->xraw_lc - 1) / h->xraw_lc);
-  hipLaunchKernelGGL(k_xw576<kXwWaves>, dim3((unsigned)((waves + kXwWaves - 1) / kXwWaves)), dim3(64 * kXwWaves),
-                     xw_lds_bytes<kXwWaves>(), h->st, a, h->dm);
+  const dim3 grid((unsigned)((waves + kXwWaves - 1) / kXwWaves)), block(64 * kXwWaves);
+  if (want_cfl)
+    hipLaunchKernelGGL((k_xw576<kXwWaves, true>), grid, block, xw_lds_bytes<kXwWaves>(), h->st, a, h->dm);
+  else
+    hipLaunchKernelGGL((k_xw576<kXwWaves, false>), grid, block, xw_lds_bytes<kXwWaves>(), h->st, a, h->dm);
 }
 constexpr int kZasmCols = LH_ZASM_COLS;
 template <int PZ, typename TR>
 static size_t zasm_lds() {
-  return ((size_t)2 * kZasmCols * (PZ + 1) + Plan<PZ>::template TwL<cx<TR>>::SIZE) * sizeof(cx<TR>);
+  return ((size_t)2 * kZasmCols * (PZ + 1) + ZasmTw<PZ, cx<TR>>::SIZE) * sizeof(cx<TR>);
 }
 template <int PZ, typename TR>
 static void launch_zasm_t(lh_handle* h, const ZASM& a, const void* tw) {
@@ -650,7 +653,9 @@ int lh_create(const lh_config* c, lh_handle** out) {
     if (use_xw(h)) {
       // one block of kXwWaves waves per CU; a chunk's start-up is three inverse transforms
       // against nine transforms per level
-      HIPCHK(h, hipFuncSetAttribute((const void*)k_xw576<kXwWaves>, hipFuncAttributeMaxDynamicSharedMemorySize,
+      HIPCHK(h, hipFuncSetAttribute((const void*)k_xw576<kXwWaves, true>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    (int)xw_lds_bytes<kXwWaves>()));
+      HIPCHK(h, hipFuncSetAttribute((const void*)k_xw576<kXwWaves, false>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                     (int)xw_lds_bytes<kXwWaves>()));
       if (h->xraw_lc <= 0)
         h->xraw_lc = chunk_levels(dm.nyu - 2, (long)dm.px / 2, ncu * kXwWaves, 4, 96, 1.0 / 3.0);

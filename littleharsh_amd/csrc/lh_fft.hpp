@@ -194,6 +194,58 @@ struct StageTwL {
   }
 };
 
+// Stage twiddles from an LDS table with one row per distinct j = b mod NS, shared by the
+// block: row j holds w^(j r N/(NS R)) for r = 1..M; a higher power r > M (at most 2M) is the
+// product of the row's entries r - M and M (one rounding more).  Rows of butterflies b < NS
+// need no modulo, so their reads are a per-thread base plus compile-time offsets.
+template <int N, int T, int R, int NS, int M, typename C>
+struct StageTwD {
+  static_assert(M >= 1 && 2 * M >= R - 1, "StageTwD: every power r < R must be r <= M or a product of two stored ones");
+  static constexpr int NB = N / R;
+  static constexpr int PER = (NB + T - 1) / T;
+  static constexpr int SIZE = NS * M;
+  const C* tab;
+  int t;
+  __device__ __forceinline__ static void fill(C* dst, const C* __restrict__ tw, int tid, int nthreads) {
+    for (int e = tid; e < SIZE; e += nthreads) {
+      const int j = e / M, r = e % M + 1;
+      dst[e] = tw[j * r * (N / (NS * R))];
+    }
+  }
+  __device__ __forceinline__ void init(const C* tb, int tt) {
+    tab = tb;
+    t = tt;
+  }
+  __device__ __forceinline__ C get(int q, int r) const {
+    const int b = t + q * T;
+    const C* row = tab + ((b < NB ? b : 0) % NS) * M;
+    if (r <= M) return row[r - 1];
+    return cmul(row[r - M - 1], row[M - 1]);
+  }
+  template <bool INV, int RR> __device__ __forceinline__ void apply(int q, C (&v)[RR]) const {
+#pragma unroll
+    for (int r = 1; r < R; ++r) v[r] = INV ? cmulc(v[r], get(q, r)) : cmul(v[r], get(q, r));
+  }
+};
+// StageTwD for both later stages of Plan<N>: the middle stage with every power in its rows,
+// the last stage of a three-stage plan with the first R/2 of them
+template <int N, int T, int RA, int RB, int RC, typename C>
+struct TwRowsPlan {
+  using S2 = StageTwD<N, T, RB, RA, RB - 1, C>;
+  using S3 = StageTwD<N, T, (RC > 1 ? RC : 2), RA * RB, (RC > 2 ? RC / 2 : 1), C>;
+  static constexpr int SIZE = S2::SIZE + (RC > 1 ? S3::SIZE : 0);
+  S2 s2;
+  S3 s3;
+  __device__ __forceinline__ static void fill(C* dst, const C* __restrict__ tw, int tid, int nthreads) {
+    S2::fill(dst, tw, tid, nthreads);
+    if constexpr (RC > 1) S3::fill(dst + S2::SIZE, tw, tid, nthreads);
+  }
+  __device__ __forceinline__ void init(const C* tb, int t) {
+    s2.init(tb, t);
+    if constexpr (RC > 1) s3.init(tb + S2::SIZE, t);
+  }
+};
+
 // Position of element e of a line stored with one pad slot after every 2^S elements
 // (S = 0: no padding); spreads the strided stage writes over the LDS banks.
 template <int S> __device__ __forceinline__ constexpr int lpad(int e) { return S ? e + (e >> S) : e; }

@@ -262,12 +262,16 @@ struct ZASM {
   int lc;                                    // levels per block
 };
 
+// twiddle table of the z assembly's transforms
+template <int PZ, typename C>
+using ZasmTw = TwRowsPlan<PZ, Plan<PZ>::T, Plan<PZ>::RA, Plan<PZ>::RB, Plan<PZ>::RC, C>;
+
 // items per load batch of the z assembly (more loads in flight against more registers)
 #ifndef LH_ZASM_ZB
 #define LH_ZASM_ZB 1
 #endif
 template <int PZ, int L, typename TR>
-__global__ void __launch_bounds__(2 * L * Plan<PZ>::T)
+__global__ void __launch_bounds__(2 * L * Plan<PZ>::T) __attribute__((amdgpu_waves_per_eu(3)))
     k_zasm(ZASM a, Dims dm, const cx<TR>* __restrict__ tw) {
   using C = cx<TR>;
   constexpr int T = Plan<PZ>::T;
@@ -315,26 +319,50 @@ __global__ void __launch_bounds__(2 * L * Plan<PZ>::T)
   const int qb = f == 0 ? 1 : (f == 1 ? 2 : 5);       // B
   double2* __restrict__ nu = f == 0 ? a.nu1 : (f == 1 ? a.nu3 : a.nu2);
   const int ln = tid / T, t = tid % T;
-  // twiddles in an LDS table behind the 2L lines (registers hold the carried rows)
-  typename Plan<PZ>::template TwL<C> w;
-  w.init(lds + 2 * L * LS, t, tw, ln == 0);
+  // twiddles: one LDS row per distinct twiddle set behind the 2L lines, shared by every
+  // line (registers hold the carried rows); read after the barrier that precedes the
+  // transform
+  using TWP = ZasmTw<PZ, C>;
+  TWP w;
+  C* const twt = lds + 2 * L * LS;
+  TWP::fill(twt, tw, tid, NT);
+  w.init(twt, t);
   // The product rows hold the packed x spectrum P of each row pair (z, z+1): rows z and z+1
   // of column k are (P[k] + conj P[px-k]) / 2 and (P[k] - conj P[px-k]) / 2i, times 1/px.
   // (1/px)(x/2) = (1/(2px)) x exactly: halving is exact
   const TR sxh = (TR)0.5 * ((TR)1 / (TR)dm.px);
-  // Item q of this thread: column i0 + e % L and row pair e / L, e = tid + q NT.  Items
-  // outside the plane read a valid element (clamped) and are masked afterwards, so that every
-  // load of a level is issued unconditionally and the level waits for memory once.  Column 0
-  // reads its packed value twice (slot mx + 0 is never written); the other columns read
-  // P[k] and P[px-k].
-  auto item_col = [&](int q) { return min(i0 + (tid + q * NT) % L, mx - 1); };
-  auto item_ok = [&](int q) {
-    const int e = tid + q * NT;
-    return (ITZ * NT == L * NPR || e < L * NPR) && i0 + e % L < mx;
+  // Item q of this thread: column i0 + l2 and row pair pr0 + q PST (NT is a multiple of L,
+  // so every item of a thread has the same column).  Row pairs past the plane read the last
+  // one (clamped) and are masked afterwards, so that every load of a level is issued
+  // unconditionally and the level waits for memory once.  Column 0 reads its packed value
+  // twice (slot mx + 0 is never written); the other columns read P[k] and P[px-k].
+  static_assert(NT % L == 0, "z assembly: the block is a whole number of column groups");
+  constexpr int PST = NT / L;
+  const int l2 = tid % L, pr0 = tid / L;
+  const bool col_ok = i0 + l2 < mx;
+  const int col = min(i0 + l2, mx - 1);
+  const bool k0 = col == 0;
+  const int dn = k0 ? 0 : mx;
+  auto pr_of = [&](int q) { return ITZ * PST == NPR ? pr0 + q * PST : min(pr0 + q * PST, NPR - 1); };
+  // element e of a row at base b (uniform): a 32-bit byte offset from a scalar base, so that
+  // the three rows of a level share each lane's offsets
+  using GB = const char __attribute__((address_space(1)))*;
+  typedef TR V2 __attribute__((ext_vector_type(2)));
+  auto ld = [](GB b, unsigned e) {
+    const V2 v = *reinterpret_cast<const V2 __attribute__((address_space(1)))*>(b + e * (unsigned)sizeof(C));
+    return mk<C>(v.x, v.y);
   };
-  auto item_ro = [&](int q) { return min((tid + q * NT) / L, NPR - 1) * 2 * mx + item_col(q); };
-  auto item_rn = [&](int q) { return item_col(q) == 0 ? item_ro(q) : item_ro(q) + mx; };
-  auto split = [&](C zk, C zn, bool k0, C& ra, C& rb) {
+  // a row base held in scalar registers (readfirstlane), which also keeps the compiler from
+  // folding each lane's offset into a 64-bit pointer of its own
+  auto sbase = [](const void* p) {
+    const unsigned long long v = reinterpret_cast<unsigned long long>(p);
+    const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)v);
+    const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(v >> 32));
+    return reinterpret_cast<GB>(((unsigned long long)hi << 32) | lo);
+  };
+  auto pr_ok = [&](int q) { return ITZ * PST == NPR || pr0 + q * PST < NPR; };
+  const TR kx = (TR)a.k1x[col];
+  auto split = [&](C zk, C zn, C& ra, C& rb) {
     ra = mk<C>(sxh * (zk.x + zn.x), sxh * (zk.y - zn.y));
     rb = mk<C>(sxh * (zk.y + zn.y), -(sxh * (zk.x - zn.x)));
     if (k0) { ra.y = 0; rb.y = 0; }
@@ -343,62 +371,57 @@ __global__ void __launch_bounds__(2 * L * Plan<PZ>::T)
   C rl[ITZ][2];
   {
     const int l_lo = f < 2 ? l0 - 1 : l0;
-    const C* row = zo + qd * ps + (size_t)max(l_lo, 0) * slot;
+    const GB row = sbase(zo + qd * ps + (size_t)max(l_lo, 0) * slot);
     C zk[ITZ], zn[ITZ];
 #pragma unroll
     for (int q = 0; q < ITZ; ++q) {
-      zk[q] = row[item_ro(q)];
-      zn[q] = row[item_rn(q)];
+      const unsigned o = pr_of(q) * 2 * mx + col;
+      zk[q] = ld(row, o);
+      zn[q] = ld(row, o + dn);
     }
 #pragma unroll
     for (int q = 0; q < ITZ; ++q) {
-      split(zk[q], zn[q], item_col(q) == 0, rl[q][0], rl[q][1]);
-      if (!item_ok(q) || l_lo < 0) rl[q][0] = rl[q][1] = mk<C>(0, 0);
+      split(zk[q], zn[q], rl[q][0], rl[q][1]);
+      if (!col_ok || l_lo < 0) rl[q][0] = rl[q][1] = mk<C>(0, 0);
     }
   }
-  // Nu stores of this thread: element e = tid + n NT of the L x NZ retained block
+  // Nu stores of this thread: mode k = pr0 + n PST of column i0 + l2
   constexpr int NW = (L * NZ + NT - 1) / NT;
   const double sz = 1.0 / (double)PZ;
   for (int l = l0; l < l1; ++l) {
     const int l_hi = f < 2 ? l : l + 1;
     const TR mlev = (TR)(dm.ddthetavi * (f < 2 ? a.dthdyu[l + 1] : a.dthdyv[l + 1]));
-    const C* row0 = zo + q0 * ps + (size_t)l * slot;
-    const C* rowb = zo + qb * ps + (size_t)l * slot;
-    const C* rowh = zo + qd * ps + (size_t)l_hi * slot;
+    const GB row0 = sbase(zo + q0 * ps + (size_t)l * slot);
+    const GB rowb = sbase(zo + qb * ps + (size_t)l * slot);
+    const GB rowh = sbase(zo + qd * ps + (size_t)l_hi * slot);
     // items in batches of ZB: the loads of a batch are issued together, then combined
 #pragma unroll
     for (int q1 = 0; q1 < ITZ; q1 += ZB) {
       C z0[ZB][2], zb[ZB][2], zh[ZB][2];
-      TR kxq[ZB];
 #pragma unroll
       for (int u = 0; u < ZB; ++u) {
-        const int q = min(q1 + u, ITZ - 1);
-        const int o = item_ro(q), on = item_rn(q);
-        z0[u][0] = row0[o];
-        z0[u][1] = row0[on];
-        zb[u][0] = rowb[o];
-        zb[u][1] = rowb[on];
-        zh[u][0] = rowh[o];
-        zh[u][1] = rowh[on];
-        kxq[u] = (TR)a.k1x[item_col(q)];
+        const unsigned o = pr_of(min(q1 + u, ITZ - 1)) * 2 * mx + col, on = o + dn;
+        z0[u][0] = ld(row0, o);
+        z0[u][1] = ld(row0, on);
+        zb[u][0] = ld(rowb, o);
+        zb[u][1] = ld(rowb, on);
+        zh[u][0] = ld(rowh, o);
+        zh[u][1] = ld(rowh, on);
       }
 #pragma unroll
       for (int u = 0; u < ZB; ++u) {
         const int q = q1 + u;
-        const int e = tid + q * NT;
-        const int l2 = e % L, pr = e / L;
-        if (q < ITZ && (ITZ * NT == L * NPR || e < L * NPR)) {
-          const bool k0 = item_col(q) == 0;
+        const int pr = pr0 + q * PST;
+        if (q < ITZ && pr_ok(q)) {
           C r0[2], rb[2], rh[2];
-          split(z0[u][0], z0[u][1], k0, r0[0], r0[1]);
-          split(zb[u][0], zb[u][1], k0, rb[0], rb[1]);
-          split(zh[u][0], zh[u][1], k0, rh[0], rh[1]);
+          split(z0[u][0], z0[u][1], r0[0], r0[1]);
+          split(zb[u][0], zb[u][1], rb[0], rb[1]);
+          split(zh[u][0], zh[u][1], rh[0], rh[1]);
 #pragma unroll
           for (int h2 = 0; h2 < 2; ++h2) {
             C A = mk<C>(0, 0), B = mk<C>(0, 0);
-            if (item_ok(q)) {
+            if (col_ok) {
               const C x0 = r0[h2], xh = rh[h2], xl = rl[q][h2];
-              const TR kx = kxq[u];
               A = mk<C>(-kx * x0.y + mlev * (xh.x - xl.x), kx * x0.x + mlev * (xh.y - xl.y));
               B = rb[h2];
               rl[q][h2] = xh;
@@ -417,17 +440,21 @@ __global__ void __launch_bounds__(2 * L * Plan<PZ>::T)
     const int s = l + 1;                      // u slot of centre j / v slot of face j
     double kzn[NW];
 #pragma unroll
-    for (int n = 0; n < NW; ++n) kzn[n] = a.k1z[min((tid + n * NT) / L, NZ - 1)];
+    for (int n = 0; n < NW; ++n) kzn[n] = a.k1z[min(pr0 + n * PST, NZ - 1)];
 #pragma unroll
     for (int n = 0; n < NW; ++n) {
-      const int e = tid + n * NT;
-      const int l2 = e % L, k = e / L, i = i0 + l2;
-      if ((NW * NT == L * NZ || e < L * NZ) && i < mx) {
+      const int k = pr0 + n * PST;
+      if ((NW * PST == NZ || k < NZ) && col_ok) {
         const int kk = k < h ? k : k + (PZ - nz);
         const C A = lds[l2 * LS + kk], B = lds[(L + l2) * LS + kk];
         const double kz = kzn[n];
-        nu[(size_t)s * ncol + k * mx + i] =
-            d2(sz * ((double)A.x - kz * (double)B.y), sz * ((double)A.y + kz * (double)B.x));
+        typedef double D2 __attribute__((ext_vector_type(2)));
+        const GB nus = sbase(nu + (size_t)s * ncol);
+        D2 r;
+        r.x = sz * ((double)A.x - kz * (double)B.y);
+        r.y = sz * ((double)A.y + kz * (double)B.x);
+        *const_cast<D2 __attribute__((address_space(1)))*>(
+            reinterpret_cast<const D2 __attribute__((address_space(1)))*>(nus + (unsigned)(k * mx + i0 + l2) * 16u)) = r;
       }
     }
     __syncthreads();

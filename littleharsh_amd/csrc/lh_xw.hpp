@@ -114,10 +114,10 @@ template <bool INV> __device__ __forceinline__ void dft8_t(C (&v)[8]) {
 }
 __device__ __host__ constexpr int brev3(int k) { return ((k & 1) << 2) | (k & 2) | ((k >> 2) & 1); }
 
-// Value of lane t ^ H, for H = 1, 2, 8 by DPP (quad_perm [1,0,3,2], [2,3,0,1]; row_ror:8),
+// Value of lane t ^ H (t: this lane), for H = 1, 2, 8 by DPP (quad_perm [1,0,3,2], [2,3,0,1]; row_ror:8),
 // 4 and 16 by ds_swizzle in bit mode (xor within 32 lanes, no memory access) and 32 by
 // v_permlane32_swap, which exchanges the two halves of the wave.
-template <int H> __device__ __forceinline__ double lane_xor(double v) {
+template <int H> __device__ __forceinline__ double lane_xor(double v, int t) {
   int lo = __double2loint(v), hi = __double2hiint(v);
   if constexpr (H == 1 || H == 2 || H == 8) {
     constexpr int ctl = H == 1 ? 0xB1 : (H == 2 ? 0x4E : 0x128);
@@ -129,7 +129,7 @@ template <int H> __device__ __forceinline__ double lane_xor(double v) {
     hi = __builtin_amdgcn_ds_swizzle(hi, off);
   } else {
     static_assert(H == 32, "lane_xor: H in 1, 2, 4, 8, 16, 32");
-    const bool upper = (__lane_id() & 32) != 0;
+    const bool upper = (t & 32) != 0;
     const auto a = __builtin_amdgcn_permlane32_swap(lo, lo, false, false);
     const auto b = __builtin_amdgcn_permlane32_swap(hi, hi, false, false);
     lo = upper ? a[0] : a[1];
@@ -137,14 +137,16 @@ template <int H> __device__ __forceinline__ double lane_xor(double v) {
   }
   return __hiloint2double(hi, lo);
 }
-template <int H> __device__ __forceinline__ C lane_xor(C v) { return mk<C>(lane_xor<H>(v.x), lane_xor<H>(v.y)); }
+template <int H> __device__ __forceinline__ C lane_xor(C v, int t) {
+  return mk<C>(lane_xor<H>(v.x, t), lane_xor<H>(v.y, t));
+}
 
 // slot 8: the 64-point DFT across the lanes, decimation in frequency (inverse sign);
 // lane t ends with output bitrev6(t).  Stage of span H: the lower lane of a pair keeps
 // v + p, the upper one (p - v) w with w = e^{+2 pi i (t mod H) / 2H}; written without
 // branches (the lower lane multiplies by table entry 0, which is 1).
 template <int H> __device__ __forceinline__ C dif_stage(C v, int t, const C* tw) {
-  const C p = lane_xor<H>(v);
+  const C p = lane_xor<H>(v, t);
   const bool up = (t & H) != 0;
   const double sg = up ? -1.0 : 1.0;
   const C d = mk<C>(fma(sg, v.x, p.x), fma(sg, v.y, p.y));
@@ -166,7 +168,7 @@ __device__ __forceinline__ C dif64_inv(C v, int t, const C* tw) {
 // its transpose with the forward sign: input in bit-reversed lane order, output natural.
 // Lower lane: v + conj(w) p; upper lane: p - conj(w) v.
 template <int H> __device__ __forceinline__ C dit_stage(C v, int t, const C* tw) {
-  const C p = lane_xor<H>(v);
+  const C p = lane_xor<H>(v, t);
   const bool up = (t & H) != 0;
   const C a = up ? p : v;
   C b = up ? v : p;
@@ -305,9 +307,10 @@ struct XW {
 };
 
 // LDS of one block of WPB waves: the twiddle table, then per wave a transpose line and a
-// park line (one field of 9 values per lane, lane-major)
+// park line (one field of 9 values per lane, lane-major), then each lane's running CFL
+// maximum (a double per lane, kept out of the registers)
 template <int WPB> constexpr size_t xw_lds_bytes() {
-  return sizeof(double2) * ((size_t)xw::TWN + (size_t)WPB * (xw::LINE + 9 * 64));
+  return sizeof(double2) * ((size_t)xw::TWN + (size_t)WPB * (xw::LINE + 9 * 64)) + sizeof(double) * WPB * 64;
 }
 
 // One wave per (row pair, chunk of lc levels); WPB waves per block.  Products of level l
@@ -317,7 +320,9 @@ template <int WPB> constexpr size_t xw_lds_bytes() {
 // stay in registers: the one the next products do not need waits in the wave's park line
 // (V(j-1) between levels, W(j) while the centre products are formed) and is read back slot
 // by slot as an operand.
-template <int WPB>
+// CFL: whether the launch forms the CFL maximum (a template flag, so that the slot loops
+// carry no branch on it)
+template <int WPB, bool CFL>
 __global__ void __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(2))) k_xw576(XW a, Dims dm) {
   using C = double2;
   extern __shared__ char smem_raw[];
@@ -338,7 +343,8 @@ __global__ void __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(2
   if (l0 >= l1) return;
   const size_t slot = (size_t)xw::N * xw::MX;
   const size_t row = (size_t)pair * 2 * xw::MX;
-  double cmax = 0.0;
+  double* const cm = reinterpret_cast<double*>(lds + xw::TWN + WPB * (xw::LINE + 9 * 64)) + wv * 64 + t;
+  if constexpr (CFL) *cm = 0.0;
   C cu[9], cw[9], p[9];
   auto park = [&](const C (&x)[9]) {
     xw::mem_fence_compiler();
@@ -362,20 +368,29 @@ __global__ void __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(2
     C vj[9];
     xw::load_line(vj, a.zv + (size_t)(l + 1) * slot + row, t);
     xw::inv576(vj, t, buf, tw);
-    const double dyi = a.dthdyu[l + 1];
 #pragma unroll
     for (int s = 0; s < 9; ++s) {
       const C vb = pk[64 * s];
-      const C vc = mk<C>(0.5 * (vb.x + vj[s].x), 0.5 * (vb.y + vj[s].y));
-      if (a.want_cfl) {
-        const double s0 = fabs(cu[s].x) * dm.dxi + fabs(vc.x) * dyi + fabs(cw[s].x) * dm.dzi;
-        const double s1 = fabs(cu[s].y) * dm.dxi + fabs(vc.y) * dyi + fabs(cw[s].y) * dm.dzi;
-        cmax = nanmax(cmax, nanmax(s0, s1));
-      }
-      p[s] = mk<C>(vc.x * vc.x, vc.y * vc.y);
+      p[s] = mk<C>(0.5 * (vb.x + vj[s].x), 0.5 * (vb.y + vj[s].y));   // v at the centre
     }
     xw::mem_fence_compiler();
     park(cw);                                     // W(j) waits while V(j) stays resident
+    if constexpr (CFL) {
+      // |u|/dx + |v|/dy + |w|/dz at the centre, W(j) read back from the park line (its
+      // registers are free from here to the end of the level)
+      const double dyi = a.dthdyu[l + 1];
+      double cmax = *cm;
+#pragma unroll
+      for (int s = 0; s < 9; ++s) {
+        const C w = pk[64 * s];
+        const double s0 = fabs(cu[s].x) * dm.dxi + fabs(p[s].x) * dyi + fabs(w.x) * dm.dzi;
+        const double s1 = fabs(cu[s].y) * dm.dxi + fabs(p[s].y) * dyi + fabs(w.y) * dm.dzi;
+        cmax = nanmax(cmax, nanmax(s0, s1));
+      }
+      *cm = cmax;
+    }
+#pragma unroll
+    for (int s = 0; s < 9; ++s) p[s] = mk<C>(p[s].x * p[s].x, p[s].y * p[s].y);
     out(3, l, p);
 #pragma unroll
     for (int s = 0; s < 9; ++s) p[s] = mk<C>(cu[s].x * cu[s].x, cu[s].y * cu[s].y);
@@ -420,7 +435,8 @@ __global__ void __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(2
 #pragma unroll
     for (int s = 0; s < 9; ++s) cw[s] = p[s];
   }
-  if (a.want_cfl) {
+  if constexpr (CFL) {
+    double cmax = *cm;
     for (int off = 32; off > 0; off >>= 1) cmax = nanmax(cmax, __shfl_down(cmax, off, 64));
     if (t == 0) atomic_max_pos(a.scal + S_CFLMAX, cmax);
   }
