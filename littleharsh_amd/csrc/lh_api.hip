@@ -2,7 +2,8 @@
 //
 // The handle owns one HIP stream, the device-resident state in [plane][column] layout and
 // all work buffers.  lh_substep issues the kernels of one RK3 substep:
-//   NL stage   : k_zinv (u,w,v) -> k_xraw2 (x transforms, six products, CFL max)
+//   NL stage   : k_zinv (u,w,v) -> k_xw576 at 576 points in x, k_xraw2 at the other sizes
+//                (x transforms, six products, CFL max)
 //                -> k_zasm (A/B recombination, forward z transforms -> Nu_new)
 //   IB stage   : k_rhs_ib (rhs and Lu_y on canopy planes) -> k_zinv -> k_xpass_ib -> k_zfwd
 //   column     : k_prep (LU of the shared velocity matrices) -> k_vel_rhs (RHS0 + rhs of
@@ -99,8 +100,10 @@ struct lh_handle {
   double *ayu = nullptr, *ayv = nullptr, *ctwork = nullptr;
   void *twx = nullptr, *twz = nullptr;         // twiddles in the NL precision
   void *twx64 = nullptr, *twz64 = nullptr;     // FP64 twiddles (IB, statistics)
-  // the one-wave-per-line x pass (FP64, 576 collocation points in x): its twiddle table
+  // the one-wave-per-line x pass (576 collocation points in x): its twiddle table, and the
+  // same rounded to FP32 in the mixed-precision mode
   double2* twxw = nullptr;
+  float2* twxw32 = nullptr;
   // canopy
   unsigned char* masks = nullptr;
   int nmask = 0;
@@ -306,21 +309,40 @@ static void launch_xraw_t(lh_handle* h, const XRAW& a, const void* tw) {
   hipLaunchKernelGGL((k_xraw2<PX, TR>), grid, dim3(6 * Plan<PX>::T), sh, h->st, a, h->dm,
                      (const cx<TR>*)tw, lc);
 }
-// the one-wave-per-line x pass: WPB waves per block, one block per CU (LDS), two waves per
-// SIMD (registers); each wave takes one row pair and xraw_lc levels
-constexpr int kXwWaves = 8;
-constexpr bool kXwOn = true;
-static bool use_xw(const lh_handle* h) { return kXwOn && h->dm.px == 576 && h->prec == LH_PREC_FP64; }
-static void launch_xw(lh_handle* h, int want_cfl) {
-  XW a{(const double2*)h->zu, (const double2*)h->zw, (const double2*)h->zv, (double2*)h->zp, h->twxw,
+// the one-wave-per-line x pass: one block per CU (LDS), LH_XW_WAVES32 waves per block in FP32
+// (8: two per SIMD; 12: three, with a few spills), 8 in FP64 (two per SIMD, the registers'
+// limit); each wave takes one row pair and xraw_lc levels
+#ifndef LH_XW_WAVES32
+#define LH_XW_WAVES32 8
+#endif
+template <typename TR> struct XwWaves { static constexpr int value = sizeof(TR) == 8 ? 8 : LH_XW_WAVES32; };
+static bool use_xw(const lh_handle* h) { return h->dm.px == 576; }
+template <typename TR>
+static void launch_xw_t(lh_handle* h, int want_cfl) {
+  XW a{h->zu, h->zw, h->zv, h->zp, sizeof(TR) == 8 ? (const void*)h->twxw : (const void*)h->twxw32,
        h->dthdyu, h->scal, want_cfl, h->xraw_lc};
   const int nlev = h->dm.nyu - 2;
   const long waves = (long)(h->dm.px / 2) * ((nlev + h->xraw_lc - 1) / h->xraw_lc);
-  const dim3 grid((unsigned)((waves + kXwWaves - 1) / kXwWaves)), block(64 * kXwWaves);
+  constexpr int W = XwWaves<TR>::value;
+  const dim3 grid((unsigned)((waves + W - 1) / W)), block(64 * W);
+  const size_t sh = xw_lds_bytes<W, TR>();
   if (want_cfl)
-    hipLaunchKernelGGL((k_xw576<kXwWaves, true>), grid, block, xw_lds_bytes<kXwWaves>(), h->st, a, h->dm);
+    hipLaunchKernelGGL((k_xw576<W, true, TR>), grid, block, sh, h->st, a, h->dm);
   else
-    hipLaunchKernelGGL((k_xw576<kXwWaves, false>), grid, block, xw_lds_bytes<kXwWaves>(), h->st, a, h->dm);
+    hipLaunchKernelGGL((k_xw576<W, false, TR>), grid, block, sh, h->st, a, h->dm);
+}
+static void launch_xw(lh_handle* h, int want_cfl) {
+  if (h->prec == LH_PREC_FP64) launch_xw_t<double>(h, want_cfl);
+  else launch_xw_t<float>(h, want_cfl);
+}
+template <typename TR>
+static hipError_t attr_xw_t() {
+  constexpr int W = XwWaves<TR>::value;
+  hipError_t e = hipFuncSetAttribute((const void*)k_xw576<W, true, TR>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                     (int)xw_lds_bytes<W, TR>());
+  if (e != hipSuccess) return e;
+  return hipFuncSetAttribute((const void*)k_xw576<W, false, TR>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                             (int)xw_lds_bytes<W, TR>());
 }
 constexpr int kZasmCols = LH_ZASM_COLS;
 template <int PZ, typename TR>
@@ -653,12 +675,11 @@ int lh_create(const lh_config* c, lh_handle** out) {
     if (use_xw(h)) {
       // one block of kXwWaves waves per CU; a chunk's start-up is three inverse transforms
       // against nine transforms per level
-      HIPCHK(h, hipFuncSetAttribute((const void*)k_xw576<kXwWaves, true>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                    (int)xw_lds_bytes<kXwWaves>()));
-      HIPCHK(h, hipFuncSetAttribute((const void*)k_xw576<kXwWaves, false>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                    (int)xw_lds_bytes<kXwWaves>()));
+      const bool f64 = h->prec == LH_PREC_FP64;
+      HIPCHK(h, f64 ? attr_xw_t<double>() : attr_xw_t<float>());
+      const int wpb = f64 ? XwWaves<double>::value : XwWaves<float>::value;
       if (h->xraw_lc <= 0)
-        h->xraw_lc = chunk_levels(dm.nyu - 2, (long)dm.px / 2, ncu * kXwWaves, 4, 96, 1.0 / 3.0);
+        h->xraw_lc = chunk_levels(dm.nyu - 2, (long)dm.px / 2, ncu * wpb, 4, 96, 1.0 / 3.0);
     }
     if (h->xraw_lc <= 0) h->xraw_lc = chunk_levels(dm.nyu - 2, dm.pz / 2, ncu * bx, 4, 64, 1.0);
     if (h->zasm_lc <= 0)
@@ -751,6 +772,13 @@ int lh_create(const lh_config* c, lh_handle** out) {
     }
     HIPCHK(h, dalloc(&h->twxw, t.size()));
     HIPCHK(h, hipMemcpy(h->twxw, t.data(), t.size() * sizeof(double2), hipMemcpyHostToDevice));
+    if (h->prec != LH_PREC_FP64) {
+      // the same table rounded to FP32 for the mixed-precision transforms
+      std::vector<float2> t32(t.size());
+      for (size_t e = 0; e < t.size(); ++e) t32[e] = make_float2((float)t[e].x, (float)t[e].y);
+      HIPCHK(h, dalloc(&h->twxw32, t32.size()));
+      HIPCHK(h, hipMemcpy(h->twxw32, t32.data(), t32.size() * sizeof(float2), hipMemcpyHostToDevice));
+    }
   }
   // canopy masks: one per distinct footprint; planes -> mask index
   {
@@ -867,7 +895,7 @@ int lh_destroy(lh_handle* h) {
                   h->k2z, h->scal, h->u1, h->u2, h->u3, h->p, h->nu1, h->nu2, h->nu3, h->nb1, h->nb2, h->nb3, h->r1,
                   h->r2, h->r3, h->psi, h->tipl, h->zu, h->zw, h->zv, h->zp, h->dg3, h->ayu, h->ayv, h->ctwork, h->masks, h->ibu_planes,
                   h->ibv_planes, h->ibflag_u, h->ibflag_v, h->ibent, h->st_psum, h->st_pfold, h->st_acc,
-                  h->cst_acc, h->sp_acc, h->fo1, h->fo2, h->fo3, h->twxw};
+                  h->cst_acc, h->sp_acc, h->fo1, h->fo2, h->fo3, h->twxw, h->twxw32};
   for (void* q : ptrs)
     if (q) hipFree(q);
   if (h->twx && h->twx != h->twx64) hipFree(h->twx);

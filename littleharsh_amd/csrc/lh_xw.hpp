@@ -1,4 +1,5 @@
-// lh_xw.hpp -- the nonlinear x pass with one wavefront per 576-point line (gfx950, FP64).
+// lh_xw.hpp -- the nonlinear x pass with one wavefront per 576-point line (gfx950; FP64, and
+// FP32 for the transforms of the mixed-precision mode).
 //
 // Each wave owns one row pair (z0, z0+1) of the Z slots and walks up a chunk of levels, as
 // the block-level pass k_xraw2 does, but every 576-point transform is done by that wave
@@ -36,22 +37,22 @@ constexpr int TW64 = 9 * 64;             // table: w64^{+j} at TW64 + j (j = 0..
 constexpr int TWN = 9 * 64 + 64;         // table entries
 constexpr int LINE = 8 * 72;             // wave-private LDS line (the larger transpose)
 
-using C = double2;
-
-__device__ __forceinline__ C cm(C a, C b) { return mk<C>(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); }
-__device__ __forceinline__ C cmc(C a, C b) { return mk<C>(a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y); }
+// C: double2 (FP64) or float2 (the FP32 transforms of the mixed-precision mode)
+template <typename C> __device__ __forceinline__ C cm(C a, C b) { return mk<C>(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); }
+template <typename C> __device__ __forceinline__ C cmc(C a, C b) { return mk<C>(a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y); }
 __device__ __forceinline__ void mem_fence_compiler() { asm volatile("" ::: "memory"); }
-__device__ __forceinline__ C cadd2(C a, C b) { return mk<C>(a.x + b.x, a.y + b.y); }
-__device__ __forceinline__ C csub2(C a, C b) { return mk<C>(a.x - b.x, a.y - b.y); }
+template <typename C> __device__ __forceinline__ C cadd2(C a, C b) { return mk<C>(a.x + b.x, a.y + b.y); }
+template <typename C> __device__ __forceinline__ C csub2(C a, C b) { return mk<C>(a.x - b.x, a.y - b.y); }
 // multiply by +i (INV) or -i
-template <bool INV> __device__ __forceinline__ C ri(C a) { return INV ? mk<C>(-a.y, a.x) : mk<C>(a.y, -a.x); }
+template <bool INV, typename C> __device__ __forceinline__ C ri(C a) { return INV ? mk<C>(-a.y, a.x) : mk<C>(a.y, -a.x); }
 
 // radix-3 in place on (a, b, c): a <- a+b+c, b <- a + w b + w^2 c, c <- a + w^2 b + w c,
 // w = e^{+2 pi i/3} (INV) or e^{-2 pi i/3}
-template <bool INV> __device__ __forceinline__ void r3(C& a, C& b, C& c) {
-  const double s3 = 0.86602540378443864676;
+template <bool INV, typename C> __device__ __forceinline__ void r3(C& a, C& b, C& c) {
+  using R = decltype(C::x);
+  const R s3 = (R)0.86602540378443864676;
   const C t1 = cadd2(b, c);
-  const C t2 = mk<C>(a.x - 0.5 * t1.x, a.y - 0.5 * t1.y);
+  const C t2 = mk<C>(a.x - (R)0.5 * t1.x, a.y - (R)0.5 * t1.y);
   const C d = csub2(b, c);
   const C r = ri<INV>(mk<C>(s3 * d.x, s3 * d.y));
   a = cadd2(a, t1);
@@ -60,7 +61,7 @@ template <bool INV> __device__ __forceinline__ void r3(C& a, C& b, C& c) {
 }
 
 // 9-point DFT in place, natural order in; X[k1 + 3 k2] ends in v[3 k1 + k2]
-template <bool INV> __device__ __forceinline__ void dft9_t(C (&v)[9]) {
+template <bool INV, typename C> __device__ __forceinline__ void dft9_t(C (&v)[9]) {
   r3<INV>(v[0], v[3], v[6]);
   r3<INV>(v[1], v[4], v[7]);
   r3<INV>(v[2], v[5], v[8]);
@@ -81,8 +82,9 @@ __device__ __host__ constexpr int perm9(int k) { return 3 * (k % 3) + k / 3; }
 
 // 8-point DFT in place (radix-2 decimation in frequency), natural order in; X[k] ends in
 // v[bitrev3(k)]
-template <bool INV> __device__ __forceinline__ void dft8_t(C (&v)[8]) {
-  const double h = 0.70710678118654752440;
+template <bool INV, typename C> __device__ __forceinline__ void dft8_t(C (&v)[8]) {
+  using R = decltype(C::x);
+  const R h = (R)0.70710678118654752440;
 #pragma unroll
   for (int j = 0; j < 4; ++j) {
     const C a = cadd2(v[j], v[j + 4]);
@@ -137,7 +139,20 @@ template <int H> __device__ __forceinline__ double lane_xor(double v, int t) {
   }
   return __hiloint2double(hi, lo);
 }
-template <int H> __device__ __forceinline__ C lane_xor(C v, int t) {
+template <int H> __device__ __forceinline__ float lane_xor(float v, int t) {
+  int x = __float_as_int(v);
+  if constexpr (H == 1 || H == 2 || H == 8) {
+    x = __builtin_amdgcn_mov_dpp(x, H == 1 ? 0xB1 : (H == 2 ? 0x4E : 0x128), 0xF, 0xF, false);
+  } else if constexpr (H == 4 || H == 16) {
+    x = __builtin_amdgcn_ds_swizzle(x, 0x1F | (H << 10));
+  } else {
+    static_assert(H == 32, "lane_xor: H in 1, 2, 4, 8, 16, 32");
+    const auto a = __builtin_amdgcn_permlane32_swap(x, x, false, false);
+    x = (t & 32) != 0 ? a[0] : a[1];
+  }
+  return __int_as_float(x);
+}
+template <int H, typename C> __device__ __forceinline__ C lane_xor(C v, int t) {
   return mk<C>(lane_xor<H>(v.x, t), lane_xor<H>(v.y, t));
 }
 
@@ -145,10 +160,11 @@ template <int H> __device__ __forceinline__ C lane_xor(C v, int t) {
 // lane t ends with output bitrev6(t).  Stage of span H: the lower lane of a pair keeps
 // v + p, the upper one (p - v) w with w = e^{+2 pi i (t mod H) / 2H}; written without
 // branches (the lower lane multiplies by table entry 0, which is 1).
-template <int H> __device__ __forceinline__ C dif_stage(C v, int t, const C* tw) {
+template <int H, typename C> __device__ __forceinline__ C dif_stage(C v, int t, const C* tw) {
+  using R = decltype(C::x);
   const C p = lane_xor<H>(v, t);
   const bool up = (t & H) != 0;
-  const double sg = up ? -1.0 : 1.0;
+  const R sg = up ? (R)-1 : (R)1;
   const C d = mk<C>(fma(sg, v.x, p.x), fma(sg, v.y, p.y));
   if constexpr (H > 1) {
     const int e = up ? (t % H) * (32 / H) : 0;
@@ -157,7 +173,7 @@ template <int H> __device__ __forceinline__ C dif_stage(C v, int t, const C* tw)
     return d;
   }
 }
-__device__ __forceinline__ C dif64_inv(C v, int t, const C* tw) {
+template <typename C> __device__ __forceinline__ C dif64_inv(C v, int t, const C* tw) {
   v = dif_stage<32>(v, t, tw);
   v = dif_stage<16>(v, t, tw);
   v = dif_stage<8>(v, t, tw);
@@ -167,16 +183,17 @@ __device__ __forceinline__ C dif64_inv(C v, int t, const C* tw) {
 }
 // its transpose with the forward sign: input in bit-reversed lane order, output natural.
 // Lower lane: v + conj(w) p; upper lane: p - conj(w) v.
-template <int H> __device__ __forceinline__ C dit_stage(C v, int t, const C* tw) {
+template <int H, typename C> __device__ __forceinline__ C dit_stage(C v, int t, const C* tw) {
+  using R = decltype(C::x);
   const C p = lane_xor<H>(v, t);
   const bool up = (t & H) != 0;
   const C a = up ? p : v;
   C b = up ? v : p;
   if constexpr (H > 1) b = cmc(b, tw[TW64 + (t % H) * (32 / H)]);
-  const double sg = up ? -1.0 : 1.0;
+  const R sg = up ? (R)-1 : (R)1;
   return mk<C>(fma(sg, b.x, a.x), fma(sg, b.y, a.y));
 }
-__device__ __forceinline__ C dit64_fwd(C v, int t, const C* tw) {
+template <typename C> __device__ __forceinline__ C dit64_fwd(C v, int t, const C* tw) {
   v = dit_stage<1>(v, t, tw);
   v = dit_stage<2>(v, t, tw);
   v = dit_stage<4>(v, t, tw);
@@ -187,7 +204,7 @@ __device__ __forceinline__ C dit64_fwd(C v, int t, const C* tw) {
 
 // inverse 576-point transform of one wave: v[r] = X[t + 64 r] in; v[s] = x at the lane
 // layout described above out.  buf: this wave's LDS line.
-__device__ __forceinline__ void inv576(C (&v)[9], int t, C* buf, const C* tw) {
+template <typename C> __device__ __forceinline__ void inv576(C (&v)[9], int t, C* buf, const C* tw) {
   dft9_t<true>(v);                                   // Y[n1] in v[perm9(n1)]
 #pragma unroll
   for (int n1 = 1; n1 < 9; ++n1) v[perm9(n1)] = cm(v[perm9(n1)], tw[TW576 + n1 * 64 + t]);
@@ -221,7 +238,7 @@ __device__ __forceinline__ void inv576(C (&v)[9], int t, C* buf, const C* tw) {
 
 // forward 576-point transform (unscaled): v in the inverse's output layout; out
 // v[r] = X[t + 64 r]
-__device__ __forceinline__ void fwd576(C (&v)[9], int t, C* buf, const C* tw) {
+template <typename C> __device__ __forceinline__ void fwd576(C (&v)[9], int t, C* buf, const C* tw) {
   const int t1 = t & 7, g = t >> 3;
   v[8] = dit64_fwd(v[8], t, tw);                     // Z[8] of lane t
   C c8[8];
@@ -260,9 +277,10 @@ __device__ __forceinline__ void fwd576(C (&v)[9], int t, C* buf, const C* tw) {
 // Every lane issues every load (no load is left to one lane), so that the line waits for
 // memory once: lane 0's mode 0 drops the imaginary parts, and mode 192 (lane 0 of slot 3;
 // the other lanes of that slot are zero padding), by factors 0 and 1.
-__device__ __forceinline__ void load_line(C (&v)[9], const C* __restrict__ A, int t) {
+template <typename C> __device__ __forceinline__ void load_line(C (&v)[9], const C* __restrict__ A, int t) {
+  using R = decltype(C::x);
   const C* B = A + MX;
-  const double lane0 = t == 0 ? 1.0 : 0.0, other = 1.0 - lane0;
+  const R lane0 = t == 0 ? (R)1 : (R)0, other = (R)1 - lane0;
   C a[3], b[3];
 #pragma unroll
   for (int r = 0; r < 3; ++r) {
@@ -274,8 +292,8 @@ __device__ __forceinline__ void load_line(C (&v)[9], const C* __restrict__ A, in
 #pragma unroll
   for (int r = 1; r < 3; ++r) v[r] = mk<C>(a[r].x - b[r].y, a[r].y + b[r].x);
   v[3] = mk<C>(lane0 * (a3.x - b3.y), lane0 * (a3.y + b3.x));
-  v[4] = mk<C>(0.0, 0.0);
-  v[5] = mk<C>(0.0, 0.0);
+  v[4] = mk<C>(0, 0);
+  v[5] = mk<C>(0, 0);
 #pragma unroll
   for (int r = 6; r < 9; ++r) {
     const int m = N - (t + 64 * r);           // 1..192
@@ -286,7 +304,7 @@ __device__ __forceinline__ void load_line(C (&v)[9], const C* __restrict__ A, in
 
 // forward spectrum P of a packed product line -> its row of the product buffer: P[k] at k
 // (k < MX) and P[576 - k] at MX + k (k = 1..MX-1)
-__device__ __forceinline__ void store_line(const C (&v)[9], C* __restrict__ dst, int t) {
+template <typename C> __device__ __forceinline__ void store_line(const C (&v)[9], C* __restrict__ dst, int t) {
 #pragma unroll
   for (int r = 0; r < 3; ++r) dst[t + 64 * r] = v[r];
   if (t == 0) dst[192] = v[3];
@@ -297,9 +315,9 @@ __device__ __forceinline__ void store_line(const C (&v)[9], C* __restrict__ dst,
 }  // namespace xw
 
 struct XW {
-  const double2 *zu, *zw, *zv;   // Z slots [slot][576][193]
-  double2* zp;                   // [6][level][288][2 * 193] packed row pairs
-  const double2* tw;             // xw::TWN twiddles
+  const void *zu, *zw, *zv;      // Z slots [slot][576][193] (cx<TR>)
+  void* zp;                      // [6][level][288][2 * 193] packed row pairs (cx<TR>)
+  const void* tw;                // xw::TWN twiddles (cx<TR>)
   const double* dthdyu;
   double* scal;
   int want_cfl;
@@ -309,9 +327,12 @@ struct XW {
 // LDS of one block of WPB waves: the twiddle table, then per wave a transpose line and a
 // park line (one field of 9 values per lane, lane-major), then each lane's running CFL
 // maximum (a double per lane, kept out of the registers)
-template <int WPB> constexpr size_t xw_lds_bytes() {
-  return sizeof(double2) * ((size_t)xw::TWN + (size_t)WPB * (xw::LINE + 9 * 64)) + sizeof(double) * WPB * 64;
+template <int WPB, typename TR> constexpr size_t xw_lds_bytes() {
+  return sizeof(cx<TR>) * ((size_t)xw::TWN + (size_t)WPB * (xw::LINE + 9 * 64)) + sizeof(double) * WPB * 64;
 }
+// waves per SIMD: FP64 two (the registers' limit); FP32 follows the block size, with one
+// block of up to 16 waves per CU
+template <typename TR, int WPB> constexpr int xw_wpe() { return sizeof(TR) == 8 ? 2 : (WPB + 3) / 4; }
 
 // One wave per (row pair, chunk of lc levels); WPB waves per block.  Products of level l
 // (u centre j = l+1): uu, uw, ww, vv at the centre, uv, vw at the face above, from U, W at
@@ -322,13 +343,15 @@ template <int WPB> constexpr size_t xw_lds_bytes() {
 // by slot as an operand.
 // CFL: whether the launch forms the CFL maximum (a template flag, so that the slot loops
 // carry no branch on it)
-template <int WPB, bool CFL>
-__global__ void __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(2))) k_xw576(XW a, Dims dm) {
-  using C = double2;
+template <int WPB, bool CFL, typename TR>
+__global__ void __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(xw_wpe<TR, WPB>())))
+    k_xw576(XW a, Dims dm) {
+  using C = cx<TR>;
+  const TR half = (TR)0.5;
   extern __shared__ char smem_raw[];
   C* lds = reinterpret_cast<C*>(smem_raw);
   C* tw = lds;                                   // twiddle table, shared by the block
-  for (int e = threadIdx.x; e < xw::TWN; e += blockDim.x) tw[e] = a.tw[e];
+  for (int e = threadIdx.x; e < xw::TWN; e += blockDim.x) tw[e] = static_cast<const C*>(a.tw)[e];
   __syncthreads();
   const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int t = threadIdx.x & 63;
@@ -343,6 +366,10 @@ __global__ void __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(2
   if (l0 >= l1) return;
   const size_t slot = (size_t)xw::N * xw::MX;
   const size_t row = (size_t)pair * 2 * xw::MX;
+  const C* zu = static_cast<const C*>(a.zu);
+  const C* zw = static_cast<const C*>(a.zw);
+  const C* zv = static_cast<const C*>(a.zv);
+  C* zp = static_cast<C*>(a.zp);
   double* const cm = reinterpret_cast<double*>(lds + xw::TWN + WPB * (xw::LINE + 9 * 64)) + wv * 64 + t;
   if constexpr (CFL) *cm = 0.0;
   C cu[9], cw[9], p[9];
@@ -354,24 +381,24 @@ __global__ void __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(2
   };
   auto out = [&](int q, int l, C (&x)[9]) {
     xw::fwd576(x, t, buf, tw);
-    xw::store_line(x, a.zp + ((size_t)q * nlev + l) * slot + row, t);
+    xw::store_line(x, zp + ((size_t)q * nlev + l) * slot + row, t);
   };
-  xw::load_line(cu, a.zu + (size_t)(l0 + 1) * slot + row, t);
+  xw::load_line(cu, zu + (size_t)(l0 + 1) * slot + row, t);
   xw::inv576(cu, t, buf, tw);
-  xw::load_line(cw, a.zw + (size_t)(l0 + 1) * slot + row, t);
+  xw::load_line(cw, zw + (size_t)(l0 + 1) * slot + row, t);
   xw::inv576(cw, t, buf, tw);
-  xw::load_line(p, a.zv + (size_t)l0 * slot + row, t);
+  xw::load_line(p, zv + (size_t)l0 * slot + row, t);
   xw::inv576(p, t, buf, tw);
   park(p);                                        // V(j-1)
   for (int l = l0; l < l1; ++l) {
     // V(j), the face above: vv and the CFL maximum at the centre, from V(j-1) parked
     C vj[9];
-    xw::load_line(vj, a.zv + (size_t)(l + 1) * slot + row, t);
+    xw::load_line(vj, zv + (size_t)(l + 1) * slot + row, t);
     xw::inv576(vj, t, buf, tw);
 #pragma unroll
     for (int s = 0; s < 9; ++s) {
       const C vb = pk[64 * s];
-      p[s] = mk<C>(0.5 * (vb.x + vj[s].x), 0.5 * (vb.y + vj[s].y));   // v at the centre
+      p[s] = mk<C>(half * (vb.x + vj[s].x), half * (vb.y + vj[s].y));   // v at the centre
     }
     xw::mem_fence_compiler();
     park(cw);                                     // W(j) waits while V(j) stays resident
@@ -383,8 +410,8 @@ __global__ void __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(2
 #pragma unroll
       for (int s = 0; s < 9; ++s) {
         const C w = pk[64 * s];
-        const double s0 = fabs(cu[s].x) * dm.dxi + fabs(p[s].x) * dyi + fabs(w.x) * dm.dzi;
-        const double s1 = fabs(cu[s].y) * dm.dxi + fabs(p[s].y) * dyi + fabs(w.y) * dm.dzi;
+        const double s0 = fabs((double)cu[s].x) * dm.dxi + fabs((double)p[s].x) * dyi + fabs((double)w.x) * dm.dzi;
+        const double s1 = fabs((double)cu[s].y) * dm.dxi + fabs((double)p[s].y) * dyi + fabs((double)w.y) * dm.dzi;
         cmax = nanmax(cmax, nanmax(s0, s1));
       }
       *cm = cmax;
@@ -408,11 +435,11 @@ __global__ void __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(2
     }
     out(2, l, p);
     // U(j+1): uv at the face, then carried
-    xw::load_line(p, a.zu + (size_t)(l + 2) * slot + row, t);
+    xw::load_line(p, zu + (size_t)(l + 2) * slot + row, t);
     xw::inv576(p, t, buf, tw);
 #pragma unroll
     for (int s = 0; s < 9; ++s) {
-      const C uf = mk<C>(0.5 * (cu[s].x + p[s].x), 0.5 * (cu[s].y + p[s].y));
+      const C uf = mk<C>(half * (cu[s].x + p[s].x), half * (cu[s].y + p[s].y));
       cu[s] = mk<C>(uf.x * vj[s].x, uf.y * vj[s].y);
     }
     out(4, l, cu);
@@ -420,14 +447,14 @@ __global__ void __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(2
     for (int s = 0; s < 9; ++s) cu[s] = p[s];
     // W(j+1): vw at the face, then carried; W(j) comes back from the park line, which then
     // takes V(j) for the next level
-    xw::load_line(p, a.zw + (size_t)(l + 2) * slot + row, t);
+    xw::load_line(p, zw + (size_t)(l + 2) * slot + row, t);
     xw::inv576(p, t, buf, tw);
     xw::mem_fence_compiler();
 #pragma unroll
     for (int s = 0; s < 9; ++s) {
       const C w = pk[64 * s];
       pk[64 * s] = vj[s];                         // V(j) parked for the next level
-      const C wf = mk<C>(0.5 * (w.x + p[s].x), 0.5 * (w.y + p[s].y));
+      const C wf = mk<C>(half * (w.x + p[s].x), half * (w.y + p[s].y));
       vj[s] = mk<C>(vj[s].x * wf.x, vj[s].y * wf.y);
     }
     xw::mem_fence_compiler();
