@@ -108,6 +108,22 @@ int lh_get_forcing(lh_handle* h, double* f1, double* f2, double* f3);
  * completion either way; the state is then not usable. */
 int lh_substep(lh_handle* h, int krk, double dt);
 
+/* Stage-level hooks for parity bisection: the three stages of substep kRK that lh_substep
+ * runs, one at a time, on the same kernels (synchronous at return).  lh_substep equals
+ * lh_nonlinear, lh_solve_u, lh_solve_p in turn.
+ *   lh_nonlinear: Nu_new of the current state (call nonlinear, littleharsh.f90:171; the
+ *     routine is absent from the reference), into the buffer of this substep, which
+ *     lh_get_nonlinear then returns; the CFL maximum at kRK = 1.
+ *   lh_solve_u: RHS0_u1/u2/u3 (littleharsh.f90:166-168, with Nu_old from the other buffer
+ *     and Nu_new from lh_nonlinear), the immersed boundaries and solveU/V/U
+ *     (littleharsh.f90:173-175; tridLU_3D_IB.f90:50-359): u, v, w before the pressure step.
+ *     RHS0 is formed inside the sweep, so it has no hook of its own.
+ *   lh_solve_p: meanflow_ctU when flag_ctpress = 0, solveP and v_corr
+ *     (littleharsh.f90:178-188): v_corr is fused into the Poisson sweep. */
+int lh_nonlinear(lh_handle* h, int krk);
+int lh_solve_u(lh_handle* h, int krk, double dt);
+int lh_solve_p(lh_handle* h, int krk, double dt);
+
 /* One full RK3 step (three substeps) followed by meanflow_ctP when flag_ctpress == 1
  * (littleharsh.f90:164-197).  Launches are replayed from a HIP graph per dt mode. */
 int lh_step(lh_handle* h, double dt);

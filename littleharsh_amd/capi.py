@@ -111,6 +111,9 @@ def load_library():
     lib.lh_get_forcing.argtypes = [vp, _dp, _dp, _dp]
     lib.lh_get_nonlinear.argtypes = [vp, _dp, _dp, _dp]
     lib.lh_substep.argtypes = [vp, ctypes.c_int, ctypes.c_double]
+    lib.lh_nonlinear.argtypes = [vp, ctypes.c_int]
+    lib.lh_solve_u.argtypes = [vp, ctypes.c_int, ctypes.c_double]
+    lib.lh_solve_p.argtypes = [vp, ctypes.c_int, ctypes.c_double]
     lib.lh_step.argtypes = [vp, ctypes.c_double]
     lib.lh_step_cfl.argtypes = [vp, ctypes.c_double, _dp]
     lib.lh_cfl_dt.argtypes = [vp, ctypes.c_double, _dp]
@@ -165,7 +168,8 @@ def load_library():
 
 EXPORTED = [
     "lh_create", "lh_destroy", "lh_last_error", "lh_set_state", "lh_get_state",
-    "lh_set_nonlinear", "lh_get_nonlinear", "lh_set_forcing", "lh_get_forcing", "lh_substep", "lh_step", "lh_step_cfl",
+    "lh_set_nonlinear", "lh_get_nonlinear", "lh_set_forcing", "lh_get_forcing", "lh_substep", "lh_nonlinear", "lh_solve_u",
+    "lh_solve_p", "lh_step", "lh_step_cfl",
     "lh_cfl_dt", "lh_get_scalars", "lh_max_div", "lh_max_div_noalloc", "lh_stats_accumulate", "lh_stats_read",
     "lh_cond_stats_size", "lh_cond_stats_read", "lh_spectra_accumulate", "lh_spectra_read",
     "lh_timing_enable", "lh_timing_reset", "lh_timing_read",

@@ -432,7 +432,10 @@ __global__ void k_add_forcing(double2* n1, double2* n2, double2* n3, const doubl
   }
 }
 
-static void issue_substep(lh_handle* h, int krk /*0-based*/, bool cfl_mode) {
+// stages of a substep: the nonlinear term, the velocity update (canopy forcing, RHS0 and
+// solveU/V, fused in the sweep), the pressure step (ctU, solveP, v_corr, fused)
+constexpr int kStageNL = 1, kStageVel = 2, kStagePres = 4, kStageAll = 7;
+static void issue_substep(lh_handle* h, int krk /*0-based*/, bool cfl_mode, int stages = kStageAll) {
   const Dims& dm = h->dm;
   const double aR = h->rk[krk], bR = h->rk[3 + krk], gR = h->rk[6 + krk], cR = h->rk[9 + krk],
                dR = h->rk[12 + krk];
@@ -447,7 +450,7 @@ static void issue_substep(lh_handle* h, int krk /*0-based*/, bool cfl_mode) {
   // nonlinear stage (timing mode and batch members keep it in line); at kRK = 1 of a CFL
   // step dt comes from the first x pass, so it forks after k_cfl_dt
   const PREP prep{h->dyu2i, h->dyv2i, h->scal, h->ayu, h->ayv, h->gw[0], h->gw[1], bre, dm.nyu, dm.nyv};
-  const bool side = !h->timing && h->in_batch == 0;
+  const bool side = !h->timing && h->in_batch == 0 && stages == kStageAll;
   auto fork_prep = [&]() {
     (void)hipEventRecord(h->ev_fork, h->st);
     (void)hipStreamWaitEvent(h->side, h->ev_fork, 0);
@@ -456,6 +459,11 @@ static void issue_substep(lh_handle* h, int krk /*0-based*/, bool cfl_mode) {
   };
   const bool late = krk == 0 && cfl_mode;
   if (side && !late) fork_prep();
+  // Nu_new into the buffer of this substep; RHS0 + rhs per column reads both
+  const bool wb = (krk % 2) == 1;
+  double2* nw[3] = {wb ? h->nb1 : h->nu1, wb ? h->nb2 : h->nu2, wb ? h->nb3 : h->nu3};
+  double2* ol[3] = {wb ? h->nu1 : h->nb1, wb ? h->nu2 : h->nb2, wb ? h->nu3 : h->nb3};
+  if (stages & kStageNL) {
   // ---- NL stage: inverse z transforms of u, w (all u planes) and v (all v planes)
   {
     ZJobs jobs{};
@@ -482,10 +490,6 @@ static void issue_substep(lh_handle* h, int krk /*0-based*/, bool cfl_mode) {
   if (krk == 0 && cfl_mode)
     TIMED(h, LH_K_MISC, 0.0, hipLaunchKernelGGL(k_cfl_dt, dim3(1), dim3(1), 0, h->st, h->scal));
   if (side && late) fork_prep();
-  // Nu_new into the buffer of this substep; RHS0 + rhs per column reads both
-  const bool wb = (krk % 2) == 1;
-  double2* nw[3] = {wb ? h->nb1 : h->nu1, wb ? h->nb2 : h->nu2, wb ? h->nb3 : h->nu3};
-  double2* ol[3] = {wb ? h->nu1 : h->nb1, wb ? h->nu2 : h->nb2, wb ? h->nu3 : h->nb3};
   h->nu_last = wb ? 1 : 0;
   {
     ZASM a{};
@@ -511,6 +515,7 @@ static void issue_substep(lh_handle* h, int krk /*0-based*/, bool cfl_mode) {
                              h->st, nw[0], nw[1], nw[2], (const double2*)h->fo1,
                              (const double2*)h->fo2, (const double2*)h->fo3, nu, nv));
   }
+  }   // NL stage
   RHS ra{};
   ra.u1 = h->u1; ra.u2 = h->u2; ra.u3 = h->u3; ra.p = h->p;
   ra.no1 = ol[0]; ra.no2 = ol[1]; ra.no3 = ol[2];
@@ -525,7 +530,7 @@ static void issue_substep(lh_handle* h, int krk /*0-based*/, bool cfl_mode) {
   ra.tip_v0 = h->tip_v[0] + dm.d; ra.tip_v1 = h->tip_v[1] + dm.d;
   ra.C1 = aR / dm.re; ra.C2 = -cR; ra.dR = dR; ra.g = gR; ra.bre = bre;
   // ---- IB stage (FP64 transforms): canopy planes of rhs u, w, v and Lu_y on tip planes
-  if (h->nibent > 0) {
+  if ((stages & kStageVel) && h->nibent > 0) {
     const int npl = h->nibu > h->nibv ? h->nibu : h->nibv;
     // in: u (3 rows), p, Nu_old, Nu_new per canopy plane; out: rhs (+ Lu_y of the tips)
     TIMED(h, LH_K_RHS, spl * 7.0 * (2.0 * h->nibu + h->nibv),
@@ -561,7 +566,7 @@ static void issue_substep(lh_handle* h, int krk /*0-based*/, bool cfl_mode) {
   // ---- column stage
   if (side)
     (void)hipStreamWaitEvent(h->st, h->ev_join, 0);
-  else
+  else if (stages & kStageVel)
     TIMED(h, LH_K_MISC, 0.0, hipLaunchKernelGGL(k_prep, dim3(1), dim3(64), 0, h->st, prep));
   {
     COL a{};
@@ -577,6 +582,7 @@ static void issue_substep(lh_handle* h, int krk /*0-based*/, bool cfl_mode) {
     // in: rhs u, w, v, p, LU factors (8 B each); out: u, w, v, p
     const double by = spl * (2.0 * nuvw + 2.0 * dm.nyp) + 3.0 * 8.0 * dm.ncol * (double)dm.nyp;
     // in: u, p, Nu_old, Nu_new (rhs on canopy planes); eliminated rows out and back; out: u
+    if (stages & kStageVel)
     TIMED(h, LH_K_VEL, spl * (7.0 * nuvw),
           if (h->vel_u == kVelUSmall) {
             hipLaunchKernelGGL(k_vel_rhs<kVelUSmall>, dim3((dm.ncol + h->col_bs - 1) / h->col_bs, 3),
@@ -587,6 +593,7 @@ static void issue_substep(lh_handle* h, int krk /*0-based*/, bool cfl_mode) {
                                dim3(h->col_bs), 0, h->st, ra, dm, (const double*)h->ayu,
                                (const double*)h->ayv);
           });
+    if (stages & kStagePres)
     TIMED(h, LH_K_PRES, by - 2.0 * nuvw * spl + 2.0 * nuvw * spl,
           hipLaunchKernelGGL(k_pres<LH_COL_U>, dim3((dm.ncol + h->col_bs - 1) / h->col_bs),
                              dim3(h->col_bs), 0, h->st, a, dm));
@@ -990,6 +997,22 @@ int lh_get_nonlinear(lh_handle* h, double* n1, double* n2, double* n3) {
   if (n3 && (r = xfer_nl(h, b ? h->nb3 : h->nu3, n3, h->dm.nyu, false))) return r;
   return LH_OK;
 }
+
+// one stage of substep kRK on the handle's stream, synchronous at return
+static int run_stage(lh_handle* h, int krk, double dt, int stage) {
+  if (!h) return LH_ERR_ARG;
+  if (krk < 1 || krk > 3) return fail(h, LH_ERR_ARG, "kRK must be 1, 2 or 3");
+  if (stage != kStageNL && !(dt > 0.0)) return fail(h, LH_ERR_ARG, "dt must be positive");
+  if (dt > 0.0) hipLaunchKernelGGL(k_set_scalar, dim3(1), dim3(1), 0, h->st, h->scal, (int)S_DT, dt);
+  issue_substep(h, krk - 1, false, stage);
+  HIPCHK(h, hipGetLastError());
+  HIPCHK(h, hipStreamSynchronize(h->st));
+  tflush(h);
+  return LH_OK;
+}
+int lh_nonlinear(lh_handle* h, int krk) { return run_stage(h, krk, 0.0, kStageNL); }
+int lh_solve_u(lh_handle* h, int krk, double dt) { return run_stage(h, krk, dt, kStageVel); }
+int lh_solve_p(lh_handle* h, int krk, double dt) { return run_stage(h, krk, dt, kStagePres); }
 
 int lh_substep(lh_handle* h, int krk, double dt) {
   if (!h) return LH_ERR_ARG;

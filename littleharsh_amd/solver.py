@@ -79,6 +79,18 @@ class Channel:
         self._chk(self._lib.lh_get_state(self._h, _p(u), _p(v), _p(w), _p(p)), "lh_get_state")
         return u, v, w, p
 
+    def nonlinear(self, krk):
+        """The nonlinear stage of substep krk (stage-level hook)."""
+        self._chk(self._lib.lh_nonlinear(self._h, int(krk)), "lh_nonlinear")
+
+    def solve_u(self, krk, dt):
+        """RHS0, the canopy forcing and solveU/V/U of substep krk (stage-level hook)."""
+        self._chk(self._lib.lh_solve_u(self._h, int(krk), float(dt)), "lh_solve_u")
+
+    def solve_p(self, krk, dt):
+        """meanflow_ctU, solveP and v_corr of substep krk (stage-level hook)."""
+        self._chk(self._lib.lh_solve_p(self._h, int(krk), float(dt)), "lh_solve_p")
+
     def get_nonlinear(self):
         s = self.s
         n1 = np.zeros((s.ncol, s.nyu - 2), complex)

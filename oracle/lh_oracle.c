@@ -881,6 +881,23 @@ void lho_hook_solvep_vcorr(lho* o, int krk1, double dt) {
   solveP(o, krk1 - 1, dt);
   v_corr(o, krk1 - 1, dt);
 }
+/* the substep split where the GPU's stage hooks split it (littleharsh.f90:166-188):
+ * the velocity part (RHS0, nonlinear, solveU/V/U) and the pressure part (meanflow_ctU when
+ * flag_ctpress = 0, solveP, v_corr) */
+void lho_hook_substep_vel(lho* o, int krk1, double dt) {
+  const int krk = krk1 - 1;
+  rhs0(o, krk);
+  nonlinear(o, krk);
+  solve_vel(o, o->u1, o->du1, o->Lu1, 1, krk, dt);
+  solve_vel(o, o->u2, o->du2, o->Lu2, 0, krk, dt);
+  solve_vel(o, o->u3, o->du3, o->Lu3, 1, krk, dt);
+}
+void lho_hook_substep_pres(lho* o, int krk1, double dt) {
+  const int krk = krk1 - 1;
+  if (o->flag_ctpress == 0) meanflow_ctU(o, krk, dt);
+  solveP(o, krk, dt);
+  v_corr(o, krk, dt);
+}
 /* LU_build + LU_dec of a velocity grid: ay (3 x rows) and axz per column */
 void lho_hook_lu_vel(lho* o, int grid_u, int krk1, double dt, double* ay, double* axz) {
   const double beta = o->bRK[krk1 - 1] * dt / o->re;

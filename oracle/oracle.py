@@ -59,6 +59,8 @@ def lib():
         L.lho_get_work.argtypes = [vp, ctypes.c_int, _dp]
         L.lho_set_nonlinear_full.argtypes = [vp, _dp, _dp, _dp]
         L.lho_hook_solvep_vcorr.argtypes = [vp, ctypes.c_int, ctypes.c_double]
+        L.lho_hook_substep_vel.argtypes = [vp, ctypes.c_int, ctypes.c_double]
+        L.lho_hook_substep_pres.argtypes = [vp, ctypes.c_int, ctypes.c_double]
         L.lho_hook_lu_vel.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_double, _dp, _dp]
         L.lho_get_dg.argtypes = [vp, _dp]
         L.lho_hook_flowrate_corr.argtypes = [vp, ctypes.c_int, ctypes.c_double, ctypes.c_double, _dp]
@@ -159,6 +161,15 @@ class Oracle:
 
     def solvep_vcorr(self, krk, dt):
         lib().lho_hook_solvep_vcorr(self._h, krk, dt)
+
+    def substep_vel(self, krk, dt):
+        """RHS0, nonlinear and solveU/V/U of substep krk (littleharsh.f90:166-175)."""
+        lib().lho_hook_substep_vel(self._h, krk, dt)
+
+    def substep_pres(self, krk, dt):
+        """meanflow_ctU (flag_ctpress = 0), solveP and v_corr of substep krk
+        (littleharsh.f90:178-188)."""
+        lib().lho_hook_substep_pres(self._h, krk, dt)
 
     def lu_vel(self, grid_u, krk, dt):
         nj = self.s.nyu if grid_u else self.s.nyv

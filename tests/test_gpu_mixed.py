@@ -46,10 +46,12 @@ def test_mixed_production_substeps_against_fp64_oracle(require_gpu):
     u32 log2(576) ~ 6e-7 per transform into Nu (u32 = 6e-8); it enters the velocity as
     dt gRK dNu and the pressure through the Poisson solve.  Near the walls the y-difference
     m (P(j) - P(j-1)) of Nu subtracts FP32 products much larger than Nu itself, which lifts
-    the max-norm error of Nu1 to about 1e-5: 9.96e-6 and 1.03e-5 in substep 2 for two builds
-    whose only difference is the contraction of A = ikx P0 + m (Ph - Pl) into FMAs (a change
-    of one ulp per product).  Stated bound: 2e-5 relative in max norm on Nu and on u, v, w, p,
-    twice the observed spread and far below the O(1e-3..1) error of a wrong row, mode or
+    the max-norm error of Nu1 to about 1e-5.  Measured in substep 2: 9.96e-6 and 1.03e-5 with
+    the block x transform (two builds that differ only in how A = ikx P0 + m (Ph - Pl) is
+    contracted into FMAs, a change of one ulp per product), 1.60e-5 with the one-wave-per-line
+    transform, which rounds more often per point (one point in nine passes six radix-2 stages
+    with twiddles).  Stated bound: 3e-5 relative in max norm on Nu and on u, v, w, p, about
+    twice the largest observed and far below the O(1e-3..1) error of a wrong row, mode or
     twiddle."""
     from littleharsh_amd.channel import config_production
     from littleharsh_amd.initial import laminar_plus_noise as lpn
@@ -69,8 +71,8 @@ def test_mixed_production_substeps_against_fp64_oracle(require_gpu):
         serr = [_rel(a, b) for a, b in zip(g.get_state(), o.get_state())]
         print(f"mixed production substep {k}: Nu", " ".join(f"{e:.2e}" for e in nerr),
               "| u v w p", " ".join(f"{e:.2e}" for e in serr), flush=True)
-        assert max(nerr) < 2e-5
-        assert max(serr) < 2e-5
+        assert max(nerr) < 3e-5
+        assert max(serr) < 3e-5
 
 
 def test_mixed_production_statistics_track_fp64(require_gpu):
