@@ -145,6 +145,24 @@ module lh_amd
       integer(c_int), value :: krk
       real(c_double), value :: dt
     end function
+    ! the three stages of substep kRK, for parity bisection (in turn they equal lh_substep)
+    integer(c_int) function lh_nonlinear(h, krk) bind(C, name="lh_nonlinear")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: h
+      integer(c_int), value :: krk
+    end function
+    integer(c_int) function lh_solve_u(h, krk, dt) bind(C, name="lh_solve_u")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: h
+      integer(c_int), value :: krk
+      real(c_double), value :: dt
+    end function
+    integer(c_int) function lh_solve_p(h, krk, dt) bind(C, name="lh_solve_p")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: h
+      integer(c_int), value :: krk
+      real(c_double), value :: dt
+    end function
     integer(c_int) function lh_step(h, dt) bind(C, name="lh_step")
       import :: c_int, c_ptr, c_double
       type(c_ptr), value :: h
