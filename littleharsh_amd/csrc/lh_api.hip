@@ -383,7 +383,9 @@ static void launch_xib_t(lh_handle* h, const XIB& a, int nent, const void* tw) {
 template <int PX, typename TR>
 static void launch_stats_t(lh_handle* h, const STATS& a) {
   dim3 grid(a.dnz, a.n);
-  size_t sh = (size_t)3 * (PX + 1) * sizeof(cx<double>) + (size_t)6 * a.dnx * sizeof(double);
+  // lines, the fold [6][dnx] and the fold parts [parts][6][dnx] (parts <= 3 T / dnx, >= 1)
+  const int parts = std::max(1, std::min(3 * Plan<PX>::T / a.dnx, a.ntx));
+  size_t sh = (size_t)3 * (PX + 1) * sizeof(cx<double>) + (size_t)6 * a.dnx * (1 + parts) * sizeof(double);
   hipLaunchKernelGGL((k_stats<PX, TR>), grid, dim3(3 * Plan<PX>::T), sh, h->st, a, h->dm,
                      (const cx<double>*)h->twx64);
 }

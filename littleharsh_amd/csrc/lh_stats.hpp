@@ -80,6 +80,7 @@ __global__ void __launch_bounds__(3 * Plan<PX>::T)
   extern __shared__ char smem_raw[];
   C* lds = reinterpret_cast<C*>(smem_raw);
   double* fold = reinterpret_cast<double*>(lds + 3 * LS);   // [6][dnx]
+  double* foldp = fold + 6 * a.dnx;                          // [part][6][dnx], NT / dnx parts
   __shared__ double red[6][NT];
   const int kk = blockIdx.x, p = blockIdx.y;
   const int mx = dm.mx, dnx = a.dnx;
@@ -93,25 +94,32 @@ __global__ void __launch_bounds__(3 * Plan<PX>::T)
     const int zA = kk + a.dnz * iz;
     const bool hasB = iz + 1 < a.ntz;
     const int zB = zA + a.dnz;
-    for (int e = threadIdx.x; e < 3 * PX; e += NT) {
-      const int f = e / PX, nn = e % PX;
-      C v = mk<C>(0.0, 0.0);
-      if (f < a.nf) {
-        const C* base = reinterpret_cast<const C*>(a.z) + (size_t)(f * a.n + p) * slot;
-        const C* A = base + (size_t)zA * mx;
-        int m = -1;
-        bool conj = false;
-        if (nn < mx) m = nn;
-        else if (nn > PX - mx) { m = PX - nn; conj = true; }
-        if (m >= 0) {
-          C ca = A[m];
-          C cb = hasB ? base[(size_t)zB * mx + m] : mk<C>(0.0, 0.0);
-          if (m == 0) v = mk<C>(ca.x, cb.x);
-          else if (!conj) v = mk<C>(ca.x - cb.y, ca.y + cb.x);
-          else v = mk<C>(ca.x + cb.y, -ca.y + cb.x);
-        }
+    // every load issued unconditionally (clamped to a valid element and masked afterwards),
+    // so that the two rows wait for memory once
+    constexpr int NE = (3 * PX + NT - 1) / NT;
+    C ca[NE], cb[NE];
+#pragma unroll
+    for (int j = 0; j < NE; ++j) {
+      const int e = min((int)threadIdx.x + j * NT, 3 * PX - 1);
+      const int f = min(e / PX, a.nf - 1), nn = e % PX;
+      const int m = nn < mx ? nn : (nn > PX - mx ? PX - nn : 0);
+      const C* base = reinterpret_cast<const C*>(a.z) + (size_t)(f * a.n + p) * slot;
+      ca[j] = base[(size_t)zA * mx + m];
+      cb[j] = base[(size_t)(hasB ? zB : zA) * mx + m];
+    }
+#pragma unroll
+    for (int j = 0; j < NE; ++j) {
+      const int e = threadIdx.x + j * NT;
+      if (NE * NT == 3 * PX || e < 3 * PX) {
+        const int f = e / PX, nn = e % PX;
+        const bool lo = nn < mx, hi = !lo && nn > PX - mx;
+        const C b = hasB ? cb[j] : mk<C>(0.0, 0.0);
+        C v = mk<C>(0.0, 0.0);
+        if (f < a.nf && nn == 0) v = mk<C>(ca[j].x, b.x);
+        else if (f < a.nf && lo) v = mk<C>(ca[j].x - b.y, ca[j].y + b.x);
+        else if (f < a.nf && hi) v = mk<C>(ca[j].x + b.y, -ca[j].y + b.x);
+        lds[f * LS + nn] = v;
       }
-      lds[f * LS + nn] = v;
     }
     lds_bar();
     Plan<PX>::template run<true>(lds + ln * LS, t, w);
@@ -129,21 +137,29 @@ __global__ void __launch_bounds__(3 * Plan<PX>::T)
           if (!(a.type == ST_V && l >= 3 && !wxin)) s[l] += q[l];
       }
     }
-    // tile fold: thread owns tile columns i
-    for (int i = threadIdx.x; i < dnx; i += NT) {
+    // tile fold: tile column i = e mod dnx of part e / dnx, the parts taking every nparts-th
+    // tile; the parts' sums are added in part order
+    const int nparts = max(1, min(NT / dnx, a.ntx));
+    for (int e = threadIdx.x; e < nparts * dnx; e += NT) {
+      const int i = e % dnx, part = e / dnx;
       double acc[6] = {0, 0, 0, 0, 0, 0};
-      int nq = 0;
-      for (int ix = 0; ix < a.ntx; ++ix) {
+      for (int ix = part; ix < a.ntx; ix += nparts) {
         const int x = i + dnx * ix;
         const C f0 = lds[x], f1 = lds[LS + x], f2 = lds[2 * LS + x];
         for (int r = 0; r < 2; ++r) {
           if (r == 1 && !hasB) break;
           double q[6];
-          nq = stats_q(a.type, r ? f0.y : f0.x, r ? f1.y : f1.x, r ? f2.y : f2.x, q);
+          const int nq = stats_q(a.type, r ? f0.y : f0.x, r ? f1.y : f1.x, r ? f2.y : f2.x, q);
           for (int l = 0; l < nq; ++l) acc[l] += q[l];
         }
       }
-      for (int l = 0; l < nq; ++l) fold[l * dnx + i] += acc[l];
+      for (int l = 0; l < 6; ++l) foldp[(part * 6 + l) * dnx + i] = acc[l];
+    }
+    lds_bar();
+    for (int e = threadIdx.x; e < 6 * dnx; e += NT) {
+      double v = fold[e];
+      for (int part = 0; part < nparts; ++part) v += foldp[part * 6 * dnx + e];
+      fold[e] = v;
     }
     lds_bar();
   }
