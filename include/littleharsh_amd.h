@@ -197,6 +197,22 @@ int lh_spectra_read(lh_handle* h, double* out, int reset);
  * the state is saved before and restored after. */
 int lh_set_launch(lh_handle* h, int xraw_lc, int zasm_lc, int col_bs);
 int lh_get_launch(lh_handle* h, int* xraw_lc, int* zasm_lc, int* col_bs);
+
+/* FFT plans (no counterpart in the reference, whose cfftf/rfftf factor each length one
+ * way, cft_buff.f and rft_buff.f).  Each transform length has plan 0, its default radix
+ * sequence, and the lengths 48, 72, 96 and 144 also plan 1:
+ *   length  plan 0 (radices, threads per line)   plan 1
+ *   48      12 x 4, 4                           6 x 8, 8
+ *   72      12 x 6, 6                           8 x 9, 9
+ *   96      4 x 4 x 6, 8                        8 x 12, 12
+ *   144     12 x 12, 12                         9 x 16, 16
+ * xplan selects the plan of the x transforms (length px), zplan that of the z transforms
+ * (length pz).  The plans agree to rounding (not bit for bit).  lh_get_plans also returns
+ * how many plans each length has (xcount, zcount; 1 or 2). */
+int lh_set_plans(lh_handle* h, int xplan, int zplan);
+int lh_get_plans(lh_handle* h, int* xplan, int* zplan, int* xcount, int* zcount);
+
+/* lh_autotune first times every (xplan, zplan) pair, then the launch parameters. */
 int lh_autotune(lh_handle* h, int steps);
 
 /* Many independent channels (handles of any sizes on one device) advanced together: one
@@ -204,7 +220,7 @@ int lh_autotune(lh_handle* h, int steps);
  * instances are parallel branches, each on its handle's stream (no counterpart in the
  * reference; the instances share nothing). */
 /* A batch re-captures its graph when the CFL number changes or when any member's own graphs
- * were dropped (lh_set_launch, lh_autotune, lh_set_forcing switching the forcing on or off,
+ * were dropped (lh_set_launch, lh_set_plans, lh_autotune, lh_set_forcing switching the forcing on or off,
  * lh_closure_create); each handle may appear once.  Destroy the batch before its members. */
 typedef struct lh_batch lh_batch;
 int lh_batch_create(int n, lh_handle* const* handles, lh_batch** out);

@@ -1,7 +1,8 @@
 """The batched multi-instance workload: 48 independent channels of different sizes and
 canopies (random_instance(0..47)), each autotuned, advanced together by one graph launch
 per RK3 step (lh_batch_step_cfl).  Reports the aggregate grid-point substeps/s next to
-stepping the instances one after another.
+stepping the instances one after another, and the FFT plans and launch parameters the
+autotuner chose.
 
     python -m littleharsh_amd.batch_bench --instances 48 --steps 20
 """
@@ -53,6 +54,12 @@ def main():
         b.step_cfl(args.cfl)
     bat = time.perf_counter() - t0
     b.close()
+    # (x length, its plan, z length, its plan) -> instances
+    plan_counts = {}
+    for s, c in zip(setups, chans):
+        p = c.get_plans()
+        key = (s.px, p["xplan"], s.pz, p["zplan"])
+        plan_counts[key] = plan_counts.get(key, 0) + 1
     out = {
         "workload": f"{len(setups)} channels, nx,nz in {{32,48,64,96}}, ny in {{65,97,129}}",
         "grid_points": G, "steps": args.steps,
@@ -60,7 +67,7 @@ def main():
         "sequential": {"value": G * 3 * args.steps / seq, "ms_per_step": seq / args.steps * 1e3},
         "unit": "grid-point substeps/s",
         "speedup": seq / bat,
-        "launch": {k: sum(1 for t in tuned if t[k] == v) for k in ("col_bs",) for v in (64,)},
+        "plans": {f"{px}:{xp},{pz}:{zp}": n for (px, xp, pz, zp), n in sorted(plan_counts.items())},
         "tuned": tuned,
     }
     print(json.dumps(out), flush=True)

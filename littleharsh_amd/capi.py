@@ -128,6 +128,8 @@ def load_library():
     lib.lh_spectra_read.argtypes = [vp, _dp, ctypes.c_int]
     lib.lh_set_launch.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int]
     lib.lh_get_launch.argtypes = [vp, _ip, _ip, _ip]
+    lib.lh_set_plans.argtypes = [vp, ctypes.c_int, ctypes.c_int]
+    lib.lh_get_plans.argtypes = [vp, _ip, _ip, _ip, _ip]
     lib.lh_autotune.argtypes = [vp, ctypes.c_int]
     lib.lh_batch_create.argtypes = [ctypes.c_int, ctypes.POINTER(vp), ctypes.POINTER(vp)]
     lib.lh_batch_step_cfl.argtypes = [vp, ctypes.c_double]
@@ -173,7 +175,7 @@ EXPORTED = [
     "lh_cfl_dt", "lh_get_scalars", "lh_max_div", "lh_max_div_noalloc", "lh_stats_accumulate", "lh_stats_read",
     "lh_cond_stats_size", "lh_cond_stats_read", "lh_spectra_accumulate", "lh_spectra_read",
     "lh_timing_enable", "lh_timing_reset", "lh_timing_read",
-    "lh_set_launch", "lh_get_launch", "lh_autotune",
+    "lh_set_launch", "lh_get_launch", "lh_set_plans", "lh_get_plans", "lh_autotune",
     "lh_batch_create", "lh_batch_step_cfl", "lh_batch_last_error", "lh_batch_destroy",
     "lh_mlp_create", "lh_mlp_destroy", "lh_mlp_last_error", "lh_mlp_param_count",
     "lh_mlp_get_params", "lh_mlp_set_params", "lh_mlp_set_data", "lh_mlp_loss_grad",

@@ -127,6 +127,7 @@ struct lh_handle {
   int xraw_lc = LH_XRAW_LC;
   int zasm_lc = LH_ZASM_LC;
   int col_bs = LH_COL_BS;
+  int xplan = 0, zplan = 0;     // FFT plan of the x and z lengths (Plan<n, v>)
   // rows per step of the velocity sweep: 1 on large grids (fewer registers, more resident
   // waves), kVelUSmall when few columns leave the sweeps latency-bound (set at creation)
   int vel_u = 1;
@@ -253,60 +254,64 @@ static bool supported_len(int n) {
 // unroll and the block size change nothing there).
 constexpr int kVelSmallCols = 16384;
 constexpr int kVelUSmall = 4;
-template <int N> constexpr int zlines() {
-  return N >= 384 ? LH_ZLINES_LONG : (Plan<N>::T >= 8 ? 16 : 32);
+template <int N, int V> constexpr int zlines() {
+  return N >= 384 ? LH_ZLINES_LONG : (Plan<N, V>::T >= 8 ? 16 : 32);
 }
 
-template <int PZ, typename TR>
+template <int PZ, typename TR, int V>
 static void launch_zinv_t(lh_handle* h, const ZJobs& jobs, int njobs, int maxn, const void* tw) {
-  constexpr int L = zlines<PZ>();
+  constexpr int L = zlines<PZ, V>();
   ZJobs jb = jobs;
   jb.maxn = maxn;
   dim3 grid(((h->dm.mx + L - 1) / L) * maxn * njobs);
   size_t sh = (size_t)L * zls<PZ>() * sizeof(cx<TR>);
-  hipLaunchKernelGGL((k_zinv<PZ, L, TR>), grid, dim3(L * Plan<PZ>::T), sh, h->st, jb, h->dm,
+  hipLaunchKernelGGL((k_zinv<PZ, L, TR, V>), grid, dim3(L * Plan<PZ, V>::T), sh, h->st, jb, h->dm,
                      (const cx<TR>*)tw);
 }
-template <int PZ, typename TR>
+template <int PZ, typename TR, int V>
 static void launch_zfwd_t(lh_handle* h, const ZJobs& jobs, int njobs, int maxn, const void* tw) {
-  constexpr int L = zlines<PZ>();
+  constexpr int L = zlines<PZ, V>();
   ZJobs jb = jobs;
   jb.maxn = maxn;
   dim3 grid(((h->dm.mx + L - 1) / L) * maxn * njobs);
   size_t sh = (size_t)L * zls<PZ>() * sizeof(cx<TR>);
-  hipLaunchKernelGGL((k_zfwd<PZ, L, TR>), grid, dim3(L * Plan<PZ>::T), sh, h->st, jb, h->dm,
+  hipLaunchKernelGGL((k_zfwd<PZ, L, TR, V>), grid, dim3(L * Plan<PZ, V>::T), sh, h->st, jb, h->dm,
                      (const cx<TR>*)tw);
 }
-template <int PZ, typename TR>
+template <int PZ, typename TR, int V>
 static void attr_z_t(int* dummy) {
-  constexpr int L = zlines<PZ>();
+  constexpr int L = zlines<PZ, V>();
   (void)dummy;
   const int sh = (int)(L * zls<PZ>() * sizeof(cx<TR>));
-  (void)hipFuncSetAttribute((const void*)k_zinv<PZ, L, TR>,
+  (void)hipFuncSetAttribute((const void*)k_zinv<PZ, L, TR, V>,
                             hipFuncAttributeMaxDynamicSharedMemorySize, sh);
-  (void)hipFuncSetAttribute((const void*)k_zfwd<PZ, L, TR>,
+  (void)hipFuncSetAttribute((const void*)k_zfwd<PZ, L, TR, V>,
                             hipFuncAttributeMaxDynamicSharedMemorySize, sh);
 }
-template <int PX, typename TR>
+template <int PX, typename TR, int V>
 static size_t xraw_lds() {
-  return ((size_t)15 * (PadLen<PX, LH_XRAW_PAD>::value + LH_XRAW_LSP) + Plan<PX>::template TwL<cx<TR>>::SIZE) *
-         sizeof(cx<TR>);
+  return ((size_t)15 * (PadLen<PX, LH_XRAW_PAD>::value + LH_XRAW_LSP) +
+          Plan<PX, V>::template TwL<cx<TR>>::SIZE) * sizeof(cx<TR>);
+}
+template <int PX, typename TR, int V>
+static void attr_xraw_t(int* dummy) {
+  (void)dummy;
+  (void)hipFuncSetAttribute((const void*)k_xraw2<PX, TR, V>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)xraw_lds<PX, TR, V>());
 }
 // resident blocks per CU of the x pass (LDS and registers)
-template <int PX, typename TR>
+template <int PX, typename TR, int V>
 static void xraw_occ_t(int* nb) {
-  (void)hipFuncSetAttribute((const void*)k_xraw2<PX, TR>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                            (int)xraw_lds<PX, TR>());
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(nb, k_xraw2<PX, TR>, 6 * Plan<PX>::T, xraw_lds<PX, TR>()) !=
-      hipSuccess)
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(nb, k_xraw2<PX, TR, V>, 6 * Plan<PX, V>::T,
+                                                   xraw_lds<PX, TR, V>()) != hipSuccess)
     *nb = 1;
 }
-template <int PX, typename TR>
+template <int PX, typename TR, int V>
 static void launch_xraw_t(lh_handle* h, const XRAW& a, const void* tw) {
-  const size_t sh = xraw_lds<PX, TR>();
+  const size_t sh = xraw_lds<PX, TR, V>();
   const int nlev = h->dm.nyu - 2, lc = h->xraw_lc;
   dim3 grid(h->dm.pz / 2, (nlev + lc - 1) / lc);
-  hipLaunchKernelGGL((k_xraw2<PX, TR>), grid, dim3(6 * Plan<PX>::T), sh, h->st, a, h->dm,
+  hipLaunchKernelGGL((k_xraw2<PX, TR, V>), grid, dim3(6 * Plan<PX, V>::T), sh, h->st, a, h->dm,
                      (const cx<TR>*)tw, lc);
 }
 // the one-wave-per-line x pass: one block per CU (LDS), LH_XW_WAVES32 waves per block in FP32
@@ -345,38 +350,38 @@ static hipError_t attr_xw_t() {
                              (int)xw_lds_bytes<W, TR>());
 }
 constexpr int kZasmCols = LH_ZASM_COLS;
-template <int PZ, typename TR>
+template <int PZ, typename TR, int V>
 static size_t zasm_lds() {
-  return ((size_t)2 * kZasmCols * (PZ + 1) + ZasmTw<PZ, cx<TR>>::SIZE) * sizeof(cx<TR>);
+  return ((size_t)2 * kZasmCols * (PZ + 1) + ZasmTw<PZ, cx<TR>, V>::SIZE) * sizeof(cx<TR>);
 }
-template <int PZ, typename TR>
+template <int PZ, typename TR, int V>
 static void launch_zasm_t(lh_handle* h, const ZASM& a, const void* tw) {
   constexpr int L = kZasmCols;
   const int nlev = h->dm.nyu - 2;
   dim3 grid(a.nib * ((nlev + a.lc - 1) / a.lc) * 3);
-  const size_t sh = zasm_lds<PZ, TR>();
-  hipLaunchKernelGGL((k_zasm<PZ, L, TR>), grid, dim3(2 * L * Plan<PZ>::T), sh, h->st, a, h->dm,
+  const size_t sh = zasm_lds<PZ, TR, V>();
+  hipLaunchKernelGGL((k_zasm<PZ, L, TR, V>), grid, dim3(2 * L * Plan<PZ, V>::T), sh, h->st, a, h->dm,
                      (const cx<TR>*)tw);
 }
-template <int PZ, typename TR>
+template <int PZ, typename TR, int V>
 static void zasm_occ_t(int* nb) {
   constexpr int L = kZasmCols;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(nb, k_zasm<PZ, L, TR>, 2 * L * Plan<PZ>::T, zasm_lds<PZ, TR>()) !=
-      hipSuccess)
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(nb, k_zasm<PZ, L, TR, V>, 2 * L * Plan<PZ, V>::T,
+                                                   zasm_lds<PZ, TR, V>()) != hipSuccess)
     *nb = 1;
 }
-template <int PZ, typename TR>
+template <int PZ, typename TR, int V>
 static void attr_zasm_t(int* dummy) {
   constexpr int L = kZasmCols;
   (void)dummy;
-  (void)hipFuncSetAttribute((const void*)k_zasm<PZ, L, TR>,
-                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)zasm_lds<PZ, TR>());
+  (void)hipFuncSetAttribute((const void*)k_zasm<PZ, L, TR, V>,
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)zasm_lds<PZ, TR, V>());
 }
-template <int PX, typename TR>
+template <int PX, typename TR, int V>
 static void launch_xib_t(lh_handle* h, const XIB& a, int nent, const void* tw) {
   dim3 grid(h->dm.pz / (a.paired ? 4 : 2), nent);
   size_t sh = (size_t)2 * (PX + 1) * sizeof(cx<TR>);
-  hipLaunchKernelGGL((k_xpass_ib<PX, TR>), grid, dim3(2 * Plan<PX>::T), sh, h->st, a, h->dm,
+  hipLaunchKernelGGL((k_xpass_ib<PX, TR, V>), grid, dim3(2 * Plan<PX, V>::T), sh, h->st, a, h->dm,
                      (const cx<TR>*)tw);
 }
 
@@ -402,19 +407,46 @@ static void launch_stats_t(lh_handle* h, const STATS& a) {
     case 576: F<576, TR>(__VA_ARGS__); break;    \
     default: break;                              \
   }
+// the same for the launchers templated on a plan: v selects Plan<n, v> where n has two
+#define LH_PLAN_CASE2(N, v, F, TR, ...)                          \
+  case N:                                                         \
+    if (v) F<N, TR, 1>(__VA_ARGS__); else F<N, TR, 0>(__VA_ARGS__); \
+    break;
+#define LH_DISPATCH_PLAN(n, v, F, TR, ...)         \
+  switch (n) {                                      \
+    case 32: F<32, TR, 0>(__VA_ARGS__); break;      \
+    LH_PLAN_CASE2(48, v, F, TR, __VA_ARGS__)        \
+    case 64: F<64, TR, 0>(__VA_ARGS__); break;      \
+    LH_PLAN_CASE2(72, v, F, TR, __VA_ARGS__)        \
+    LH_PLAN_CASE2(96, v, F, TR, __VA_ARGS__)        \
+    LH_PLAN_CASE2(144, v, F, TR, __VA_ARGS__)       \
+    case 384: F<384, TR, 0>(__VA_ARGS__); break;    \
+    case 576: F<576, TR, 0>(__VA_ARGS__); break;    \
+    default: break;                                 \
+  }
+// number of FFT plans compiled for length n
+static int plans_of_len(int n) {
+  switch (n) {
+    case 48: return plan_count<48>();
+    case 72: return plan_count<72>();
+    case 96: return plan_count<96>();
+    case 144: return plan_count<144>();
+    default: return 1;
+  }
+}
 
 static void launch_zinv(lh_handle* h, const ZJobs& jobs, int njobs, int maxn, bool fp64) {
   if (fp64 || h->prec == LH_PREC_FP64) {
-    LH_DISPATCH_LEN(h->dm.pz, launch_zinv_t, double, h, jobs, njobs, maxn, fp64 ? h->twz64 : h->twz)
+    LH_DISPATCH_PLAN(h->dm.pz, h->zplan, launch_zinv_t, double, h, jobs, njobs, maxn, fp64 ? h->twz64 : h->twz)
   } else {
-    LH_DISPATCH_LEN(h->dm.pz, launch_zinv_t, float, h, jobs, njobs, maxn, h->twz)
+    LH_DISPATCH_PLAN(h->dm.pz, h->zplan, launch_zinv_t, float, h, jobs, njobs, maxn, h->twz)
   }
 }
 static void launch_zfwd(lh_handle* h, const ZJobs& jobs, int njobs, int maxn, bool fp64) {
   if (fp64 || h->prec == LH_PREC_FP64) {
-    LH_DISPATCH_LEN(h->dm.pz, launch_zfwd_t, double, h, jobs, njobs, maxn, fp64 ? h->twz64 : h->twz)
+    LH_DISPATCH_PLAN(h->dm.pz, h->zplan, launch_zfwd_t, double, h, jobs, njobs, maxn, fp64 ? h->twz64 : h->twz)
   } else {
-    LH_DISPATCH_LEN(h->dm.pz, launch_zfwd_t, float, h, jobs, njobs, maxn, h->twz)
+    LH_DISPATCH_PLAN(h->dm.pz, h->zplan, launch_zfwd_t, float, h, jobs, njobs, maxn, h->twz)
   }
 }
 
@@ -483,9 +515,9 @@ static void issue_substep(lh_handle* h, int krk /*0-based*/, bool cfl_mode, int 
       if (use_xw(h)) {
         launch_xw(h, krk == 0 ? 1 : 0);
       } else if (f64) {
-        LH_DISPATCH_LEN(dm.px, launch_xraw_t, double, h, a, h->twx)
+        LH_DISPATCH_PLAN(dm.px, h->xplan, launch_xraw_t, double, h, a, h->twx)
       } else {
-        LH_DISPATCH_LEN(dm.px, launch_xraw_t, float, h, a, h->twx)
+        LH_DISPATCH_PLAN(dm.px, h->xplan, launch_xraw_t, float, h, a, h->twx)
       }
     });
   }
@@ -504,9 +536,9 @@ static void issue_substep(lh_handle* h, int krk /*0-based*/, bool cfl_mode, int 
     const double by = 6.0 * nlev * zsl + spl * (2.0 * nlev + (nlev - 1));
     TIMED(h, LH_K_ASM, by, {
       if (f64) {
-        LH_DISPATCH_LEN(dm.pz, launch_zasm_t, double, h, a, h->twz)
+        LH_DISPATCH_PLAN(dm.pz, h->zplan, launch_zasm_t, double, h, a, h->twz)
       } else {
-        LH_DISPATCH_LEN(dm.pz, launch_zasm_t, float, h, a, h->twz)
+        LH_DISPATCH_PLAN(dm.pz, h->zplan, launch_zasm_t, float, h, a, h->twz)
       }
     });
   }
@@ -553,9 +585,9 @@ static void issue_substep(lh_handle* h, int krk /*0-based*/, bool cfl_mode, int 
     XIB a{h->zp, h->ibent, h->masks, h->scal, bre, 0};
     XIB b{h->zp, h->ibent + h->nibtip, h->masks, h->scal, bre, 1};
     TIMED(h, LH_K_XIB, (2.0 * nib + 6.0) * z64, {
-      if (h->nibtip > 0) LH_DISPATCH_LEN(dm.px, launch_xib_t, double, h, a, h->nibtip, h->twx64)
+      if (h->nibtip > 0) LH_DISPATCH_PLAN(dm.px, h->xplan, launch_xib_t, double, h, a, h->nibtip, h->twx64)
       if (h->nibent > h->nibtip)
-        LH_DISPATCH_LEN(dm.px, launch_xib_t, double, h, b, h->nibent - h->nibtip, h->twx64)
+        LH_DISPATCH_PLAN(dm.px, h->xplan, launch_xib_t, double, h, b, h->nibent - h->nibtip, h->twx64)
     });
     ZJobs fj{};
     slot = 0;
@@ -661,10 +693,16 @@ int lh_create(const lh_config* c, lh_handle** out) {
   HIPCHK(h, hipEventCreateWithFlags(&h->ev_join, hipEventDisableTiming));
   {
     int dummy = 0;
-    LH_DISPATCH_LEN(c->pz, attr_zasm_t, double, &dummy)
-    LH_DISPATCH_LEN(c->pz, attr_zasm_t, float, &dummy)
-    LH_DISPATCH_LEN(c->pz, attr_z_t, double, &dummy)
-    LH_DISPATCH_LEN(c->pz, attr_z_t, float, &dummy)
+    for (int v = 0; v < plans_of_len(c->pz); ++v) {
+      LH_DISPATCH_PLAN(c->pz, v, attr_zasm_t, double, &dummy)
+      LH_DISPATCH_PLAN(c->pz, v, attr_zasm_t, float, &dummy)
+      LH_DISPATCH_PLAN(c->pz, v, attr_z_t, double, &dummy)
+      LH_DISPATCH_PLAN(c->pz, v, attr_z_t, float, &dummy)
+    }
+    for (int v = 0; v < plans_of_len(c->px); ++v) {
+      LH_DISPATCH_PLAN(c->px, v, attr_xraw_t, double, &dummy)
+      LH_DISPATCH_PLAN(c->px, v, attr_xraw_t, float, &dummy)
+    }
   }
   {
     int ncu = 0;
@@ -673,11 +711,11 @@ int lh_create(const lh_config* c, lh_handle** out) {
     // slots = CUs x resident blocks per CU of the kernel at its precision
     int bx = 1, bz = 1;
     if (h->prec == LH_PREC_MIXED) {
-      LH_DISPATCH_LEN(dm.px, xraw_occ_t, float, &bx)
-      LH_DISPATCH_LEN(dm.pz, zasm_occ_t, float, &bz)
+      LH_DISPATCH_PLAN(dm.px, h->xplan, xraw_occ_t, float, &bx)
+      LH_DISPATCH_PLAN(dm.pz, h->zplan, zasm_occ_t, float, &bz)
     } else {
-      LH_DISPATCH_LEN(dm.px, xraw_occ_t, double, &bx)
-      LH_DISPATCH_LEN(dm.pz, zasm_occ_t, double, &bz)
+      LH_DISPATCH_PLAN(dm.px, h->xplan, xraw_occ_t, double, &bx)
+      LH_DISPATCH_PLAN(dm.pz, h->zplan, zasm_occ_t, double, &bz)
     }
     if (bx < 1) bx = 1;
     if (bz < 1) bz = 1;
@@ -1120,9 +1158,9 @@ int lh_cfl_dt(lh_handle* h, double cfl, double* dt) {
   if (use_xw(h)) {
     launch_xw(h, 1);
   } else if (h->prec == LH_PREC_FP64) {
-    LH_DISPATCH_LEN(dm.px, launch_xraw_t, double, h, a, h->twx)
+    LH_DISPATCH_PLAN(dm.px, h->xplan, launch_xraw_t, double, h, a, h->twx)
   } else {
-    LH_DISPATCH_LEN(dm.px, launch_xraw_t, float, h, a, h->twx)
+    LH_DISPATCH_PLAN(dm.px, h->xplan, launch_xraw_t, float, h, a, h->twx)
   }
   hipLaunchKernelGGL(k_cfl_dt, dim3(1), dim3(1), 0, h->st, h->scal);
   HIPCHK(h, hipGetLastError());
@@ -1248,9 +1286,30 @@ int lh_get_launch(lh_handle* h, int* xraw_lc, int* zasm_lc, int* col_bs) {
   return LH_OK;
 }
 
-// Coordinate search over the launch parameters: every candidate runs `steps` CFL steps
-// (after one warm-up step) timed with events on the handle's stream; the state, the
-// nonlinear terms and the device scalars are saved first and restored afterwards.
+int lh_set_plans(lh_handle* h, int xplan, int zplan) {
+  if (!h) return LH_ERR_ARG;
+  if (xplan < 0 || xplan >= plans_of_len(h->dm.px) || zplan < 0 || zplan >= plans_of_len(h->dm.pz))
+    return fail(h, LH_ERR_ARG, "lh_set_plans: plan out of range for this length (lh_get_plans gives the counts)");
+  HIPCHK(h, hipStreamSynchronize(h->st));
+  h->xplan = xplan;
+  h->zplan = zplan;
+  drop_graphs(h);
+  return LH_OK;
+}
+
+int lh_get_plans(lh_handle* h, int* xplan, int* zplan, int* xcount, int* zcount) {
+  if (!h) return LH_ERR_ARG;
+  if (xplan) *xplan = h->xplan;
+  if (zplan) *zplan = h->zplan;
+  if (xcount) *xcount = plans_of_len(h->dm.px);
+  if (zcount) *zcount = plans_of_len(h->dm.pz);
+  return LH_OK;
+}
+
+// Search over the FFT plans (every x/z pair), then coordinate search over the launch
+// parameters: every candidate runs `steps` CFL steps (after one warm-up step) timed with
+// events on the handle's stream; the state, the nonlinear terms and the device scalars
+// are saved first and restored afterwards.
 int lh_autotune(lh_handle* h, int steps) {
   if (!h) return LH_ERR_ARG;
   if (steps < 1) steps = 2;
@@ -1285,13 +1344,29 @@ int lh_autotune(lh_handle* h, int steps) {
     HIPCHK(h, hipEventElapsedTime(ms, e0, e1));
     return LH_OK;
   };
+  int r = LH_OK;
+  // every pair of FFT plans (at most 2 x 2), the current pair first
+  {
+    const int nxp = plans_of_len(dm.px), nzp = plans_of_len(dm.pz);
+    const int x00 = h->xplan, z00 = h->zplan;
+    int bx = x00, bz = z00;
+    float best_ms = 1e30f;
+    for (int c = 0; c < nxp * nzp && r == LH_OK; ++c) {
+      h->xplan = (x00 + c / nzp) % nxp;
+      h->zplan = (z00 + c % nzp) % nzp;
+      float ms = 0.f;
+      r = trial(&ms);
+      if (ms < best_ms) { best_ms = ms; bx = h->xplan; bz = h->zplan; }
+    }
+    h->xplan = bx;
+    h->zplan = bz;
+  }
   // candidates of each knob, the current value first (the x pass default comes from the
   // dispatch-round model, its neighbours bracket it)
   const int x0 = h->xraw_lc, z0 = h->zasm_lc;
   const std::vector<int> cands[3] = {{h->col_bs, 64, 128, 256},
                                      {x0, x0 / 2 > 0 ? x0 / 2 : 1, 2 * x0, 16},
                                      {z0, 1, 2, 4, 8}};
-  int r = LH_OK;
   for (int pass = 0; pass < 3 && r == LH_OK; ++pass) {
     int* knob = pass == 0 ? &h->col_bs : (pass == 1 ? &h->xraw_lc : &h->zasm_lc);
     int best = *knob;

@@ -335,9 +335,11 @@ __device__ __forceinline__ void stage_load_regs(const C* src, int b, int q, cons
 
 // Plans: threads per line T, the radix sequence, and the per-thread twiddle set Tw<C>.
 // run() works in place; run_pp() ping-pongs between two lines and leaves the result in
-// the second one when NSTAGE is odd, in the first one when it is even.
-#define LH_PLAN2(NN, TT, R1, R2)                                                        \
-  template <> struct Plan<NN> {                                                         \
+// the second one when NSTAGE is odd, in the first one when it is even.  Plan<N, 0> is the
+// default factorisation of length N; Plan<N, 1>, where a length has one, is an
+// alternative (other radices and threads per line) that the autotuner may choose.
+#define LH_PLAN2(NN, VV, TT, R1, R2)                                                    \
+  template <> struct Plan<NN, VV> {                                                     \
     static constexpr int T = TT;                                                        \
     static constexpr int RA = R1, RB = R2, RC = 1;                                      \
     template <typename C> struct Tw {                                                   \
@@ -363,8 +365,8 @@ __device__ __forceinline__ void stage_load_regs(const C* src, int b, int q, cons
       stage_pp<NN, TT, R2, R1, INV, S>(b, a, t, w.s2);                                  \
     }                                                                                   \
   };
-#define LH_PLAN3(NN, TT, R1, R2, R3)                                                    \
-  template <> struct Plan<NN> {                                                         \
+#define LH_PLAN3(NN, VV, TT, R1, R2, R3)                                                \
+  template <> struct Plan<NN, VV> {                                                     \
     static constexpr int T = TT;                                                        \
     static constexpr int RA = R1, RB = R2, RC = R3;                                     \
     template <typename C> struct Tw {                                                   \
@@ -397,17 +399,25 @@ __device__ __forceinline__ void stage_load_regs(const C* src, int b, int q, cons
     }                                                                                   \
   };
 
-template <int N> struct Plan;
+template <int N, int V = 0> struct Plan;
 
-LH_PLAN3(576, 48, 12, 12, 4)
-LH_PLAN3(384, 48, 8, 12, 4)
-LH_PLAN2(144, 12, 12, 12)
-LH_PLAN3(96, 8, 4, 4, 6)
-LH_PLAN2(72, 6, 12, 6)
-LH_PLAN2(64, 4, 16, 4)
-LH_PLAN2(48, 4, 12, 4)
-LH_PLAN2(32, 4, 8, 4)
+LH_PLAN3(576, 0, 48, 12, 12, 4)
+LH_PLAN3(384, 0, 48, 8, 12, 4)
+LH_PLAN2(144, 0, 12, 12, 12)
+LH_PLAN3(96, 0, 8, 4, 4, 6)
+LH_PLAN2(72, 0, 6, 12, 6)
+LH_PLAN2(64, 0, 4, 16, 4)
+LH_PLAN2(48, 0, 4, 12, 4)
+LH_PLAN2(32, 0, 4, 8, 4)
+// alternatives for the lengths of the small channels (nx, nz of 32..96 give 48..144)
+LH_PLAN2(144, 1, 16, 9, 16)
+LH_PLAN2(96, 1, 12, 8, 12)
+LH_PLAN2(72, 1, 9, 8, 9)
+LH_PLAN2(48, 1, 8, 6, 8)
 #undef LH_PLAN2
 #undef LH_PLAN3
+
+// number of plans of length N (1 or 2)
+template <int N> constexpr int plan_count() { return (N == 48 || N == 72 || N == 96 || N == 144) ? 2 : 1; }
 
 }  // namespace lh

@@ -83,8 +83,8 @@ __device__ __forceinline__ ZBlock z_block(int ncg, int maxn) {
 // takes its inputs straight from global memory into registers and the last stage stores
 // its outputs straight to global memory; only the middle stage (three-stage plans) reads
 // and writes the LDS line in place.
-template <int PZ> struct ZShape {
-  using P = Plan<PZ>;
+template <int PZ, int V = 0> struct ZShape {
+  using P = Plan<PZ, V>;
   static constexpr int T = P::T, R1 = P::RA, R2 = P::RB, R3 = P::RC;
   static constexpr bool THREE = R3 > 1;
   static constexpr int RL = THREE ? R3 : R2, NSL = PZ / RL;   // last stage
@@ -94,9 +94,9 @@ template <int PZ> struct ZShape {
 };
 
 // middle stage of a three-stage plan, in place on the line
-template <int PZ, bool INV, int PS, typename C, typename W>
+template <int PZ, int V, bool INV, int PS, typename C, typename W>
 __device__ __forceinline__ void z_middle(C* line, int t, const W& w) {
-  using Z = ZShape<PZ>;
+  using Z = ZShape<PZ, V>;
   C v[Z::PER2][Z::R2];
 #pragma unroll
   for (int q = 0; q < Z::PER2; ++q) {
@@ -116,19 +116,19 @@ __device__ __forceinline__ void z_middle(C* line, int t, const W& w) {
   lds_bar();
 }
 // last stage: butterfly q of this thread, outputs at positions b + r * NSL
-template <int PZ, bool INV, int PS, typename C, typename W>
-__device__ __forceinline__ void z_last(const C* line, int b, int q, const W& w, C (&v)[ZShape<PZ>::RL]) {
-  using Z = ZShape<PZ>;
+template <int PZ, int V, bool INV, int PS, typename C, typename W>
+__device__ __forceinline__ void z_last(const C* line, int b, int q, const W& w, C (&v)[ZShape<PZ, V>::RL]) {
+  using Z = ZShape<PZ, V>;
   if constexpr (Z::THREE) stage_load_regs<PZ, Z::RL, Z::R1 * Z::R2, INV, PS>(line, b, q, w.s3, v);
   else stage_load_regs<PZ, Z::RL, Z::R1, INV, PS>(line, b, q, w.s2, v);
 }
 
 // inverse: spectral plane (nz x mx), zero-padded to pz in k, inverse cft along z
-template <int PZ, int L, typename TR>
-__global__ void __launch_bounds__(L* Plan<PZ>::T) __attribute__((amdgpu_waves_per_eu(LH_Z_WPE)))
+template <int PZ, int L, typename TR, int V = 0>
+__global__ void __launch_bounds__((L * Plan<PZ, V>::T)) __attribute__((amdgpu_waves_per_eu(LH_Z_WPE)))
     k_zinv(ZJobs jobs, Dims dm, const cx<TR>* __restrict__ tw) {
   using C = cx<TR>;
-  using Z = ZShape<PZ>;
+  using Z = ZShape<PZ, V>;
   constexpr int LS = zls<PZ>();
   constexpr int PS = LH_Z_PAD;
   extern __shared__ char smem_raw[];
@@ -144,7 +144,7 @@ __global__ void __launch_bounds__(L* Plan<PZ>::T) __attribute__((amdgpu_waves_pe
   const bool ok = i < mx;
   const double2* src = jb.src + (size_t)plane * dm.ncol + i;
   C* line = lds + l * LS;
-  typename Plan<PZ>::template Tw<C> w;
+  typename Plan<PZ, V>::template Tw<C> w;
   w.load(t, tw);
   // position n of the padded line holds mode n (n < h), the zero padding, or mode n-(pz-nz)
 #pragma unroll
@@ -166,14 +166,14 @@ __global__ void __launch_bounds__(L* Plan<PZ>::T) __attribute__((amdgpu_waves_pe
     }
   }
   lds_bar();
-  if constexpr (Z::THREE) z_middle<PZ, true, PS>(line, t, w);
+  if constexpr (Z::THREE) z_middle<PZ, V, true, PS>(line, t, w);
   C* dst = reinterpret_cast<C*>(jb.dst) + (size_t)(jb.slot0 + p) * PZ * mx + i;
 #pragma unroll
   for (int q = 0; q < Z::PERL; ++q) {
     const int b = t + q * Z::T;
     if (Z::PERL * Z::T == Z::NSL || b < Z::NSL) {
       C v[Z::RL];
-      z_last<PZ, true, PS>(line, b, q, w, v);
+      z_last<PZ, V, true, PS>(line, b, q, w, v);
       if (ok) {
 #pragma unroll
         for (int r = 0; r < Z::RL; ++r) dst[(size_t)(b + r * Z::NSL) * mx] = v[r];
@@ -183,11 +183,11 @@ __global__ void __launch_bounds__(L* Plan<PZ>::T) __attribute__((amdgpu_waves_pe
 }
 
 // forward: Z slot -> forward cft along z, x 1/pz, truncation to nz modes -> spectral FP64
-template <int PZ, int L, typename TR>
-__global__ void __launch_bounds__(L* Plan<PZ>::T) __attribute__((amdgpu_waves_per_eu(LH_Z_WPE)))
+template <int PZ, int L, typename TR, int V = 0>
+__global__ void __launch_bounds__((L * Plan<PZ, V>::T)) __attribute__((amdgpu_waves_per_eu(LH_Z_WPE)))
     k_zfwd(ZJobs jobs, Dims dm, const cx<TR>* __restrict__ tw) {
   using C = cx<TR>;
-  using Z = ZShape<PZ>;
+  using Z = ZShape<PZ, V>;
   constexpr int LS = zls<PZ>();
   constexpr int PS = LH_Z_PAD;
   extern __shared__ char smem_raw[];
@@ -203,7 +203,7 @@ __global__ void __launch_bounds__(L* Plan<PZ>::T) __attribute__((amdgpu_waves_pe
   const bool ok = i < mx;
   const C* src = reinterpret_cast<const C*>(jb.src) + (size_t)(jb.slot0 + p) * PZ * mx + i;
   C* line = lds + l * LS;
-  typename Plan<PZ>::template Tw<C> w;
+  typename Plan<PZ, V>::template Tw<C> w;
   w.load(t, tw);
 #pragma unroll
   for (int q = 0; q < Z::PER1; ++q) {
@@ -216,7 +216,7 @@ __global__ void __launch_bounds__(L* Plan<PZ>::T) __attribute__((amdgpu_waves_pe
     }
   }
   lds_bar();
-  if constexpr (Z::THREE) z_middle<PZ, false, PS>(line, t, w);
+  if constexpr (Z::THREE) z_middle<PZ, V, false, PS>(line, t, w);
   double2* dst = reinterpret_cast<double2*>(jb.dst) + (size_t)plane * dm.ncol + i;
   const double sz = 1.0 / (double)PZ;
 #pragma unroll
@@ -224,7 +224,7 @@ __global__ void __launch_bounds__(L* Plan<PZ>::T) __attribute__((amdgpu_waves_pe
     const int b = t + q * Z::T;
     if (Z::PERL * Z::T == Z::NSL || b < Z::NSL) {
       C v[Z::RL];
-      z_last<PZ, false, PS>(line, b, q, w, v);
+      z_last<PZ, V, false, PS>(line, b, q, w, v);
       if (ok) {
 #pragma unroll
         for (int r = 0; r < Z::RL; ++r) {
@@ -302,11 +302,12 @@ struct XIB {
                                 // 1: other entries (two row pairs z0, z0+2 per block)
 };
 
-template <int PX, typename TR>
-__global__ void __launch_bounds__(2 * Plan<PX>::T)
+template <int PX, typename TR, int V = 0>
+__global__ void __launch_bounds__((2 * Plan<PX, V>::T))
     k_xpass_ib(XIB a, Dims dm, const cx<TR>* __restrict__ tw) {
   using C = cx<TR>;
-  constexpr int T = Plan<PX>::T;
+  using P = Plan<PX, V>;
+  constexpr int T = P::T;
   constexpr int LS = PX + 1;
   extern __shared__ char smem_raw[];
   C* lds = reinterpret_cast<C*>(smem_raw);
@@ -326,9 +327,9 @@ __global__ void __launch_bounds__(2 * Plan<PX>::T)
   }
   __syncthreads();
   const int ln = threadIdx.x / T, t = threadIdx.x % T;
-  typename Plan<PX>::template Tw<C> w;
+  typename P::template Tw<C> w;
   w.load(t, tw);
-  Plan<PX>::template run<true>(lds + ln * LS, t, w);
+  P::template run<true>(lds + ln * LS, t, w);
   const double beta = a.bre * a.scal[S_DT];
   const unsigned char* m0 = a.masks + ((size_t)en.mask * pz + z0) * dm.px;
   if (paired) {
@@ -353,7 +354,7 @@ __global__ void __launch_bounds__(2 * Plan<PX>::T)
     }
   }
   __syncthreads();
-  Plan<PX>::template run<false>(lds + ln * LS, t, w);
+  P::template run<false>(lds + ln * LS, t, w);
   C* dst = z + en.slot * slot + (size_t)z0 * mx;
   const int nout = paired ? 2 : 1;
   for (int e = threadIdx.x; e < nout * mx; e += blockDim.x) {

@@ -65,12 +65,14 @@ struct XRAW {
 #ifndef LH_XRAW2_WPE32
 #define LH_XRAW2_WPE32 2
 #endif
-template <int PX, typename TR>
-__global__ void __launch_bounds__(6 * Plan<PX>::T)
+template <int PX, typename TR, int V = 0>
+__global__ void __launch_bounds__((6 * Plan<PX, V>::T))
     __attribute__((amdgpu_waves_per_eu(sizeof(TR) == 4 ? LH_XRAW2_WPE32 : LH_XRAW2_WPE)))
     k_xraw2(XRAW a, Dims dm, const cx<TR>* __restrict__ tw, int lc) {
   using C = cx<TR>;
-  constexpr int T = Plan<PX>::T;
+  using P = Plan<PX, V>;
+  using ZS = ZShape<PX, V>;
+  constexpr int T = P::T;
   constexpr int PS = LH_XRAW_PAD;                  // line padding (one slot per 2^PS)
   constexpr int LS = PadLen<PX, PS>::value + LH_XRAW_LSP;
   constexpr int NT = 6 * T;
@@ -97,13 +99,13 @@ __global__ void __launch_bounds__(6 * Plan<PX>::T)
   C* const keep = lds + 12 * LS;
   // An inverse transform of line f starts in in_line(f) and ends in out_line(f): lines 3..5
   // (the upper level of a pair) end in `keep` whatever the parity of the stage count.
-  constexpr bool ODD = Plan<PX>::NSTAGE % 2 == 1;
+  constexpr bool ODD = P::NSTAGE % 2 == 1;
   auto in_line = [&](int f) { return ODD || f < 3 ? b0 + f * LS : keep + (f - 3) * LS; };
   auto other_line = [&](int f) { return !ODD || f < 3 ? b1 + f * LS : keep + (f - 3) * LS; };
   C* const bres = ODD ? b1 : b0;               // where the transforms of lines 0..2 end
   C* const mine_a = in_line(ln);
   C* const mine_b = other_line(ln);
-  typename Plan<PX>::template TwL<C> w;
+  typename P::template TwL<C> w;
   w.init(lds + 15 * LS, t, tw, ln == 0);
 
   // packed rows (z0, z0+1) of n Z slots -> line slots 0..n-1
@@ -178,19 +180,19 @@ __global__ void __launch_bounds__(6 * Plan<PX>::T)
     // all stages but the last ping-pong between pa and pb; the last one leaves P in registers
     C* const ha = pa + ln * LS;
     C* const hb = pb + ln * LS;
-    stage_pp<PX, T, ZShape<PX>::R1, 1, false, PS>(ha, hb, t, NoTw{});
-    if constexpr (ZShape<PX>::THREE) stage_pp<PX, T, ZShape<PX>::R2, ZShape<PX>::R1, false, PS>(hb, ha, t, w.s2);
-    const C* const hl = ZShape<PX>::THREE ? ha : hb;
+    stage_pp<PX, T, ZS::R1, 1, false, PS>(ha, hb, t, NoTw{});
+    if constexpr (ZS::THREE) stage_pp<PX, T, ZS::R2, ZS::R1, false, PS>(hb, ha, t, w.s2);
+    const C* const hl = ZS::THREE ? ha : hb;
     C* dst = zp + ((size_t)ln * nlev + l) * slot + (size_t)z0 * mx;
 #pragma unroll
-    for (int q = 0; q < ZShape<PX>::PERL; ++q) {
+    for (int q = 0; q < ZS::PERL; ++q) {
       const int b = t + q * T;
-      if (ZShape<PX>::PERL * T == ZShape<PX>::NSL || b < ZShape<PX>::NSL) {
-        C v[ZShape<PX>::RL];
-        z_last<PX, false, PS>(hl, b, q, w, v);
+      if (ZS::PERL * T == ZS::NSL || b < ZS::NSL) {
+        C v[ZS::RL];
+        z_last<PX, V, false, PS>(hl, b, q, w, v);
 #pragma unroll
-        for (int r = 0; r < ZShape<PX>::RL; ++r) {
-          const int k = b + r * ZShape<PX>::NSL;
+        for (int r = 0; r < ZS::RL; ++r) {
+          const int k = b + r * ZS::NSL;
           if (k < mx) dst[k] = v[r];
           else if (k > PX - mx) dst[mx + (PX - k)] = v[r];
         }
@@ -204,7 +206,7 @@ __global__ void __launch_bounds__(6 * Plan<PX>::T)
     const int su = l0 + 1;
     const C* s[3] = {zu + su * slot, zw + su * slot, zv + (su - 1) * slot};
     load_lines(s, 3);
-    Plan<PX>::template run_pp<true, PS>(mine_a, mine_b, t, w);
+    P::template run_pp<true, PS>(mine_a, mine_b, t, w);
     read_lines(bres, c0);
     if constexpr (!ODD) lds_bar();   // odd: read b1 while the next writes go to b0
   }
@@ -214,7 +216,7 @@ __global__ void __launch_bounds__(6 * Plan<PX>::T)
     const C* s[6] = {zu + (su + 1) * slot, zw + (su + 1) * slot, zv + su * slot,
                      zu + (su + 2) * slot, zw + (su + 2) * slot, zv + (su + 1) * slot};
     load_lines(s, 6);
-    Plan<PX>::template run_pp<true, PS>(mine_a, mine_b, t, w);
+    P::template run_pp<true, PS>(mine_a, mine_b, t, w);
     read_lines(bres, n1);
     if constexpr (!ODD) lds_bar();
     level_out(l, c0, n1, b0, b1);
@@ -227,7 +229,7 @@ __global__ void __launch_bounds__(6 * Plan<PX>::T)
     const int su = l + 1;
     const C* s[3] = {zu + (su + 1) * slot, zw + (su + 1) * slot, zv + su * slot};
     load_lines(s, 3);
-    Plan<PX>::template run_pp<true, PS>(mine_a, mine_b, t, w);
+    P::template run_pp<true, PS>(mine_a, mine_b, t, w);
     read_lines(bres, n1);
     if constexpr (!ODD) lds_bar();
     level_out(l, c0, n1, b0, b1);
@@ -263,18 +265,19 @@ struct ZASM {
 };
 
 // twiddle table of the z assembly's transforms
-template <int PZ, typename C>
-using ZasmTw = TwRowsPlan<PZ, Plan<PZ>::T, Plan<PZ>::RA, Plan<PZ>::RB, Plan<PZ>::RC, C>;
+template <int PZ, typename C, int V = 0>
+using ZasmTw = TwRowsPlan<PZ, Plan<PZ, V>::T, Plan<PZ, V>::RA, Plan<PZ, V>::RB, Plan<PZ, V>::RC, C>;
 
 // items per load batch of the z assembly (more loads in flight against more registers)
 #ifndef LH_ZASM_ZB
 #define LH_ZASM_ZB 1
 #endif
-template <int PZ, int L, typename TR>
-__global__ void __launch_bounds__(2 * L * Plan<PZ>::T) __attribute__((amdgpu_waves_per_eu(3)))
+template <int PZ, int L, typename TR, int V = 0>
+__global__ void __launch_bounds__((2 * L * Plan<PZ, V>::T)) __attribute__((amdgpu_waves_per_eu(3)))
     k_zasm(ZASM a, Dims dm, const cx<TR>* __restrict__ tw) {
   using C = cx<TR>;
-  constexpr int T = Plan<PZ>::T;
+  using P = Plan<PZ, V>;
+  constexpr int T = P::T;
   constexpr int NT = 2 * L * T;
   constexpr int LS = PZ + 1;
   constexpr int NPR = PZ / 2;                          // row pairs
@@ -322,7 +325,7 @@ __global__ void __launch_bounds__(2 * L * Plan<PZ>::T) __attribute__((amdgpu_wav
   // twiddles: one LDS row per distinct twiddle set behind the 2L lines, shared by every
   // line (registers hold the carried rows); read after the barrier that precedes the
   // transform
-  using TWP = ZasmTw<PZ, C>;
+  using TWP = ZasmTw<PZ, C, V>;
   TWP w;
   C* const twt = lds + 2 * L * LS;
   TWP::fill(twt, tw, tid, NT);
@@ -435,7 +438,7 @@ __global__ void __launch_bounds__(2 * L * Plan<PZ>::T) __attribute__((amdgpu_wav
       }
     }
     __syncthreads();
-    Plan<PZ>::template run<false>(lds + ln * LS, t, w);
+    P::template run<false>(lds + ln * LS, t, w);
     // Nu = (F(A) + ikz F(B)) / pz on the retained modes
     const int s = l + 1;                      // u slot of centre j / v slot of face j
     double kzn[NW];

@@ -186,9 +186,21 @@ class Channel:
                   "lh_get_launch")
         return dict(xraw_lc=a.value, zasm_lc=b.value, col_bs=c.value)
 
+    def set_plans(self, xplan: int, zplan: int):
+        """FFT plan of the x and of the z transforms (0 = the default radices; 1 = the
+        alternative that lengths 48, 72, 96 and 144 have)."""
+        self._chk(self._lib.lh_set_plans(self._h, int(xplan), int(zplan)), "lh_set_plans")
+
+    def get_plans(self):
+        v = [ctypes.c_int() for _ in range(4)]
+        self._chk(self._lib.lh_get_plans(self._h, *(ctypes.byref(x) for x in v)), "lh_get_plans")
+        return dict(xplan=v[0].value, zplan=v[1].value, xcount=v[2].value, zcount=v[3].value)
+
     def autotune(self, steps: int = 2):
+        """Times every pair of FFT plans, then the launch parameters; returns the choice."""
         self._chk(self._lib.lh_autotune(self._h, int(steps)), "lh_autotune")
-        return self.get_launch()
+        p = self.get_plans()
+        return dict(self.get_launch(), xplan=p["xplan"], zplan=p["zplan"])
 
     # ---- per-kernel timing (HIP events on the handle's stream)
     def timing(self, enable):
