@@ -110,13 +110,14 @@ def test_constant_flow_rate_tiny(require_gpu):
     assert abs(gm - om) < bound
 
 
-def test_one_substep_production(require_gpu):
-    """The 384 x 289 x 384 channel (Ngal 576, 48 x 24 posts): one substep of each kind."""
+def test_one_step_production(require_gpu):
+    """The 384 x 289 x 384 channel (Ngal 576, 48 x 24 posts): one full RK3 step, each of its
+    three substeps compared."""
     from littleharsh_amd.channel import config_production
     s = config_production()
     g, o, _ = make_pair(s, seed=5)
     dt = 1e-4
-    for k in (1, 2):
+    for k in (1, 2, 3):
         g.substep(k, dt)
         o.substep(k, dt)
         errs = report(f"production substep{k}", g.get_state(), o.get_state())
@@ -124,14 +125,37 @@ def test_one_substep_production(require_gpu):
 
 
 def test_200_steps_small(require_gpu):
-    """64 x 97 x 64 canopy channel, 200 RK3 steps replayed from the HIP graph."""
+    """64 x 97 x 64 canopy channel, 200 RK3 steps replayed from the HIP graph, with the
+    statistics of stats.f90:1-108 (plain and canopy-conditioned) sampled every 10 steps on
+    both sides and compared after the last step."""
+    from oracle.oracle import split_cond
     s = small()
     g, o, _ = make_pair(s, seed=11)
     dt = 2e-4
     for n in range(200):
+        if n % 10 == 0:
+            g.stats_accumulate()
+            o.stats_accumulate()
         g.step(dt)
         o.step(dt)
         if (n + 1) % 50 == 0:
             errs = report(f"small step {n + 1}", g.get_state(), o.get_state())
             assert max(errs.values()) < TOL
     assert abs(g.scalars()["qx"] - o.flowrate()) < 1e-10 * abs(o.flowrate())
+    gs, gn = g.stats()
+    os_, on = o.stats()
+    assert gn == on == 20
+    npts = s.px * s.pz
+    worst = 0.0
+    for q in range(13):
+        # sums of gn samples over npts points: the error bar is the sum of magnitudes
+        scale = max(np.abs(os_[q]).max(), gn * npts * 1e-3)
+        worst = max(worst, np.abs(gs[q] - os_[q]).max() / scale)
+    gc = split_cond(s, g.cond_stats_flat())
+    oc = o.cond_stats()
+    for q in range(13):
+        scale = max(np.abs(oc[q]).max(), gn * 1e-3 * s.ntilex * s.ntilez)
+        worst = max(worst, np.abs(gc[q] - oc[q]).max() / scale)
+    print(f"small statistics after 200 steps, 20 samples: worst row error {worst:.3e}",
+          flush=True)
+    assert worst < TOL
