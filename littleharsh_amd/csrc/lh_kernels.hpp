@@ -246,20 +246,6 @@ __global__ void __launch_bounds__((L * Plan<PZ, V>::T)) __attribute__((amdgpu_wa
 // x pass helpers: rows z0, z0+1 of a Z slot packed into one complex line
 //   line = A + iB (A, B Hermitian half spectra) -> inverse -> Re = row z0, Im = row z0+1
 // ------------------------------------------------------------------------------------
-template <int PX, typename C>
-__device__ __forceinline__ C pack_rows(const C* __restrict__ A, const C* __restrict__ B, int n, int mx) {
-  if (n == 0) return mk<C>(A[0].x, B[0].x);
-  if (n < mx) {
-    const C a = A[n], b = B[n];
-    return mk<C>(a.x - b.y, a.y + b.x);
-  }
-  if (n > PX - mx) {
-    const int m = PX - n;
-    const C a = A[m], b = B[m];
-    return mk<C>(a.x + b.y, -a.y + b.x);
-  }
-  return mk<C>(0, 0);
-}
 
 // forward unpack: Z_k of a+ib -> half spectra of a and b, x 1/px (rft_buff.f:63-73)
 template <int PX, typename C>
@@ -321,36 +307,74 @@ __global__ void __launch_bounds__((2 * Plan<PX, V>::T))
   // rows (z0+2, z0+3)
   const C* A = z + en.slot * slot + (size_t)z0 * mx;
   const C* B = paired ? A + 2 * mx : z + en.lslot * slot + (size_t)z0 * mx;
-  for (int e = threadIdx.x; e < 2 * PX; e += blockDim.x) {
-    const int ln = e / PX, n = e % PX;
-    lds[ln * LS + n] = pack_rows<PX>(ln ? B : A, (ln ? B : A) + mx, n, mx);
+  const unsigned char* m0 = a.masks + ((size_t)en.mask * pz + z0) * dm.px;
+  // Every load of the block is issued before any is used, so the block waits for memory
+  // once: the canopy mask bytes of its rows as 32-bit words (px is a multiple of 4), and
+  // the half spectra of the two lines, item (line, k) for k < mx.  The host holds
+  // px = 3 nx / 2, so mx = px / 3 + 1 and 2 mx items cover both lines.
+  constexpr int NT = 2 * T;
+  constexpr int WR = PX / 4;                            // mask words per row
+  constexpr int MW = (4 * WR + NT - 1) / NT;            // mask words per thread (4 rows)
+  constexpr int NI = 2 * (PX / 3 + 1);                  // items of the two lines
+  constexpr int MH = (NI + NT - 1) / NT;                // items per thread
+  const int nw = (paired ? 4 : 2) * WR;
+  const unsigned* mwp = reinterpret_cast<const unsigned*>(m0);
+  unsigned mw[MW];
+  {
+    C va[MH], vb[MH];
+#pragma unroll
+    for (int q = 0; q < MW; ++q) mw[q] = mwp[min((int)threadIdx.x + q * NT, nw - 1)];
+#pragma unroll
+    for (int q = 0; q < MH; ++q) {
+      const int it = min((int)threadIdx.x + q * NT, 2 * mx - 1);
+      const int ln = it / mx, k = it % mx;
+      const C* R = ln ? B : A;
+      va[q] = R[k];
+      vb[q] = R[mx + k];
+    }
+    // position k holds a + ib, position px - k its mirror (rft_buff.f packing of two rows)
+#pragma unroll
+    for (int q = 0; q < MH; ++q) {
+      const int it = threadIdx.x + q * NT;
+      if (it < 2 * mx) {
+        const int ln = it / mx, k = it % mx;
+        const C x = va[q], y = vb[q];
+        if (k == 0) {
+          lds[ln * LS] = mk<C>(x.x, y.x);
+        } else {
+          lds[ln * LS + k] = mk<C>(x.x - y.y, x.y + y.x);
+          lds[ln * LS + PX - k] = mk<C>(x.x + y.y, -x.y + y.x);
+        }
+      }
+    }
+    // zero between the halves: positions mx .. px - mx of both lines
+    const int nzr = PX - 2 * mx + 1;
+    for (int e = threadIdx.x; e < 2 * nzr; e += NT) lds[(e / nzr) * LS + mx + e % nzr] = mk<C>(0, 0);
   }
-  __syncthreads();
   const int ln = threadIdx.x / T, t = threadIdx.x % T;
   typename P::template Tw<C> w;
-  w.load(t, tw);
+  w.load(t, tw);                 // in flight across the barrier
+  __syncthreads();
   P::template run<true>(lds + ln * LS, t, w);
   const double beta = a.bre * a.scal[S_DT];
-  const unsigned char* m0 = a.masks + ((size_t)en.mask * pz + z0) * dm.px;
-  if (paired) {
-    // rhs = 0 on the canopy points of the four rows
-    for (int e = threadIdx.x; e < 2 * PX; e += blockDim.x) {
-      const int l2 = e / PX, x = e % PX;
-      const unsigned char* m = m0 + (size_t)(2 * l2) * dm.px;
-      C v = lds[l2 * LS + x];
-      if (m[x]) v.x = 0;
-      if (m[dm.px + x]) v.y = 0;
-      lds[l2 * LS + x] = v;
-    }
-  } else {
-    // tip plane: rhs = -beta Lu_y on the canopy points
-    const unsigned char* m1 = m0 + dm.px;
-    for (int x = threadIdx.x; x < PX; x += blockDim.x) {
-      C v = lds[x];
-      const C l = lds[LS + x];
-      if (m0[x]) v.x = (TR)(-beta * (double)l.x);
-      if (m1[x]) v.y = (TR)(-beta * (double)l.y);
-      lds[x] = v;
+  // mask word q covers bytes x0 .. x0 + 3 of block row r; row r is component r % 2 of
+  // line r / 2 (paired: rhs = 0 there), or, on a tip plane (rows z0, z0 + 1 of line 0),
+  // rhs = -beta Lu_y with Lu_y the same component of line 1.  Each thread writes single
+  // components, so no two threads touch the same value.
+  TR* ldsr = reinterpret_cast<TR*>(lds);
+#pragma unroll
+  for (int q = 0; q < MW; ++q) {
+    const int wd = threadIdx.x + q * NT;
+    if (wd < nw && mw[q] != 0u) {
+      const int r = wd / WR, x0 = 4 * (wd % WR);
+      const int c = r & 1, l0 = paired ? r >> 1 : 0;
+#pragma unroll
+      for (int b = 0; b < 4; ++b) {
+        if ((mw[q] >> (8 * b)) & 0xFFu) {
+          const int o = 2 * (l0 * LS + x0 + b) + c;
+          ldsr[o] = paired ? (TR)0 : (TR)(-beta * (double)ldsr[o + 2 * LS]);
+        }
+      }
     }
   }
   __syncthreads();

@@ -81,7 +81,8 @@ __global__ void __launch_bounds__(3 * Plan<PX>::T)
   C* lds = reinterpret_cast<C*>(smem_raw);
   double* fold = reinterpret_cast<double*>(lds + 3 * LS);   // [6][dnx]
   double* foldp = fold + 6 * a.dnx;                          // [part][6][dnx], NT / dnx parts
-  __shared__ double red[6][NT];
+  constexpr int NW = (NT + 63) / 64;
+  __shared__ double red[6][NW];
   const int kk = blockIdx.x, p = blockIdx.y;
   const int mx = dm.mx, dnx = a.dnx;
   const size_t slot = (size_t)dm.pz * mx;
@@ -163,14 +164,27 @@ __global__ void __launch_bounds__(3 * Plan<PX>::T)
     }
     lds_bar();
   }
-  // block reduction of the plain sums, in thread order
+  // block reduction of the plain sums: a shuffle tree inside each wavefront (lane 0 ends
+  // with the sum of the wave's active lanes), then the waves' sums in wave order; the order
+  // is fixed, so the result does not depend on scheduling
+  {
+    const int wv = threadIdx.x / 64, lane = threadIdx.x % 64;
+    const int act = min(64, NT - wv * 64);
 #pragma unroll
-  for (int l = 0; l < 6; ++l) red[l][threadIdx.x] = s[l];
+    for (int l = 0; l < 6; ++l) {
+      double v = s[l];
+      for (int off = 32; off > 0; off >>= 1) {
+        const double o = __shfl_down(v, off, 64);
+        if (lane + off < act) v += o;
+      }
+      if (lane == 0) red[l][wv] = v;
+    }
+  }
   __syncthreads();
   const size_t pk = (size_t)p * a.dnz + kk;
   if (threadIdx.x < 6) {
     double v = 0.0;
-    for (int q = 0; q < NT; ++q) v += red[threadIdx.x][q];
+    for (int q = 0; q < NW; ++q) v += red[threadIdx.x][q];
     a.psum[pk * 6 + threadIdx.x] = v;
   }
   for (int e = threadIdx.x; e < 6 * dnx; e += NT) a.pfold[pk * 6 * dnx + e] = fold[e];
