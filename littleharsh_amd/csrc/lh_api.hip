@@ -388,9 +388,10 @@ static void launch_xib_t(lh_handle* h, const XIB& a, int nent, const void* tw) {
 template <int PX, typename TR>
 static void launch_stats_t(lh_handle* h, const STATS& a) {
   dim3 grid(a.dnz, a.n);
-  // lines, the fold [6][dnx] and the fold parts [parts][6][dnx] (parts <= 3 T / dnx, >= 1)
-  const int parts = std::max(1, std::min(3 * Plan<PX>::T / a.dnx, a.ntx));
-  size_t sh = (size_t)3 * (PX + 1) * sizeof(cx<double>) + (size_t)6 * a.dnx * (1 + parts) * sizeof(double);
+  // lines, twiddle rows, the fold [6][dnx] and the fold parts [parts][6][dnx]
+  const size_t sh = stats_lds_bytes<PX>(a.dnx, a.ntx);
+  // more than 64 KB when a tile is wide (dnx = px without a canopy lattice)
+  (void)hipFuncSetAttribute((const void*)k_stats<PX, TR>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh);
   hipLaunchKernelGGL((k_stats<PX, TR>), grid, dim3(3 * Plan<PX>::T), sh, h->st, a, h->dm,
                      (const cx<double>*)h->twx64);
 }

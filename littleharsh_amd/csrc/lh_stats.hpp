@@ -56,119 +56,157 @@ struct STATS {
   double* pfold;      // [n][dnz][6][dnx]
 };
 
-// local quantities of one point; returns how many
-__device__ __forceinline__ int stats_q(int type, double a, double b, double c, double* q) {
+// local quantities of one point: q[0..5], zero beyond the type's count (6, 5 or 2), so
+// that every loop over them has a fixed length and the arrays stay in registers
+__device__ __forceinline__ void stats_q(int type, double a, double b, double c, double (&q)[6]) {
+  q[0] = a;
+  q[1] = a * a;
   if (type == ST_U) {
-    q[0] = a; q[1] = a * a; q[2] = b; q[3] = b * b; q[4] = a * c; q[5] = a * b;
-    return 6;
+    q[2] = b; q[3] = b * b; q[4] = a * c; q[5] = a * b;
+  } else if (type == ST_V) {
+    q[2] = a * b; q[3] = c; q[4] = c * c; q[5] = 0.0;
+  } else {
+    q[2] = 0.0; q[3] = 0.0; q[4] = 0.0; q[5] = 0.0;
   }
-  if (type == ST_V) {
-    q[0] = a; q[1] = a * a; q[2] = a * b; q[3] = c; q[4] = c * c;
-    return 5;
-  }
-  q[0] = a; q[1] = a * a;
-  return 2;
 }
 
+// LDS lines of k_stats: one pad slot after every 2^LH_STATS_PAD values, so that the strided
+// stage writes of the transforms spread over the banks
+#ifndef LH_STATS_PAD
+#define LH_STATS_PAD 5
+#endif
+template <int PX> constexpr int stats_ls() { return PadLen<PX, LH_STATS_PAD>::value + 1; }
+// twiddles of the statistics transforms: one LDS table per block
+template <int PX>
+using StatsTw = TwRowsPlan<PX, Plan<PX>::T, Plan<PX>::RA, Plan<PX>::RB, Plan<PX>::RC, cx<double>>;
+// parts of the canopy-tile fold: each of the 3 T threads owns one (tile column, part) slot
+// when dnx <= 3 T, the parts taking every nparts-th tile
+__host__ __device__ constexpr int stats_parts(int nt, int dnx, int ntx) {
+  return nt / dnx < 1 ? 1 : (nt / dnx < ntx ? nt / dnx : ntx);
+}
+template <int PX> constexpr size_t stats_lds_bytes(int dnx, int ntx) {
+  return (size_t)(3 * stats_ls<PX>() + StatsTw<PX>::SIZE) * sizeof(cx<double>) +
+         (size_t)6 * dnx * (1 + stats_parts(3 * Plan<PX>::T, dnx, ntx)) * sizeof(double);
+}
+
+// One block per (plane, tile row class kk): the rows z = kk + dnz iz, two per pass as one
+// complex line per field.  Per pass: the half spectra of the three fields are loaded by
+// (field, k) item, all before any is used; the lines are transformed to physical space; then
+// every thread takes its fold slots (tile column i, part) and adds the quantities of the
+// points x = i + dnx ix of its part to its own partial of the tile fold (kept in LDS across
+// the passes, so no barrier guards it) and to its plain sums.  The tiles cover the line
+// (dnx ntilex = px), so the fold visits every x once.  At the end the parts are added in
+// part order and the plain sums by a shuffle tree per wavefront, then in wave order.
 template <int PX, typename TR>
 __global__ void __launch_bounds__(3 * Plan<PX>::T)
     k_stats(STATS a, Dims dm, const cx<double>* __restrict__ tw) {
   using C = cx<double>;
-  constexpr int T = Plan<PX>::T;
-  constexpr int LS = PX + 1;
+  using P = Plan<PX>;
+  using TWP = StatsTw<PX>;
+  constexpr int T = P::T;
+  constexpr int PS = LH_STATS_PAD;
+  constexpr int LS = stats_ls<PX>();
   constexpr int NT = 3 * T;
   extern __shared__ char smem_raw[];
   C* lds = reinterpret_cast<C*>(smem_raw);
-  double* fold = reinterpret_cast<double*>(lds + 3 * LS);   // [6][dnx]
-  double* foldp = fold + 6 * a.dnx;                          // [part][6][dnx], NT / dnx parts
+  C* const twt = lds + 3 * LS;
+  double* const fold = reinterpret_cast<double*>(twt + TWP::SIZE);   // [6][dnx]
+  double* const foldp = fold + 6 * a.dnx;                             // [part][6][dnx]
   constexpr int NW = (NT + 63) / 64;
   __shared__ double red[6][NW];
+  const int tid = threadIdx.x;
   const int kk = blockIdx.x, p = blockIdx.y;
   const int mx = dm.mx, dnx = a.dnx;
   const size_t slot = (size_t)dm.pz * mx;
-  for (int e = threadIdx.x; e < 6 * dnx; e += NT) fold[e] = 0.0;
+  const int nparts = stats_parts(NT, dnx, a.ntx);
+  for (int e = tid; e < nparts * 6 * dnx; e += NT) foldp[e] = 0.0;
   double s[6] = {0, 0, 0, 0, 0, 0};
-  const int ln = threadIdx.x / T, t = threadIdx.x % T;
-  typename Plan<PX>::template Tw<C> w;
-  w.load(t, tw);
+  const int ln = tid / T, t = tid % T;
+  TWP w;
+  TWP::fill(twt, tw, tid, NT);
+  w.init(twt, t);
+  const C* const zbase = reinterpret_cast<const C*>(a.z);
+  // the host holds px = 3 nx / 2, so mx = px / 3 + 1
+  constexpr int NI = 3 * (PX / 3 + 1);
+  constexpr int MI = (NI + NT - 1) / NT;
+  const int nzr = PX - 2 * mx + 1;                    // zero positions mx .. px - mx
   for (int iz = 0; iz < a.ntz; iz += 2) {
     const int zA = kk + a.dnz * iz;
     const bool hasB = iz + 1 < a.ntz;
-    const int zB = zA + a.dnz;
-    // every load issued unconditionally (clamped to a valid element and masked afterwards),
-    // so that the two rows wait for memory once
-    constexpr int NE = (3 * PX + NT - 1) / NT;
-    C ca[NE], cb[NE];
+    const int zB = hasB ? zA + a.dnz : zA;
+    {
+      C ca[MI], cb[MI];
 #pragma unroll
-    for (int j = 0; j < NE; ++j) {
-      const int e = min((int)threadIdx.x + j * NT, 3 * PX - 1);
-      const int f = min(e / PX, a.nf - 1), nn = e % PX;
-      const int m = nn < mx ? nn : (nn > PX - mx ? PX - nn : 0);
-      const C* base = reinterpret_cast<const C*>(a.z) + (size_t)(f * a.n + p) * slot;
-      ca[j] = base[(size_t)zA * mx + m];
-      cb[j] = base[(size_t)(hasB ? zB : zA) * mx + m];
-    }
-#pragma unroll
-    for (int j = 0; j < NE; ++j) {
-      const int e = threadIdx.x + j * NT;
-      if (NE * NT == 3 * PX || e < 3 * PX) {
-        const int f = e / PX, nn = e % PX;
-        const bool lo = nn < mx, hi = !lo && nn > PX - mx;
-        const C b = hasB ? cb[j] : mk<C>(0.0, 0.0);
-        C v = mk<C>(0.0, 0.0);
-        if (f < a.nf && nn == 0) v = mk<C>(ca[j].x, b.x);
-        else if (f < a.nf && lo) v = mk<C>(ca[j].x - b.y, ca[j].y + b.x);
-        else if (f < a.nf && hi) v = mk<C>(ca[j].x + b.y, -ca[j].y + b.x);
-        lds[f * LS + nn] = v;
+      for (int q = 0; q < MI; ++q) {
+        const int it = min(tid + q * NT, 3 * mx - 1);
+        const int f = min(it / mx, a.nf - 1), k = it % mx;
+        const C* base = zbase + (size_t)(f * a.n + p) * slot;
+        ca[q] = base[(size_t)zA * mx + k];
+        cb[q] = base[(size_t)zB * mx + k];
       }
+#pragma unroll
+      for (int q = 0; q < MI; ++q) {
+        const int it = tid + q * NT;
+        if (it < 3 * mx) {
+          const int f = it / mx, k = it % mx;
+          const bool on = f < a.nf;
+          const C x = on ? ca[q] : mk<C>(0.0, 0.0);
+          const C y = on && hasB ? cb[q] : mk<C>(0.0, 0.0);
+          C* line = lds + f * LS;
+          if (k == 0) {
+            line[0] = mk<C>(x.x, y.x);
+          } else {
+            line[lpad<PS>(k)] = mk<C>(x.x - y.y, x.y + y.x);
+            line[lpad<PS>(PX - k)] = mk<C>(x.x + y.y, -x.y + y.x);
+          }
+        }
+      }
+      for (int e = tid; e < 3 * nzr; e += NT) lds[(e / nzr) * LS + lpad<PS>(mx + e % nzr)] = mk<C>(0.0, 0.0);
     }
     lds_bar();
-    Plan<PX>::template run<true>(lds + ln * LS, t, w);
-    lds_bar();
-    // plain sums (wx rows: x < nx, z < nz only)
-    for (int x = threadIdx.x; x < PX; x += NT) {
-      const C f0 = lds[x], f1 = lds[LS + x], f2 = lds[2 * LS + x];
-      for (int r = 0; r < 2; ++r) {
-        if (r == 1 && !hasB) break;
-        const int zr = r ? zB : zA;
-        double q[6];
-        const int nq = stats_q(a.type, r ? f0.y : f0.x, r ? f1.y : f1.x, r ? f2.y : f2.x, q);
-        const bool wxin = x < dm.nx && zr < dm.nz;
-        for (int l = 0; l < nq; ++l)
-          if (!(a.type == ST_V && l >= 3 && !wxin)) s[l] += q[l];
-      }
-    }
-    // tile fold: tile column i = e mod dnx of part e / dnx, the parts taking every nparts-th
-    // tile; the parts' sums are added in part order
-    const int nparts = max(1, min(NT / dnx, a.ntx));
-    for (int e = threadIdx.x; e < nparts * dnx; e += NT) {
+    P::template run<true, PS>(lds + ln * LS, t, w);
+    // fold slots of this thread; plain sums of wx (type V, quantities 3 and 4) only for
+    // x < nx, z < nz
+    for (int e = tid; e < nparts * dnx; e += NT) {
       const int i = e % dnx, part = e / dnx;
       double acc[6] = {0, 0, 0, 0, 0, 0};
       for (int ix = part; ix < a.ntx; ix += nparts) {
         const int x = i + dnx * ix;
-        const C f0 = lds[x], f1 = lds[LS + x], f2 = lds[2 * LS + x];
-        for (int r = 0; r < 2; ++r) {
-          if (r == 1 && !hasB) break;
-          double q[6];
-          const int nq = stats_q(a.type, r ? f0.y : f0.x, r ? f1.y : f1.x, r ? f2.y : f2.x, q);
-          for (int l = 0; l < nq; ++l) acc[l] += q[l];
+        const int xp = lpad<PS>(x);
+        const C f0 = lds[xp], f1 = lds[LS + xp], f2 = lds[2 * LS + xp];
+        double q[6];
+        stats_q(a.type, f0.x, f1.x, f2.x, q);
+        const bool wxa = x < dm.nx && zA < dm.nz, wxb = x < dm.nx && zB < dm.nz;
+#pragma unroll
+        for (int l = 0; l < 6; ++l) {
+          acc[l] += q[l];
+          if (a.type != ST_V || l < 3 || wxa) s[l] += q[l];
+        }
+        if (hasB) {
+          stats_q(a.type, f0.y, f1.y, f2.y, q);
+#pragma unroll
+          for (int l = 0; l < 6; ++l) {
+            acc[l] += q[l];
+            if (a.type != ST_V || l < 3 || wxb) s[l] += q[l];
+          }
         }
       }
-      for (int l = 0; l < 6; ++l) foldp[(part * 6 + l) * dnx + i] = acc[l];
+#pragma unroll
+      for (int l = 0; l < 6; ++l) foldp[(part * 6 + l) * dnx + i] += acc[l];
     }
     lds_bar();
-    for (int e = threadIdx.x; e < 6 * dnx; e += NT) {
-      double v = fold[e];
-      for (int part = 0; part < nparts; ++part) v += foldp[part * 6 * dnx + e];
-      fold[e] = v;
-    }
-    lds_bar();
+  }
+  // the fold: parts added in part order
+  for (int e = tid; e < 6 * dnx; e += NT) {
+    double v = 0.0;
+    for (int part = 0; part < nparts; ++part) v += foldp[part * 6 * dnx + e];
+    fold[e] = v;
   }
   // block reduction of the plain sums: a shuffle tree inside each wavefront (lane 0 ends
   // with the sum of the wave's active lanes), then the waves' sums in wave order; the order
   // is fixed, so the result does not depend on scheduling
   {
-    const int wv = threadIdx.x / 64, lane = threadIdx.x % 64;
+    const int wv = tid / 64, lane = tid % 64;
     const int act = min(64, NT - wv * 64);
 #pragma unroll
     for (int l = 0; l < 6; ++l) {
@@ -182,12 +220,12 @@ __global__ void __launch_bounds__(3 * Plan<PX>::T)
   }
   __syncthreads();
   const size_t pk = (size_t)p * a.dnz + kk;
-  if (threadIdx.x < 6) {
+  if (tid < 6) {
     double v = 0.0;
-    for (int q = 0; q < NW; ++q) v += red[threadIdx.x][q];
-    a.psum[pk * 6 + threadIdx.x] = v;
+    for (int q = 0; q < NW; ++q) v += red[tid][q];
+    a.psum[pk * 6 + tid] = v;
   }
-  for (int e = threadIdx.x; e < 6 * dnx; e += NT) a.pfold[pk * 6 * dnx + e] = fold[e];
+  for (int e = tid; e < 6 * dnx; e += NT) a.pfold[pk * 6 * dnx + e] = fold[e];
 }
 
 struct STFIN {
