@@ -1549,7 +1549,7 @@ int lh_stats_accumulate(lh_handle* h) {
     HIPCHK(h, hipMemset(h->cst_acc, 0, (size_t)g.clen * sizeof(double)));
   }
   // spectral v_itp -> r1, w_itp -> r2, wx -> r3 (the rhs buffers are free between substeps)
-  hipLaunchKernelGGL(k_stats_prep, dim3((dm.ncol + 255) / 256), dim3(256), 0, h->st, h->u2, h->u3,
+  hipLaunchKernelGGL(k_stats_prep, dim3((dm.ncol + 255) / 256, dm.nyu), dim3(256), 0, h->st, h->u2, h->u3,
                      h->dthdyv, h->k1z, h->r1, h->r2, h->r3, dm);
   static const int qmap[3][6] = {{0, 1, 4, 5, 8, 9}, {2, 3, 10, 11, 12, 0}, {6, 7, 0, 0, 0, 0}};
   static const double tsg[3][6] = {{1, 1, 1, 1, -1, 1}, {-1, 1, -1, 1, 1, 1}, {1, 1, 1, 1, 1, 1}};

@@ -55,6 +55,11 @@ struct ZJobs {
 #ifndef LH_Z_WPE
 #define LH_Z_WPE 3
 #endif
+// FP32 z transforms of the long lengths (pz >= 384): 6 waves per SIMD, two 768-thread
+// blocks per CU (as the FP32 z assembly)
+#ifndef LH_Z_WPE32
+#define LH_Z_WPE32 6
+#endif
 // LDS line stride of the z transforms (lines padded by one slot per 2^LH_Z_PAD elements);
 // LH_Z_LSP = 2 puts the 8 columns of a 16-lane group on distinct banks.
 template <int PZ> constexpr int zls() { return PadLen<PZ, LH_Z_PAD>::value + LH_Z_LSP; }
@@ -125,7 +130,8 @@ __device__ __forceinline__ void z_last(const C* line, int b, int q, const W& w, 
 
 // inverse: spectral plane (nz x mx), zero-padded to pz in k, inverse cft along z
 template <int PZ, int L, typename TR, int V = 0>
-__global__ void __launch_bounds__((L * Plan<PZ, V>::T)) __attribute__((amdgpu_waves_per_eu(LH_Z_WPE)))
+__global__ void __launch_bounds__((L * Plan<PZ, V>::T))
+    __attribute__((amdgpu_waves_per_eu(sizeof(TR) == 4 && PZ >= 384 ? LH_Z_WPE32 : LH_Z_WPE)))
     k_zinv(ZJobs jobs, Dims dm, const cx<TR>* __restrict__ tw) {
   using C = cx<TR>;
   using Z = ZShape<PZ, V>;
@@ -184,7 +190,8 @@ __global__ void __launch_bounds__((L * Plan<PZ, V>::T)) __attribute__((amdgpu_wa
 
 // forward: Z slot -> forward cft along z, x 1/pz, truncation to nz modes -> spectral FP64
 template <int PZ, int L, typename TR, int V = 0>
-__global__ void __launch_bounds__((L * Plan<PZ, V>::T)) __attribute__((amdgpu_waves_per_eu(LH_Z_WPE)))
+__global__ void __launch_bounds__((L * Plan<PZ, V>::T))
+    __attribute__((amdgpu_waves_per_eu(sizeof(TR) == 4 && PZ >= 384 ? LH_Z_WPE32 : LH_Z_WPE)))
     k_zfwd(ZJobs jobs, Dims dm, const cx<TR>* __restrict__ tw) {
   using C = cx<TR>;
   using Z = ZShape<PZ, V>;

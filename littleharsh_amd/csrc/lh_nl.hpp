@@ -272,8 +272,15 @@ using ZasmTw = TwRowsPlan<PZ, Plan<PZ, V>::T, Plan<PZ, V>::RA, Plan<PZ, V>::RB, 
 #ifndef LH_ZASM_ZB
 #define LH_ZASM_ZB 1
 #endif
+// FP32 lines of the long transforms (pz >= 384) at 6 waves per SIMD (80 VGPRs, a few
+// spilled): two 768-thread blocks then share a CU (77 KB of LDS each), so one block's
+// transforms overlap the other's loads (mixed precision at 576: 3.51 -> 3.27 ms per step)
+#ifndef LH_ZASM_WPE32
+#define LH_ZASM_WPE32 6
+#endif
 template <int PZ, int L, typename TR, int V = 0>
-__global__ void __launch_bounds__((2 * L * Plan<PZ, V>::T)) __attribute__((amdgpu_waves_per_eu(3)))
+__global__ void __launch_bounds__((2 * L * Plan<PZ, V>::T))
+    __attribute__((amdgpu_waves_per_eu(sizeof(TR) == 4 && PZ >= 384 ? LH_ZASM_WPE32 : 3)))
     k_zasm(ZASM a, Dims dm, const cx<TR>* __restrict__ tw) {
   using C = cx<TR>;
   using P = Plan<PZ, V>;

@@ -21,25 +21,25 @@ namespace lh {
 
 enum { ST_U = 0, ST_V = 1, ST_P = 2 };
 
-// v_itp (u planes) -> a, w_itp (v planes) -> b, wx (v planes) -> c
+// v_itp (u planes) -> a, w_itp (v planes) -> b, wx (v planes) -> c; one thread per
+// (column, plane s), planes on grid y
 __global__ void __launch_bounds__(256) k_stats_prep(const double2* __restrict__ u2,
                                                     const double2* __restrict__ u3,
                                                     const double* __restrict__ dthdyv,
                                                     const double* __restrict__ k1z, double2* a,
                                                     double2* b, double2* c, Dims dm) {
   const int col = blockIdx.x * blockDim.x + threadIdx.x;
+  const int s = blockIdx.y;
   if (col >= dm.ncol) return;
   const size_t n = dm.ncol;
-  const double kz = k1z[col / dm.mx];
-  for (int s = 0; s < dm.nyu; ++s) {
-    double2 vi = make_double2(0.0, 0.0);
-    if (s >= 1 && s <= dm.nyu - 2) {
-      const double2 v0 = u2[(s - 1) * n + col], v1 = u2[s * n + col];
-      vi = make_double2(0.5 * (v0.x + v1.x), 0.5 * (v0.y + v1.y));
-    }
-    a[s * n + col] = vi;
+  double2 vi = make_double2(0.0, 0.0);
+  if (s >= 1 && s <= dm.nyu - 2) {
+    const double2 v0 = u2[(s - 1) * n + col], v1 = u2[s * n + col];
+    vi = make_double2(0.5 * (v0.x + v1.x), 0.5 * (v0.y + v1.y));
   }
-  for (int s = 0; s < dm.nyv; ++s) {
+  a[s * n + col] = vi;
+  if (s < dm.nyv) {
+    const double kz = k1z[col / dm.mx];
     const double2 w0 = u3[s * n + col], w1 = u3[(s + 1) * n + col], v = u2[s * n + col];
     b[s * n + col] = make_double2(0.5 * (w0.x + w1.x), 0.5 * (w0.y + w1.y));
     const double m = dm.ddthetavi * dthdyv[s];
