@@ -188,7 +188,8 @@ int lh_spectra_read(lh_handle* h, double* out, int reset);
  * not depend on them: each only changes how work is split into blocks.
  *   xraw_lc  levels per block of the sliding x pass        (>= 1)
  *   zasm_lc  levels per block of the z assembly            (>= 1)
- *   col_bs   threads per block of the column kernels       (32..256, multiple of 32)
+ *   col_bs   threads per block of the canopy rhs kernel    (32..256, multiple of 32; the
+ *            column stage itself runs 64 columns per block)
  * lh_create sets the two level counts from the grid and the device: each block of those
  * kernels holds a CU's LDS, so the launch runs in ceil(blocks / resident slots) rounds,
  * and the count minimising rounds x (levels + start-up) is taken.
@@ -216,9 +217,11 @@ int lh_get_plans(lh_handle* h, int* xplan, int* zplan, int* xcount, int* zcount)
 int lh_autotune(lh_handle* h, int steps);
 
 /* Many independent channels (handles of any sizes on one device) advanced together: one
- * CFL step of every instance per call, issued as one captured HIP graph in which the
- * instances are parallel branches, each on its handle's stream (no counterpart in the
- * reference; the instances share nothing). */
+ * CFL step of every instance per call, issued as one captured HIP graph (no counterpart in
+ * the reference; the instances share nothing).  Per substep, the nonlinear and canopy
+ * stages of the instances are parallel branches, each on its handle's stream; the column
+ * stage (LU of the velocity matrices, velocity sweeps, pressure step) of all instances is one
+ * launch of each kernel.  A member computes exactly what it computes stepped alone. */
 /* A batch re-captures its graph when the CFL number changes or when any member's own graphs
  * were dropped (lh_set_launch, lh_set_plans, lh_autotune, lh_set_forcing switching the forcing on or off,
  * lh_closure_create); each handle may appear once.  Destroy the batch before its members. */

@@ -8,7 +8,9 @@
 //   IB stage   : k_rhs_ib (rhs and Lu_y on canopy planes) -> k_zinv -> k_xpass_ib -> k_zfwd
 //   column     : k_prep (LU of the shared velocity matrices) -> k_vel_rhs (RHS0 + rhs of
 //                solveU/V on the fly, Thomas u, w, v)
-//                -> k_pres (constant-flow-rate correction, solveP, v_corr)
+//                -> k_pres (constant-flow-rate correction, solveP, v_corr); these read the
+//                channel's parameters from a job table, so that a batch launches them once
+//                for all its members
 // Statistics and spectra (lh_stats.hpp) run on request, outside the substep.
 // lh_step replays a captured HIP graph of three substeps (+ meanflow_ctP).
 #include <hip/hip_runtime.h>
@@ -32,11 +34,14 @@
 
 using namespace lh;
 
+struct ColJob;
+
 // Default launch parameters (per handle; lh_autotune may change them):
 //   LH_XRAW_LC  levels per block of the sliding x pass (0: chosen per grid, see chunk_levels)
 //   LH_ZASM_LC  levels per block of the z assembly (0: chosen per grid)
-//   LH_COL_BS   threads per block of the column kernels (one column per thread: small blocks
-//               spread the 74k columns of the production grid evenly over the 256 CUs)
+//   LH_COL_BS   threads per block of the canopy rhs kernel (one column per thread; the column
+//               stage itself runs 64 columns per block, which spread the 74k columns of the
+//               production grid evenly over the 256 CUs)
 #ifndef LH_XRAW_LC
 #define LH_XRAW_LC 0
 #endif
@@ -98,6 +103,8 @@ struct lh_handle {
   void *zu = nullptr, *zw = nullptr, *zv = nullptr, *zp = nullptr;
   double* dg3 = nullptr;
   double *ayu = nullptr, *ayv = nullptr, *ctwork = nullptr;
+  ColJob* coljobs = nullptr;    // the column-stage jobs of the three substeps
+  int* colmap = nullptr;        // block -> channel (all 0)
   void *twx = nullptr, *twz = nullptr;         // twiddles in the NL precision
   void *twx64 = nullptr, *twz64 = nullptr;     // FP64 twiddles (IB, statistics)
   // the one-wave-per-line x pass (576 collocation points in x): its twiddle table, and the
@@ -135,9 +142,9 @@ struct lh_handle {
   // event pair around each replay of the captured step graph
   bool timing = false;
   bool timing_steps = false;
-  // batches the handle belongs to: while any, k_prep stays on the handle's stream, since the
-  // batch's graph already runs the instances side by side and a side branch per substep and
-  // instance (48 x 3 cross-stream joins per step) serialised them
+  // batches the handle belongs to: while any, the handle's own graphs keep k_prep on its
+  // stream (a batch runs the column stage of all its members itself; a side branch per
+  // substep and instance, 48 x 3 cross-stream joins per step, serialised the instances)
   int in_batch = 0;
   // bumped whenever the captured graphs are dropped (launch parameters, forcing switched on
   // or off, batch membership): a batch re-captures when a member's generation changed
@@ -467,14 +474,135 @@ __global__ void k_add_forcing(double2* n1, double2* n2, double2* n3, const doubl
   }
 }
 
-// stages of a substep: the nonlinear term, the velocity update (canopy forcing, RHS0 and
-// solveU/V, fused in the sweep), the pressure step (ctU, solveP, v_corr, fused)
-constexpr int kStageNL = 1, kStageVel = 2, kStagePres = 4, kStageAll = 7;
-static void issue_substep(lh_handle* h, int krk /*0-based*/, bool cfl_mode, int stages = kStageAll) {
+// stages of a substep: the nonlinear term, the velocity update (the canopy forcing, then
+// RHS0 and solveU/V fused in the sweep), the pressure step (ctU, solveP, v_corr, fused).
+// The nonlinear and the canopy stage are split into their launches (z, x, assembly; rhs,
+// z, x, z) so that a batch can issue each launch for all its members in turn.
+constexpr int kStageNLz = 1, kStageIBr = 2, kStagePres = 4, kStageVelCol = 8, kStageNLx = 16,
+              kStageNLa = 32, kStageIBz = 64, kStageIBx = 128, kStageIBf = 256;
+constexpr int kStageNL = kStageNLz | kStageNLx | kStageNLa;
+constexpr int kStageIB = kStageIBr | kStageIBz | kStageIBx | kStageIBf;
+constexpr int kStageVel = kStageIB | kStageVelCol, kStageAll = kStageNL | kStageVel | kStagePres;
+
+// parameters of the column stage of substep krk (0-based); the batch builds them for every
+// member and launches the column kernels of all members at once
+static void nu_buffers(const lh_handle* h, int krk, double2* nw[3], double2* ol[3]) {
+  // Nu_new into the buffer of this substep; RHS0 + rhs per column reads both
+  const bool wb = (krk % 2) == 1;
+  nw[0] = wb ? h->nb1 : h->nu1; nw[1] = wb ? h->nb2 : h->nu2; nw[2] = wb ? h->nb3 : h->nu3;
+  ol[0] = wb ? h->nu1 : h->nb1; ol[1] = wb ? h->nu2 : h->nb2; ol[2] = wb ? h->nu3 : h->nb3;
+}
+static PREP make_prep(const lh_handle* h, int krk) {
+  const Dims& dm = h->dm;
+  const double bre = h->rk[3 + krk] / dm.re;
+  return PREP{h->dyu2i, h->dyv2i, h->scal, h->ayu, h->ayv, h->gw[0], h->gw[1], bre, dm.nyu, dm.nyv};
+}
+static RHS make_rhs(const lh_handle* h, int krk) {
   const Dims& dm = h->dm;
   const double aR = h->rk[krk], bR = h->rk[3 + krk], gR = h->rk[6 + krk], cR = h->rk[9 + krk],
                dR = h->rk[12 + krk];
-  const double bre = bR / dm.re;
+  double2 *nw[3], *ol[3];
+  nu_buffers(h, krk, nw, ol);
+  RHS ra{};
+  ra.u1 = h->u1; ra.u2 = h->u2; ra.u3 = h->u3; ra.p = h->p;
+  ra.no1 = ol[0]; ra.no2 = ol[1]; ra.no3 = ol[2];
+  ra.nn1 = nw[0]; ra.nn2 = nw[1]; ra.nn3 = nw[2];
+  ra.r1 = h->r1; ra.r2 = h->r2; ra.r3 = h->r3; ra.tipl = h->tipl;
+  ra.dyu2i = h->dyu2i; ra.dyv2i = h->dyv2i; ra.dthdyv = h->dthdyv;
+  ra.k1x = h->k1x; ra.k2x = h->k2x; ra.k1z = h->k1z; ra.k2z = h->k2z; ra.scal = h->scal;
+  ra.planes_u = h->ibu_planes; ra.planes_v = h->ibv_planes;
+  ra.ib_u = h->ibflag_u; ra.ib_v = h->ibflag_v;
+  ra.np_u = h->nibu; ra.np_v = h->nibv;
+  ra.tip_u0 = h->tip_u[0] + dm.d; ra.tip_u1 = h->tip_u[1] + dm.d;
+  ra.tip_v0 = h->tip_v[0] + dm.d; ra.tip_v1 = h->tip_v[1] + dm.d;
+  ra.C1 = aR / dm.re; ra.C2 = -cR; ra.dR = dR; ra.g = gR; ra.bre = bR / dm.re;
+  return ra;
+}
+static COL make_col(const lh_handle* h, int krk) {
+  const Dims& dm = h->dm;
+  const double aR = h->rk[krk], bR = h->rk[3 + krk], dR = h->rk[12 + krk];
+  COL a{};
+  a.r1 = h->r1; a.r2 = h->r2; a.r3 = h->r3;
+  a.u1 = h->u1; a.u2 = h->u2; a.u3 = h->u3; a.p = h->p; a.psi = h->psi;
+  a.ayu = h->ayu; a.ayv = h->ayv; a.dg3 = h->dg3;
+  a.k2x = h->k2x; a.k2z = h->k2z; a.yu = h->yu;
+  a.dthdyu = h->dthdyu; a.dthdyv = h->dthdyv; a.k1x = h->k1x; a.k1z = h->k1z;
+  a.scal = h->scal; a.ab = aR + bR; a.gw0 = h->gw[0]; a.gw1 = h->gw[1];
+  a.ctu = h->flag_ctpress == 0 ? 1 : 0;
+  a.ct = CTU{h->u1, h->scal, h->ctwork, h->dyu2i, h->yv, h->gw[0], h->gw[1], bR / dm.re, dR, dm.ny,
+             dm.d, dm.ncol};
+  return a;
+}
+
+// The column stage (k_prep, the velocity sweeps, the pressure step, the flow rate) runs from
+// job tables: a job holds one channel's parameters for one substep, and block b of a launch
+// works on channel colmap[b], columns (b - blk0) * 64 + lane.  A single channel launches
+// them with its own table (one job per substep); a batch launches them once for all its
+// members.  Small channels have a few thousand spectral columns, so one channel's column
+// kernels fill a few percent of the device for the time of their sequential sweep over the
+// planes; launched together, the members' sweeps run side by side.  Both launch the same
+// kernels, so a batch member computes exactly what the channel computes alone.
+struct ColJob {
+  RHS ra;
+  COL ca;
+  PREP prep;
+  Dims dm;
+  const double* yv;
+  int vel_u;        // rows per step of the member's velocity sweep
+  int blk0;         // first block of the member in the launches of its sweep kernels
+  int flowrate;     // flag_ctpress == 1: flowrateRe after the step
+};
+constexpr int kMultiCols = 64;
+__global__ void __launch_bounds__(64) k_prep(const ColJob* __restrict__ jobs) {
+  prep_body(jobs[blockIdx.x].prep, threadIdx.x);
+}
+// one kernel per row unroll of the velocity sweep (their register needs differ: 72 VGPRs at
+// one row per step, 150 at four); a batch orders its members' blocks by unroll and launches
+// each variant over its range, blocks boff, boff + 1, ...
+template <int U>
+__global__ void __launch_bounds__(kMultiCols) k_vel_rhs(const ColJob* __restrict__ jobs,
+                                                          const int* __restrict__ colmap, int boff) {
+  const int b = blockIdx.x + boff;
+  const ColJob& J = jobs[colmap[b]];
+  const Dims dm = J.dm;                // loaded once, ahead of the component branches
+  const int c = (b - J.blk0) * kMultiCols + threadIdx.x;
+  if (c >= dm.ncol) return;
+  const RHS ra = J.ra;
+  vel_column<U>(ra, dm, c, blockIdx.y, J.ca.ayu, J.ca.ayv);
+}
+__global__ void __launch_bounds__(kMultiCols) k_pres(const ColJob* __restrict__ jobs,
+                                                           const int* __restrict__ colmap) {
+  const ColJob& J = jobs[colmap[blockIdx.x]];
+  const Dims dm = J.dm;
+  const int c = (blockIdx.x - J.blk0) * kMultiCols + threadIdx.x;
+  if (c >= dm.ncol) return;
+  const COL ca = J.ca;
+  pres_column<LH_COL_U>(ca, dm, c);
+}
+__global__ void __launch_bounds__(256) k_flowrate(const ColJob* __restrict__ jobs) {
+  __shared__ double part[256];
+  const ColJob& J = jobs[blockIdx.x];
+  if (!J.flowrate) return;
+  flowrate_body(J.ca.u1, J.yv, J.ca.scal, J.dm.ny, J.dm.d, J.dm.ncol, part);
+}
+
+static ColJob make_job(const lh_handle* h, int krk, int blk0) {
+  ColJob J;
+  J.ra = make_rhs(h, krk);
+  J.ca = make_col(h, krk);
+  J.prep = make_prep(h, krk);
+  J.dm = h->dm;
+  J.yv = h->yv;
+  J.vel_u = h->vel_u;
+  J.blk0 = blk0;
+  J.flowrate = h->flag_ctpress == 1 ? 1 : 0;
+  return J;
+}
+static int col_blocks(const lh_handle* h) { return (h->dm.ncol + kMultiCols - 1) / kMultiCols; }
+
+static void issue_substep(lh_handle* h, int krk /*0-based*/, bool cfl_mode, int stages = kStageAll) {
+  const Dims& dm = h->dm;
+  const double bre = h->rk[3 + krk] / dm.re;
   const int nlev = dm.nyu - 2;
   const bool f64 = h->prec == LH_PREC_FP64;
   const double zel = f64 ? 16.0 : 8.0;                       // bytes per Z element
@@ -484,29 +612,28 @@ static void issue_substep(lh_handle* h, int krk /*0-based*/, bool cfl_mode, int 
   // k_prep needs only dt: in graph and direct issue it runs on the side stream beside the
   // nonlinear stage (timing mode and batch members keep it in line); at kRK = 1 of a CFL
   // step dt comes from the first x pass, so it forks after k_cfl_dt
-  const PREP prep{h->dyu2i, h->dyv2i, h->scal, h->ayu, h->ayv, h->gw[0], h->gw[1], bre, dm.nyu, dm.nyv};
   const bool side = !h->timing && h->in_batch == 0 && stages == kStageAll;
   auto fork_prep = [&]() {
     (void)hipEventRecord(h->ev_fork, h->st);
     (void)hipStreamWaitEvent(h->side, h->ev_fork, 0);
-    hipLaunchKernelGGL(k_prep, dim3(1), dim3(64), 0, h->side, prep);
+    hipLaunchKernelGGL(k_prep, dim3(1), dim3(64), 0, h->side, (const ColJob*)h->coljobs + krk);
     (void)hipEventRecord(h->ev_join, h->side);
   };
   const bool late = krk == 0 && cfl_mode;
   if (side && !late) fork_prep();
-  // Nu_new into the buffer of this substep; RHS0 + rhs per column reads both
+  double2 *nw[3], *ol[3];
+  nu_buffers(h, krk, nw, ol);
   const bool wb = (krk % 2) == 1;
-  double2* nw[3] = {wb ? h->nb1 : h->nu1, wb ? h->nb2 : h->nu2, wb ? h->nb3 : h->nu3};
-  double2* ol[3] = {wb ? h->nu1 : h->nb1, wb ? h->nu2 : h->nb2, wb ? h->nu3 : h->nb3};
   if (stages & kStageNL) {
   // ---- NL stage: inverse z transforms of u, w (all u planes) and v (all v planes)
-  {
+  if (stages & kStageNLz) {
     ZJobs jobs{};
     jobs.j[0] = ZJob{h->u1, h->zu, nullptr, 0, dm.nyu, 0};
     jobs.j[1] = ZJob{h->u3, h->zw, nullptr, 0, dm.nyu, 0};
     jobs.j[2] = ZJob{h->u2, h->zv, nullptr, 0, dm.nyv, 0};
     TIMED(h, LH_K_ZINV, nuvw * (spl + zsl), launch_zinv(h, jobs, 3, dm.nyu, false));
   }
+  if (stages & kStageNLx) {
   if (krk == 0)
     TIMED(h, LH_K_MISC, 0.0,
           hipLaunchKernelGGL(k_set_scalar, dim3(1), dim3(1), 0, h->st, h->scal, (int)S_CFLMAX, 0.0));
@@ -525,6 +652,8 @@ static void issue_substep(lh_handle* h, int krk /*0-based*/, bool cfl_mode, int 
   if (krk == 0 && cfl_mode)
     TIMED(h, LH_K_MISC, 0.0, hipLaunchKernelGGL(k_cfl_dt, dim3(1), dim3(1), 0, h->st, h->scal));
   if (side && late) fork_prep();
+  }
+  if (stages & kStageNLa) {
   h->nu_last = wb ? 1 : 0;
   {
     ZASM a{};
@@ -550,24 +679,14 @@ static void issue_substep(lh_handle* h, int krk /*0-based*/, bool cfl_mode, int 
                              h->st, nw[0], nw[1], nw[2], (const double2*)h->fo1,
                              (const double2*)h->fo2, (const double2*)h->fo3, nu, nv));
   }
+  }
   }   // NL stage
-  RHS ra{};
-  ra.u1 = h->u1; ra.u2 = h->u2; ra.u3 = h->u3; ra.p = h->p;
-  ra.no1 = ol[0]; ra.no2 = ol[1]; ra.no3 = ol[2];
-  ra.nn1 = nw[0]; ra.nn2 = nw[1]; ra.nn3 = nw[2];
-  ra.r1 = h->r1; ra.r2 = h->r2; ra.r3 = h->r3; ra.tipl = h->tipl;
-  ra.dyu2i = h->dyu2i; ra.dyv2i = h->dyv2i; ra.dthdyv = h->dthdyv;
-  ra.k1x = h->k1x; ra.k2x = h->k2x; ra.k1z = h->k1z; ra.k2z = h->k2z; ra.scal = h->scal;
-  ra.planes_u = h->ibu_planes; ra.planes_v = h->ibv_planes;
-  ra.ib_u = h->ibflag_u; ra.ib_v = h->ibflag_v;
-  ra.np_u = h->nibu; ra.np_v = h->nibv;
-  ra.tip_u0 = h->tip_u[0] + dm.d; ra.tip_u1 = h->tip_u[1] + dm.d;
-  ra.tip_v0 = h->tip_v[0] + dm.d; ra.tip_v1 = h->tip_v[1] + dm.d;
-  ra.C1 = aR / dm.re; ra.C2 = -cR; ra.dR = dR; ra.g = gR; ra.bre = bre;
+  const RHS ra = make_rhs(h, krk);
   // ---- IB stage (FP64 transforms): canopy planes of rhs u, w, v and Lu_y on tip planes
-  if ((stages & kStageVel) && h->nibent > 0) {
+  if ((stages & kStageIB) && h->nibent > 0) {
     const int npl = h->nibu > h->nibv ? h->nibu : h->nibv;
     // in: u (3 rows), p, Nu_old, Nu_new per canopy plane; out: rhs (+ Lu_y of the tips)
+    if (stages & kStageIBr)
     TIMED(h, LH_K_RHS, spl * 7.0 * (2.0 * h->nibu + h->nibv),
           hipLaunchKernelGGL(k_rhs_ib, dim3((dm.ncol + h->col_bs - 1) / h->col_bs, npl, 3),
                              dim3(h->col_bs), 0, h->st, ra, dm));
@@ -581,10 +700,12 @@ static void issue_substep(lh_handle* h, int krk /*0-based*/, bool cfl_mode, int 
     jobs.j[3] = ZJob{h->tipl, h->zp, nullptr, 0, 6, slot};
     int maxn = h->nibu > h->nibv ? h->nibu : h->nibv;
     if (maxn < 6) maxn = 6;
+    if (stages & kStageIBz)
     TIMED(h, LH_K_IBZ, (nib + 6.0) * (spl + z64), launch_zinv(h, jobs, 4, maxn, true));
     // entries are sorted tip first: the tips take one row pair per block, the rest two
     XIB a{h->zp, h->ibent, h->masks, h->scal, bre, 0};
     XIB b{h->zp, h->ibent + h->nibtip, h->masks, h->scal, bre, 1};
+    if (stages & kStageIBx)
     TIMED(h, LH_K_XIB, (2.0 * nib + 6.0) * z64, {
       if (h->nibtip > 0) LH_DISPATCH_PLAN(dm.px, h->xplan, launch_xib_t, double, h, a, h->nibtip, h->twx64)
       if (h->nibent > h->nibtip)
@@ -595,43 +716,35 @@ static void issue_substep(lh_handle* h, int krk /*0-based*/, bool cfl_mode, int 
     fj.j[0] = ZJob{(const double2*)h->zp, h->r1, h->ibu_planes, 0, h->nibu, slot}; slot += h->nibu;
     fj.j[1] = ZJob{(const double2*)h->zp, h->r3, h->ibu_planes, 0, h->nibu, slot}; slot += h->nibu;
     fj.j[2] = ZJob{(const double2*)h->zp, h->r2, h->ibv_planes, 0, h->nibv, slot};
+    if (stages & kStageIBf)
     TIMED(h, LH_K_IBZ, nib * (spl + z64),
           launch_zfwd(h, fj, 3, h->nibu > h->nibv ? h->nibu : h->nibv, true));
   }
   // ---- column stage
   if (side)
     (void)hipStreamWaitEvent(h->st, h->ev_join, 0);
-  else if (stages & kStageVel)
-    TIMED(h, LH_K_MISC, 0.0, hipLaunchKernelGGL(k_prep, dim3(1), dim3(64), 0, h->st, prep));
+  else if (stages & kStageVelCol)
+    TIMED(h, LH_K_MISC, 0.0,
+          hipLaunchKernelGGL(k_prep, dim3(1), dim3(64), 0, h->st, (const ColJob*)h->coljobs + krk));
   {
-    COL a{};
-    a.r1 = h->r1; a.r2 = h->r2; a.r3 = h->r3;
-    a.u1 = h->u1; a.u2 = h->u2; a.u3 = h->u3; a.p = h->p; a.psi = h->psi;
-    a.ayu = h->ayu; a.ayv = h->ayv; a.dg3 = h->dg3;
-    a.k2x = h->k2x; a.k2z = h->k2z; a.yu = h->yu;
-    a.dthdyu = h->dthdyu; a.dthdyv = h->dthdyv; a.k1x = h->k1x; a.k1z = h->k1z;
-    a.scal = h->scal; a.ab = aR + bR; a.gw0 = h->gw[0]; a.gw1 = h->gw[1];
-    a.ctu = h->flag_ctpress == 0 ? 1 : 0;
-    a.ct = CTU{h->u1, h->scal, h->ctwork, h->dyu2i, h->yv, h->gw[0], h->gw[1], bre, dR, dm.ny,
-               dm.d, dm.ncol};
+    const ColJob* jk = h->coljobs + krk;
+    const int nb = col_blocks(h);
     // in: rhs u, w, v, p, LU factors (8 B each); out: u, w, v, p
     const double by = spl * (2.0 * nuvw + 2.0 * dm.nyp) + 3.0 * 8.0 * dm.ncol * (double)dm.nyp;
     // in: u, p, Nu_old, Nu_new (rhs on canopy planes); eliminated rows out and back; out: u
-    if (stages & kStageVel)
-    TIMED(h, LH_K_VEL, spl * (7.0 * nuvw),
-          if (h->vel_u == kVelUSmall) {
-            hipLaunchKernelGGL(k_vel_rhs<kVelUSmall>, dim3((dm.ncol + h->col_bs - 1) / h->col_bs, 3),
-                               dim3(h->col_bs), 0, h->st, ra, dm, (const double*)h->ayu,
-                               (const double*)h->ayv);
-          } else {
-            hipLaunchKernelGGL(k_vel_rhs<LH_VEL_U>, dim3((dm.ncol + h->col_bs - 1) / h->col_bs, 3),
-                               dim3(h->col_bs), 0, h->st, ra, dm, (const double*)h->ayu,
-                               (const double*)h->ayv);
-          });
+    if (stages & kStageVelCol)
+    TIMED(h, LH_K_VEL, spl * (7.0 * nuvw), {
+      if (h->vel_u == kVelUSmall)
+        hipLaunchKernelGGL(k_vel_rhs<kVelUSmall>, dim3(nb, 3), dim3(kMultiCols), 0, h->st, jk,
+                           (const int*)h->colmap, 0);
+      else
+        hipLaunchKernelGGL(k_vel_rhs<LH_VEL_U>, dim3(nb, 3), dim3(kMultiCols), 0, h->st, jk,
+                           (const int*)h->colmap, 0);
+    });
     if (stages & kStagePres)
     TIMED(h, LH_K_PRES, by - 2.0 * nuvw * spl + 2.0 * nuvw * spl,
-          hipLaunchKernelGGL(k_pres<LH_COL_U>, dim3((dm.ncol + h->col_bs - 1) / h->col_bs),
-                             dim3(h->col_bs), 0, h->st, a, dm));
+          hipLaunchKernelGGL(k_pres, dim3(nb), dim3(kMultiCols), 0, h->st, jk,
+                             (const int*)h->colmap));
   }
 }
 
@@ -639,8 +752,7 @@ static void issue_step(lh_handle* h, bool cfl_mode) {
   for (int k = 0; k < 3; ++k) issue_substep(h, k, cfl_mode);
   if (h->flag_ctpress == 1)
     TIMED(h, LH_K_MISC, 0.0,
-          hipLaunchKernelGGL(k_flowrate, dim3(1), dim3(256), 0, h->st, h->u1, h->yv, h->scal,
-                             h->dm.ny, h->dm.d, h->dm.ncol));
+          hipLaunchKernelGGL(k_flowrate, dim3(1), dim3(256), 0, h->st, (const ColJob*)h->coljobs));
 }
 
 // ---------------------------------------------------------------------------------------
@@ -920,6 +1032,16 @@ int lh_create(const lh_config* c, lh_handle** out) {
     HIPCHK(h, hipMemcpy(h->ibflag_u, fu.data(), dm.nyu, hipMemcpyHostToDevice));
     HIPCHK(h, hipMemcpy(h->ibflag_v, fv.data(), dm.nyv, hipMemcpyHostToDevice));
   }
+  // the column-stage jobs of the three substeps (every buffer they name exists now)
+  {
+    std::vector<ColJob> jobs;
+    for (int k = 0; k < 3; ++k) jobs.push_back(make_job(h, k, 0));
+    const std::vector<int> map(col_blocks(h), 0);
+    HIPCHK(h, hipMalloc((void**)&h->coljobs, jobs.size() * sizeof(ColJob)));
+    HIPCHK(h, hipMalloc((void**)&h->colmap, map.size() * sizeof(int)));
+    HIPCHK(h, hipMemcpy(h->coljobs, jobs.data(), jobs.size() * sizeof(ColJob), hipMemcpyHostToDevice));
+    HIPCHK(h, hipMemcpy(h->colmap, map.data(), map.size() * sizeof(int), hipMemcpyHostToDevice));
+  }
   // pressure LU factors, once (start.f90:628-630)
   {
     BUILDP a{h->dg3, h->dthdyu, h->dthdyv, h->yu, h->k2x, h->k2z};
@@ -943,7 +1065,7 @@ int lh_destroy(lh_handle* h) {
                   h->k2z, h->scal, h->u1, h->u2, h->u3, h->p, h->nu1, h->nu2, h->nu3, h->nb1, h->nb2, h->nb3, h->r1,
                   h->r2, h->r3, h->psi, h->tipl, h->zu, h->zw, h->zv, h->zp, h->dg3, h->ayu, h->ayv, h->ctwork, h->masks, h->ibu_planes,
                   h->ibv_planes, h->ibflag_u, h->ibflag_v, h->ibent, h->st_psum, h->st_pfold, h->st_acc,
-                  h->cst_acc, h->sp_acc, h->fo1, h->fo2, h->fo3, h->twxw, h->twxw32};
+                  h->cst_acc, h->sp_acc, h->fo1, h->fo2, h->fo3, h->twxw, h->twxw32, h->coljobs, h->colmap};
   for (void* q : ptrs)
     if (q) hipFree(q);
   if (h->twx && h->twx != h->twx64) hipFree(h->twx);
@@ -1062,8 +1184,7 @@ int lh_substep(lh_handle* h, int krk, double dt) {
   issue_substep(h, krk - 1, false);
   // meanflow_ctP follows the last substep of a step (littleharsh.f90:193-197)
   if (krk == 3 && h->flag_ctpress == 1)
-    hipLaunchKernelGGL(k_flowrate, dim3(1), dim3(256), 0, h->st, h->u1, h->yv, h->scal, h->dm.ny,
-                       h->dm.d, h->dm.ncol);
+    hipLaunchKernelGGL(k_flowrate, dim3(1), dim3(256), 0, h->st, (const ColJob*)h->coljobs);
   HIPCHK(h, hipGetLastError());
   HIPCHK(h, hipStreamSynchronize(h->st));
   tflush(h);
@@ -1404,7 +1525,10 @@ struct lh_batch {
   hipStream_t master = nullptr;
   hipGraphExec_t g = nullptr;
   double cfl = -1.0;
-  std::vector<hipEvent_t> ev;
+  std::vector<hipEvent_t> ev;   // 3 forks (one per substep), then 3 x n joins
+  ColJob* jobs = nullptr;       // [3][n] column-stage jobs of the captured graph
+  int* colmap = nullptr;        // block -> member of the column launches
+  int nblk = 0;
   bool joined = false;          // the handles count this batch (in_batch)
   std::string err;
 };
@@ -1420,7 +1544,7 @@ int lh_batch_create(int n, lh_handle* const* hs, lh_batch** out) {
     b->hs.push_back(hs[i]);
   }
   if (hipStreamCreate(&b->master) != hipSuccess) { delete b; return LH_ERR_HIP; }
-  b->ev.assign(n + 1, nullptr);
+  b->ev.assign(3 * (n + 1), nullptr);
   for (auto& e : b->ev)
     if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) {
       lh_batch_destroy(b);
@@ -1451,16 +1575,66 @@ int lh_batch_step_cfl(lh_batch* b, double cfl) {
       if (e != hipSuccess) b->err = std::string(what) + ": " + hipGetErrorString(e);
       return e == hipSuccess;
     };
+    // the column-stage jobs of the three substeps, before the capture (they are copied to
+    // the device, where the graph's launches read them)
+    const int n = (int)b->hs.size();
+    std::vector<ColJob> jobs(3 * (size_t)n);
+    // blocks of the members with the small-grid sweep (kVelUSmall rows per step) first
+    std::vector<int> colmap;
+    int nblk = 0, nsmall = 0;
+    for (int pass = 0; pass < 2; ++pass)
+      for (int i = 0; i < n; ++i) {
+        lh_handle* h = b->hs[i];
+        if ((h->vel_u == kVelUSmall) != (pass == 0)) continue;
+        for (int k = 0; k < 3; ++k) jobs[(size_t)k * n + i] = make_job(h, k, nblk);
+        const int nb = col_blocks(h);
+        colmap.insert(colmap.end(), nb, i);
+        nblk += nb;
+        if (pass == 0) nsmall = nblk;
+      }
+    if (b->jobs) { (void)hipFree(b->jobs); b->jobs = nullptr; }
+    if (b->colmap) { (void)hipFree(b->colmap); b->colmap = nullptr; }
+    if (!chk(hipMalloc((void**)&b->jobs, jobs.size() * sizeof(ColJob)), "job table") ||
+        !chk(hipMalloc((void**)&b->colmap, colmap.size() * sizeof(int)), "block map") ||
+        !chk(hipMemcpy(b->jobs, jobs.data(), jobs.size() * sizeof(ColJob), hipMemcpyHostToDevice), "job upload") ||
+        !chk(hipMemcpy(b->colmap, colmap.data(), colmap.size() * sizeof(int), hipMemcpyHostToDevice), "map upload"))
+      return LH_ERR_HIP;
+    b->nblk = nblk;
+    // Per substep: each member's nonlinear and canopy stages on its own stream (parallel
+    // branches), then, joined on the master stream, the column stage of all members in one
+    // launch each of k_prep, the velocity sweeps and the pressure step.
     if (!chk(hipStreamBeginCapture(b->master, hipStreamCaptureModeGlobal), "capture")) return LH_ERR_HIP;
-    bool ok = chk(hipEventRecord(b->ev[0], b->master), "fork");
-    for (size_t i = 0; i < b->hs.size() && ok; ++i) {
-      lh_handle* h = b->hs[i];
-      ok = chk(hipStreamWaitEvent(h->st, b->ev[0], 0), "fork wait");
-      hipLaunchKernelGGL(k_set_scalar, dim3(1), dim3(1), 0, h->st, h->scal, (int)S_CFL, cfl);
-      issue_step(h, true);
-      ok = ok && chk(hipEventRecord(b->ev[i + 1], h->st), "join");
-      ok = ok && chk(hipStreamWaitEvent(b->master, b->ev[i + 1], 0), "join wait");
+    // Within a substep the members' launches are issued launch by launch (each launch for
+    // every member, then the next), so that consecutive nodes of the graph are independent.
+    static const int phases[] = {kStageNLz, kStageNLx, kStageNLa, kStageIBr, kStageIBz, kStageIBx, kStageIBf};
+    bool ok = true;
+    for (int k = 0; k < 3 && ok; ++k) {
+      ok = chk(hipEventRecord(b->ev[k], b->master), "fork");
+      for (int i = 0; i < n && ok; ++i) {
+        lh_handle* h = b->hs[i];
+        ok = chk(hipStreamWaitEvent(h->st, b->ev[k], 0), "fork wait");
+        if (k == 0) hipLaunchKernelGGL(k_set_scalar, dim3(1), dim3(1), 0, h->st, h->scal, (int)S_CFL, cfl);
+      }
+      for (int ph : phases)
+        for (int i = 0; i < n && ok; ++i) issue_substep(b->hs[i], k, true, ph);
+      for (int i = 0; i < n && ok; ++i) {
+        lh_handle* h = b->hs[i];
+        hipEvent_t join = b->ev[3 + (size_t)k * n + i];
+        ok = chk(hipEventRecord(join, h->st), "join");
+        ok = ok && chk(hipStreamWaitEvent(b->master, join, 0), "join wait");
+      }
+      const ColJob* jk = b->jobs + (size_t)k * n;
+      hipLaunchKernelGGL(k_prep, dim3(n), dim3(64), 0, b->master, jk);
+      if (nsmall > 0)
+        hipLaunchKernelGGL(k_vel_rhs<kVelUSmall>, dim3(nsmall, 3), dim3(kMultiCols), 0, b->master, jk,
+                           (const int*)b->colmap, 0);
+      if (nblk > nsmall)
+        hipLaunchKernelGGL(k_vel_rhs<LH_VEL_U>, dim3(nblk - nsmall, 3), dim3(kMultiCols), 0, b->master,
+                           jk, (const int*)b->colmap, nsmall);
+      hipLaunchKernelGGL(k_pres, dim3(nblk), dim3(kMultiCols), 0, b->master, jk,
+                         (const int*)b->colmap);
     }
+    hipLaunchKernelGGL(k_flowrate, dim3(n), dim3(256), 0, b->master, (const ColJob*)b->jobs);
     const bool ended = chk(hipStreamEndCapture(b->master, &g), "end capture");
     if (!ok || !ended) return LH_ERR_HIP;
     if (!chk(hipGraphInstantiate(&b->g, g, nullptr, nullptr, 0), "instantiate")) {
@@ -1496,6 +1670,8 @@ int lh_batch_destroy(lh_batch* b) {
       drop_graphs(h);                               // captured with k_prep in line
     }
   if (b->g) (void)hipGraphExecDestroy(b->g);
+  if (b->jobs) (void)hipFree(b->jobs);
+  if (b->colmap) (void)hipFree(b->colmap);
   for (auto e : b->ev)
     if (e) (void)hipEventDestroy(e);
   if (b->master) (void)hipStreamDestroy(b->master);

@@ -94,14 +94,16 @@ struct RhsCol {
 __device__ __forceinline__ RhsCol rhs_col(const RHS& a, const Dims& dm, int c, int f) {
   RhsCol r;
   const int i = c % dm.mx, k = c / dm.mx;
-  r.dt = a.scal[S_DT];
+  const gptr<const double> scal = gp(a.scal);
+  r.dt = scal[S_DT];
   const double beta = a.bre * r.dt;
-  r.kx = a.k1x[i];
-  r.kz = a.k1z[k];
-  r.k2 = a.k2x[i] + a.k2z[k];
-  r.axzi = 1.0 / (1.0 - beta * (a.k2x[i] + a.k2z[k]));
+  r.kx = gp(a.k1x)[i];
+  r.kz = gp(a.k1z)[k];
+  const double k2x = gp(a.k2x)[i], k2z = gp(a.k2z)[k];
+  r.k2 = k2x + k2z;
+  r.axzi = 1.0 / (1.0 - beta * (k2x + k2z));
   r.kp = -a.dR * (f == 0 ? r.kx : r.kz);
-  r.mp = (f == 0 && c == 0) ? -a.dR * a.scal[S_MPGX] : 0.0;
+  r.mp = (f == 0 && c == 0) ? -a.dR * scal[S_MPGX] : 0.0;
   return r;
 }
 
@@ -110,14 +112,14 @@ __device__ __forceinline__ RhsCol rhs_col(const RHS& a, const Dims& dm, int c, i
 __device__ __forceinline__ double2 rhs_at(const RHS& a, const Dims& dm, const RhsCol& k, int f,
                                           int s, double2 um, double2 u0, double2 up, double2 pl,
                                           double2 pr, double2 vo, double2 vn) {
-  const double* wy = (f < 2 ? a.dyu2i : a.dyv2i) + 3 * s;
+  const gptr<const double> wy = gp(f < 2 ? a.dyu2i : a.dyv2i) + 3 * s;
   const double2 Lu = dadd(dadd(drm(wy[0], um), drm(k.k2 + wy[1], u0)), drm(wy[2], up));
   double2 du;
   if (f < 2) {
     du = dadd(dadd(drm(a.C1, Lu), drm(a.C2, vo)), dim(k.kp, pl));
     du.x += k.mp;
   } else {
-    const double C3v = -a.dR * dm.ddthetavi * a.dthdyv[s];
+    const double C3v = -a.dR * dm.ddthetavi * gp(a.dthdyv)[s];
     du = dadd(dadd(drm(a.C1, Lu), drm(a.C2, vo)), drm(C3v, dsub(pr, pl)));
   }
   du = dsub(du, drm(a.g, vn));
@@ -157,14 +159,15 @@ __global__ void __launch_bounds__(256) k_rhs_ib(RHS a, Dims dm) {
 // back substitution into u.  F: 0 u1, 1 u3, 2 u2.
 template <int U, int F>
 __device__ __forceinline__ void solve_rhs(const RHS& a, const Dims& dm, int c,
-                                          const double* __restrict__ ay) {
+                                          const double* __restrict__ ayp) {
   const int ncol = dm.ncol;
-  const double2* __restrict__ no = F == 0 ? a.no1 : (F == 1 ? a.no3 : a.no2);
-  const double2* __restrict__ nn = F == 0 ? a.nn1 : (F == 1 ? a.nn3 : a.nn2);
-  double2* __restrict__ uu = F == 0 ? a.u1 : (F == 1 ? a.u3 : a.u2);
-  double2* __restrict__ x = F == 0 ? a.r1 : (F == 1 ? a.r3 : a.r2);
-  const unsigned char* __restrict__ ib = F < 2 ? a.ib_u : a.ib_v;
-  const double2* __restrict__ pp = a.p;
+  const gptr<const double> ay = gp(ayp);
+  const G2 no = g2(F == 0 ? a.no1 : (F == 1 ? a.no3 : a.no2));
+  const G2 nn = g2(F == 0 ? a.nn1 : (F == 1 ? a.nn3 : a.nn2));
+  const G2 uu = g2(F == 0 ? a.u1 : (F == 1 ? a.u3 : a.u2));
+  const G2 x = g2(F == 0 ? a.r1 : (F == 1 ? a.r3 : a.r2));
+  const gptr<const unsigned char> ib = gp(F < 2 ? a.ib_u : a.ib_v);
+  const G2 pp = g2(a.p);
   const int nj = F < 2 ? dm.nyu : dm.nyv;
   const RhsCol k = rhs_col(a, dm, c, F);
   const double2 zero = d2(0.0, 0.0);
@@ -228,45 +231,43 @@ __device__ __forceinline__ void solve_rhs(const RHS& a, const Dims& dm, int c,
 }
 
 template <int U>
-__global__ void __launch_bounds__(256) k_vel_rhs(RHS a, Dims dm, const double* __restrict__ ayu,
-                                                 const double* __restrict__ ayv) {
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= dm.ncol) return;
-  if (blockIdx.y == 0) solve_rhs<U, 0>(a, dm, c, ayu);
-  else if (blockIdx.y == 1) solve_rhs<U, 1>(a, dm, c, ayu);
+__device__ __forceinline__ void vel_column(const RHS& a, const Dims& dm, int c, int f,
+                                           const double* __restrict__ ayu, const double* __restrict__ ayv) {
+  if (f == 0) solve_rhs<U, 0>(a, dm, c, ayu);
+  else if (f == 1) solve_rhs<U, 1>(a, dm, c, ayu);
   else solve_rhs<U, 2>(a, dm, c, ayv);
 }
 
 
+
 // constant-flow-rate correction (column 0), solveP and v_corr, one column per thread
 template <int U>
-__global__ void __launch_bounds__(256) k_pres(COL a, Dims dm) {
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void pres_column(const COL& a, const Dims& dm, int c) {
   const int ncol = dm.ncol;
-  if (c >= ncol) return;
   if (a.ctu && c == 0) ctu_column(a.ct);
   // ---- solveP: divergence, psi = dtirk*div, forward elimination of LUsolP
   const int i = c % dm.mx, k = c / dm.mx;
-  const double kx = a.k1x[i], kz = a.k1z[k];
-  const double dt = a.scal[S_DT];
+  const double kx = gp(a.k1x)[i], kz = gp(a.k1z)[k];
+  const double dt = gp(a.scal)[S_DT];
   const double dtirk = (1.0 / dt) / a.ab;
   const int np = dm.nyp;
   const double dv = dm.ddthetavi;
-  double2* __restrict__ u1 = a.u1;
-  double2* __restrict__ u2 = a.u2;
-  double2* __restrict__ u3 = a.u3;
-  double2* __restrict__ psi = a.psi;
-  double2* __restrict__ pp = a.p;
+  const G2 u1 = g2(a.u1);
+  const G2 u2 = g2(a.u2);
+  const G2 u3 = g2(a.u3);
+  const G2 psi = g2(a.psi);
+  const G2 pp = g2(a.p);
+  const gptr<const double> dthdyu = gp(a.dthdyu), dthdyv = gp(a.dthdyv), dg3 = gp(a.dg3);
   double2 prev = d2(0.0, 0.0);
   double2 vlow = u2[c];   // v face below p row 0
   // LU_decP on the fly: the lower and the inverse pivot of each row come from the operator
   // row and the running modified upper a3p (the same operations k_buildP used for dg3)
-  const double k2x = a.k2x[i], k2z = a.k2z[k];
+  const double k2x = gp(a.k2x)[i], k2z = gp(a.k2z)[k];
   double a3p = 0.0;
   int r = 0;
   // rows in chunks of U; every load of a chunk is issued (rows past the last one read the
   // last one again), so that a chunk waits for memory once
-  const double dypin = a.yu[2 + dm.d] - a.yu[1 + dm.d];
+  const double dypin = gp(a.yu)[2 + dm.d] - gp(a.yu)[1 + dm.d];
   for (; r < np; r += U) {
     double2 a1[U], a3[U], a2[U];
     double hu[U], hv[U + 1];              // dthdyu(rr+1), dthdyv(rr) of the chunk's rows
@@ -277,10 +278,10 @@ __global__ void __launch_bounds__(256) k_pres(COL a, Dims dm) {
       a1[q] = u1[ou];
       a3[q] = u3[ou];
       a2[q] = u2[ou];
-      hu[q] = a.dthdyu[rq + 1];
+      hu[q] = dthdyu[rq + 1];
     }
 #pragma unroll
-    for (int q = 0; q <= U; ++q) hv[q] = a.dthdyv[min(r + q, np)];
+    for (int q = 0; q <= U; ++q) hv[q] = dthdyv[min(r + q, np)];
 #pragma unroll
     for (int q = 0; q < U; ++q) {
       const int rr = r + q;
@@ -320,12 +321,12 @@ __global__ void __launch_bounds__(256) k_pres(COL a, Dims dm) {
       const int rr = max(r - q, 0);
       const size_t o = (size_t)rr * ncol + c, ou = (size_t)(rr + 1) * ncol + c;
       sv[q] = psi[o];
-      g3[q] = a.dg3[o];
+      g3[q] = dg3[o];
       pv[q] = pp[o];
       a1[q] = u1[ou];
       a3[q] = u3[ou];
       a2[q] = u2[ou];
-      hv[q] = a.dthdyv[rr + 1];
+      hv[q] = dthdyv[rr + 1];
     }
 #pragma unroll
     for (int q = 0; q < U; ++q) {

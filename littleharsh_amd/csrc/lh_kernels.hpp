@@ -399,6 +399,35 @@ __global__ void __launch_bounds__((2 * Plan<PX, V>::T))
 }
 
 __device__ __forceinline__ double2 d2(double a, double b) { double2 r; r.x = a; r.y = b; return r; }
+// A device buffer held as a global-address-space pointer.  The column kernels take their
+// buffers from a job table, and a pointer loaded from memory is generic to the compiler,
+// which then uses flat loads and stores; through a gptr every access is a global one.
+template <typename T> using gptr = __attribute__((address_space(1))) T*;
+template <typename T> __device__ __forceinline__ gptr<T> gp(T* p) { return (gptr<T>)p; }
+// the same for complex (double2) buffers: g2(p)[i] reads and writes element i as two doubles
+// through a global pointer (double2 itself cannot be copied through one)
+typedef double gd2v __attribute__((ext_vector_type(2)));
+struct G2ref {
+  __attribute__((address_space(1))) gd2v* q;
+  __device__ __forceinline__ operator double2() const {
+    const gd2v v = *q;
+    return d2(v.x, v.y);
+  }
+  __device__ __forceinline__ const G2ref& operator=(double2 v) const {
+    gd2v w;
+    w.x = v.x;
+    w.y = v.y;
+    *q = w;
+    return *this;
+  }
+};
+struct G2 {
+  __attribute__((address_space(1))) gd2v* p;
+  __device__ __forceinline__ G2ref operator[](size_t i) const { return G2ref{p + i}; }
+};
+__device__ __forceinline__ G2 g2(const double2* p) {
+  return G2{(__attribute__((address_space(1))) gd2v*)const_cast<double2*>(p)};
+}
 __device__ __forceinline__ double2 dadd(double2 a, double2 b) { return d2(a.x + b.x, a.y + b.y); }
 __device__ __forceinline__ double2 dsub(double2 a, double2 b) { return d2(a.x - b.x, a.y - b.y); }
 __device__ __forceinline__ double2 drm(double s, double2 a) { return d2(s * a.x, s * a.y); }
@@ -418,13 +447,12 @@ struct PREP {
 // LU_build + LU_dec of the two shared y matrices (tridLU_3D_IB.f90:427-493, 601-640), one
 // thread per grid.  The recurrence runs in registers (the previous row's scaled upper
 // entry is the only carried value) with the operations in the order of LU_dec.
-__global__ void k_prep(PREP a) {
-  const int t = threadIdx.x;
+__device__ __forceinline__ void prep_body(const PREP& a, int t) {
   if (t > 1) return;
   const int nj = t == 0 ? a.nyu : a.nyv;
-  const double* dd = t == 0 ? a.dyu2i : a.dyv2i;
-  double* ay = t == 0 ? a.ayu : a.ayv;
-  const double beta = a.bre * a.scal[S_DT];
+  const gptr<const double> dd = gp(t == 0 ? a.dyu2i : a.dyv2i);
+  const gptr<double> ay = gp(t == 0 ? a.ayu : a.ayv);
+  const double beta = a.bre * gp(a.scal)[S_DT];
   // row 0: (0, 1, gw(1)) on the u grid, Dirichlet (0, 1, 0) on the v grid
   double up = t == 0 ? a.gw0 : 0.0;
   double di = 1.0 / 1.0;
@@ -530,9 +558,11 @@ __global__ void __launch_bounds__(256) k_buildP(BUILDP a, Dims dm) {
 
 // meanflow_ctP (littleharsh.f90:505-521): Qx of the (0,0) column, one block of 256 threads
 // (strided partial sums, then a fixed-order tree: the same sum on every run)
-__global__ void __launch_bounds__(256) k_flowrate(const double2* u1, const double* yv, double* scal, int ny,
-                                                  int d, int ncol) {
-  __shared__ double part[256];
+// flowrateRe of u1 into scal[S_QX], one block of 256 threads (part: 256 doubles of LDS)
+__device__ __forceinline__ void flowrate_body(const double2* u1p, const double* yvp, double* scal, int ny,
+                                              int d, int ncol, double* part) {
+  const double2* u1 = u1p;
+  const gptr<const double> yv = gp(yvp);
   double q = 0.0;
   for (int j = 1 + (int)threadIdx.x; j <= ny - 1; j += 256)
     q += (yv[j + d] - yv[j - 1 + d]) * u1[(size_t)(j + d) * ncol].x;
@@ -542,7 +572,7 @@ __global__ void __launch_bounds__(256) k_flowrate(const double2* u1, const doubl
     if ((int)threadIdx.x < s) part[threadIdx.x] += part[threadIdx.x + s];
     __syncthreads();
   }
-  if (threadIdx.x == 0) scal[S_QX] = part[0];
+  if (threadIdx.x == 0) gp(scal)[S_QX] = part[0];
 }
 
 // error (littleharsh.f90:323-352): max |div| over the p grid
