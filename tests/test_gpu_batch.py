@@ -63,6 +63,34 @@ def test_batch_matches_single_instances(require_gpu):
     b.close()
 
 
+def test_batch_with_a_large_member(require_gpu):
+    """A batch whose members use both velocity sweeps: two small channels (4 rows per step)
+    and a 256 x 33 x 256 one (33024 spectral columns, 1 row per step, px = pz = 384), so the
+    column launches cover two block ranges; every member bit-identical to its single steps."""
+    from littleharsh_amd.channel import make_setup
+    from littleharsh_amd.solver import Batch, Channel
+    big = make_setup(nx=256, nz=256, ny_total=33, dsty=4, re=180.0, alp=1.0, bet=2.0,
+                     mpgx=-1.0, posth=0.2, dyq=4.0, ntilex=16, ntilez=8, dstx=6, dstz=12)
+    setups = [random_instance(3), big, random_instance(11)]
+    chans, refs = [], []
+    for i, s in enumerate(setups):
+        st = laminar_plus_noise(s, seed=60 + i)
+        c, r = Channel(s), Channel(s)
+        c.set_state(*st)
+        r.set_state(*st)
+        chans.append(c)
+        refs.append(r)
+    b = Batch(chans)
+    for _ in range(2):
+        b.step_cfl(0.5)
+        for r in refs:
+            r.step_cfl(0.5)
+    for i, (c, r) in enumerate(zip(chans, refs)):
+        for x, y in zip(c.get_state(), r.get_state()):
+            assert np.array_equal(x, y), i
+    b.close()
+
+
 def test_batch_recaptures_when_a_member_changes(require_gpu):
     """Switching a member's forcing on (and off) after the batch graph was captured must be
     seen by the next batch step: the batch re-captures when a member's graphs were dropped."""
