@@ -11,12 +11,16 @@ from cases import small
 pytestmark = pytest.mark.gpu
 
 
-def test_mixed_precision_tracks_fp64(require_gpu):
+# plans (1, 1): the FP32 transform kernels of the alternative radix sequences (8 x 12 at
+# 96 points), which the autotuner may pick for a mixed-precision channel
+@pytest.mark.parametrize("plans", [(0, 0), (1, 1)])
+def test_mixed_precision_tracks_fp64(require_gpu, plans):
     from littleharsh_amd.solver import Channel
     s = small()
     st = laminar_plus_noise(s, seed=31)
     g64 = Channel(s)
     g32 = Channel(s, precision=capi.LH_PREC_MIXED)
+    g32.set_plans(*plans)
     g64.set_state(*st)
     g32.set_state(*st)
     for _ in range(20):
