@@ -188,8 +188,9 @@ int lh_spectra_read(lh_handle* h, double* out, int reset);
  * not depend on them: each only changes how work is split into blocks.
  *   xraw_lc  levels per block of the sliding x pass        (>= 1)
  *   zasm_lc  levels per block of the z assembly            (>= 1)
- *   col_bs   threads per block of the canopy rhs kernel    (32..256, multiple of 32; the
- *            column stage itself runs 64 columns per block)
+ *   col_bs   threads per block of the canopy rhs kernel    (32..256, multiple of 32; used
+ *            by channels with an x or z transform length above 144 only: the others, and
+ *            the column stage, run 64 columns per block)
  * lh_create sets the two level counts from the grid and the device: each block of those
  * kernels holds a CU's LDS, so the launch runs in ceil(blocks / resident slots) rounds,
  * and the count minimising rounds x (levels + start-up) is taken.

@@ -35,6 +35,7 @@
 using namespace lh;
 
 struct ColJob;
+struct IbSets;
 
 // Default launch parameters (per handle; lh_autotune may change them):
 //   LH_XRAW_LC  levels per block of the sliding x pass (0: chosen per grid, see chunk_levels)
@@ -105,6 +106,7 @@ struct lh_handle {
   double *ayu = nullptr, *ayv = nullptr, *ctwork = nullptr;
   ColJob* coljobs = nullptr;    // the column-stage jobs of the three substeps
   int* colmap = nullptr;        // block -> channel (all 0)
+  IbSets* ib = nullptr;         // canopy-stage job tables (small channels with a canopy)
   void *twx = nullptr, *twz = nullptr;         // twiddles in the NL precision
   void *twx64 = nullptr, *twz64 = nullptr;     // FP64 twiddles (IB, statistics)
   // the one-wave-per-line x pass (576 collocation points in x): its twiddle table, and the
@@ -432,6 +434,17 @@ static void launch_stats_t(lh_handle* h, const STATS& a) {
     case 576: F<576, TR, 0>(__VA_ARGS__); break;    \
     default: break;                                 \
   }
+// the same over the lengths that run from job tables (up to 144)
+#define LH_DISPATCH_PLAN_SMALL(n, v, F, TR, ...)    \
+  switch (n) {                                      \
+    case 32: F<32, TR, 0>(__VA_ARGS__); break;      \
+    LH_PLAN_CASE2(48, v, F, TR, __VA_ARGS__)        \
+    case 64: F<64, TR, 0>(__VA_ARGS__); break;      \
+    LH_PLAN_CASE2(72, v, F, TR, __VA_ARGS__)        \
+    LH_PLAN_CASE2(96, v, F, TR, __VA_ARGS__)        \
+    LH_PLAN_CASE2(144, v, F, TR, __VA_ARGS__)       \
+    default: break;                                 \
+  }
 // number of FFT plans compiled for length n
 static int plans_of_len(int n) {
   switch (n) {
@@ -600,6 +613,193 @@ static ColJob make_job(const lh_handle* h, int krk, int blk0) {
 }
 static int col_blocks(const lh_handle* h) { return (h->dm.ncol + kMultiCols - 1) / kMultiCols; }
 
+// ---- the canopy stage from job tables (channels whose transform lengths are all <= 144)
+// A channel of that size runs its canopy stage from tables, alone or in a batch; within a
+// batch the channels that share the kernel, the length and the plan form one launch.
+static bool small_channel(const lh_handle* h) { return h->dm.px <= 144 && h->dm.pz <= 144; }
+static bool ib_jobs(const lh_handle* h) { return small_channel(h) && h->nibent > 0; }
+template <int N, typename TR, int V> static void zlines_of_t(int* l) { *l = zlines<N, V>(); }
+static int zlines_of(int n, int v) {
+  int l = 16;
+  LH_DISPATCH_PLAN_SMALL(n, v, zlines_of_t, double, &l)
+  return l;
+}
+
+// Groups of one kernel: channels that share (length, plan); per group a job table per
+// substep (nk = 3) or one for all substeps (nk = 1), and one block map.
+template <typename J>
+struct GroupSet {
+  struct G { int len, plan; std::vector<int> mem; size_t job0[3]; size_t map0; int nblk; };
+  std::vector<G> gs;
+  J* djobs = nullptr;
+  int* dmap = nullptr;
+  // key(h, &len, &plan) -> whether the channel takes part; blocks(h); make(h, k, blk0)
+  template <class Key, class Blocks, class Make>
+  hipError_t build(const std::vector<lh_handle*>& hs, int nk, Key key, Blocks blocks, Make make) {
+    release();
+    for (int i = 0; i < (int)hs.size(); ++i) {
+      int len = 0, plan = 0;
+      if (!key(hs[i], &len, &plan)) continue;
+      size_t g = 0;
+      while (g < gs.size() && !(gs[g].len == len && gs[g].plan == plan)) ++g;
+      if (g == gs.size()) gs.push_back(G{len, plan, {}, {0, 0, 0}, 0, 0});
+      gs[g].mem.push_back(i);
+    }
+    std::vector<J> jobs;
+    std::vector<int> map;
+    for (auto& G : gs) {
+      for (int k = 0; k < nk; ++k) {
+        G.job0[k] = jobs.size();
+        int blk = 0;
+        for (int i : G.mem) {
+          jobs.push_back(make(hs[i], k, blk));
+          blk += blocks(hs[i]);
+        }
+        G.nblk = blk;
+      }
+      for (int k = nk; k < 3; ++k) G.job0[k] = G.job0[0];
+      G.map0 = map.size();
+      for (size_t j = 0; j < G.mem.size(); ++j) map.insert(map.end(), blocks(hs[G.mem[j]]), (int)j);
+    }
+    if (jobs.empty()) return hipSuccess;
+    hipError_t e = hipMalloc((void**)&djobs, jobs.size() * sizeof(J));
+    if (e == hipSuccess) e = hipMalloc((void**)&dmap, map.size() * sizeof(int));
+    if (e == hipSuccess) e = hipMemcpy(djobs, jobs.data(), jobs.size() * sizeof(J), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(dmap, map.data(), map.size() * sizeof(int), hipMemcpyHostToDevice);
+    return e;
+  }
+  void release() {
+    if (djobs) (void)hipFree(djobs);
+    if (dmap) (void)hipFree(dmap);
+    djobs = nullptr;
+    dmap = nullptr;
+    gs.clear();
+  }
+};
+
+template <int PZ, typename TR, int V>
+static void launch_zinv_jobs_t(hipStream_t st, const ZJobT* jt, const int* map, int nblk) {
+  constexpr int L = zlines<PZ, V>();
+  const size_t sh = (size_t)L * zls<PZ>() * sizeof(cx<TR>);
+  hipLaunchKernelGGL((k_zinv_jobs<PZ, L, TR, V>), dim3(nblk), dim3(L * Plan<PZ, V>::T), sh, st, jt, map);
+}
+template <int PZ, typename TR, int V>
+static void launch_zfwd_jobs_t(hipStream_t st, const ZJobT* jt, const int* map, int nblk) {
+  constexpr int L = zlines<PZ, V>();
+  const size_t sh = (size_t)L * zls<PZ>() * sizeof(cx<TR>);
+  hipLaunchKernelGGL((k_zfwd_jobs<PZ, L, TR, V>), dim3(nblk), dim3(L * Plan<PZ, V>::T), sh, st, jt, map);
+}
+template <int PX, typename TR, int V>
+static void launch_xib_jobs_t(hipStream_t st, const XibJob* jt, const int* map, int nblk) {
+  const size_t sh = (size_t)2 * (PX + 1) * sizeof(cx<TR>);
+  hipLaunchKernelGGL((k_xpass_ib_jobs<PX, TR, V>), dim3(nblk), dim3(2 * Plan<PX, V>::T), sh, st, jt, map);
+}
+
+// the canopy-stage jobs of one channel (the launches of the by-value path, same arguments)
+static int ib_npl(const lh_handle* h) { return h->nibu > h->nibv ? h->nibu : h->nibv; }
+static int ib_maxn(const lh_handle* h) { return ib_npl(h) > 6 ? ib_npl(h) : 6; }
+static ZJobs ib_zinv_jobs(const lh_handle* h) {
+  ZJobs jobs{};
+  int slot = 0;
+  jobs.j[0] = ZJob{h->r1, h->zp, h->ibu_planes, 0, h->nibu, slot}; slot += h->nibu;
+  jobs.j[1] = ZJob{h->r3, h->zp, h->ibu_planes, 0, h->nibu, slot}; slot += h->nibu;
+  jobs.j[2] = ZJob{h->r2, h->zp, h->ibv_planes, 0, h->nibv, slot}; slot += h->nibv;
+  jobs.j[3] = ZJob{h->tipl, h->zp, nullptr, 0, 6, slot};
+  jobs.maxn = ib_maxn(h);
+  return jobs;
+}
+static ZJobs ib_zfwd_jobs(const lh_handle* h) {
+  ZJobs fj{};
+  int slot = 0;
+  fj.j[0] = ZJob{(const double2*)h->zp, h->r1, h->ibu_planes, 0, h->nibu, slot}; slot += h->nibu;
+  fj.j[1] = ZJob{(const double2*)h->zp, h->r3, h->ibu_planes, 0, h->nibu, slot}; slot += h->nibu;
+  fj.j[2] = ZJob{(const double2*)h->zp, h->r2, h->ibv_planes, 0, h->nibv, slot};
+  fj.maxn = ib_npl(h);
+  return fj;
+}
+static XIB ib_xib(const lh_handle* h, int krk, int paired) {
+  const double bre = h->rk[3 + krk] / h->dm.re;
+  return paired ? XIB{h->zp, h->ibent + h->nibtip, h->masks, h->scal, bre, 1}
+                : XIB{h->zp, h->ibent, h->masks, h->scal, bre, 0};
+}
+static int zgrid_cols(const lh_handle* h) { return (h->dm.mx + zlines_of(h->dm.pz, h->zplan) - 1) / zlines_of(h->dm.pz, h->zplan); }
+
+struct IbSets {
+  GroupSet<RhsIbJob> ri;
+  GroupSet<ZJobT> zi, zf;
+  GroupSet<XibJob> xt, xp;
+  hipError_t build(const std::vector<lh_handle*>& hs) {
+    hipError_t e = ri.build(
+        hs, 3, [](const lh_handle* h, int* l, int* v) { *l = 0; *v = 0; return ib_jobs(h); },
+        [](const lh_handle* h) { return ((h->dm.ncol + 63) / 64) * ib_npl(h) * 3; },
+        [](const lh_handle* h, int k, int blk0) {
+          return RhsIbJob{make_rhs(h, k), h->dm, blk0, (h->dm.ncol + 63) / 64, ib_npl(h)};
+        });
+    if (e == hipSuccess)
+      e = zi.build(
+          hs, 1, [](const lh_handle* h, int* l, int* v) { *l = h->dm.pz; *v = h->zplan; return ib_jobs(h); },
+          [](const lh_handle* h) { return zgrid_cols(h) * ib_maxn(h) * 4; },
+          [](const lh_handle* h, int, int blk0) {
+            return ZJobT{ib_zinv_jobs(h), h->dm, h->twz64, blk0, zgrid_cols(h) * ib_maxn(h) * 4};
+          });
+    if (e == hipSuccess)
+      e = zf.build(
+          hs, 1, [](const lh_handle* h, int* l, int* v) { *l = h->dm.pz; *v = h->zplan; return ib_jobs(h); },
+          [](const lh_handle* h) { return zgrid_cols(h) * ib_npl(h) * 3; },
+          [](const lh_handle* h, int, int blk0) {
+            return ZJobT{ib_zfwd_jobs(h), h->dm, h->twz64, blk0, zgrid_cols(h) * ib_npl(h) * 3};
+          });
+    if (e == hipSuccess)
+      e = xt.build(
+          hs, 3,
+          [](const lh_handle* h, int* l, int* v) { *l = h->dm.px; *v = h->xplan; return ib_jobs(h) && h->nibtip > 0; },
+          [](const lh_handle* h) { return (h->dm.pz / 2) * h->nibtip; },
+          [](const lh_handle* h, int k, int blk0) { return XibJob{ib_xib(h, k, 0), h->dm, h->twx64, blk0, h->dm.pz / 2}; });
+    if (e == hipSuccess)
+      e = xp.build(
+          hs, 3,
+          [](const lh_handle* h, int* l, int* v) {
+            *l = h->dm.px; *v = h->xplan; return ib_jobs(h) && h->nibent > h->nibtip;
+          },
+          [](const lh_handle* h) { return (h->dm.pz / 4) * (h->nibent - h->nibtip); },
+          [](const lh_handle* h, int k, int blk0) { return XibJob{ib_xib(h, k, 1), h->dm, h->twx64, blk0, h->dm.pz / 4}; });
+    return e;
+  }
+  void release() { ri.release(); zi.release(); zf.release(); xt.release(); xp.release(); }
+  // the launches of the canopy stage of substep k of every channel in the sets, in stage
+  // order: rhs, inverse z, x (tip entries, then the others), forward z
+  void launch_rhs(int k, hipStream_t st) const {
+    for (const auto& G : ri.gs)
+      hipLaunchKernelGGL(k_rhs_ib_jobs, dim3(G.nblk), dim3(64), 0, st, ri.djobs + G.job0[k], ri.dmap + G.map0);
+  }
+  void launch_zinv(int k, hipStream_t st) const {
+    for (const auto& G : zi.gs)
+      LH_DISPATCH_PLAN_SMALL(G.len, G.plan, launch_zinv_jobs_t, double, st, zi.djobs + G.job0[k], zi.dmap + G.map0, G.nblk)
+  }
+  void launch_x(int k, hipStream_t st) const {
+    for (const auto& G : xt.gs)
+      LH_DISPATCH_PLAN_SMALL(G.len, G.plan, launch_xib_jobs_t, double, st, xt.djobs + G.job0[k], xt.dmap + G.map0, G.nblk)
+    for (const auto& G : xp.gs)
+      LH_DISPATCH_PLAN_SMALL(G.len, G.plan, launch_xib_jobs_t, double, st, xp.djobs + G.job0[k], xp.dmap + G.map0, G.nblk)
+  }
+  void launch_zfwd(int k, hipStream_t st) const {
+    for (const auto& G : zf.gs)
+      LH_DISPATCH_PLAN_SMALL(G.len, G.plan, launch_zfwd_jobs_t, double, st, zf.djobs + G.job0[k], zf.dmap + G.map0, G.nblk)
+  }
+  void launch(int k, hipStream_t st) const {
+    launch_rhs(k, st);
+    launch_zinv(k, st);
+    launch_x(k, st);
+    launch_zfwd(k, st);
+  }
+};
+// (re)build a small channel's own canopy tables: at creation and whenever its plans change
+static hipError_t refresh_ib(lh_handle* h) {
+  if (!ib_jobs(h)) return hipSuccess;
+  if (!h->ib) h->ib = new IbSets();
+  return h->ib->build(std::vector<lh_handle*>{h});
+}
+
 static void issue_substep(lh_handle* h, int krk /*0-based*/, bool cfl_mode, int stages = kStageAll) {
   const Dims& dm = h->dm;
   const double bre = h->rk[3 + krk] / dm.re;
@@ -683,7 +883,14 @@ static void issue_substep(lh_handle* h, int krk /*0-based*/, bool cfl_mode, int 
   }   // NL stage
   const RHS ra = make_rhs(h, krk);
   // ---- IB stage (FP64 transforms): canopy planes of rhs u, w, v and Lu_y on tip planes
-  if ((stages & kStageIB) && h->nibent > 0) {
+  if ((stages & kStageIB) && ib_jobs(h) && h->ib) {
+    const double z64 = 16.0 * dm.pz * dm.mx;
+    const double nib = 2.0 * h->nibu + h->nibv;
+    if (stages & kStageIBr) TIMED(h, LH_K_RHS, spl * 7.0 * nib, h->ib->launch_rhs(krk, h->st));
+    if (stages & kStageIBz) TIMED(h, LH_K_IBZ, (nib + 6.0) * (spl + z64), h->ib->launch_zinv(krk, h->st));
+    if (stages & kStageIBx) TIMED(h, LH_K_XIB, (2.0 * nib + 6.0) * z64, h->ib->launch_x(krk, h->st));
+    if (stages & kStageIBf) TIMED(h, LH_K_IBZ, nib * (spl + z64), h->ib->launch_zfwd(krk, h->st));
+  } else if ((stages & kStageIB) && h->nibent > 0) {
     const int npl = h->nibu > h->nibv ? h->nibu : h->nibv;
     // in: u (3 rows), p, Nu_old, Nu_new per canopy plane; out: rhs (+ Lu_y of the tips)
     if (stages & kStageIBr)
@@ -1042,6 +1249,7 @@ int lh_create(const lh_config* c, lh_handle** out) {
     HIPCHK(h, hipMemcpy(h->coljobs, jobs.data(), jobs.size() * sizeof(ColJob), hipMemcpyHostToDevice));
     HIPCHK(h, hipMemcpy(h->colmap, map.data(), map.size() * sizeof(int), hipMemcpyHostToDevice));
   }
+  HIPCHK(h, refresh_ib(h));
   // pressure LU factors, once (start.f90:628-630)
   {
     BUILDP a{h->dg3, h->dthdyu, h->dthdyv, h->yu, h->k2x, h->k2z};
@@ -1061,6 +1269,11 @@ int lh_destroy(lh_handle* h) {
   for (auto& g : h->gexec)
     if (g) (void)hipGraphExecDestroy(g);
   for (auto e : h->evpool) (void)hipEventDestroy(e);
+  if (h->ib) {
+    h->ib->release();
+    delete h->ib;
+    h->ib = nullptr;
+  }
   void* ptrs[] = {h->yu, h->yv, h->dthdyu, h->dthdyv, h->dyu2i, h->dyv2i, h->k1x, h->k2x, h->k1z,
                   h->k2z, h->scal, h->u1, h->u2, h->u3, h->p, h->nu1, h->nu2, h->nu3, h->nb1, h->nb2, h->nb3, h->r1,
                   h->r2, h->r3, h->psi, h->tipl, h->zu, h->zw, h->zv, h->zp, h->dg3, h->ayu, h->ayv, h->ctwork, h->masks, h->ibu_planes,
@@ -1415,6 +1628,7 @@ int lh_set_plans(lh_handle* h, int xplan, int zplan) {
   HIPCHK(h, hipStreamSynchronize(h->st));
   h->xplan = xplan;
   h->zplan = zplan;
+  HIPCHK(h, refresh_ib(h));
   drop_graphs(h);
   return LH_OK;
 }
@@ -1476,12 +1690,14 @@ int lh_autotune(lh_handle* h, int steps) {
     for (int c = 0; c < nxp * nzp && r == LH_OK; ++c) {
       h->xplan = (x00 + c / nzp) % nxp;
       h->zplan = (z00 + c % nzp) % nzp;
+      if (refresh_ib(h) != hipSuccess) { r = fail(h, LH_ERR_HIP, "lh_autotune: job table"); break; }
       float ms = 0.f;
       r = trial(&ms);
       if (ms < best_ms) { best_ms = ms; bx = h->xplan; bz = h->zplan; }
     }
     h->xplan = bx;
     h->zplan = bz;
+    if (refresh_ib(h) != hipSuccess && r == LH_OK) r = fail(h, LH_ERR_HIP, "lh_autotune: job table");
   }
   // candidates of each knob, the current value first (the x pass default comes from the
   // dispatch-round model, its neighbours bracket it)
@@ -1490,6 +1706,7 @@ int lh_autotune(lh_handle* h, int steps) {
                                      {x0, x0 / 2 > 0 ? x0 / 2 : 1, 2 * x0, 16},
                                      {z0, 1, 2, 4, 8}};
   for (int pass = 0; pass < 3 && r == LH_OK; ++pass) {
+    if (pass == 0 && (h->nibent == 0 || ib_jobs(h))) continue;   // no by-value canopy rhs launch
     int* knob = pass == 0 ? &h->col_bs : (pass == 1 ? &h->xraw_lc : &h->zasm_lc);
     int best = *knob;
     float best_ms = 1e30f;
@@ -1529,6 +1746,7 @@ struct lh_batch {
   ColJob* jobs = nullptr;       // [3][n] column-stage jobs of the captured graph
   int* colmap = nullptr;        // block -> member of the column launches
   int nblk = 0;
+  IbSets ib;                    // canopy-stage job tables of the members with small grids
   bool joined = false;          // the handles count this batch (in_batch)
   std::string err;
 };
@@ -1600,9 +1818,12 @@ int lh_batch_step_cfl(lh_batch* b, double cfl) {
         !chk(hipMemcpy(b->colmap, colmap.data(), colmap.size() * sizeof(int), hipMemcpyHostToDevice), "map upload"))
       return LH_ERR_HIP;
     b->nblk = nblk;
-    // Per substep: each member's nonlinear and canopy stages on its own stream (parallel
-    // branches), then, joined on the master stream, the column stage of all members in one
-    // launch each of k_prep, the velocity sweeps and the pressure step.
+    if (!chk(b->ib.build(b->hs), "canopy job tables")) return LH_ERR_HIP;
+    // Per substep: each member's nonlinear stage on its own stream (parallel branches), with
+    // the canopy stage of the members with large grids; joined on the master stream, the
+    // canopy stage of the members with small grids in one launch per kernel and plan, then the
+    // column stage of all members in one launch each of k_prep, the velocity sweeps and the
+    // pressure step.
     if (!chk(hipStreamBeginCapture(b->master, hipStreamCaptureModeGlobal), "capture")) return LH_ERR_HIP;
     // Within a substep the members' launches are issued launch by launch (each launch for
     // every member, then the next), so that consecutive nodes of the graph are independent.
@@ -1616,13 +1837,15 @@ int lh_batch_step_cfl(lh_batch* b, double cfl) {
         if (k == 0) hipLaunchKernelGGL(k_set_scalar, dim3(1), dim3(1), 0, h->st, h->scal, (int)S_CFL, cfl);
       }
       for (int ph : phases)
-        for (int i = 0; i < n && ok; ++i) issue_substep(b->hs[i], k, true, ph);
+        for (int i = 0; i < n && ok; ++i)
+          if (!((ph & kStageIB) && ib_jobs(b->hs[i]))) issue_substep(b->hs[i], k, true, ph);
       for (int i = 0; i < n && ok; ++i) {
         lh_handle* h = b->hs[i];
         hipEvent_t join = b->ev[3 + (size_t)k * n + i];
         ok = chk(hipEventRecord(join, h->st), "join");
         ok = ok && chk(hipStreamWaitEvent(b->master, join, 0), "join wait");
       }
+      b->ib.launch(k, b->master);
       const ColJob* jk = b->jobs + (size_t)k * n;
       hipLaunchKernelGGL(k_prep, dim3(n), dim3(64), 0, b->master, jk);
       if (nsmall > 0)
@@ -1672,6 +1895,7 @@ int lh_batch_destroy(lh_batch* b) {
   if (b->g) (void)hipGraphExecDestroy(b->g);
   if (b->jobs) (void)hipFree(b->jobs);
   if (b->colmap) (void)hipFree(b->colmap);
+  b->ib.release();
   for (auto e : b->ev)
     if (e) (void)hipEventDestroy(e);
   if (b->master) (void)hipStreamDestroy(b->master);

@@ -127,29 +127,53 @@ __device__ __forceinline__ double2 rhs_at(const RHS& a, const Dims& dm, const Rh
 }
 
 // rhs on the canopy planes (grid: columns x planes x component) and Lu_y on the tip planes
-__global__ void __launch_bounds__(256) k_rhs_ib(RHS a, Dims dm) {
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  const int f = blockIdx.z;
-  const int q = blockIdx.y;
+// The component is a template argument: picked at run time among the fields of the RHS
+// argument, the buffers would keep the whole argument in per-lane scratch memory.
+template <int f>
+__device__ __forceinline__ void rhs_ib_f(const RHS& a, const Dims& dm, int c, int q) {
   const int ncol = dm.ncol;
   if (c >= ncol || q >= (f < 2 ? a.np_u : a.np_v)) return;
-  const int s = f < 2 ? a.planes_u[q] : a.planes_v[q];
-  const double2* __restrict__ uu = f == 0 ? a.u1 : (f == 1 ? a.u3 : a.u2);
-  const double2* __restrict__ no = f == 0 ? a.no1 : (f == 1 ? a.no3 : a.no2);
-  const double2* __restrict__ nn = f == 0 ? a.nn1 : (f == 1 ? a.nn3 : a.nn2);
-  double2* __restrict__ rr = f == 0 ? a.r1 : (f == 1 ? a.r3 : a.r2);
+  const int s = gp(f < 2 ? a.planes_u : a.planes_v)[q];
+  const G2 uu = g2(f == 0 ? a.u1 : (f == 1 ? a.u3 : a.u2));
+  const G2 no = g2(f == 0 ? a.no1 : (f == 1 ? a.no3 : a.no2));
+  const G2 nn = g2(f == 0 ? a.nn1 : (f == 1 ? a.nn3 : a.nn2));
+  const G2 rr = g2(f == 0 ? a.r1 : (f == 1 ? a.r3 : a.r2));
+  const G2 pp = g2(a.p);
   const RhsCol k = rhs_col(a, dm, c, f);
   const size_t o = (size_t)s * ncol + c;
   const double2 um = uu[o - ncol], u0 = uu[o], up = uu[o + ncol];
-  const double2 pl = a.p[o - ncol], pr = f == 2 ? a.p[o] : d2(0.0, 0.0);
-  const double2 vo = a.C2 != 0.0 ? no[o] : d2(0.0, 0.0);   // C2 = -cRK is 0 in substep 1
+  const double2 pl = pp[o - ncol], pr = f == 2 ? (double2)pp[o] : d2(0.0, 0.0);
+  const double2 vo = a.C2 != 0.0 ? (double2)no[o] : d2(0.0, 0.0);   // C2 = -cRK is 0 in substep 1
   rr[o] = rhs_at(a, dm, k, f, s, um, u0, up, pl, pr, vo, nn[o]);
   const bool tip = f < 2 ? (s == a.tip_u0 || s == a.tip_u1) : (s == a.tip_v0 || s == a.tip_v1);
   if (tip) {
-    const double* wy = (f < 2 ? a.dyu2i : a.dyv2i) + 3 * s;
+    const gptr<const double> wy = gp(f < 2 ? a.dyu2i : a.dyv2i) + 3 * s;
     const int tipr = f < 2 ? 2 * f + (s == a.tip_u0 ? 0 : 1) : 4 + (s == a.tip_v0 ? 0 : 1);
-    a.tipl[(size_t)tipr * ncol + c] = dadd(dadd(drm(wy[0], um), drm(wy[1], u0)), drm(wy[2], up));
+    g2(a.tipl)[(size_t)tipr * ncol + c] = dadd(dadd(drm(wy[0], um), drm(wy[1], u0)), drm(wy[2], up));
   }
+}
+// f: 0 u1, 1 u3, 2 u2 (uniform over the block)
+__device__ __forceinline__ void rhs_ib_body(const RHS& a, const Dims& dm, int c, int q, int f) {
+  if (f == 0) rhs_ib_f<0>(a, dm, c, q);
+  else if (f == 1) rhs_ib_f<1>(a, dm, c, q);
+  else rhs_ib_f<2>(a, dm, c, q);
+}
+__global__ void __launch_bounds__(256) k_rhs_ib(RHS a, Dims dm) {
+  rhs_ib_body(a, dm, blockIdx.x * blockDim.x + threadIdx.x, blockIdx.y, blockIdx.z);
+}
+// the canopy rhs from a job table: 64 columns per block, a channel's grid is gx column blocks
+// x npl planes x 3 components
+struct RhsIbJob {
+  RHS a;
+  Dims dm;
+  int blk0, gx, gy;
+};
+__global__ void __launch_bounds__(64) k_rhs_ib_jobs(const RhsIbJob* __restrict__ jt, const int* __restrict__ map) {
+  const RhsIbJob& J = jt[map[blockIdx.x]];
+  const int lb = blockIdx.x - J.blk0;
+  const RHS a = J.a;
+  const Dims dm = J.dm;
+  rhs_ib_body(a, dm, (lb % J.gx) * 64 + threadIdx.x, (lb / J.gx) % J.gy, lb / (J.gx * J.gy));
 }
 
 // solveU/solveV with the rhs built on the fly: forward elimination of LUsolU/LUsolV

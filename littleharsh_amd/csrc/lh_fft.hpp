@@ -19,9 +19,32 @@ template <typename T> struct cvec;
 template <> struct cvec<double> { using type = double2; };
 template <> struct cvec<float> { using type = float2; };
 template <typename T> using cx = typename cvec<T>::type;
+template <typename C> struct cscalar;
+template <> struct cscalar<double2> { using type = double; };
+template <> struct cscalar<float2> { using type = float; };
 
 template <typename C> __device__ __forceinline__ C mk(decltype(C::x) a, decltype(C::x) b) {
   C r; r.x = a; r.y = b; return r;
+}
+
+// A device buffer held as a global-address-space pointer.  A pointer that a kernel loads
+// from memory (a job table) is generic to the compiler, which then uses flat loads and
+// stores; through a gptr every access is a global one.
+template <typename T> using gptr = __attribute__((address_space(1))) T*;
+template <typename T> __device__ __forceinline__ gptr<T> gp(T* p) { return (gptr<T>)p; }
+// element i of a complex buffer in global memory, read or written as a two-element vector
+// (double2 and float2 themselves cannot be copied through a gptr)
+template <typename C> __device__ __forceinline__ C gld(const C* p, size_t i) {
+  typedef typename cscalar<C>::type S __attribute__((ext_vector_type(2)));
+  const S v = ((const __attribute__((address_space(1))) S*)p)[i];
+  return mk<C>(v.x, v.y);
+}
+template <typename C> __device__ __forceinline__ void gst(C* p, size_t i, C a) {
+  typedef typename cscalar<C>::type S __attribute__((ext_vector_type(2)));
+  S v;
+  v.x = a.x;
+  v.y = a.y;
+  ((__attribute__((address_space(1))) S*)p)[i] = v;
 }
 template <typename C> __device__ __forceinline__ C cadd(C a, C b) { return mk<C>(a.x + b.x, a.y + b.y); }
 template <typename C> __device__ __forceinline__ C csub(C a, C b) { return mk<C>(a.x - b.x, a.y - b.y); }
@@ -158,7 +181,7 @@ struct StageTw {
       const int b = t + q * T;
       const int j = (b < NB ? b : 0) % NS;
 #pragma unroll
-      for (int r = 1; r < R; ++r) w[q][r - 1] = tw[j * r * (N / (NS * R))];
+      for (int r = 1; r < R; ++r) w[q][r - 1] = gld(tw, j * r * (N / (NS * R)));
     }
   }
   __device__ __forceinline__ C get(int q, int r) const { return w[q][r - 1]; }

@@ -70,10 +70,9 @@ template <int PZ> constexpr int zls() { return PadLen<PZ, LH_Z_PAD>::value + LH_
 // remapped so that consecutive column groups of a plane run on one XCD (blocks are dealt
 // round-robin over the 8 XCDs).
 struct ZBlock { int cg, p, job; };
-__device__ __forceinline__ ZBlock z_block(int ncg, int maxn) {
-  int bid = blockIdx.x;
+__device__ __forceinline__ ZBlock z_block(int ncg, int maxn, int bid, int nblk) {
   if (LH_Z_XCD) {
-    const int nblk = gridDim.x, per = nblk / 8;
+    const int per = nblk / 8;
     if (bid < per * 8) bid = (bid % 8) * per + bid / 8;
   }
   ZBlock r;
@@ -129,21 +128,20 @@ __device__ __forceinline__ void z_last(const C* line, int b, int q, const W& w, 
 }
 
 // inverse: spectral plane (nz x mx), zero-padded to pz in k, inverse cft along z
-template <int PZ, int L, typename TR, int V = 0>
-__global__ void __launch_bounds__((L * Plan<PZ, V>::T))
-    __attribute__((amdgpu_waves_per_eu(sizeof(TR) == 4 && PZ >= 384 ? LH_Z_WPE32 : LH_Z_WPE)))
-    k_zinv(ZJobs jobs, Dims dm, const cx<TR>* __restrict__ tw) {
+template <int PZ, int L, typename TR, int V>
+__device__ __forceinline__ void zinv_body(const ZJobs& jobs, const Dims& dm, const cx<TR>* __restrict__ tw,
+                                          int bid, int nblk) {
   using C = cx<TR>;
   using Z = ZShape<PZ, V>;
   constexpr int LS = zls<PZ>();
   constexpr int PS = LH_Z_PAD;
   extern __shared__ char smem_raw[];
   C* lds = reinterpret_cast<C*>(smem_raw);
-  const ZBlock zb = z_block((dm.mx + L - 1) / L, jobs.maxn);
+  const ZBlock zb = z_block((dm.mx + L - 1) / L, jobs.maxn, bid, nblk);
   const ZJob jb = jobs.j[zb.job];
   const int p = zb.p;
   if (p >= jb.n) return;
-  const int plane = jb.planes ? jb.planes[p] : jb.first + p;
+  const int plane = jb.planes ? gp(jb.planes)[p] : jb.first + p;
   const int nz = dm.nz, mx = dm.mx, h = nz / 2;
   const int l = threadIdx.x % L, t = threadIdx.x / L;
   const int i = zb.cg * L + l;
@@ -164,7 +162,7 @@ __global__ void __launch_bounds__((L * Plan<PZ, V>::T))
         const int k = n < h ? n : n - (PZ - nz);
         v[r] = mk<C>(0, 0);
         if (ok && (n < h || k >= h)) {
-          const double2 s = src[(size_t)k * mx];
+          const double2 s = gld(src, (size_t)k * mx);
           v[r] = mk<C>((TR)s.x, (TR)s.y);
         }
       }
@@ -182,28 +180,27 @@ __global__ void __launch_bounds__((L * Plan<PZ, V>::T))
       z_last<PZ, V, true, PS>(line, b, q, w, v);
       if (ok) {
 #pragma unroll
-        for (int r = 0; r < Z::RL; ++r) dst[(size_t)(b + r * Z::NSL) * mx] = v[r];
+        for (int r = 0; r < Z::RL; ++r) gst(dst, (size_t)(b + r * Z::NSL) * mx, v[r]);
       }
     }
   }
 }
 
 // forward: Z slot -> forward cft along z, x 1/pz, truncation to nz modes -> spectral FP64
-template <int PZ, int L, typename TR, int V = 0>
-__global__ void __launch_bounds__((L * Plan<PZ, V>::T))
-    __attribute__((amdgpu_waves_per_eu(sizeof(TR) == 4 && PZ >= 384 ? LH_Z_WPE32 : LH_Z_WPE)))
-    k_zfwd(ZJobs jobs, Dims dm, const cx<TR>* __restrict__ tw) {
+template <int PZ, int L, typename TR, int V>
+__device__ __forceinline__ void zfwd_body(const ZJobs& jobs, const Dims& dm, const cx<TR>* __restrict__ tw,
+                                          int bid, int nblk) {
   using C = cx<TR>;
   using Z = ZShape<PZ, V>;
   constexpr int LS = zls<PZ>();
   constexpr int PS = LH_Z_PAD;
   extern __shared__ char smem_raw[];
   C* lds = reinterpret_cast<C*>(smem_raw);
-  const ZBlock zb = z_block((dm.mx + L - 1) / L, jobs.maxn);
+  const ZBlock zb = z_block((dm.mx + L - 1) / L, jobs.maxn, bid, nblk);
   const ZJob jb = jobs.j[zb.job];
   const int p = zb.p;
   if (p >= jb.n) return;
-  const int plane = jb.planes ? jb.planes[p] : jb.first + p;
+  const int plane = jb.planes ? gp(jb.planes)[p] : jb.first + p;
   const int nz = dm.nz, mx = dm.mx, h = nz / 2;
   const int l = threadIdx.x % L, t = threadIdx.x / L;
   const int i = zb.cg * L + l;
@@ -218,7 +215,7 @@ __global__ void __launch_bounds__((L * Plan<PZ, V>::T))
     if (Z::PER1 * Z::T == Z::NB1 || b < Z::NB1) {
       C v[Z::R1];
 #pragma unroll
-      for (int r = 0; r < Z::R1; ++r) v[r] = ok ? src[(size_t)(b + r * Z::NB1) * mx] : mk<C>(0, 0);
+      for (int r = 0; r < Z::R1; ++r) v[r] = ok ? gld(src, (size_t)(b + r * Z::NB1) * mx) : mk<C>(0, 0);
       stage_regs_store<PZ, Z::R1, 1, false, PS>(v, b, q, NoTw{}, line);
     }
   }
@@ -241,12 +238,55 @@ __global__ void __launch_bounds__((L * Plan<PZ, V>::T))
             double2 o;
             o.x = sz * (double)v[r].x;
             o.y = sz * (double)v[r].y;
-            dst[(size_t)k * mx] = o;
+            gst(dst, (size_t)k * mx, o);
           }
         }
       }
     }
   }
+}
+
+template <int PZ, int L, typename TR, int V = 0>
+__global__ void __launch_bounds__((L * Plan<PZ, V>::T))
+    __attribute__((amdgpu_waves_per_eu(sizeof(TR) == 4 && PZ >= 384 ? LH_Z_WPE32 : LH_Z_WPE)))
+    k_zinv(ZJobs jobs, Dims dm, const cx<TR>* __restrict__ tw) {
+  zinv_body<PZ, L, TR, V>(jobs, dm, tw, blockIdx.x, gridDim.x);
+}
+template <int PZ, int L, typename TR, int V = 0>
+__global__ void __launch_bounds__((L * Plan<PZ, V>::T))
+    __attribute__((amdgpu_waves_per_eu(sizeof(TR) == 4 && PZ >= 384 ? LH_Z_WPE32 : LH_Z_WPE)))
+    k_zfwd(ZJobs jobs, Dims dm, const cx<TR>* __restrict__ tw) {
+  zfwd_body<PZ, L, TR, V>(jobs, dm, tw, blockIdx.x, gridDim.x);
+}
+
+// Job tables: the launches of many small channels (lengths up to 144) as one launch per
+// kernel, length and plan.  Job j covers blocks blk0 .. blk0 + nblk - 1 of the launch; block
+// b works on the channel map[b] as block b - blk0 of that channel's own grid.  A single
+// small channel launches the same kernels with a table of its own, so that it computes
+// exactly what it computes inside a batch.
+struct ZJobT {
+  ZJobs jobs;
+  Dims dm;
+  const void* tw;
+  int blk0, nblk;
+};
+template <int PZ, int L, typename TR, int V = 0>
+__global__ void __launch_bounds__((L * Plan<PZ, V>::T)) k_zinv_jobs(const ZJobT* __restrict__ jt,
+                                                                     const int* __restrict__ map) {
+  const ZJobT& J = jt[map[blockIdx.x]];
+  const Dims dm = J.dm;
+  // the job list is read where it lies: a local copy, indexed at run time, would sit in
+  // per-lane scratch memory
+  zinv_body<PZ, L, TR, V>(J.jobs, dm, (const cx<TR>*)J.tw, blockIdx.x - J.blk0, J.nblk);
+}
+template <int PZ, int L, typename TR, int V = 0>
+__global__ void __launch_bounds__((L * Plan<PZ, V>::T)) k_zfwd_jobs(const ZJobT* __restrict__ jt,
+                                                                     const int* __restrict__ map) {
+  const ZJobT& J = jt[map[blockIdx.x]];
+  const Dims dm = J.dm;
+  // the job list is read where it lies: a local copy, indexed at run time, would sit in
+  // per-lane scratch memory
+  zfwd_body<PZ, L, TR, V>(J.jobs, dm, (const cx<TR>*)J.tw, blockIdx.x - J.blk0, J.nblk);
 }
 
 // ------------------------------------------------------------------------------------
@@ -295,9 +335,9 @@ struct XIB {
                                 // 1: other entries (two row pairs z0, z0+2 per block)
 };
 
-template <int PX, typename TR, int V = 0>
-__global__ void __launch_bounds__((2 * Plan<PX, V>::T))
-    k_xpass_ib(XIB a, Dims dm, const cx<TR>* __restrict__ tw) {
+template <int PX, typename TR, int V>
+__device__ __forceinline__ void xpass_ib_body(const XIB& a, const Dims& dm, const cx<TR>* __restrict__ tw,
+                                              int bx, int by) {
   using C = cx<TR>;
   using P = Plan<PX, V>;
   constexpr int T = P::T;
@@ -305,8 +345,14 @@ __global__ void __launch_bounds__((2 * Plan<PX, V>::T))
   extern __shared__ char smem_raw[];
   C* lds = reinterpret_cast<C*>(smem_raw);
   const bool paired = a.paired != 0;
-  const int z0 = (paired ? 4 : 2) * blockIdx.x;
-  const IBEntry en = a.ent[blockIdx.y];
+  const int z0 = (paired ? 4 : 2) * bx;
+  IBEntry en;
+  {
+    const gptr<const int> e4 = gp(reinterpret_cast<const int*>(a.ent + by));
+    en.slot = e4[0];
+    en.lslot = e4[1];
+    en.mask = e4[2];
+  }
   const int mx = dm.mx, pz = dm.pz;
   const size_t slot = (size_t)pz * mx;
   C* z = reinterpret_cast<C*>(a.z);
@@ -330,14 +376,14 @@ __global__ void __launch_bounds__((2 * Plan<PX, V>::T))
   {
     C va[MH], vb[MH];
 #pragma unroll
-    for (int q = 0; q < MW; ++q) mw[q] = mwp[min((int)threadIdx.x + q * NT, nw - 1)];
+    for (int q = 0; q < MW; ++q) mw[q] = gp(mwp)[min((int)threadIdx.x + q * NT, nw - 1)];
 #pragma unroll
     for (int q = 0; q < MH; ++q) {
       const int it = min((int)threadIdx.x + q * NT, 2 * mx - 1);
       const int ln = it / mx, k = it % mx;
       const C* R = ln ? B : A;
-      va[q] = R[k];
-      vb[q] = R[mx + k];
+      va[q] = gld(R, k);
+      vb[q] = gld(R, mx + k);
     }
     // position k holds a + ib, position px - k its mirror (rft_buff.f packing of two rows)
 #pragma unroll
@@ -363,7 +409,7 @@ __global__ void __launch_bounds__((2 * Plan<PX, V>::T))
   w.load(t, tw);                 // in flight across the barrier
   __syncthreads();
   P::template run<true>(lds + ln * LS, t, w);
-  const double beta = a.bre * a.scal[S_DT];
+  const double beta = a.bre * gp(a.scal)[S_DT];
   // mask word q covers bytes x0 .. x0 + 3 of block row r; row r is component r % 2 of
   // line r / 2 (paired: rhs = 0 there), or, on a tip plane (rows z0, z0 + 1 of line 0),
   // rhs = -beta Lu_y with Lu_y the same component of line 1.  Each thread writes single
@@ -393,19 +439,35 @@ __global__ void __launch_bounds__((2 * Plan<PX, V>::T))
     C va, vb;
     unpack_rows<PX>(lds + l2 * LS, k, va, vb);
     C* d = dst + (size_t)(2 * l2) * mx;
-    d[k] = va;
-    d[mx + k] = vb;
+    gst(d, k, va);
+    gst(d, mx + k, vb);
   }
+}
+template <int PX, typename TR, int V = 0>
+__global__ void __launch_bounds__((2 * Plan<PX, V>::T))
+    k_xpass_ib(XIB a, Dims dm, const cx<TR>* __restrict__ tw) {
+  xpass_ib_body<PX, TR, V>(a, dm, tw, blockIdx.x, blockIdx.y);
+}
+// the canopy x pass from a job table (a channel's grid: gx row-pair blocks x entries)
+struct XibJob {
+  XIB a;
+  Dims dm;
+  const void* tw;
+  int blk0, gx;
+};
+template <int PX, typename TR, int V = 0>
+__global__ void __launch_bounds__((2 * Plan<PX, V>::T)) k_xpass_ib_jobs(const XibJob* __restrict__ jt,
+                                                                          const int* __restrict__ map) {
+  const XibJob& J = jt[map[blockIdx.x]];
+  const int lb = blockIdx.x - J.blk0;
+  const XIB a = J.a;
+  const Dims dm = J.dm;
+  xpass_ib_body<PX, TR, V>(a, dm, (const cx<TR>*)J.tw, lb % J.gx, lb / J.gx);
 }
 
 __device__ __forceinline__ double2 d2(double a, double b) { double2 r; r.x = a; r.y = b; return r; }
-// A device buffer held as a global-address-space pointer.  The column kernels take their
-// buffers from a job table, and a pointer loaded from memory is generic to the compiler,
-// which then uses flat loads and stores; through a gptr every access is a global one.
-template <typename T> using gptr = __attribute__((address_space(1))) T*;
-template <typename T> __device__ __forceinline__ gptr<T> gp(T* p) { return (gptr<T>)p; }
-// the same for complex (double2) buffers: g2(p)[i] reads and writes element i as two doubles
-// through a global pointer (double2 itself cannot be copied through one)
+// Complex (double2) buffers through a global pointer (gptr): g2(p)[i] reads and writes
+// element i as two doubles (double2 itself cannot be copied through one)
 typedef double gd2v __attribute__((ext_vector_type(2)));
 struct G2ref {
   __attribute__((address_space(1))) gd2v* q;
