@@ -215,8 +215,7 @@ static int fail(lh_handle* h, int code, const std::string& msg) {
 // state at the start of the step with a reduction that keeps NaN, so a non-finite value
 // anywhere in u, v or w shows there.  With need_dt the device CFL time step must also be
 // finite and positive.  Reads back the device scalars into s.
-static int numeric_check(lh_handle* h, bool need_dt, double* s) {
-  HIPCHK(h, hipMemcpy(s, h->scal, S_NSCAL * sizeof(double), hipMemcpyDeviceToHost));
+static int check_scalars(lh_handle* h, bool need_dt, const double* s) {
   char msg[160];
   if (!std::isfinite(s[S_CFLMAX])) {
     snprintf(msg, sizeof msg, "non-finite velocity at the start of the step (CFL maximum %g)", s[S_CFLMAX]);
@@ -228,6 +227,10 @@ static int numeric_check(lh_handle* h, bool need_dt, double* s) {
     return fail(h, LH_ERR_NUMERIC, msg);
   }
   return LH_OK;
+}
+static int numeric_check(lh_handle* h, bool need_dt, double* s) {
+  HIPCHK(h, hipMemcpy(s, h->scal, S_NSCAL * sizeof(double), hipMemcpyDeviceToHost));
+  return check_scalars(h, need_dt, s);
 }
 
 template <typename T>
@@ -493,6 +496,9 @@ __global__ void k_add_forcing(double2* n1, double2* n2, double2* n3, const doubl
 // z, x, z) so that a batch can issue each launch for all its members in turn.
 constexpr int kStageNLz = 1, kStageIBr = 2, kStagePres = 4, kStageVelCol = 8, kStageNLx = 16,
               kStageNLa = 32, kStageIBz = 64, kStageIBx = 128, kStageIBf = 256;
+// with a stage: the caller clears the CFL maximum and forms the time step of a CFL step
+// (a batch does both for all its members in one launch each)
+constexpr int kCallerCfl = 512;
 constexpr int kStageNL = kStageNLz | kStageNLx | kStageNLa;
 constexpr int kStageIB = kStageIBr | kStageIBz | kStageIBx | kStageIBf;
 constexpr int kStageVel = kStageIB | kStageVelCol, kStageAll = kStageNL | kStageVel | kStagePres;
@@ -597,6 +603,28 @@ __global__ void __launch_bounds__(256) k_flowrate(const ColJob* __restrict__ job
   const ColJob& J = jobs[blockIdx.x];
   if (!J.flowrate) return;
   flowrate_body(J.ca.u1, J.yv, J.ca.scal, J.dm.ny, J.dm.d, J.dm.ncol, part);
+}
+// The scalars of the n channels of a job table, one thread per channel: the set-up of a CFL
+// step (the CFL number, and the maximum cleared for the x pass to accumulate), its time
+// step (the division k_cfl_dt makes), and a copy of every scalar into out[n][S_NSCAL].
+__global__ void k_jobs_cfl_begin(const ColJob* __restrict__ jobs, int n, double cfl) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const gptr<double> s = gp(jobs[i].ca.scal);
+  s[S_CFL] = cfl;
+  s[S_CFLMAX] = 0.0;
+}
+__global__ void k_jobs_cfl_dt(const ColJob* __restrict__ jobs, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const gptr<double> s = gp(jobs[i].ca.scal);
+  s[S_DT] = s[S_CFL] / s[S_CFLMAX];
+}
+__global__ void k_jobs_scalars(const ColJob* __restrict__ jobs, int n, double* __restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const gptr<const double> s = gp(jobs[i].ca.scal);
+  for (int q = 0; q < S_NSCAL; ++q) out[(size_t)i * S_NSCAL + q] = s[q];
 }
 
 static ColJob make_job(const lh_handle* h, int krk, int blk0) {
@@ -834,7 +862,7 @@ static void issue_substep(lh_handle* h, int krk /*0-based*/, bool cfl_mode, int 
     TIMED(h, LH_K_ZINV, nuvw * (spl + zsl), launch_zinv(h, jobs, 3, dm.nyu, false));
   }
   if (stages & kStageNLx) {
-  if (krk == 0)
+  if (krk == 0 && !(stages & kCallerCfl))
     TIMED(h, LH_K_MISC, 0.0,
           hipLaunchKernelGGL(k_set_scalar, dim3(1), dim3(1), 0, h->st, h->scal, (int)S_CFLMAX, 0.0));
   {
@@ -849,7 +877,7 @@ static void issue_substep(lh_handle* h, int krk /*0-based*/, bool cfl_mode, int 
       }
     });
   }
-  if (krk == 0 && cfl_mode)
+  if (krk == 0 && cfl_mode && !(stages & kCallerCfl))
     TIMED(h, LH_K_MISC, 0.0, hipLaunchKernelGGL(k_cfl_dt, dim3(1), dim3(1), 0, h->st, h->scal));
   if (side && late) fork_prep();
   }
@@ -1747,6 +1775,7 @@ struct lh_batch {
   int* colmap = nullptr;        // block -> member of the column launches
   int nblk = 0;
   IbSets ib;                    // canopy-stage job tables of the members with small grids
+  double* dscal = nullptr;      // [n][S_NSCAL] the members' scalars after a step
   bool joined = false;          // the handles count this batch (in_batch)
   std::string err;
 };
@@ -1819,6 +1848,8 @@ int lh_batch_step_cfl(lh_batch* b, double cfl) {
       return LH_ERR_HIP;
     b->nblk = nblk;
     if (!chk(b->ib.build(b->hs), "canopy job tables")) return LH_ERR_HIP;
+    if (!b->dscal && !chk(hipMalloc((void**)&b->dscal, (size_t)n * S_NSCAL * sizeof(double)), "scalars"))
+      return LH_ERR_HIP;
     // Per substep: each member's nonlinear stage on its own stream (parallel branches), with
     // the canopy stage of the members with large grids; joined on the master stream, the
     // canopy stage of the members with small grids in one launch per kernel and plan, then the
@@ -1829,22 +1860,28 @@ int lh_batch_step_cfl(lh_batch* b, double cfl) {
     // every member, then the next), so that consecutive nodes of the graph are independent.
     static const int phases[] = {kStageNLz, kStageNLx, kStageNLa, kStageIBr, kStageIBz, kStageIBx, kStageIBf};
     bool ok = true;
+    const int sb = (n + 63) / 64;
     for (int k = 0; k < 3 && ok; ++k) {
+      // the CFL number and the cleared maximum of every member, before the fork
+      if (k == 0) hipLaunchKernelGGL(k_jobs_cfl_begin, dim3(sb), dim3(64), 0, b->master, (const ColJob*)b->jobs, n, cfl);
       ok = chk(hipEventRecord(b->ev[k], b->master), "fork");
-      for (int i = 0; i < n && ok; ++i) {
-        lh_handle* h = b->hs[i];
-        ok = chk(hipStreamWaitEvent(h->st, b->ev[k], 0), "fork wait");
-        if (k == 0) hipLaunchKernelGGL(k_set_scalar, dim3(1), dim3(1), 0, h->st, h->scal, (int)S_CFL, cfl);
-      }
+      for (int i = 0; i < n && ok; ++i)
+        ok = chk(hipStreamWaitEvent(b->hs[i]->st, b->ev[k], 0), "fork wait");
+      // members with small grids leave the time step to the launch after the join (nothing
+      // of theirs before it needs dt); the others form it on their branch, before their
+      // canopy stage
       for (int ph : phases)
-        for (int i = 0; i < n && ok; ++i)
-          if (!((ph & kStageIB) && ib_jobs(b->hs[i]))) issue_substep(b->hs[i], k, true, ph);
+        for (int i = 0; i < n && ok; ++i) {
+          lh_handle* h = b->hs[i];
+          if (!((ph & kStageIB) && ib_jobs(h))) issue_substep(h, k, true, ph | (small_channel(h) ? kCallerCfl : 0));
+        }
       for (int i = 0; i < n && ok; ++i) {
         lh_handle* h = b->hs[i];
         hipEvent_t join = b->ev[3 + (size_t)k * n + i];
         ok = chk(hipEventRecord(join, h->st), "join");
         ok = ok && chk(hipStreamWaitEvent(b->master, join, 0), "join wait");
       }
+      if (k == 0) hipLaunchKernelGGL(k_jobs_cfl_dt, dim3(sb), dim3(64), 0, b->master, (const ColJob*)b->jobs, n);
       b->ib.launch(k, b->master);
       const ColJob* jk = b->jobs + (size_t)k * n;
       hipLaunchKernelGGL(k_prep, dim3(n), dim3(64), 0, b->master, jk);
@@ -1858,6 +1895,7 @@ int lh_batch_step_cfl(lh_batch* b, double cfl) {
                          (const int*)b->colmap);
     }
     hipLaunchKernelGGL(k_flowrate, dim3(n), dim3(256), 0, b->master, (const ColJob*)b->jobs);
+    hipLaunchKernelGGL(k_jobs_scalars, dim3(sb), dim3(64), 0, b->master, (const ColJob*)b->jobs, n, b->dscal);
     const bool ended = chk(hipStreamEndCapture(b->master, &g), "end capture");
     if (!ok || !ended) return LH_ERR_HIP;
     if (!chk(hipGraphInstantiate(&b->g, g, nullptr, nullptr, 0), "instantiate")) {
@@ -1873,9 +1911,14 @@ int lh_batch_step_cfl(lh_batch* b, double cfl) {
     b->err = "batch graph launch failed";
     return LH_ERR_HIP;
   }
+  // every member's scalars in one copy
+  std::vector<double> sc(b->hs.size() * S_NSCAL);
+  if (hipMemcpy(sc.data(), b->dscal, sc.size() * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess) {
+    b->err = "batch scalars: copy failed";
+    return LH_ERR_HIP;
+  }
   for (size_t i = 0; i < b->hs.size(); ++i) {
-    double s[S_NSCAL];
-    const int r = numeric_check(b->hs[i], true, s);
+    const int r = check_scalars(b->hs[i], true, sc.data() + i * S_NSCAL);
     if (r != LH_OK) {
       b->err = "instance " + std::to_string(i) + ": " + b->hs[i]->err;
       return r;
@@ -1895,6 +1938,7 @@ int lh_batch_destroy(lh_batch* b) {
   if (b->g) (void)hipGraphExecDestroy(b->g);
   if (b->jobs) (void)hipFree(b->jobs);
   if (b->colmap) (void)hipFree(b->colmap);
+  if (b->dscal) (void)hipFree(b->dscal);
   b->ib.release();
   for (auto e : b->ev)
     if (e) (void)hipEventDestroy(e);
