@@ -219,10 +219,14 @@ int lh_autotune(lh_handle* h, int steps);
 
 /* Many independent channels (handles of any sizes on one device) advanced together: one
  * CFL step of every instance per call, issued as one captured HIP graph (no counterpart in
- * the reference; the instances share nothing).  Per substep, the nonlinear and canopy
- * stages of the instances are parallel branches, each on its handle's stream; the column
- * stage (LU of the velocity matrices, velocity sweeps, pressure step) of all instances is one
- * launch of each kernel.  A member computes exactly what it computes stepped alone. */
+ * the reference; the instances share nothing).  Per substep, the nonlinear stages of the
+ * instances are parallel branches, each on its handle's stream (with the canopy stage of
+ * instances whose x or z length exceeds 144); after them the canopy stage of the other
+ * instances is one launch per kernel and (length, FFT plan) group, and the column stage
+ * (LU of the velocity matrices, velocity sweeps, pressure step) of all instances one launch
+ * of each kernel.  A member computes exactly what it computes stepped alone.  After the
+ * step every member's scalars are checked as by lh_step_cfl; the error names the first
+ * member that failed. */
 /* A batch re-captures its graph when the CFL number changes or when any member's own graphs
  * were dropped (lh_set_launch, lh_set_plans, lh_autotune, lh_set_forcing switching the forcing on or off,
  * lh_closure_create); each handle may appear once.  Destroy the batch before its members. */
