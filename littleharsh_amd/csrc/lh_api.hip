@@ -361,6 +361,18 @@ static hipError_t attr_xw_t() {
   return hipFuncSetAttribute((const void*)k_xw576<W, false, TR>, hipFuncAttributeMaxDynamicSharedMemorySize,
                              (int)xw_lds_bytes<W, TR>());
 }
+// the canopy x pass at 576 points: one wave per (entry, row pair), LH_XIBW_WAVES per block
+// (12: three per SIMD at 132 VGPRs, 121 KB of LDS; 16 would spill)
+#ifndef LH_XIBW_WAVES
+#define LH_XIBW_WAVES 12
+#endif
+static size_t xibw_lds_bytes() { return sizeof(double2) * ((size_t)xw::TWN + (size_t)LH_XIBW_WAVES * xw::LINE); }
+static void launch_xib576(lh_handle* h, const XIB& a, int nent) {
+  constexpr int W = LH_XIBW_WAVES;
+  const long waves = (long)nent * (h->dm.pz / 2);
+  hipLaunchKernelGGL((k_xpass_ib576<W>), dim3((unsigned)((waves + W - 1) / W)), dim3(64 * W), xibw_lds_bytes(),
+                     h->st, a, h->dm, nent, (const double2*)h->twxw);
+}
 constexpr int kZasmCols = LH_ZASM_COLS;
 template <int PZ, typename TR, int V>
 static size_t zasm_lds() {
@@ -942,9 +954,14 @@ static void issue_substep(lh_handle* h, int krk /*0-based*/, bool cfl_mode, int 
     XIB b{h->zp, h->ibent + h->nibtip, h->masks, h->scal, bre, 1};
     if (stages & kStageIBx)
     TIMED(h, LH_K_XIB, (2.0 * nib + 6.0) * z64, {
-      if (h->nibtip > 0) LH_DISPATCH_PLAN(dm.px, h->xplan, launch_xib_t, double, h, a, h->nibtip, h->twx64)
-      if (h->nibent > h->nibtip)
-        LH_DISPATCH_PLAN(dm.px, h->xplan, launch_xib_t, double, h, b, h->nibent - h->nibtip, h->twx64)
+      if (use_xw(h)) {
+        if (h->nibtip > 0) launch_xib576(h, a, h->nibtip);
+        if (h->nibent > h->nibtip) launch_xib576(h, b, h->nibent - h->nibtip);
+      } else {
+        if (h->nibtip > 0) LH_DISPATCH_PLAN(dm.px, h->xplan, launch_xib_t, double, h, a, h->nibtip, h->twx64)
+        if (h->nibent > h->nibtip)
+          LH_DISPATCH_PLAN(dm.px, h->xplan, launch_xib_t, double, h, b, h->nibent - h->nibtip, h->twx64)
+      }
     });
     ZJobs fj{};
     slot = 0;
@@ -1072,6 +1089,8 @@ int lh_create(const lh_config* c, lh_handle** out) {
       // against nine transforms per level
       const bool f64 = h->prec == LH_PREC_FP64;
       HIPCHK(h, f64 ? attr_xw_t<double>() : attr_xw_t<float>());
+      HIPCHK(h, hipFuncSetAttribute((const void*)k_xpass_ib576<LH_XIBW_WAVES>,
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)xibw_lds_bytes()));
       const int wpb = f64 ? XwWaves<double>::value : XwWaves<float>::value;
       if (h->xraw_lc <= 0)
         h->xraw_lc = chunk_levels(dm.nyu - 2, (long)dm.px / 2, ncu * wpb, 4, 96, 1.0 / 3.0);

@@ -469,5 +469,75 @@ __global__ void __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(x
   }
 }
 
+// Canopy x pass at 576 points, one wavefront per (entry, row pair): the rhs rows (z0, z0+1)
+// of a canopy plane to physical space, rhs = 0 on the canopy points of a plane below the
+// tips (XIB::paired = 1) or -beta Lu_y there on a tip plane (paired = 0, the Lu_y rows of
+// the entry's lslot transformed alongside), then back to Fourier and the two rows separated
+// (tridLU_3D_IB.f90:224-245, 306-325; the operations of k_xpass_ib on the transforms of
+// k_xw576).  After inv576 lane t holds x = (t>>3) + 9 ((t&7) + 8 s) in slot s < 8 and
+// x = 8 + 9 bitrev6(t) in slot 8, which is where its mask bytes are read.
+template <int WPB>
+__global__ void __launch_bounds__(64 * WPB) k_xpass_ib576(XIB a, Dims dm, int nent,
+                                                          const double2* __restrict__ twg) {
+  using C = double2;
+  extern __shared__ char smem_raw[];
+  C* lds = reinterpret_cast<C*>(smem_raw);
+  C* tw = lds;                                   // twiddle table, shared by the block
+  for (int e = threadIdx.x; e < xw::TWN; e += blockDim.x) tw[e] = twg[e];
+  __syncthreads();
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int t = threadIdx.x & 63;
+  C* buf = lds + xw::TWN + wv * xw::LINE;
+  const int npair = dm.pz / 2;
+  const int item = blockIdx.x * WPB + wv;
+  if (item >= nent * npair) return;              // whole waves (no barrier follows)
+  const int e = item / npair, z0 = 2 * (item % npair);
+  const IBEntry en = a.ent[e];
+  const size_t slot = (size_t)dm.pz * xw::MX;
+  C* z = reinterpret_cast<C*>(a.z);
+  C* const row = z + en.slot * slot + (size_t)z0 * xw::MX;
+  const bool tip = a.paired == 0;
+  C v[9], w[9];
+  xw::load_line(v, row, t);
+  xw::inv576(v, t, buf, tw);
+  if (tip) {
+    xw::load_line(w, z + en.lslot * slot + (size_t)z0 * xw::MX, t);
+    xw::inv576(w, t, buf, tw);
+  }
+  // canopy points: mask bytes of rows z0 (real parts) and z0 + 1 (imaginary parts)
+  const unsigned char* m0 = a.masks + ((size_t)en.mask * dm.pz + z0) * xw::N;
+  const double beta = a.bre * a.scal[S_DT];
+  const int n1 = t >> 3, m2 = t & 7;
+  const int x8 = 8 + 9 * (int)(__brev((unsigned)t) >> 26);
+#pragma unroll
+  for (int s = 0; s < 9; ++s) {
+    const int x = s < 8 ? n1 + 9 * (m2 + 8 * s) : x8;
+    const bool c0 = m0[x] != 0, c1 = m0[xw::N + x] != 0;
+    if (c0) v[s].x = tip ? -beta * w[s].x : 0.0;
+    if (c1) v[s].y = tip ? -beta * w[s].y : 0.0;
+  }
+  xw::fwd576(v, t, buf, tw);                     // v[r] = P[t + 64 r]
+  // rows z0, z0+1 from P[k] and P[576 - k], x 1/576, k = t + 64 r < 193: P[576 - k] is
+  // slot 8 - r of lane 64 - t, or, in lane 0, slot 9 - r (P[0] itself for k = 0)
+  const double sx = 1.0 / (double)xw::N;
+  auto put = [&](int k, C zk, C zn) {
+    C ra = mk<C>(0.5 * (zk.x + zn.x), 0.5 * (zk.y - zn.y));
+    C rb = mk<C>(0.5 * (zk.y + zn.y), -0.5 * (zk.x - zn.x));
+    ra = mk<C>(sx * ra.x, sx * ra.y);
+    rb = mk<C>(sx * rb.x, sx * rb.y);
+    if (k == 0) { ra.y = 0; rb.y = 0; }
+    row[k] = ra;
+    row[xw::MX + k] = rb;
+  };
+  const int src = (64 - t) & 63;
+#pragma unroll
+  for (int r = 0; r < 3; ++r) {
+    C zn = mk<C>(__shfl(v[8 - r].x, src, 64), __shfl(v[8 - r].y, src, 64));
+    if (t == 0) zn = r == 0 ? v[0] : v[9 - r];
+    put(t + 64 * r, v[r], zn);
+  }
+  if (t == 0) put(192, v[3], v[6]);
+}
+
 
 }  // namespace lh
