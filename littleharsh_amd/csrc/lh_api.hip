@@ -665,41 +665,45 @@ static int zlines_of(int n, int v) {
   return l;
 }
 
-// Groups of one kernel: channels that share (length, plan); per group a job table per
-// substep (nk = 3) or one for all substeps (nk = 1), and one block map.
+// Groups of one kernel: job units that share (length, plan), a unit being one job of a
+// channel (a channel has nu of them, u = 0 .. nu-1); per group a job table per substep
+// (nk = 3) or one for all substeps (nk = 1), and one block map.
 template <typename J>
 struct GroupSet {
-  struct G { int len, plan; std::vector<int> mem; size_t job0[3]; size_t map0; int nblk; };
+  struct Unit { lh_handle* h; int u; };
+  struct G { int len, plan; std::vector<Unit> mem; size_t job0[3]; size_t map0; int nblk; };
   std::vector<G> gs;
   J* djobs = nullptr;
   int* dmap = nullptr;
-  // key(h, &len, &plan) -> whether the channel takes part; blocks(h); make(h, k, blk0)
+  // key(h, u, &len, &plan) -> whether unit u of the channel takes part; blocks(h, u);
+  // make(h, u, k, blk0)
   template <class Key, class Blocks, class Make>
-  hipError_t build(const std::vector<lh_handle*>& hs, int nk, Key key, Blocks blocks, Make make) {
+  hipError_t build(const std::vector<lh_handle*>& hs, int nu, int nk, Key key, Blocks blocks, Make make) {
     release();
-    for (int i = 0; i < (int)hs.size(); ++i) {
-      int len = 0, plan = 0;
-      if (!key(hs[i], &len, &plan)) continue;
-      size_t g = 0;
-      while (g < gs.size() && !(gs[g].len == len && gs[g].plan == plan)) ++g;
-      if (g == gs.size()) gs.push_back(G{len, plan, {}, {0, 0, 0}, 0, 0});
-      gs[g].mem.push_back(i);
-    }
+    for (int i = 0; i < (int)hs.size(); ++i)
+      for (int u = 0; u < nu; ++u) {
+        int len = 0, plan = 0;
+        if (!key(hs[i], u, &len, &plan)) continue;
+        size_t g = 0;
+        while (g < gs.size() && !(gs[g].len == len && gs[g].plan == plan)) ++g;
+        if (g == gs.size()) gs.push_back(G{len, plan, {}, {0, 0, 0}, 0, 0});
+        gs[g].mem.push_back(Unit{hs[i], u});
+      }
     std::vector<J> jobs;
     std::vector<int> map;
     for (auto& G : gs) {
       for (int k = 0; k < nk; ++k) {
         G.job0[k] = jobs.size();
         int blk = 0;
-        for (int i : G.mem) {
-          jobs.push_back(make(hs[i], k, blk));
-          blk += blocks(hs[i]);
+        for (const Unit& m : G.mem) {
+          jobs.push_back(make(m.h, m.u, k, blk));
+          blk += blocks(m.h, m.u);
         }
         G.nblk = blk;
       }
       for (int k = nk; k < 3; ++k) G.job0[k] = G.job0[0];
       G.map0 = map.size();
-      for (size_t j = 0; j < G.mem.size(); ++j) map.insert(map.end(), blocks(hs[G.mem[j]]), (int)j);
+      for (size_t j = 0; j < G.mem.size(); ++j) map.insert(map.end(), blocks(G.mem[j].h, G.mem[j].u), (int)j);
     }
     if (jobs.empty()) return hipSuccess;
     hipError_t e = hipMalloc((void**)&djobs, jobs.size() * sizeof(J));
@@ -767,47 +771,48 @@ static int zgrid_cols(const lh_handle* h) { return (h->dm.mx + zlines_of(h->dm.p
 struct IbSets {
   GroupSet<RhsIbJob> ri;
   GroupSet<ZJobT> zi, zf;
-  GroupSet<XibJob> xt, xp;
+  GroupSet<XibJob> xb;
   hipError_t build(const std::vector<lh_handle*>& hs) {
     hipError_t e = ri.build(
-        hs, 3, [](const lh_handle* h, int* l, int* v) { *l = 0; *v = 0; return ib_jobs(h); },
-        [](const lh_handle* h) { return ((h->dm.ncol + 63) / 64) * ib_npl(h) * 3; },
-        [](const lh_handle* h, int k, int blk0) {
+        hs, 1, 3, [](const lh_handle* h, int, int* l, int* v) { *l = 0; *v = 0; return ib_jobs(h); },
+        [](const lh_handle* h, int) { return ((h->dm.ncol + 63) / 64) * ib_npl(h) * 3; },
+        [](const lh_handle* h, int, int k, int blk0) {
           return RhsIbJob{make_rhs(h, k), h->dm, blk0, (h->dm.ncol + 63) / 64, ib_npl(h)};
         });
     if (e == hipSuccess)
       e = zi.build(
-          hs, 1, [](const lh_handle* h, int* l, int* v) { *l = h->dm.pz; *v = h->zplan; return ib_jobs(h); },
-          [](const lh_handle* h) { return zgrid_cols(h) * ib_maxn(h) * 4; },
-          [](const lh_handle* h, int, int blk0) {
+          hs, 1, 1, [](const lh_handle* h, int, int* l, int* v) { *l = h->dm.pz; *v = h->zplan; return ib_jobs(h); },
+          [](const lh_handle* h, int) { return zgrid_cols(h) * ib_maxn(h) * 4; },
+          [](const lh_handle* h, int, int, int blk0) {
             return ZJobT{ib_zinv_jobs(h), h->dm, h->twz64, blk0, zgrid_cols(h) * ib_maxn(h) * 4};
           });
     if (e == hipSuccess)
       e = zf.build(
-          hs, 1, [](const lh_handle* h, int* l, int* v) { *l = h->dm.pz; *v = h->zplan; return ib_jobs(h); },
-          [](const lh_handle* h) { return zgrid_cols(h) * ib_npl(h) * 3; },
-          [](const lh_handle* h, int, int blk0) {
+          hs, 1, 1, [](const lh_handle* h, int, int* l, int* v) { *l = h->dm.pz; *v = h->zplan; return ib_jobs(h); },
+          [](const lh_handle* h, int) { return zgrid_cols(h) * ib_npl(h) * 3; },
+          [](const lh_handle* h, int, int, int blk0) {
             return ZJobT{ib_zfwd_jobs(h), h->dm, h->twz64, blk0, zgrid_cols(h) * ib_npl(h) * 3};
           });
+    // x pass: unit 0 the tip entries (one row pair per block), unit 1 the others (two), in one
+    // launch per group
     if (e == hipSuccess)
-      e = xt.build(
-          hs, 3,
-          [](const lh_handle* h, int* l, int* v) { *l = h->dm.px; *v = h->xplan; return ib_jobs(h) && h->nibtip > 0; },
-          [](const lh_handle* h) { return (h->dm.pz / 2) * h->nibtip; },
-          [](const lh_handle* h, int k, int blk0) { return XibJob{ib_xib(h, k, 0), h->dm, h->twx64, blk0, h->dm.pz / 2}; });
-    if (e == hipSuccess)
-      e = xp.build(
-          hs, 3,
-          [](const lh_handle* h, int* l, int* v) {
-            *l = h->dm.px; *v = h->xplan; return ib_jobs(h) && h->nibent > h->nibtip;
+      e = xb.build(
+          hs, 2, 3,
+          [](const lh_handle* h, int u, int* l, int* v) {
+            *l = h->dm.px; *v = h->xplan;
+            return ib_jobs(h) && (u == 0 ? h->nibtip > 0 : h->nibent > h->nibtip);
           },
-          [](const lh_handle* h) { return (h->dm.pz / 4) * (h->nibent - h->nibtip); },
-          [](const lh_handle* h, int k, int blk0) { return XibJob{ib_xib(h, k, 1), h->dm, h->twx64, blk0, h->dm.pz / 4}; });
+          [](const lh_handle* h, int u) {
+            return u == 0 ? (h->dm.pz / 2) * h->nibtip : (h->dm.pz / 4) * (h->nibent - h->nibtip);
+          },
+          [](const lh_handle* h, int u, int k, int blk0) {
+            return XibJob{ib_xib(h, k, u), h->dm, h->twx64, blk0, h->dm.pz / (u == 0 ? 2 : 4)};
+          });
     return e;
   }
-  void release() { ri.release(); zi.release(); zf.release(); xt.release(); xp.release(); }
+  void release() { ri.release(); zi.release(); zf.release(); xb.release(); }
   // the launches of the canopy stage of substep k of every channel in the sets, in stage
-  // order: rhs, inverse z, x (tip entries, then the others), forward z
+  // order: rhs, inverse z, x (tip and other entries together), forward z
   void launch_rhs(int k, hipStream_t st) const {
     for (const auto& G : ri.gs)
       hipLaunchKernelGGL(k_rhs_ib_jobs, dim3(G.nblk), dim3(64), 0, st, ri.djobs + G.job0[k], ri.dmap + G.map0);
@@ -817,10 +822,8 @@ struct IbSets {
       LH_DISPATCH_PLAN_SMALL(G.len, G.plan, launch_zinv_jobs_t, double, st, zi.djobs + G.job0[k], zi.dmap + G.map0, G.nblk)
   }
   void launch_x(int k, hipStream_t st) const {
-    for (const auto& G : xt.gs)
-      LH_DISPATCH_PLAN_SMALL(G.len, G.plan, launch_xib_jobs_t, double, st, xt.djobs + G.job0[k], xt.dmap + G.map0, G.nblk)
-    for (const auto& G : xp.gs)
-      LH_DISPATCH_PLAN_SMALL(G.len, G.plan, launch_xib_jobs_t, double, st, xp.djobs + G.job0[k], xp.dmap + G.map0, G.nblk)
+    for (const auto& G : xb.gs)
+      LH_DISPATCH_PLAN_SMALL(G.len, G.plan, launch_xib_jobs_t, double, st, xb.djobs + G.job0[k], xb.dmap + G.map0, G.nblk)
   }
   void launch_zfwd(int k, hipStream_t st) const {
     for (const auto& G : zf.gs)
